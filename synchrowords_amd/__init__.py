@@ -1,0 +1,362 @@
+"""synchrowords_amd — MI355X-native shortest-reset-word search (SynchroLib's Exact path).
+
+Python mirror of the C ABI in ``include/synchrowords_amd.h`` (ctypes).  The
+HIP library ``libsynchrowords_amd.so`` is built in-tree by
+``python -m synchrowords_amd.build``; importing this package does not need it,
+but every call that computes does, and fails loudly when it is missing: there
+is no CPU path behind these functions.
+
+Sets are numpy ``uint64`` arrays of shape ``(count, W)``, ``W = ceil(n/64)``,
+state q at bit q%64 of word q/64 — the reference's ``Subset<N>`` layout.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "Context", "ExactConfig", "ExactResult", "PipelineResult", "Leaf", "load", "words", "DeviceArray",
+]
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libsynchrowords_amd.so"
+
+SW_HOST = 0
+SW_DEVICE = 1
+
+_lib = None
+
+
+class SwError(RuntimeError):
+    """A failed call into the HIP library (message from sw_last_error)."""
+
+
+def words(n: int) -> int:
+    return (n + 63) // 64
+
+
+class _ExactConfig(C.Structure):
+    _fields_ = [("max_memory_mb", C.c_uint64), ("dominance_pairs", C.c_uint64), ("forward_cap", C.c_uint64),
+                ("dfs", C.c_int32), ("verbose", C.c_int32)]
+
+
+class _ExactResult(C.Structure):
+    _fields_ = [("found", C.c_int32), ("bound_word", C.c_int32), ("length", C.c_uint64), ("word_len", C.c_uint64),
+                ("steps_forward", C.c_uint64), ("steps_backward", C.c_uint64), ("sets_forward", C.c_uint64),
+                ("sets_backward", C.c_uint64), ("pair_tests", C.c_uint64), ("hash_probes", C.c_uint64),
+                ("ms_total", C.c_double)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("non_synchro", C.c_int32), ("n_algorithms", C.c_int32), ("lower", C.c_uint64),
+                ("upper", C.c_uint64), ("word_len", C.c_uint64), ("eppstein_bound", C.c_uint64),
+                ("beam_bound", C.c_uint64), ("ms", C.c_double * 8), ("names", (C.c_char * 16) * 8),
+                ("exact", _ExactResult)]
+
+
+_P = C.c_void_p
+_U64P = C.POINTER(C.c_uint64)
+_U32P = C.POINTER(C.c_uint32)
+
+_SIGS = {
+    "sw_last_error": (C.c_char_p, []),
+    "sw_version": (C.c_int, []),
+    "sw_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "sw_ctx_destroy": (None, [_P]),
+    "sw_device_info": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_int), _U64P]),
+    "sw_dev_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "sw_dev_free": (C.c_int, [_P, _P]),
+    "sw_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "sw_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "sw_sync": (C.c_int, [_P]),
+    "sw_stream_timer": (C.c_int, [_P, C.c_int, C.POINTER(C.c_float)]),
+    "sw_transition_apply": (C.c_int, [_P, C.c_uint32, _P, _P, _P, C.c_uint64, C.c_int]),
+    "sw_expand": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_int, _P, C.c_uint64, _P, C.c_int]),
+    "sw_leaf_create": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
+    "sw_leaf_run": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_uint32, _P, C.c_uint32, _P]),
+    "sw_leaf_destroy": (None, [_P]),
+    "sw_subset_check": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, _P, C.c_uint64, C.c_uint32, C.c_int, _P, _U64P,
+                                  C.POINTER(C.c_int32)]),
+    "sw_count_bits": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, _P, C.c_int]),
+    "sw_exact_solve": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint64, C.c_uint64, _P, C.c_uint64,
+                                 C.POINTER(_ExactConfig), C.POINTER(_ExactResult), _P, C.c_uint64]),
+    "sw_pipeline_solve": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_char_p, C.POINTER(_Result), _P,
+                                    C.c_uint64]),
+}
+
+
+def load(path: str | os.PathLike | None = None):
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise SwError(f"{p} is missing: build it with `python -m synchrowords_amd.build` "
+                      "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = load().sw_last_error()
+        raise SwError(f"synchrowords_amd error {rc}: {msg.decode() if msg else ''}")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p) if a is not None and a.size else None
+
+
+def _sets(a, n: int) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    if a.ndim == 1:
+        a = a.reshape(-1, words(n)) if a.size else a.reshape(0, words(n))
+    if a.shape[1] != words(n):
+        raise ValueError(f"sets must have {words(n)} words per row for n={n}")
+    return a
+
+
+def _delta(n: int, k: int, delta) -> np.ndarray:
+    d = np.ascontiguousarray(delta, dtype=np.uint32).reshape(-1)
+    if d.size != n * k:
+        raise ValueError("delta must have n*k entries")
+    return d
+
+
+@dataclass
+class ExactConfig:
+    max_memory_mb: int = 0
+    dominance_pairs: int = 0
+    forward_cap: int = 0
+    dfs: bool = True
+    verbose: bool = False
+
+    def _c(self) -> _ExactConfig:
+        return _ExactConfig(self.max_memory_mb, self.dominance_pairs, self.forward_cap, int(self.dfs),
+                            int(self.verbose))
+
+
+@dataclass
+class ExactResult:
+    found: bool
+    length: int
+    word: list[int] | None
+    bound_word: bool
+    stats: dict = field(default_factory=dict)
+
+
+@dataclass
+class PipelineResult:
+    non_synchro: bool
+    lower: int
+    upper: int
+    word: list[int] | None
+    eppstein_bound: int
+    beam_bound: int
+    times_ms: list[tuple[str, float]]
+    exact: dict
+
+
+class DeviceArray:
+    """A device buffer owned by a context (for inputs resident in HBM)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check(load().sw_dev_alloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, ctx: "Context", a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.nbytes)
+        if a.nbytes:
+            _check(load().sw_memcpy_h2d(ctx.handle, d.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+        return d
+
+    def to_host(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        _check(load().sw_memcpy_d2h(self.ctx.handle, out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            _check(load().sw_dev_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.handle:
+                load().sw_dev_free(self.ctx.handle, self.ptr)
+        except Exception:
+            pass
+
+
+class Leaf:
+    """The reference's SubsetsImplicitTrieKernel (allocate / run / deallocate)."""
+
+    def __init__(self, ctx: "Context", n: int, small, large_cnt: int):
+        self.ctx = ctx
+        self.n = n
+        s = _sets(small, n)
+        h = C.c_void_p()
+        _check(load().sw_leaf_create(ctx.handle, n, _ptr(s), s.shape[0], large_cnt, C.byref(h)))
+        self.handle = h
+
+    def run(self, small_pos: int, small_cnt: int, large, proper: bool = False) -> np.ndarray:
+        lg = _sets(large, self.n)
+        ret = np.zeros(lg.shape[0], dtype=np.uint8)
+        _check(load().sw_leaf_run(self.handle, int(proper), small_pos, small_cnt, _ptr(lg), lg.shape[0], _ptr(ret)))
+        return ret.astype(bool)
+
+    def close(self) -> None:
+        if self.handle:
+            load().sw_leaf_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device, one stream (sw_ctx)."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.sw_ctx_create(device, C.byref(h)))
+        self.handle = h
+
+    def close(self) -> None:
+        if self.handle:
+            load().sw_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self) -> dict:
+        name = C.create_string_buffer(64)
+        cus = C.c_int()
+        mem = C.c_uint64()
+        _check(load().sw_device_info(self.handle, name, 64, C.byref(cus), C.byref(mem)))
+        return {"name": name.value.decode(), "cus": cus.value, "mem_bytes": mem.value}
+
+    def sync(self) -> None:
+        _check(load().sw_sync(self.handle))
+
+    def timer_start(self) -> None:
+        """Record a HIP event on the context's stream."""
+        _check(load().sw_stream_timer(self.handle, 0, None))
+
+    def timer_stop(self) -> float:
+        """Milliseconds on the context's stream since timer_start()."""
+        ms = C.c_float()
+        _check(load().sw_stream_timer(self.handle, 1, C.byref(ms)))
+        return float(ms.value)
+
+    # ------------------------------------------------ kernel-level calls
+    def transition_apply(self, n: int, table, sets) -> np.ndarray:
+        s = _sets(sets, n)
+        t = np.ascontiguousarray(table, dtype=np.uint64)
+        out = np.zeros_like(s)
+        _check(load().sw_transition_apply(self.handle, n, _ptr(t), _ptr(s), _ptr(out), s.shape[0], SW_HOST))
+        return out
+
+    def expand(self, n: int, k: int, delta, sets, backward: bool = False) -> np.ndarray:
+        s = _sets(sets, n)
+        d = _delta(n, k, delta)
+        out = np.zeros((k * s.shape[0], words(n)), dtype=np.uint64)
+        _check(load().sw_expand(self.handle, n, k, _ptr(d), int(backward), _ptr(s), s.shape[0], _ptr(out), SW_HOST))
+        return out
+
+    def subset_check(self, n: int, small, large, thresh: int = 1, per_row: bool = False, count: bool = True):
+        """(results or None, pair_count or None, any) of the list-vs-list checker."""
+        s = _sets(small, n)
+        lg = _sets(large, n)
+        res = np.zeros(s.shape[0], dtype=np.uint8) if per_row else None
+        cnt = C.c_uint64()
+        anyf = C.c_int32()
+        _check(load().sw_subset_check(self.handle, n, _ptr(s), s.shape[0], _ptr(lg), lg.shape[0], thresh, SW_HOST,
+                                      _ptr(res) if per_row else None,
+                                      C.byref(cnt) if (count or per_row) else None, C.byref(anyf)))
+        return (res.astype(bool) if per_row else None), (cnt.value if (count or per_row) else None), bool(anyf.value)
+
+    def subset_check_device(self, n: int, small: DeviceArray, small_cnt: int, large: DeviceArray, large_cnt: int,
+                            count: bool = True):
+        cnt = C.c_uint64()
+        anyf = C.c_int32()
+        _check(load().sw_subset_check(self.handle, n, small.ptr, small_cnt, large.ptr, large_cnt, 1, SW_DEVICE,
+                                      None, C.byref(cnt) if count else None, C.byref(anyf)))
+        return (cnt.value if count else None), bool(anyf.value)
+
+    def count_bits(self, n: int, sets) -> np.ndarray:
+        s = _sets(sets, n)
+        acc = np.zeros(n, dtype=np.uint64)
+        _check(load().sw_count_bits(self.handle, n, _ptr(s), s.shape[0], _ptr(acc), SW_HOST))
+        return acc
+
+    def leaf(self, n: int, small, large_cnt: int) -> Leaf:
+        return Leaf(self, n, small, large_cnt)
+
+    # ------------------------------------------------- algorithm-level calls
+    def exact(self, n: int, k: int, delta, lower: int = 0, upper: int | None = None, upper_word=None,
+              cfg: ExactConfig | None = None) -> ExactResult:
+        d = _delta(n, k, delta)
+        if upper is None:
+            upper = (n - 1) * (n - 1) + 1
+        uw = np.ascontiguousarray(upper_word if upper_word is not None else [], dtype=np.uint32)
+        cap = max(int(upper), 1)
+        word = np.zeros(min(cap, 1 << 24), dtype=np.uint32)
+        r = _ExactResult()
+        c = (cfg or ExactConfig())._c()
+        _check(load().sw_exact_solve(self.handle, n, k, _ptr(d), lower, upper, _ptr(uw), uw.size, C.byref(c),
+                                     C.byref(r), _ptr(word), word.size))
+        w = [int(x) for x in word[: r.word_len]] if r.word_len else None
+        stats = {f: getattr(r, f) for f, _ in _ExactResult._fields_}
+        return ExactResult(bool(r.found), int(r.length), w, bool(r.bound_word), stats)
+
+    def pipeline(self, n: int, k: int, delta, plan: dict | str | None = None) -> PipelineResult:
+        d = _delta(n, k, delta)
+        if isinstance(plan, dict):
+            plan = json.dumps(plan)
+        word = np.zeros(max((n - 1) * (n - 1) + 1, 1), dtype=np.uint32)
+        r = _Result()
+        _check(load().sw_pipeline_solve(self.handle, n, k, _ptr(d), plan.encode() if plan else None, C.byref(r),
+                                        _ptr(word), word.size))
+        names = [bytes(r.names[i]).split(b"\0")[0].decode() for i in range(r.n_algorithms)]
+        times = [(names[i], float(r.ms[i])) for i in range(r.n_algorithms)]
+        w = [int(x) for x in word[: r.word_len]] if r.word_len else None
+        ex = {f: getattr(r.exact, f) for f, _ in _ExactResult._fields_}
+        return PipelineResult(bool(r.non_synchro), int(r.lower), int(r.upper), w, int(r.eppstein_bound),
+                              int(r.beam_bound), times, ex)
+
+
+def replay(n: int, k: int, delta, word) -> int:
+    """|delta(Q, word)| computed directly (host check of a returned word)."""
+    d = np.asarray(delta, dtype=np.int64).reshape(n, k)
+    cur = np.ones(n, dtype=bool)
+    for a in word:
+        nxt = np.zeros(n, dtype=bool)
+        nxt[d[cur, int(a)]] = True
+        cur = nxt
+    return int(cur.sum())

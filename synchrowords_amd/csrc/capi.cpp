@@ -1,0 +1,398 @@
+// C ABI: housekeeping and the kernel-level entry points.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "sw_host.hpp"
+#include "sw_kernels.hpp"
+
+namespace sw {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+void use(sw_ctx* ctx) { SW_CHECK_HIP(hipSetDevice(ctx->device)); }
+
+}  // namespace sw
+
+using sw::DevBuf;
+using sw::Error;
+
+#define SW_API_BEGIN try {
+#define SW_API_END                      \
+  }                                     \
+  catch (const Error& e) {              \
+    sw::set_error(e.msg);               \
+    return e.code;                      \
+  }                                     \
+  catch (const std::bad_alloc&) {       \
+    sw::set_error("host out of memory");\
+    return SW_ERR_NOMEM;                \
+  }                                     \
+  catch (const std::exception& e) {     \
+    sw::set_error(e.what());            \
+    return SW_ERR_INVALID;              \
+  }                                     \
+  return SW_OK;
+
+namespace {
+
+uint32_t words_of(uint32_t n) { return (n + 63) / 64; }
+
+void check_n(uint32_t n) {
+  SW_REQUIRE(n >= 1, "n must be at least 1");
+  if (n > 512) throw Error{SW_ERR_UNSUPPORTED, "n above 512 states is not compiled"};
+}
+
+// Device view of a caller list: the pointer itself (SW_DEVICE) or a copy in
+// scratch buffer `buf` (SW_HOST).
+const uint64_t* to_device(sw_ctx* ctx, DevBuf& buf, const uint64_t* p, uint64_t words, int where) {
+  if (where == SW_DEVICE) return p;
+  SW_REQUIRE(where == SW_HOST, "where must be SW_HOST or SW_DEVICE");
+  buf.ensure(words * 8 + 8);
+  if (words) SW_CHECK_HIP(hipMemcpyAsync(buf.p, p, words * 8, hipMemcpyHostToDevice, ctx->stream));
+  return buf.as<uint64_t>();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sw_last_error(void) { return sw::g_error.c_str(); }
+
+int sw_version(void) { return 10000; }
+
+int sw_ctx_create(int device, sw_ctx** out) {
+  SW_API_BEGIN
+  SW_REQUIRE(out, "out is NULL");
+  int count = 0;
+  SW_CHECK_HIP(hipGetDeviceCount(&count));
+  SW_REQUIRE(device >= 0 && device < count, "no such HIP device");
+  SW_CHECK_HIP(hipSetDevice(device));
+  auto ctx = std::make_unique<sw_ctx>();
+  ctx->device = device;
+  SW_CHECK_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  hipDeviceProp_t prop;
+  SW_CHECK_HIP(hipGetDeviceProperties(&prop, device));
+  ctx->name = prop.gcnArchName;
+  ctx->cus = prop.multiProcessorCount;
+  ctx->mem = prop.totalGlobalMem;
+  *out = ctx.release();
+  SW_API_END
+}
+
+void sw_ctx_destroy(sw_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->a.release();
+  ctx->b.release();
+  ctx->c.release();
+  ctx->d.release();
+  ctx->e.release();
+  if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+  if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int sw_device_info(sw_ctx* ctx, char* name, size_t name_cap, int* cus, uint64_t* mem_bytes) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx, "ctx is NULL");
+  if (name && name_cap) {
+    std::snprintf(name, name_cap, "%s", ctx->name.c_str());
+  }
+  if (cus) *cus = ctx->cus;
+  if (mem_bytes) *mem_bytes = ctx->mem;
+  SW_API_END
+}
+
+int sw_dev_alloc(sw_ctx* ctx, size_t bytes, void** out) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx && out, "NULL argument");
+  sw::use(ctx);
+  SW_CHECK_HIP(hipMalloc(out, bytes ? bytes : 1));
+  SW_API_END
+}
+
+int sw_dev_free(sw_ctx* ctx, void* p) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx, "ctx is NULL");
+  sw::use(ctx);
+  if (p) SW_CHECK_HIP(hipFree(p));
+  SW_API_END
+}
+
+int sw_memcpy_h2d(sw_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx, "ctx is NULL");
+  sw::use(ctx);
+  SW_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  SW_API_END
+}
+
+int sw_memcpy_d2h(sw_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx, "ctx is NULL");
+  sw::use(ctx);
+  SW_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  SW_API_END
+}
+
+int sw_sync(sw_ctx* ctx) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx, "ctx is NULL");
+  sw::use(ctx);
+  SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  SW_API_END
+}
+
+int sw_stream_timer(sw_ctx* ctx, int op, float* ms) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx, "ctx is NULL");
+  sw::use(ctx);
+  if (!ctx->ev_start) {
+    SW_CHECK_HIP(hipEventCreate(&ctx->ev_start));
+    SW_CHECK_HIP(hipEventCreate(&ctx->ev_stop));
+  }
+  if (op == 0) {
+    SW_CHECK_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+  } else {
+    SW_REQUIRE(ms, "ms is NULL");
+    SW_CHECK_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+    SW_CHECK_HIP(hipEventSynchronize(ctx->ev_stop));
+    SW_CHECK_HIP(hipEventElapsedTime(ms, ctx->ev_start, ctx->ev_stop));
+  }
+  SW_API_END
+}
+
+int sw_transition_apply(sw_ctx* ctx, uint32_t n, const uint64_t* table, const uint64_t* from, uint64_t* to,
+    uint64_t cnt, int where) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx && table && (from || !cnt) && (to || !cnt), "NULL argument");
+  check_n(n);
+  sw::use(ctx);
+  const uint32_t W = words_of(n), S = (n + 7) / 8;
+  const size_t tw = size_t(S) * 256 * W;
+  ctx->a.ensure(tw * 8);
+  SW_CHECK_HIP(hipMemcpyAsync(ctx->a.p, table, tw * 8, hipMemcpyHostToDevice, ctx->stream));
+  const uint64_t* din = to_device(ctx, ctx->b, from, cnt * W, where);
+  uint64_t* dout = to;
+  if (where == SW_HOST) {
+    ctx->c.ensure(cnt * W * 8 + 8);
+    dout = ctx->c.as<uint64_t>();
+  }
+  SW_CHECK_HIP(sw::launch_expand(int(W), din, cnt, ctx->a.as<uint64_t>(), S, 1, nullptr, n, 0, dout, nullptr,
+      ctx->stream));
+  if (where == SW_HOST && cnt)
+    SW_CHECK_HIP(hipMemcpyAsync(to, dout, cnt * W * 8, hipMemcpyDeviceToHost, ctx->stream));
+  SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  SW_API_END
+}
+
+int sw_expand(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, int dir, const uint64_t* in, uint64_t cnt,
+    uint64_t* out, int where) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx && delta && (in || !cnt) && (out || !cnt), "NULL argument");
+  SW_REQUIRE(dir == 0 || dir == 1, "dir must be 0 or 1");
+  SW_REQUIRE(k >= 1 && k <= 255, "k must be in [1, 255]");
+  check_n(n);
+  for (size_t i = 0; i < size_t(n) * k; ++i) SW_REQUIRE(delta[i] < n, "transition out of range");
+  sw::use(ctx);
+  const uint32_t W = words_of(n), S = (n + 7) / 8;
+  auto t = sw::build_tables(n, k, delta, dir);
+  ctx->a.ensure(t.size() * 8);
+  SW_CHECK_HIP(hipMemcpyAsync(ctx->a.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  const uint64_t* din = to_device(ctx, ctx->b, in, cnt * W, where);
+  uint64_t* dout = out;
+  if (where == SW_HOST) {
+    ctx->c.ensure(cnt * k * W * 8 + 8);
+    dout = ctx->c.as<uint64_t>();
+  }
+  SW_CHECK_HIP(sw::launch_expand(int(W), din, cnt, ctx->a.as<uint64_t>(), S, k, nullptr, n, 0, dout, nullptr,
+      ctx->stream));
+  if (where == SW_HOST && cnt)
+    SW_CHECK_HIP(hipMemcpyAsync(out, dout, cnt * k * W * 8, hipMemcpyDeviceToHost, ctx->stream));
+  SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  SW_API_END
+}
+
+}  // extern "C"
+
+struct sw_leaf {
+  sw_ctx* ctx;
+  uint32_t n;
+  uint32_t small_cnt, large_cap;
+  DevBuf small, large, flags;
+};
+
+extern "C" {
+
+int sw_leaf_create(sw_ctx* ctx, uint32_t n, const uint64_t* small, uint32_t small_cnt, uint32_t large_cnt,
+    sw_leaf** out) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx && out && (small || !small_cnt), "NULL argument");
+  check_n(n);
+  sw::use(ctx);
+  auto leaf = std::make_unique<sw_leaf>();
+  leaf->ctx = ctx;
+  leaf->n = n;
+  leaf->small_cnt = small_cnt;
+  leaf->large_cap = large_cnt;
+  const uint32_t W = words_of(n);
+  leaf->small.ensure(size_t(small_cnt) * W * 8 + 8);
+  if (small_cnt)
+    SW_CHECK_HIP(hipMemcpyAsync(leaf->small.p, small, size_t(small_cnt) * W * 8, hipMemcpyHostToDevice, ctx->stream));
+  leaf->large.ensure(size_t(large_cnt) * W * 8 + 8);
+  leaf->flags.ensure(sw::vsub_flag_bytes(int(W), large_cnt) + 8);
+  SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  *out = leaf.release();
+  SW_API_END
+}
+
+int sw_leaf_run(sw_leaf* leaf, int proper, uint32_t small_pos, uint32_t small_cnt, const uint64_t* large,
+    uint32_t large_cnt, uint8_t* ret) {
+  SW_API_BEGIN
+  SW_REQUIRE(leaf && (large || !large_cnt) && (ret || !large_cnt), "NULL argument");
+  SW_REQUIRE(uint64_t(small_pos) + small_cnt <= leaf->small_cnt, "small range out of bounds");
+  SW_REQUIRE(large_cnt <= leaf->large_cap, "large_cnt above the allocated size");
+  sw_ctx* ctx = leaf->ctx;
+  sw::use(ctx);
+  const uint32_t W = words_of(leaf->n);
+  if (!large_cnt) return SW_OK;
+  SW_CHECK_HIP(hipMemcpyAsync(leaf->large.p, large, size_t(large_cnt) * W * 8, hipMemcpyHostToDevice, ctx->stream));
+  const size_t fb = sw::vsub_flag_bytes(int(W), large_cnt);
+  SW_CHECK_HIP(hipMemsetAsync(leaf->flags.p, 0, fb, ctx->stream));
+  sw::VsubParams p;
+  p.stream = leaf->small.as<uint64_t>() + size_t(small_pos) * W;
+  p.s_count = small_cnt;
+  p.tile = leaf->large.as<uint64_t>();
+  p.t_count = large_cnt;
+  p.n = leaf->n;
+  p.mode = sw::VSUB_FLAGS;
+  p.proper_by_size = proper ? 1 : 0;
+  p.flags = leaf->flags.p;
+  SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
+  std::unique_ptr<uint8_t[]> bits(new uint8_t[fb]);
+  SW_CHECK_HIP(hipMemcpyAsync(bits.get(), leaf->flags.p, fb, hipMemcpyDeviceToHost, ctx->stream));
+  SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  for (uint32_t j = 0; j < large_cnt; ++j) ret[j] = (bits[j >> 3] >> (j & 7)) & 1;
+  SW_API_END
+}
+
+void sw_leaf_destroy(sw_leaf* leaf) {
+  if (!leaf) return;
+  (void)hipSetDevice(leaf->ctx->device);
+  (void)hipStreamSynchronize(leaf->ctx->stream);
+  delete leaf;
+}
+
+int sw_subset_check(sw_ctx* ctx, uint32_t n, const uint64_t* small, uint64_t small_cnt, const uint64_t* large,
+    uint64_t large_cnt, uint32_t thresh, int where, uint8_t* results, uint64_t* pair_count, int32_t* any) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx && (small || !small_cnt) && (large || !large_cnt), "NULL argument");
+  check_n(n);
+  SW_REQUIRE(small_cnt < (1ull << 32) && large_cnt < (1ull << 32), "lists above 2^32 sets");
+  sw::use(ctx);
+  const uint32_t W = words_of(n);
+  const uint64_t* ds = to_device(ctx, ctx->a, small, small_cnt * W, where);
+  const uint64_t* dl = to_device(ctx, ctx->b, large, large_cnt * W, where);
+  // Scratch: [0] pair count, [8] found flag, [16] witness.
+  ctx->c.ensure(64);
+  unsigned long long* dcount = ctx->c.as<unsigned long long>();
+  int* dfound = reinterpret_cast<int*>(ctx->c.as<char>() + 8);
+  unsigned long long* dwit = reinterpret_cast<unsigned long long*>(ctx->c.as<char>() + 16);
+  SW_CHECK_HIP(hipMemsetAsync(ctx->c.p, 0, 16, ctx->stream));
+  SW_CHECK_HIP(hipMemsetAsync(dwit, 0xff, 8, ctx->stream));
+  sw::VsubParams p;
+  p.stream = ds;
+  p.s_count = small_cnt;
+  p.tile = dl;
+  p.t_count = large_cnt;
+  p.n = n;
+  uint64_t total = 0;
+  int32_t found = 0;
+  if (results) {
+    ctx->d.ensure(small_cnt * 4 + 8);
+    SW_CHECK_HIP(hipMemsetAsync(ctx->d.p, 0, small_cnt * 4, ctx->stream));
+    p.mode = sw::VSUB_SCOUNT;
+    p.scount = ctx->d.as<uint32_t>();
+    SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
+    std::unique_ptr<uint32_t[]> c(new uint32_t[small_cnt + 1]);
+    if (small_cnt)
+      SW_CHECK_HIP(hipMemcpyAsync(c.get(), ctx->d.p, small_cnt * 4, hipMemcpyDeviceToHost, ctx->stream));
+    SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint64_t i = 0; i < small_cnt; ++i) {
+      results[i] = c[i] >= thresh;
+      total += c[i];
+    }
+    found = total > 0;
+  } else if (pair_count) {
+    p.mode = sw::VSUB_COUNT;
+    p.count_out = dcount;
+    SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
+    unsigned long long c = 0;
+    SW_CHECK_HIP(hipMemcpyAsync(&c, dcount, 8, hipMemcpyDeviceToHost, ctx->stream));
+    SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    total = c;
+    found = c > 0;
+  } else {
+    p.mode = sw::VSUB_ANY;
+    p.found = dfound;
+    p.witness = dwit;
+    SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
+    SW_CHECK_HIP(hipMemcpyAsync(&found, dfound, 4, hipMemcpyDeviceToHost, ctx->stream));
+    SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  if (pair_count) *pair_count = total;
+  if (any) *any = found ? 1 : 0;
+  SW_API_END
+}
+
+int sw_count_bits(sw_ctx* ctx, uint32_t n, const uint64_t* sets, uint64_t cnt, uint64_t* acc, int where) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx && acc && (sets || !cnt), "NULL argument");
+  check_n(n);
+  sw::use(ctx);
+  const uint32_t W = words_of(n);
+  const uint64_t* d = to_device(ctx, ctx->a, sets, cnt * W, where);
+  ctx->c.ensure(size_t(n) * 8);
+  SW_CHECK_HIP(hipMemsetAsync(ctx->c.p, 0, size_t(n) * 8, ctx->stream));
+  SW_CHECK_HIP(sw::launch_count_bits(int(W), d, cnt, n, ctx->c.as<unsigned long long>(), ctx->stream));
+  SW_CHECK_HIP(hipMemcpyAsync(acc, ctx->c.p, size_t(n) * 8, hipMemcpyDeviceToHost, ctx->stream));
+  SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  SW_API_END
+}
+
+int sw_exact_solve(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint64_t lower, uint64_t upper,
+    const uint32_t* upper_word, uint64_t upper_word_len, const sw_exact_config* cfg, sw_exact_result* out,
+    uint32_t* word, uint64_t word_cap) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx && delta && out, "NULL argument");
+  SW_REQUIRE(k >= 1 && k <= 255, "k must be in [1, 255]");
+  check_n(n);
+  for (size_t i = 0; i < size_t(n) * k; ++i) SW_REQUIRE(delta[i] < n, "transition out of range");
+  SW_REQUIRE(lower <= upper, "lower above upper");
+  sw_exact_config c{};
+  if (cfg) c = *cfg;
+  if (c.dfs == 0 && !cfg) c.dfs = 1;
+  auto res = sw::exact_search(ctx, n, k, delta, lower, upper, c);
+  *out = res.r;
+  if (res.r.bound_word) {
+    // The known word of the upper bound is optimal.
+    res.word.assign(upper_word ? upper_word : nullptr, upper_word ? upper_word + upper_word_len : nullptr);
+    if (!upper_word || upper_word_len != upper) res.word.clear();
+  }
+  out->word_len = res.word.size();
+  if (word && res.word.size() <= word_cap) std::copy(res.word.begin(), res.word.end(), word);
+  SW_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,453 @@
+// Device-resident exact search for the shortest reset word.
+//
+// The search is the reference's meet in the middle (Exact<N,K>,
+// synchrolib/algorithm/exact/exact.cpp:36-97; meet_in_the_middle.cpp:57-344)
+// re-planned for one GPU:
+//
+//   forward side   F_d = images of Q under words of length d,
+//   backward side  B_d = preimages of sink-component singletons under words
+//                  of length d, kept complemented (C = Q \ B; preimage commutes
+//                  with complement, so the same tables expand C),
+//   one step       expand the chosen side through slice tables, drop exact
+//                  duplicates of anything the side ever produced (device hash
+//                  set), then — while the pair budget allows — drop sets
+//                  dominated by a produced set of the same side (forward:
+//                  a proper subset; backward: a proper superset),
+//   goal check     some forward set inside some backward set, i.e. disjoint
+//                  from its complement (the vertical subset kernel, ANY mode).
+//
+// Every set the search ever keeps stays in a per-side arena with its parent
+// row and letter, so the meeting pair (A, B) is turned into a word u.v: u
+// spells the forward path to A, v the backward path from B to its singleton.
+//
+// Why the pruning is exact: a set X dropped at depth d has a witness Y kept at
+// depth <= d with Y inside X (forward) — any word continuing X continues Y at
+// no greater length — so a shortest word, if it passed through X, yields one
+// through Y of no greater length; induction on the depth keeps a witness in
+// each frontier.  Dropping backward sets of at most one state is the
+// reference's own rule (dfs.cpp:285-287): on a shortest word every backward
+// set strictly contains the forward image it meets, which has >= 2 states.
+//
+// The reference's cost model (meet_in_the_middle.cpp:346-551) and its DFS
+// phase (dfs.cpp) exist to stay inside host memory; with hundreds of GB of HBM
+// the device keeps whole levels, expands the smaller frontier, and freezes the
+// forward side past a size cap (the backward-only continuation, the role of
+// the reference's DFS).  Choices of side never change the answer.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <climits>
+
+#include "sw_host.hpp"
+#include "sw_kernels.hpp"
+
+namespace sw {
+
+namespace {
+
+struct Side {
+  DevBuf arena, parent, letter, table, front, front_idx;
+  uint64_t rows = 0;
+  uint64_t tcap = 0;
+  std::vector<uint64_t> level_end;  // arena rows after each level
+  uint64_t fcount = 0, prev_fcount = 0;
+  uint64_t steps = 0;
+  bool frozen = false;
+};
+
+class Search {
+ public:
+  Search(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, const sw_exact_config& cfg)
+      : ctx_(ctx), st_(ctx->stream), n_(n), k_(k), W_((n + 63) / 64), slices_((n + 7) / 8), delta_(delta),
+        cfg_(cfg) {
+    budget_pairs_ = cfg.dominance_pairs ? cfg.dominance_pairs : 100000000000ull;
+    forward_cap_ = cfg.forward_cap ? cfg.forward_cap : (1ull << 22);
+    mem_cap_ = (cfg.max_memory_mb ? cfg.max_memory_mb : 65536ull) << 20;
+  }
+
+  ExactOutcome run(uint64_t lower, uint64_t upper) {
+    auto t0 = std::chrono::steady_clock::now();
+    ExactOutcome out;
+    sw_exact_result& r = out.r;
+    std::memset(&r, 0, sizeof(r));
+    if (n_ == 1) {
+      r.found = 1;
+      r.length = 0;
+      return out;
+    }
+    upload_tables();
+    init_forward();
+    init_backward();
+    uint64_t t = 0;
+    bool found = false;
+    uint64_t wit = 0;
+    // The supplied upper bound is the length of a known word: search below it.
+    while (t + 1 < upper) {
+      if (F_.fcount == 0 || B_.fcount == 0) break;  // every continuation pruned
+      bool fwd;
+      if (F_.frozen) fwd = false;
+      else fwd = F_.fcount <= B_.fcount;
+      ++t;
+      step(fwd ? F_ : B_, !fwd, t);
+      if (cfg_.verbose)
+        std::fprintf(stderr, "[sw exact] len %llu %s |F| %llu |B| %llu rows F %llu B %llu\n",
+            (unsigned long long)t, fwd ? "F" : "B", (unsigned long long)F_.fcount,
+            (unsigned long long)B_.fcount, (unsigned long long)F_.rows, (unsigned long long)B_.rows);
+      if (goal_check(&wit)) {
+        found = true;
+        break;
+      }
+      if (!cfg_.dfs && F_.frozen) break;
+    }
+    r.steps_forward = F_.steps;
+    r.steps_backward = B_.steps;
+    r.sets_forward = F_.rows;
+    r.sets_backward = B_.rows;
+    r.pair_tests = pair_tests_;
+    r.hash_probes = read_probes();
+    if (found) {
+      r.found = 1;
+      r.length = t;
+      out.word = reconstruct(wit);
+      r.word_len = out.word.size();
+    } else if (t + 1 >= upper || F_.fcount == 0 || B_.fcount == 0) {
+      // No word shorter than the upper bound: the bound is the length.
+      r.found = 1;
+      r.bound_word = 1;
+      r.length = upper;
+    } else {
+      r.found = 0;
+      r.length = std::max<uint64_t>(lower, t + 1);
+    }
+    r.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return out;
+  }
+
+  void set_memory_error() { throw Error{SW_ERR_NOMEM, "exact search: device memory budget exceeded"}; }
+
+ private:
+  sw_ctx* ctx_;
+  hipStream_t st_;
+  uint32_t n_, k_, W_, slices_;
+  const uint32_t* delta_;
+  sw_exact_config cfg_;
+  uint64_t budget_pairs_, forward_cap_, mem_cap_;
+  uint64_t pair_tests_ = 0;
+  DevBuf tables_[2];
+  Side F_, B_;
+  DevBuf cand_, valid_, slot_, keep_, pos_, small_, scan_tmp_, flags_, probes_;
+  size_t scan_bytes_ = 0;
+
+  uint64_t row_bytes() const { return uint64_t(W_) * 8; }
+
+  void upload_tables() {
+    for (int dir = 0; dir < 2; ++dir) {
+      auto t = build_tables(n_, k_, delta_, dir);
+      tables_[dir].ensure(t.size() * 8);
+      SW_CHECK_HIP(hipMemcpyAsync(tables_[dir].p, t.data(), t.size() * 8, hipMemcpyHostToDevice, st_));
+    }
+    small_.ensure(64);
+    probes_.ensure(8);
+    SW_CHECK_HIP(hipMemsetAsync(probes_.p, 0, 8, st_));
+    SW_CHECK_HIP(hipStreamSynchronize(st_));
+  }
+
+  uint64_t read_probes() {
+    unsigned long long v = 0;
+    SW_CHECK_HIP(hipMemcpyAsync(&v, probes_.p, 8, hipMemcpyDeviceToHost, st_));
+    SW_CHECK_HIP(hipStreamSynchronize(st_));
+    return v;
+  }
+
+  uint32_t read_u32(const void* dev) {
+    uint32_t v = 0;
+    SW_CHECK_HIP(hipMemcpyAsync(&v, dev, 4, hipMemcpyDeviceToHost, st_));
+    SW_CHECK_HIP(hipStreamSynchronize(st_));
+    return v;
+  }
+
+  void ensure_table(Side& S, uint64_t incoming) {
+    const uint64_t need = (S.rows + incoming) * 2 + 1024;
+    if (S.tcap >= need) return;
+    uint64_t cap = 1024;
+    while (cap < need) cap <<= 1;
+    SW_REQUIRE(cap <= (1ull << 32), "exact search: hash table over 2^32 slots");
+    S.table.release();
+    S.table.ensure(cap * 4);
+    S.tcap = cap;
+    SW_CHECK_HIP(launch_fill_u32(S.table.as<uint32_t>(), kEmpty, cap, st_));
+    SW_CHECK_HIP(launch_hash_rebuild(int(W_), S.arena.as<uint64_t>(), S.rows, S.table.as<uint32_t>(), cap - 1, st_));
+  }
+
+  void ensure_rows(Side& S, uint64_t rows) {
+    if (rows * row_bytes() > mem_cap_ / 4) set_memory_error();
+    SW_REQUIRE(rows < 0x7fffffffull, "exact search: more than 2^31 sets on one side");
+    S.arena.ensure(rows * row_bytes(), st_, S.rows * row_bytes());
+    S.parent.ensure(rows * 4, st_, S.rows * 4);
+    S.letter.ensure(rows, st_, S.rows);
+  }
+
+  void ensure_scan(uint64_t count) {
+    size_t bytes = 0;
+    SW_CHECK_HIP(launch_scan(nullptr, nullptr, count, nullptr, nullptr, &bytes, st_));
+    if (bytes > scan_bytes_) {
+      scan_tmp_.release();
+      scan_tmp_.ensure(bytes);
+      scan_bytes_ = bytes;
+    }
+  }
+
+  // Seed a side with host rows (level 0).
+  void seed(Side& S, const std::vector<uint64_t>& rows) {
+    const uint64_t cnt = rows.size() / W_;
+    ensure_rows(S, cnt);
+    SW_CHECK_HIP(hipMemcpyAsync(S.arena.p, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, st_));
+    SW_CHECK_HIP(hipMemsetAsync(S.parent.p, 0xff, cnt * 4, st_));
+    SW_CHECK_HIP(hipMemsetAsync(S.letter.p, 0xff, cnt, st_));
+    S.rows = cnt;
+    S.level_end.push_back(cnt);
+    ensure_table(S, 0);
+    S.front.ensure(cnt * row_bytes());
+    S.front_idx.ensure(cnt * 4);
+    SW_CHECK_HIP(launch_select(int(W_), S.arena.as<uint64_t>(), 0, nullptr, nullptr, cnt, S.front.as<uint64_t>(),
+        S.front_idx.as<uint32_t>(), st_));
+    S.fcount = cnt;
+    S.prev_fcount = cnt;
+  }
+
+  void init_forward() {
+    std::vector<uint64_t> q(W_, 0);
+    for (uint32_t s = 0; s < n_; ++s) q[s >> 6] |= 1ull << (s & 63);
+    seed(F_, q);
+  }
+
+  // Sink-component states with two preimages under some letter
+  // (exact.cpp:148-164 of the reference); stored complemented.
+  void init_backward() {
+    const std::vector<char> sink = sink_states(n_, k_, delta_);
+    std::vector<uint32_t> indeg(size_t(n_) * k_, 0);
+    for (uint32_t q = 0; q < n_; ++q)
+      for (uint32_t a = 0; a < k_; ++a) indeg[size_t(delta_[size_t(q) * k_ + a]) * k_ + a]++;
+    std::vector<uint64_t> rows;
+    for (uint32_t q = 0; q < n_; ++q) {
+      if (!sink[q]) continue;
+      bool ok = false;
+      for (uint32_t a = 0; a < k_; ++a) ok |= indeg[size_t(q) * k_ + a] > 1;
+      if (!ok) continue;
+      for (uint32_t w = 0; w < W_; ++w) {
+        uint64_t x = ~0ull;
+        if (n_ < (w + 1) * 64) x = (n_ > w * 64) ? ((1ull << (n_ - w * 64)) - 1) : 0;
+        if ((q >> 6) == w) x &= ~(1ull << (q & 63));
+        rows.push_back(x);
+      }
+    }
+    seed(B_, rows);
+  }
+
+  // One expansion step of side S (backward: complemented sets).
+  void step(Side& S, bool backward, uint64_t t) {
+    (void)t;
+    const uint64_t m = S.fcount;
+    const uint64_t cnt = m * k_;
+    SW_REQUIRE(cnt < 0x7fffffffull, "exact search: step over 2^31 candidates");
+    if ((S.rows + cnt) * row_bytes() * 2 > mem_cap_) set_memory_error();
+    cand_.ensure(cnt * row_bytes());
+    valid_.ensure(cnt * 4);
+    slot_.ensure(cnt * 4);
+    keep_.ensure(cnt * 4 + 4);
+    pos_.ensure(cnt * 4 + 4);
+    ensure_scan(cnt);
+    const int W = int(W_);
+    SW_CHECK_HIP(launch_expand(W, S.front.as<uint64_t>(), m, tables_[backward ? 1 : 0].as<uint64_t>(), slices_, k_,
+        nullptr, n_, backward ? 1 : 0, cand_.as<uint64_t>(), valid_.as<uint32_t>(), st_));
+    ensure_table(S, cnt);
+    SW_CHECK_HIP(launch_hash_insert(W, cand_.as<uint64_t>(), valid_.as<uint32_t>(), cnt, S.table.as<uint32_t>(),
+        S.tcap - 1, S.arena.as<uint64_t>(), slot_.as<uint32_t>(), probes_.as<unsigned long long>(), st_));
+    SW_CHECK_HIP(launch_hash_resolve(slot_.as<uint32_t>(), S.table.as<uint32_t>(), cnt, keep_.as<uint32_t>(), st_));
+    uint32_t* total = small_.as<uint32_t>();
+    SW_CHECK_HIP(launch_scan(keep_.as<uint32_t>(), pos_.as<uint32_t>(), cnt, total, scan_tmp_.p, &scan_bytes_, st_));
+    const uint64_t nnew = read_u32(total);
+    ensure_rows(S, S.rows + nnew);
+    const uint64_t lo = S.rows;
+    SW_CHECK_HIP(launch_commit(W, cand_.as<uint64_t>(), keep_.as<uint32_t>(), pos_.as<uint32_t>(),
+        slot_.as<uint32_t>(), cnt, m, S.front_idx.as<uint32_t>(), 0, nullptr, lo, S.arena.as<uint64_t>(),
+        S.parent.as<uint32_t>(), S.letter.as<uint8_t>(), S.table.as<uint32_t>(), st_));
+    S.rows = lo + nnew;
+    S.level_end.push_back(S.rows);
+    S.steps++;
+    S.prev_fcount = S.fcount;
+    S.front.ensure(nnew * row_bytes() + 8);
+    S.front_idx.ensure(nnew * 4 + 4);
+    const bool dominance = nnew > 0 && S.rows * nnew <= budget_pairs_;
+    if (dominance) {
+      // Tile: the new level; stream: every row of the side (the new level
+      // included; a row is never its own witness).
+      const size_t fb = vsub_flag_bytes(W, nnew);
+      flags_.ensure(fb);
+      SW_CHECK_HIP(hipMemsetAsync(flags_.p, 0, fb, st_));
+      VsubParams p;
+      p.stream = S.arena.as<uint64_t>();
+      p.s_count = S.rows;
+      p.tile = S.arena.as<uint64_t>() + lo * W_;
+      p.t_count = nnew;
+      p.n = n_;
+      p.mode = VSUB_FLAGS;
+      p.diag = int64_t(lo);
+      p.flags = flags_.p;
+      SW_CHECK_HIP(launch_vsub(W, p, st_));
+      pair_tests_ += S.rows * nnew;
+      SW_CHECK_HIP(launch_alive(W, flags_.p, nnew, keep_.as<uint32_t>(), st_));
+      ensure_scan(nnew);
+      SW_CHECK_HIP(launch_scan(keep_.as<uint32_t>(), pos_.as<uint32_t>(), nnew, total, scan_tmp_.p, &scan_bytes_, st_));
+      SW_CHECK_HIP(launch_select(W, S.arena.as<uint64_t>(), lo, keep_.as<uint32_t>(), pos_.as<uint32_t>(), nnew,
+          S.front.as<uint64_t>(), S.front_idx.as<uint32_t>(), st_));
+      S.fcount = read_u32(total);
+    } else {
+      SW_CHECK_HIP(launch_select(W, S.arena.as<uint64_t>(), lo, nullptr, nullptr, nnew, S.front.as<uint64_t>(),
+          S.front_idx.as<uint32_t>(), st_));
+      S.fcount = nnew;
+    }
+    if (!backward && S.fcount >= forward_cap_) S.frozen = true;
+  }
+
+  // Some forward frontier set disjoint from some backward complement.
+  bool goal_check(uint64_t* witness) {
+    if (!F_.fcount || !B_.fcount) return false;
+    int* found = small_.as<int>() + 2;
+    unsigned long long* wit = reinterpret_cast<unsigned long long*>(small_.as<char>() + 16);
+    SW_CHECK_HIP(hipMemsetAsync(found, 0, 4, st_));
+    SW_CHECK_HIP(hipMemsetAsync(wit, 0xff, 8, st_));
+    VsubParams p;
+    p.stream = F_.front.as<uint64_t>();
+    p.s_count = F_.fcount;
+    p.tile = B_.front.as<uint64_t>();
+    p.t_count = B_.fcount;
+    p.n = n_;
+    p.complement_tile = 1;
+    p.mode = VSUB_ANY;
+    p.found = found;
+    p.witness = wit;
+    SW_CHECK_HIP(launch_vsub(int(W_), p, st_));
+    pair_tests_ += F_.fcount * B_.fcount;
+    int f = 0;
+    unsigned long long w = 0;
+    SW_CHECK_HIP(hipMemcpyAsync(&f, found, 4, hipMemcpyDeviceToHost, st_));
+    SW_CHECK_HIP(hipMemcpyAsync(&w, wit, 8, hipMemcpyDeviceToHost, st_));
+    SW_CHECK_HIP(hipStreamSynchronize(st_));
+    *witness = w;
+    return f != 0;
+  }
+
+  std::vector<uint32_t> path(Side& S, uint32_t frontier_pos) {
+    uint32_t row = 0;
+    SW_CHECK_HIP(hipMemcpyAsync(&row, S.front_idx.as<uint32_t>() + frontier_pos, 4, hipMemcpyDeviceToHost, st_));
+    std::vector<uint32_t> parent(S.rows);
+    std::vector<uint8_t> letter(S.rows);
+    SW_CHECK_HIP(hipMemcpyAsync(parent.data(), S.parent.p, S.rows * 4, hipMemcpyDeviceToHost, st_));
+    SW_CHECK_HIP(hipMemcpyAsync(letter.data(), S.letter.p, S.rows, hipMemcpyDeviceToHost, st_));
+    SW_CHECK_HIP(hipStreamSynchronize(st_));
+    std::vector<uint32_t> w;
+    const uint64_t level0 = S.level_end.front();
+    while (row >= level0) {
+      w.push_back(letter[row]);
+      row = parent[row];
+    }
+    return w;  // letters from the set back to level 0
+  }
+
+  std::vector<uint32_t> reconstruct(uint64_t witness) {
+    const uint32_t s = uint32_t(witness >> 32), t = uint32_t(witness & 0xffffffffu);
+    std::vector<uint32_t> u = path(F_, s);
+    std::reverse(u.begin(), u.end());  // forward letters from Q
+    std::vector<uint32_t> v = path(B_, t);  // backward letters towards the singleton
+    u.insert(u.end(), v.begin(), v.end());
+    return u;
+  }
+};
+
+}  // namespace
+
+// The sink component: the strongly connected component closed first by a
+// depth-first search from state 0 (utils/connectivity.cpp:105-160 of the
+// reference).  For a synchronizing automaton it is the unique sink component.
+std::vector<char> sink_states(uint32_t n, uint32_t k, const uint32_t* delta) {
+  std::vector<uint32_t> idx(n, 0), low(n, 0), stack;
+  std::vector<char> on(n, 0), sink(n, 0);
+  struct Frame {
+    uint32_t v, a;
+  };
+  uint32_t counter = 1;
+  std::vector<Frame> call{{0, 0}};
+  idx[0] = low[0] = counter++;
+  stack.push_back(0);
+  on[0] = 1;
+  while (!call.empty()) {
+    Frame& f = call.back();
+    if (f.a < k) {
+      const uint32_t w = delta[size_t(f.v) * k + f.a];
+      f.a++;
+      if (!idx[w]) {
+        idx[w] = low[w] = counter++;
+        stack.push_back(w);
+        on[w] = 1;
+        call.push_back({w, 0});
+      } else if (on[w]) {
+        low[f.v] = std::min(low[f.v], idx[w]);
+      }
+      continue;
+    }
+    const uint32_t v = f.v;
+    call.pop_back();
+    if (!call.empty()) low[call.back().v] = std::min(low[call.back().v], low[v]);
+    if (low[v] == idx[v]) {
+      uint32_t w;
+      do {
+        w = stack.back();
+        stack.pop_back();
+        on[w] = 0;
+        sink[w] = 1;
+      } while (w != v);
+      return sink;  // the first component closed has no edge leaving it
+    }
+  }
+  return sink;
+}
+
+std::vector<uint64_t> build_tables(uint32_t n, uint32_t k, const uint32_t* delta, int dir) {
+  const uint32_t W = (n + 63) / 64, S = (n + 7) / 8;
+  std::vector<uint64_t> t(size_t(k) * S * 256 * W, 0);
+  // Preimage lists per letter.
+  std::vector<std::vector<uint32_t>> pre;
+  if (dir == 1) {
+    pre.assign(size_t(n) * k, {});
+    for (uint32_t q = 0; q < n; ++q)
+      for (uint32_t a = 0; a < k; ++a) pre[size_t(delta[size_t(q) * k + a]) * k + a].push_back(q);
+  }
+  for (uint32_t a = 0; a < k; ++a)
+    for (uint32_t s = 0; s < S; ++s)
+      for (uint32_t b = 1; b < 256; ++b) {
+        uint64_t* row = &t[((size_t(a) * S + s) * 256 + b) * W];
+        for (uint32_t j = 0; j < 8; ++j) {
+          const uint32_t q = s * 8 + j;
+          if (q >= n || !((b >> j) & 1)) continue;
+          if (dir == 0) {
+            const uint32_t x = delta[size_t(q) * k + a];
+            row[x >> 6] |= 1ull << (x & 63);
+          } else {
+            for (uint32_t p : pre[size_t(q) * k + a]) row[p >> 6] |= 1ull << (p & 63);
+          }
+        }
+      }
+  return t;
+}
+
+ExactOutcome exact_search(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint64_t lower, uint64_t upper,
+    const sw_exact_config& cfg) {
+  use(ctx);
+  Search s(ctx, n, k, delta, cfg);
+  return s.run(lower, upper);
+}
+
+}  // namespace sw

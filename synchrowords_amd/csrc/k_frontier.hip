@@ -1,0 +1,341 @@
+// Frontier kernels: expansion through slice tables, exact dedup in an
+// open-addressing hash set, stream compaction, per-state counts.
+#include "sw_device.hpp"
+#include "sw_kernels.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+namespace sw {
+
+// ---------------------------------------------------------------- expansion
+// One thread per child.  The child is the OR over slices s of table row
+// [letter][s][byte s of the parent] (the reference's
+// PreprocessedTransition::apply, preprocessed_transition.hpp:31-49); a zero
+// byte selects the empty row and is skipped.
+template <int W>
+__global__ __launch_bounds__(256) void k_expand(const uint64_t* __restrict__ parents, uint64_t m,
+    const uint64_t* __restrict__ tables, uint32_t slices, uint32_t letters, const uint32_t* __restrict__ letter_list,
+    uint32_t n, int drop_mode, uint64_t* __restrict__ out, uint32_t* __restrict__ valid) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m * letters) return;
+  const uint64_t ai = i / m, j = i - ai * m;
+  const uint32_t a = letter_list ? letter_list[ai] : uint32_t(ai);
+  uint64_t x[W], r[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    x[w] = parents[j * W + w];
+    r[w] = 0;
+  }
+  const uint64_t* tab = tables + uint64_t(a) * slices * 256 * W;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    for (int b = 0; b < 8; ++b) {
+      const uint32_t s = uint32_t(w) * 8 + b;
+      if (s >= slices) break;
+      const uint32_t byte = uint32_t(x[w] >> (b * 8)) & 255u;
+      if (!byte) continue;
+      const uint64_t* row = tab + (uint64_t(s) * 256 + byte) * W;
+#pragma unroll
+      for (int v = 0; v < W; ++v) r[v] |= row[v];
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w) out[i * W + w] = r[w];
+  if (valid) {
+    uint32_t ok = 1;
+    if (drop_mode == 1) ok = set_popc<W>(r) + 1 < int(n);
+    valid[i] = ok;
+  }
+}
+
+template <int W>
+static hipError_t expand_w(const uint64_t* parents, uint64_t m, const uint64_t* tables, uint32_t slices,
+    uint32_t letters, const uint32_t* letter_list, uint32_t n, int drop_mode, uint64_t* out, uint32_t* valid,
+    hipStream_t st) {
+  const uint64_t total = m * letters;
+  if (!total) return hipSuccess;
+  const uint64_t blocks = (total + 255) / 256;
+  hipLaunchKernelGGL(k_expand<W>, dim3(unsigned(blocks)), dim3(256), 0, st, parents, m, tables, slices, letters,
+      letter_list, n, drop_mode, out, valid);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------- hashing
+// Linear probing.  A candidate claims an empty slot with CAS; on meeting an
+// equal set it is a duplicate, unless that set is a candidate of this step
+// with a larger index, which it then displaces — so the survivor of a group
+// of equal new sets is the one with the smallest index, independent of
+// timing.
+template <int W>
+__global__ __launch_bounds__(256) void k_hash_insert(const uint64_t* __restrict__ cand,
+    const uint32_t* __restrict__ valid, uint64_t count, uint32_t* table, uint64_t mask,
+    const uint64_t* __restrict__ arena, uint32_t* __restrict__ slot_of, unsigned long long* probes) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint32_t nprobe = 0;
+  if (i < count) {
+    uint32_t res = kEmpty;
+    if (!valid || valid[i]) {
+      uint64_t s[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) s[w] = cand[i * W + w];
+      const uint32_t mine = kCand | uint32_t(i);
+      uint64_t pos = set_hash<W>(s) & mask;
+      while (true) {
+        ++nprobe;
+        uint32_t v = __hip_atomic_load(&table[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool advance = false;
+        while (!advance) {
+          if (v == kEmpty) {
+            uint32_t prev = atomicCAS(&table[pos], kEmpty, mine);
+            if (prev == kEmpty) {
+              res = uint32_t(pos);
+              goto done;
+            }
+            v = prev;
+            continue;
+          }
+          const uint64_t* other = (v & kCand) ? cand + uint64_t(v & ~kCand) * W : arena + uint64_t(v) * W;
+          if (!set_eq<W>(s, other)) {
+            advance = true;
+            break;
+          }
+          if ((v & kCand) && (v & ~kCand) > uint32_t(i)) {
+            uint32_t prev = atomicCAS(&table[pos], v, mine);
+            if (prev == v) {
+              res = uint32_t(pos);
+              goto done;
+            }
+            v = prev;
+            continue;
+          }
+          goto done;  // equal to an older row or a smaller candidate
+        }
+        pos = (pos + 1) & mask;
+      }
+    done:
+      slot_of[i] = res;
+    } else {
+      slot_of[i] = kEmpty;
+    }
+  }
+  if (probes) {
+    uint64_t c = nprobe;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(probes, (unsigned long long)c);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_hash_resolve(const uint32_t* __restrict__ slot_of,
+    const uint32_t* __restrict__ table, uint64_t count, uint32_t* __restrict__ keep) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t s = slot_of[i];
+  keep[i] = (s != kEmpty && table[s] == (kCand | uint32_t(i))) ? 1u : 0u;
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_commit(const uint64_t* __restrict__ cand, const uint32_t* __restrict__ keep,
+    const uint32_t* __restrict__ pos, const uint32_t* __restrict__ slot_of, uint64_t count, uint64_t m,
+    const uint32_t* __restrict__ parent_idx, uint64_t parent_base, const uint32_t* __restrict__ letter_list,
+    uint64_t base, uint64_t* __restrict__ arena, uint32_t* __restrict__ parent, uint8_t* __restrict__ letter,
+    uint32_t* table) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count || !keep[i]) return;
+  const uint64_t dst = base + pos[i];
+#pragma unroll
+  for (int w = 0; w < W; ++w) arena[dst * W + w] = cand[i * W + w];
+  const uint64_t ai = i / m;
+  if (parent) parent[dst] = parent_idx ? parent_idx[i - ai * m] : uint32_t(parent_base + (i - ai * m));
+  if (letter) letter[dst] = uint8_t(letter_list ? letter_list[ai] : ai);
+  if (table) table[slot_of[i]] = uint32_t(dst);
+}
+
+__global__ __launch_bounds__(256) void k_alive(const uint8_t* __restrict__ dead, uint64_t count,
+    uint32_t* __restrict__ keep) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  keep[i] = ((dead[i >> 3] >> (i & 7)) & 1) ? 0u : 1u;
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_select(const uint64_t* __restrict__ rows, uint64_t lo,
+    const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos, uint64_t count, uint64_t* __restrict__ out,
+    uint32_t* __restrict__ idx_out) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count || (keep && !keep[i])) return;
+  const uint64_t j = pos ? pos[i] : i;
+#pragma unroll
+  for (int w = 0; w < W; ++w) out[j * W + w] = rows[(lo + i) * W + w];
+  if (idx_out) idx_out[j] = uint32_t(lo + i);
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_count_bits(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n,
+    unsigned long long* acc) {
+  __shared__ uint32_t hist[kMaxW * 64];
+  for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) hist[q] = 0;
+  __syncthreads();
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      uint64_t x = sets[i * W + w];
+      while (x) {
+        atomicAdd(&hist[w * 64 + __builtin_ctzll(x)], 1u);
+        x &= x - 1;
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t q = threadIdx.x; q < n; q += blockDim.x)
+    if (hist[q]) atomicAdd(&acc[q], (unsigned long long)hist[q]);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t count) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_scan_total(const uint32_t* keep, const uint32_t* pos, uint64_t count, uint32_t* total) {
+  *total = count ? pos[count - 1] + keep[count - 1] : 0u;
+}
+
+// ------------------------------------------------------------------ launchers
+#define SW_DISPATCH_W(W, CALL)      \
+  switch (W) {                      \
+    case 1: { constexpr int w_ = 1; return CALL; } \
+    case 2: { constexpr int w_ = 2; return CALL; } \
+    case 3: { constexpr int w_ = 3; return CALL; } \
+    case 4: { constexpr int w_ = 4; return CALL; } \
+    case 5: { constexpr int w_ = 5; return CALL; } \
+    case 6: { constexpr int w_ = 6; return CALL; } \
+    case 7: { constexpr int w_ = 7; return CALL; } \
+    case 8: { constexpr int w_ = 8; return CALL; } \
+  }                                 \
+  return hipErrorInvalidValue;
+
+static inline unsigned blocks_for(uint64_t count, unsigned bs = 256) { return unsigned((count + bs - 1) / bs); }
+
+hipError_t launch_expand(int W, const uint64_t* parents, uint64_t m, const uint64_t* tables, uint32_t slices,
+    uint32_t letters, const uint32_t* letter_list, uint32_t n, int drop_mode, uint64_t* out, uint32_t* valid,
+    hipStream_t st) {
+  SW_DISPATCH_W(W, expand_w<w_>(parents, m, tables, slices, letters, letter_list, n, drop_mode, out, valid, st))
+}
+
+template <int W>
+static hipError_t hash_insert_w(const uint64_t* cand, const uint32_t* valid, uint64_t count, uint32_t* table,
+    uint64_t mask, const uint64_t* arena, uint32_t* slot_of, unsigned long long* probes, hipStream_t st) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_hash_insert<W>, dim3(blocks_for(count)), dim3(256), 0, st, cand, valid, count, table, mask,
+      arena, slot_of, probes);
+  return hipGetLastError();
+}
+
+hipError_t launch_hash_insert(int W, const uint64_t* cand, const uint32_t* valid, uint64_t count, uint32_t* table,
+    uint64_t mask, const uint64_t* arena, uint32_t* slot_of, unsigned long long* probes, hipStream_t st) {
+  SW_DISPATCH_W(W, hash_insert_w<w_>(cand, valid, count, table, mask, arena, slot_of, probes, st))
+}
+
+hipError_t launch_hash_resolve(const uint32_t* slot_of, const uint32_t* table, uint64_t count, uint32_t* keep,
+    hipStream_t st) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_hash_resolve, dim3(blocks_for(count)), dim3(256), 0, st, slot_of, table, count, keep);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(const uint32_t* keep, uint32_t* pos, uint64_t count, uint32_t* total, void* temp,
+    size_t* temp_bytes, hipStream_t st) {
+  if (count > 0x7fffffffull) return hipErrorInvalidValue;
+  if (!temp) return hipcub::DeviceScan::ExclusiveSum(nullptr, *temp_bytes, keep, pos, int(count ? count : 1), st);
+  if (count) {
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, *temp_bytes, keep, pos, int(count), st);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, st, keep, pos, count, total);
+  return hipGetLastError();
+}
+
+template <int W>
+static hipError_t commit_w(const uint64_t* cand, const uint32_t* keep, const uint32_t* pos, const uint32_t* slot_of,
+    uint64_t count, uint64_t m, const uint32_t* parent_idx, uint64_t parent_base, const uint32_t* letter_list,
+    uint64_t base, uint64_t* arena, uint32_t* parent, uint8_t* letter, uint32_t* table, hipStream_t st) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_commit<W>, dim3(blocks_for(count)), dim3(256), 0, st, cand, keep, pos, slot_of, count, m,
+      parent_idx, parent_base, letter_list, base, arena, parent, letter, table);
+  return hipGetLastError();
+}
+
+hipError_t launch_commit(int W, const uint64_t* cand, const uint32_t* keep, const uint32_t* pos,
+    const uint32_t* slot_of, uint64_t count, uint64_t m, const uint32_t* parent_idx, uint64_t parent_base,
+    const uint32_t* letter_list, uint64_t base, uint64_t* arena, uint32_t* parent, uint8_t* letter, uint32_t* table,
+    hipStream_t st) {
+  SW_DISPATCH_W(W, commit_w<w_>(cand, keep, pos, slot_of, count, m, parent_idx, parent_base, letter_list, base,
+      arena, parent, letter, table, st))
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_hash_rebuild(const uint64_t* __restrict__ arena, uint64_t rows,
+    uint32_t* table, uint64_t mask) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  uint64_t pos = set_hash<W>(arena + i * W) & mask;
+  while (atomicCAS(&table[pos], kEmpty, uint32_t(i)) != kEmpty) pos = (pos + 1) & mask;
+}
+
+template <int W>
+static hipError_t rebuild_w(const uint64_t* arena, uint64_t rows, uint32_t* table, uint64_t mask, hipStream_t st) {
+  if (!rows) return hipSuccess;
+  hipLaunchKernelGGL(k_hash_rebuild<W>, dim3(blocks_for(rows)), dim3(256), 0, st, arena, rows, table, mask);
+  return hipGetLastError();
+}
+
+hipError_t launch_hash_rebuild(int W, const uint64_t* arena, uint64_t rows, uint32_t* table, uint64_t mask,
+    hipStream_t st) {
+  SW_DISPATCH_W(W, rebuild_w<w_>(arena, rows, table, mask, st))
+}
+
+hipError_t launch_alive(int W, const void* dead_bits, uint64_t count, uint32_t* keep, hipStream_t st) {
+  (void)W;
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_alive, dim3(blocks_for(count)), dim3(256), 0, st, static_cast<const uint8_t*>(dead_bits),
+      count, keep);
+  return hipGetLastError();
+}
+
+template <int W>
+static hipError_t select_w(const uint64_t* rows, uint64_t lo, const uint32_t* keep, const uint32_t* pos,
+    uint64_t count, uint64_t* out, uint32_t* idx_out, hipStream_t st) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_select<W>, dim3(blocks_for(count)), dim3(256), 0, st, rows, lo, keep, pos, count, out,
+      idx_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_select(int W, const uint64_t* rows, uint64_t lo, const uint32_t* keep, const uint32_t* pos,
+    uint64_t count, uint64_t* out, uint32_t* idx_out, hipStream_t st) {
+  SW_DISPATCH_W(W, select_w<w_>(rows, lo, keep, pos, count, out, idx_out, st))
+}
+
+template <int W>
+static hipError_t count_bits_w(const uint64_t* sets, uint64_t count, uint32_t n, unsigned long long* acc,
+    hipStream_t st) {
+  if (!count) return hipSuccess;
+  unsigned blocks = blocks_for(count);
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_count_bits<W>, dim3(blocks), dim3(256), 0, st, sets, count, n, acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_bits(int W, const uint64_t* sets, uint64_t count, uint32_t n, unsigned long long* acc,
+    hipStream_t st) {
+  SW_DISPATCH_W(W, count_bits_w<w_>(sets, count, n, acc, st))
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t count, hipStream_t st) {
+  if (!count) return hipSuccess;
+  unsigned blocks = blocks_for(count);
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, st, p, v, count);
+  return hipGetLastError();
+}
+
+}  // namespace sw

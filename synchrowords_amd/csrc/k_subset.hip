@@ -1,0 +1,253 @@
+// List-vs-list subset test on CDNA4: the "vertical" kernel.
+//
+// Question answered (the reference's leaf and checker kernels,
+// cuda/subsets_implicit_trie_kernel.cu:12-91, cuda/subsets_checker_kernel.cu:8-33):
+// for stream sets S_i and tile sets T_j, which pairs have S_i a subset of T_j?
+//
+// Layout.  A workgroup owns a tile of 64*MB tile sets (MB = 64 for N <= 256,
+// 32 above).  The tile is transposed into LDS: row q, lane L holds an MB-bit
+// mask whose bit b says "state q is in tile set L*MB + b".  A stream set S is
+// wave-uniform (scalar loads); the wave walks only the states q of S and ANDs
+// row q into a per-lane accumulator, so after |S| steps lane L knows, for all
+// MB of its tile sets at once, which contain S.  One LDS row read per state of
+// S tests 64*MB pairs; the stream set's bits are iterated on the scalar unit.
+//
+// Modes: COUNT (number of pairs), ANY (first pair found; every wave polls a
+// flag and stops), FLAGS (per tile set: does some stream set lie inside it),
+// SCOUNT (per stream set: how many tile sets contain it).
+#include "sw_device.hpp"
+#include "sw_kernels.hpp"
+
+#include <climits>
+
+namespace sw {
+
+template <int W>
+struct VsubMask {
+  using type = uint64_t;
+};
+template <>
+struct VsubMask<5> {
+  using type = uint32_t;
+};
+template <>
+struct VsubMask<6> {
+  using type = uint32_t;
+};
+template <>
+struct VsubMask<7> {
+  using type = uint32_t;
+};
+template <>
+struct VsubMask<8> {
+  using type = uint32_t;
+};
+
+template <int W>
+constexpr int vsub_mask_bits() {
+  return sizeof(typename VsubMask<W>::type) * 8;
+}
+
+size_t vsub_tile_sets(int W) { return W <= 4 ? 64 * 64 : 64 * 32; }
+
+size_t vsub_lds_bytes(uint32_t n, int W, bool proper_by_size) {
+  const size_t mb = W <= 4 ? 8 : 4;
+  size_t b = size_t(n + 1) * 64 * mb;
+  if (proper_by_size) b += vsub_tile_sets(W) * sizeof(uint16_t);
+  return b;
+}
+
+template <typename M>
+__device__ inline int mpopc(M x) {
+  return sizeof(M) == 8 ? __popcll(uint64_t(x)) : __popc(uint32_t(x));
+}
+
+template <int W>
+__global__ __launch_bounds__(1024) void k_vsub(VsubParams p) {
+  using M = typename VsubMask<W>::type;
+  constexpr int MB = sizeof(M) * 8;
+  constexpr int TILE = 64 * MB;
+  extern __shared__ __align__(16) unsigned char smem[];
+  M* rows = reinterpret_cast<M*>(smem);  // [(n + 1) rows][64 lanes]
+  const uint32_t n = p.n;
+  uint16_t* tsize = reinterpret_cast<uint16_t*>(smem + size_t(n + 1) * 64 * sizeof(M));
+  const int lane = threadIdx.x & 63;
+  const int wave = rfl(threadIdx.x >> 6);
+  const int nwaves = blockDim.x >> 6;
+  const uint64_t tile_base = uint64_t(blockIdx.x) * TILE;
+  const uint64_t tile_left = p.t_count - tile_base;  // > 0 by grid construction
+
+  // ---- transpose the tile: group g = tile sets [g*64, g*64+64), one per lane.
+  constexpr int NGROUPS = TILE / 64;
+  for (int g = wave; g < NGROUPS; g += nwaves) {
+    const uint64_t t = uint64_t(g) * 64 + lane;
+    const bool valid = t < tile_left;
+    int pc = 0;
+    for (int w = 0; w < W; ++w) {
+      uint64_t x = 0;
+      if (valid) {
+        x = p.tile[(tile_base + t) * W + w];
+        if (p.complement_tile) x = ~x & word_mask(n, w);
+      }
+      pc += __popcll(x);
+      uint64_t kept = 0;  // lane j keeps the ballot of state w*64 + j
+      for (int b = 0; b < 64; ++b) {
+        const uint64_t m = __builtin_amdgcn_ballot_w64((x >> b) & 1ull);
+        kept = (lane == b) ? m : kept;
+      }
+      const uint32_t q = uint32_t(w) * 64 + lane;
+      if (q < n) {
+        if (MB == 64) {
+          rows[size_t(q) * 64 + g] = M(kept);
+        } else {
+          rows[size_t(q) * 64 + 2 * g] = M(kept);
+          rows[size_t(q) * 64 + 2 * g + 1] = M(kept >> 32);
+        }
+      }
+    }
+    if (p.proper_by_size) tsize[t] = uint16_t(valid ? pc : 0xffff);
+  }
+  if (wave == 0) rows[size_t(n) * 64 + lane] = M(~M(0));  // padding row
+  __syncthreads();
+
+  // Lane L's tile sets that exist.
+  M valid;
+  {
+    const int64_t c = int64_t(tile_left < uint64_t(TILE) ? tile_left : uint64_t(TILE)) - int64_t(lane) * MB;
+    valid = c >= MB ? M(~M(0)) : (c <= 0 ? M(0) : M((M(1) << c) - 1));
+  }
+
+  const uint64_t s_begin = uint64_t(blockIdx.y) * p.s_chunk;
+  uint64_t s_end = s_begin + p.s_chunk;
+  if (s_end > p.s_count) s_end = p.s_count;
+
+  uint32_t cnt = 0;
+  M hits = 0;
+  uint32_t iter = 0;
+  for (uint64_t s = s_begin + wave; s < s_end; s += nwaves, ++iter) {
+    if (p.mode == VSUB_ANY && (iter & 15) == 0) {
+      if (rfl(__hip_atomic_load(p.found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
+    }
+    const uint64_t* a = p.stream + s * W;
+    uint64_t aw[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) aw[w] = rfl64(a[w]) & word_mask(n, w);  // states >= n have no row
+    M r = valid;
+    bool go = true;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      if (!go) continue;
+      uint64_t bits = aw[w];
+      const uint32_t base = uint32_t(w) * 64;
+      while (__popcll(bits) >= 4) {
+        const uint32_t q0 = base + __builtin_ctzll(bits);
+        bits &= bits - 1;
+        const uint32_t q1 = base + __builtin_ctzll(bits);
+        bits &= bits - 1;
+        const uint32_t q2 = base + __builtin_ctzll(bits);
+        bits &= bits - 1;
+        const uint32_t q3 = base + __builtin_ctzll(bits);
+        bits &= bits - 1;
+        const M m0 = rows[size_t(q0) * 64 + lane];
+        const M m1 = rows[size_t(q1) * 64 + lane];
+        const M m2 = rows[size_t(q2) * 64 + lane];
+        const M m3 = rows[size_t(q3) * 64 + lane];
+        r &= (m0 & m1) & (m2 & m3);
+      }
+      while (bits) {
+        const uint32_t q0 = base + __builtin_ctzll(bits);
+        bits &= bits - 1;
+        r &= rows[size_t(q0) * 64 + lane];
+      }
+      if (p.early_exit && w + 1 < W) go = __builtin_amdgcn_ballot_w64(r != 0) != 0;
+    }
+    if (p.diag != INT64_MIN) {
+      // Skip the pair (s, t) with s - t == diag: the same element of one list.
+      const int64_t t = int64_t(s) - p.diag - int64_t(tile_base);
+      if (t >= 0 && t < TILE && lane == int(t / MB)) r &= ~(M(1) << (t % MB));
+    }
+    if (p.proper_by_size && r) {
+      // A stream set inside a tile set of equal cardinality is that set.
+      int c = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) c += __popcll(aw[w]);
+      M rr = r;
+      while (rr) {
+        const int b = sizeof(M) == 8 ? __builtin_ctzll(uint64_t(rr)) : __builtin_ctz(uint32_t(rr));
+        rr &= rr - 1;
+        if (tsize[lane * MB + b] == c) r &= ~(M(1) << b);
+      }
+    }
+    if (p.mode == VSUB_COUNT) {
+      cnt += mpopc(r);
+    } else if (p.mode == VSUB_SCOUNT) {
+      // Per stream set: pairs summed over the wave's lanes.
+      if (__builtin_amdgcn_ballot_w64(r != 0)) {
+        uint32_t c = mpopc(r);
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+        if (lane == 0) atomicAdd(&p.scount[s], c);
+      }
+    } else if (p.mode == VSUB_FLAGS) {
+      hits |= r;
+    } else if (r) {
+      const int b = sizeof(M) == 8 ? __builtin_ctzll(uint64_t(r)) : __builtin_ctz(uint32_t(r));
+      const unsigned long long wv = (s << 32) | (tile_base + uint64_t(lane) * MB + b);
+      atomicMin(p.witness, wv);
+      __hip_atomic_store(p.found, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+
+  if (p.mode == VSUB_COUNT) {
+    uint64_t c = cnt;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if (lane == 0 && c) atomicAdd(p.count_out, (unsigned long long)c);
+  } else if (p.mode == VSUB_FLAGS) {
+    if (hits) {
+      M* f = reinterpret_cast<M*>(p.flags);
+      atomicOr(&f[tile_base / MB + lane], hits);
+    }
+  }
+}
+
+template <int W>
+static hipError_t launch_vsub_w(const VsubParams& p0, hipStream_t st) {
+  VsubParams p = p0;
+  if (p.t_count == 0 || p.s_count == 0) return hipSuccess;
+  const size_t tile = vsub_tile_sets(W);
+  const uint64_t gx = (p.t_count + tile - 1) / tile;
+  if (p.s_chunk == 0) {
+    // Aim for about 1024 workgroups, at least 64 stream sets per workgroup.
+    uint64_t gy = (1024 + gx - 1) / gx;
+    uint64_t chunk = (p.s_count + gy - 1) / gy;
+    if (chunk < 64) chunk = 64;
+    p.s_chunk = chunk;
+  }
+  const uint64_t gy = (p.s_count + p.s_chunk - 1) / p.s_chunk;
+  if (gx > 0x7fffffffull || gy > 65535ull) return hipErrorInvalidValue;
+  const size_t lds = vsub_lds_bytes(p.n, W, p.proper_by_size != 0);
+  static bool attr_set[kMaxW + 1] = {};
+  if (!attr_set[W]) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vsub<W>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    attr_set[W] = true;
+  }
+  hipLaunchKernelGGL(k_vsub<W>, dim3(unsigned(gx), unsigned(gy)), dim3(1024), lds, st, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_vsub(int W, const VsubParams& p, hipStream_t st) {
+  switch (W) {
+    case 1: return launch_vsub_w<1>(p, st);
+    case 2: return launch_vsub_w<2>(p, st);
+    case 3: return launch_vsub_w<3>(p, st);
+    case 4: return launch_vsub_w<4>(p, st);
+    case 5: return launch_vsub_w<5>(p, st);
+    case 6: return launch_vsub_w<6>(p, st);
+    case 7: return launch_vsub_w<7>(p, st);
+    case 8: return launch_vsub_w<8>(p, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace sw

@@ -1,0 +1,693 @@
+// The plan pipeline around the device exact search: the reference's run()
+// (jit/jitmain.cpp:11-49) over a config-JSON plan (configs/exact.json).
+//
+// Brute, Eppstein and Beam are the bound suppliers that sit before Exact in
+// the reference's plans; they run on the host here as they do in the
+// reference.  Exact runs on the device (exact.cpp).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <queue>
+#include <string>
+#include <vector>
+
+#include "sw_host.hpp"
+
+namespace sw {
+namespace {
+
+// ------------------------------------------------------------- tiny JSON
+struct Json {
+  enum Kind { NUL, BOOL, NUM, STR, ARR, OBJ } kind = NUL;
+  bool b = false;
+  double num = 0;
+  std::string str;
+  std::vector<Json> arr;
+  std::vector<std::pair<std::string, Json>> obj;
+  const Json* get(const std::string& k) const {
+    for (auto& kv : obj)
+      if (kv.first == k) return &kv.second;
+    return nullptr;
+  }
+};
+
+struct JsonParser {
+  const char* p;
+  const char* end;
+  void ws() {
+    while (p < end && (*p == ' ' || *p == '\n' || *p == '\t' || *p == '\r')) ++p;
+  }
+  [[noreturn]] void fail(const char* what) { throw Error{SW_ERR_INVALID, std::string("plan JSON: ") + what}; }
+  Json parse() {
+    ws();
+    if (p >= end) fail("unexpected end");
+    Json j;
+    if (*p == '{') {
+      j.kind = Json::OBJ;
+      ++p;
+      ws();
+      if (*p == '}') {
+        ++p;
+        return j;
+      }
+      while (true) {
+        ws();
+        Json k = parse();
+        if (k.kind != Json::STR) fail("object key");
+        ws();
+        if (*p != ':') fail("expected ':'");
+        ++p;
+        j.obj.emplace_back(k.str, parse());
+        ws();
+        if (*p == ',') {
+          ++p;
+          continue;
+        }
+        if (*p == '}') {
+          ++p;
+          break;
+        }
+        fail("expected ',' or '}'");
+      }
+    } else if (*p == '[') {
+      j.kind = Json::ARR;
+      ++p;
+      ws();
+      if (*p == ']') {
+        ++p;
+        return j;
+      }
+      while (true) {
+        j.arr.push_back(parse());
+        ws();
+        if (*p == ',') {
+          ++p;
+          continue;
+        }
+        if (*p == ']') {
+          ++p;
+          break;
+        }
+        fail("expected ',' or ']'");
+      }
+    } else if (*p == '"') {
+      j.kind = Json::STR;
+      ++p;
+      while (p < end && *p != '"') {
+        if (*p == '\\' && p + 1 < end) ++p;
+        j.str.push_back(*p++);
+      }
+      if (p >= end) fail("unterminated string");
+      ++p;
+    } else if (!strncmp(p, "true", 4)) {
+      j.kind = Json::BOOL;
+      j.b = true;
+      p += 4;
+    } else if (!strncmp(p, "false", 5)) {
+      j.kind = Json::BOOL;
+      p += 5;
+    } else if (!strncmp(p, "null", 4)) {
+      p += 4;
+    } else {
+      j.kind = Json::NUM;
+      char* e = nullptr;
+      j.num = strtod(p, &e);
+      if (e == p) fail("bad value");
+      p = e;
+    }
+    return j;
+  }
+};
+
+// ------------------------------------------------- expression evaluation
+// Plan values may be C++ expressions over AUT_N and AUT_K
+// (docs/config.md:9-12), e.g. "AUT_N * std::log2(AUT_N)".  Integer operands
+// keep integer arithmetic (C++ division), anything else is long double.
+struct Val {
+  bool is_int;
+  long long i;
+  long double f;
+  long double as_f() const { return is_int ? (long double)i : f; }
+};
+
+struct Expr {
+  const char* p;
+  long long N, K;
+  [[noreturn]] void fail() { throw Error{SW_ERR_INVALID, std::string("plan expression: cannot parse near '") + p + "'"}; }
+  void ws() {
+    while (*p == ' ') ++p;
+  }
+  static Val I(long long v) { return {true, v, 0}; }
+  static Val F(long double v) { return {false, 0, v}; }
+  Val prim() {
+    ws();
+    if (*p == '(') {
+      ++p;
+      Val v = sum();
+      ws();
+      if (*p != ')') fail();
+      ++p;
+      return v;
+    }
+    if (*p == '-') {
+      ++p;
+      Val v = prim();
+      return v.is_int ? I(-v.i) : F(-v.f);
+    }
+    if ((*p >= '0' && *p <= '9') || *p == '.') {
+      const char* s = p;
+      bool flt = false;
+      while ((*p >= '0' && *p <= '9') || *p == '.' || *p == 'e' || *p == 'E') {
+        if (*p == '.' || *p == 'e' || *p == 'E') flt = true;
+        ++p;
+      }
+      std::string tok(s, p);
+      while (*p == 'u' || *p == 'U' || *p == 'l' || *p == 'L' || *p == 'f' || *p == 'F') ++p;
+      return flt ? F(strtold(tok.c_str(), nullptr)) : I(strtoll(tok.c_str(), nullptr, 10));
+    }
+    const char* s = p;
+    while ((*p >= 'a' && *p <= 'z') || (*p >= 'A' && *p <= 'Z') || *p == '_' || *p == ':' || (*p >= '0' && *p <= '9'))
+      ++p;
+    std::string id(s, p);
+    if (id == "AUT_N") return I(N);
+    if (id == "AUT_K") return I(K);
+    if (id == "true") return I(1);
+    if (id == "false") return I(0);
+    std::string fn = id.rfind("std::", 0) == 0 ? id.substr(5) : id;
+    ws();
+    if (*p != '(') fail();
+    ++p;
+    Val a = sum();
+    ws();
+    Val b = I(0);
+    if (*p == ',') {
+      ++p;
+      b = sum();
+      ws();
+    }
+    if (*p != ')') fail();
+    ++p;
+    long double x = a.as_f();
+    if (fn == "log2") return F(std::log2(x));
+    if (fn == "log") return F(std::log(x));
+    if (fn == "sqrt") return F(std::sqrt(x));
+    if (fn == "pow") return F(std::pow(x, b.as_f()));
+    if (fn == "ceil") return F(std::ceil(x));
+    if (fn == "floor") return F(std::floor(x));
+    fail();
+  }
+  Val prod() {
+    Val v = prim();
+    while (true) {
+      ws();
+      char op = *p;
+      if (op != '*' && op != '/' && op != '%') return v;
+      ++p;
+      Val r = prim();
+      if (v.is_int && r.is_int) {
+        if (op == '*') v.i *= r.i;
+        else if (r.i == 0) fail();
+        else if (op == '/') v.i /= r.i;
+        else v.i %= r.i;
+      } else {
+        long double a = v.as_f(), b = r.as_f();
+        v = F(op == '*' ? a * b : a / b);
+      }
+    }
+  }
+  Val sum() {
+    Val v = prod();
+    while (true) {
+      ws();
+      char op = *p;
+      if (op != '+' && op != '-') return v;
+      ++p;
+      Val r = prod();
+      if (v.is_int && r.is_int) v.i = op == '+' ? v.i + r.i : v.i - r.i;
+      else v = F(op == '+' ? v.as_f() + r.as_f() : v.as_f() - r.as_f());
+    }
+  }
+};
+
+long double eval_num(const Json* j, const char* dflt, uint32_t n, uint32_t k) {
+  if (j && j->kind == Json::NUM) return j->num;
+  if (j && j->kind == Json::BOOL) return j->b ? 1 : 0;
+  std::string s = (j && j->kind == Json::STR) ? j->str : std::string(dflt);
+  Expr e{s.c_str(), n, k};
+  Val v = e.sum();
+  e.ws();
+  if (*e.p) e.fail();
+  return v.as_f();
+}
+
+bool eval_bool(const Json* j, bool dflt, uint32_t n, uint32_t k) {
+  if (!j) return dflt;
+  if (j->kind == Json::BOOL) return j->b;
+  return eval_num(j, "0", n, k) != 0;
+}
+
+// ----------------------------------------------------------------- Brute
+// Power-set BFS (algorithm/brute/brute.hpp:88-122).  -1: not synchronizing.
+long brute(uint32_t n, uint32_t k, const uint32_t* d) {
+  if (n == 1) return 0;
+  const uint32_t full = n == 32 ? 0xffffffffu : ((1u << n) - 1);
+  std::vector<uint8_t> dist(size_t(1) << n, 255);
+  std::queue<uint32_t> q;
+  dist[full] = 0;
+  q.push(full);
+  while (!q.empty()) {
+    uint32_t m = q.front();
+    q.pop();
+    for (uint32_t a = 0; a < k; ++a) {
+      uint32_t v = 0;
+      for (uint32_t x = m; x; x &= x - 1) v |= 1u << d[size_t(__builtin_ctz(x)) * k + a];
+      if (dist[v] == 255) {
+        dist[v] = uint8_t(dist[m] + 1);
+        q.push(v);
+        if (__builtin_popcount(v) == 1) return dist[v];
+      }
+    }
+  }
+  return -1;
+}
+
+// -------------------------------------------------------------- Eppstein
+// Pair graph BFS backwards from merged pairs, then greedy merging of the
+// closest pair (algorithm/eppstein/eppstein.hpp:25-114, pairs_tree.hpp:25-65).
+struct EppOut {
+  bool non_synchro = false;
+  bool improved = false;
+  uint64_t bound = 0;
+  std::vector<uint32_t> word;
+};
+
+EppOut eppstein(uint32_t n, uint32_t k, const uint32_t* d, uint64_t upper) {
+  EppOut o;
+  o.bound = upper;
+  std::vector<uint32_t> beg(size_t(n) * k + 1), edges(size_t(n) * k);
+  // CSR of preimages per (state, letter), sources ascending.
+  std::vector<uint32_t> cnt(size_t(n) * k, 0);
+  for (uint32_t q = 0; q < n; ++q)
+    for (uint32_t a = 0; a < k; ++a) cnt[size_t(a) * n + d[size_t(q) * k + a]]++;
+  beg[0] = 0;
+  for (size_t i = 0; i < size_t(n) * k; ++i) beg[i + 1] = beg[i] + cnt[i];
+  std::vector<uint32_t> fill(beg.begin(), beg.end() - 1);
+  for (uint32_t a = 0; a < k; ++a)
+    for (uint32_t q = 0; q < n; ++q) {
+      size_t t = size_t(a) * n + d[size_t(q) * k + a];
+      edges[fill[t]++] = q;
+    }
+  auto B = [&](uint32_t q, uint32_t a) { return beg[size_t(a) * n + q]; };
+  auto E = [&](uint32_t q, uint32_t a) { return beg[size_t(a) * n + q + 1]; };
+  std::vector<uint32_t> len(size_t(n) * n, 0), let(size_t(n) * n, 0);
+  std::vector<std::pair<uint32_t, uint32_t>> queue;
+  for (uint32_t q = 0; q < n; ++q)
+    for (uint32_t a = 0; a < k; ++a)
+      for (uint32_t i1 = B(q, a); i1 < E(q, a); ++i1)
+        for (uint32_t i2 = i1 + 1; i2 < E(q, a); ++i2) {
+          uint32_t x = edges[i1], y = edges[i2];
+          if (len[size_t(x) * n + y]) continue;
+          len[size_t(x) * n + y] = len[size_t(y) * n + x] = 1;
+          let[size_t(x) * n + y] = let[size_t(y) * n + x] = a;
+          queue.emplace_back(x, y);
+        }
+  const size_t all = size_t(n) * (n - 1) / 2;
+  size_t bottom = 0;
+  while (queue.size() < all && queue.size() > bottom) {
+    auto [v1, v2] = queue[bottom++];
+    const uint32_t l = len[size_t(v1) * n + v2] + 1;
+    for (uint32_t a = 0; a < k; ++a)
+      for (uint32_t i1 = B(v1, a); i1 < E(v1, a); ++i1)
+        for (uint32_t i2 = B(v2, a); i2 < E(v2, a); ++i2) {
+          uint32_t x = edges[i1], y = edges[i2];
+          if (len[size_t(x) * n + y]) continue;
+          len[size_t(x) * n + y] = len[size_t(y) * n + x] = l;
+          let[size_t(x) * n + y] = let[size_t(y) * n + x] = a;
+          queue.emplace_back(x, y);
+        }
+  }
+  if (queue.size() < all) {
+    o.non_synchro = true;
+    return o;
+  }
+  std::vector<char> cur(n, 1), nx(n);
+  uint32_t size = n;
+  uint64_t total = 0;
+  std::vector<uint32_t> w;
+  while (size > 1) {
+    uint32_t n1 = 0, n2 = 0, best = n * n;
+    for (uint32_t a = 0; a + 1 < n; ++a)
+      if (cur[a])
+        for (uint32_t b = a + 1; b < n; ++b)
+          if (cur[b] && len[size_t(a) * n + b] < best) {
+            n1 = a;
+            n2 = b;
+            best = len[size_t(a) * n + b];
+          }
+    total += best;
+    if (total > upper) return o;
+    while (n1 != n2) {
+      const uint32_t x = let[size_t(n1) * n + n2];
+      w.push_back(x);
+      std::fill(nx.begin(), nx.end(), 0);
+      size = 0;
+      for (uint32_t q = 0; q < n; ++q)
+        if (cur[q] && !nx[d[size_t(q) * k + x]]) {
+          nx[d[size_t(q) * k + x]] = 1;
+          size++;
+        }
+      cur.swap(nx);
+      n1 = d[size_t(n1) * k + x];
+      n2 = d[size_t(n2) * k + x];
+    }
+  }
+  o.improved = true;
+  o.bound = total;
+  o.word = std::move(w);
+  return o;
+}
+
+// ------------------------------------------------------------------ Beam
+// Inverse BFS from the sink singletons keeping the `beam` largest preimages
+// (ties by per-word numeric order); the first depth reaching all N states
+// bounds the length (algorithm/beam/beam.hpp:98-188).  Parents are kept so
+// the bound comes with its word.
+template <int W>
+struct BeamRunner {
+  uint32_t n, k;
+  std::vector<uint32_t> d;  // permuted automaton
+  std::vector<uint64_t> tab;
+  uint32_t slices;
+  std::vector<uint64_t> seed;
+
+  struct Out {
+    uint64_t len;
+    std::vector<uint32_t> word;
+  };
+
+  void apply(const uint64_t* x, uint32_t a, uint64_t* r) const {
+    for (int w = 0; w < W; ++w) r[w] = 0;
+    for (uint32_t s = 0; s < slices; ++s) {
+      uint32_t byte = uint32_t(x[s >> 3] >> ((s & 7) * 8)) & 255u;
+      if (!byte) continue;
+      const uint64_t* row = &tab[((size_t(a) * slices + s) * 256 + byte) * W];
+      for (int w = 0; w < W; ++w) r[w] |= row[w];
+    }
+  }
+
+  Out run(uint32_t beam, uint64_t max_len) {
+    struct Node {
+      uint64_t v[W];
+      uint32_t parent;
+      uint32_t letter;
+      uint32_t size;
+    };
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> hist;  // per depth: (parent, letter)
+    std::vector<Node> list;
+    for (size_t i = 0; i < seed.size() / W; ++i) {
+      Node nd{};
+      std::memcpy(nd.v, &seed[i * W], 8 * W);
+      list.push_back(nd);
+    }
+    std::vector<Node> next;
+    for (uint64_t len = 1; len < max_len; ++len) {
+      next.resize(size_t(k) * list.size());
+      for (uint32_t a = 0; a < k; ++a)
+        for (size_t i = 0; i < list.size(); ++i) {
+          Node& c = next[a * list.size() + i];
+          apply(list[i].v, a, c.v);
+          c.parent = uint32_t(i);
+          c.letter = a;
+          int sz = 0;
+          for (int w = 0; w < W; ++w) sz += __builtin_popcountll(c.v[w]);
+          c.size = uint32_t(sz);
+        }
+      std::stable_sort(next.begin(), next.end(), [](const Node& x, const Node& y) {
+        if (x.size != y.size) return x.size > y.size;
+        for (int w = 0; w < W; ++w)
+          if (x.v[w] != y.v[w]) return x.v[w] < y.v[w];
+        return false;
+      });
+      std::swap(list, next);
+      std::vector<std::pair<uint32_t, uint32_t>> h;
+      if (list[0].size == n) {
+        Out o;
+        o.len = len;
+        o.word.push_back(list[0].letter);
+        uint32_t p = list[0].parent;
+        for (size_t depth = hist.size(); depth-- > 0;) {
+          o.word.push_back(hist[depth][p].second);
+          p = hist[depth][p].first;
+        }
+        return o;
+      }
+      auto eq = [](const Node& x, const Node& y) {
+        for (int w = 0; w < W; ++w)
+          if (x.v[w] != y.v[w]) return false;
+        return true;
+      };
+      list.erase(std::unique(list.begin(), list.end(), eq), list.end());
+      if (list.size() > beam) list.resize(beam);
+      for (auto& nd : list) h.emplace_back(nd.parent, nd.letter);
+      hist.push_back(std::move(h));
+    }
+    return Out{max_len, {}};
+  }
+};
+
+struct BeamSettings {
+  bool dynamic = true;
+  long double beam_size = 0, min_beam = 0, max_beam = 0, ratio = 0.01;
+  bool presort_indeg = true;
+  long long max_iter = -1;
+};
+
+struct BeamResult {
+  uint64_t bound;
+  std::vector<uint32_t> word;
+};
+
+template <int W>
+BeamResult beam_w(uint32_t n, uint32_t k, const uint32_t* d0, uint64_t upper, const BeamSettings& bs) {
+  // Presort "indeg": relabel states by ascending in-degree, ties by index
+  // (beam.hpp:351-369).
+  std::vector<uint32_t> order(n);
+  if (bs.presort_indeg) {
+    std::vector<std::pair<double, int>> indeg(n);
+    for (uint32_t q = 0; q < n; ++q) indeg[q] = {0, int(q)};
+    for (uint32_t q = 0; q < n; ++q)
+      for (uint32_t a = 0; a < k; ++a) indeg[d0[size_t(q) * k + a]].first++;
+    std::sort(indeg.begin(), indeg.end());
+    for (uint32_t i = 0; i < n; ++i) order[indeg[i].second] = i;
+  } else {
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+  }
+  BeamRunner<W> br;
+  br.n = n;
+  br.k = k;
+  br.d.assign(size_t(n) * k, 0);
+  for (uint32_t q = 0; q < n; ++q)
+    for (uint32_t a = 0; a < k; ++a) br.d[size_t(order[q]) * k + a] = order[d0[size_t(q) * k + a]];
+  br.tab = build_tables(n, k, br.d.data(), 1);
+  br.slices = (n + 7) / 8;
+  const std::vector<char> sink = sink_states(n, k, br.d.data());
+  std::vector<uint32_t> indeg(size_t(n) * k, 0);
+  for (uint32_t q = 0; q < n; ++q)
+    for (uint32_t a = 0; a < k; ++a) indeg[size_t(br.d[size_t(q) * k + a]) * k + a]++;
+  for (uint32_t q = 0; q < n; ++q) {
+    if (!sink[q]) continue;
+    bool ok = false;
+    for (uint32_t a = 0; a < k; ++a) ok |= indeg[size_t(q) * k + a] > 1;
+    if (!ok) continue;
+    for (int w = 0; w < W; ++w) br.seed.push_back(uint32_t(w) == q / 64 ? (1ull << (q % 64)) : 0ull);
+  }
+  uint64_t maxlen = upper;
+  if (bs.max_iter >= 0 && uint64_t(bs.max_iter) + 1 < maxlen) maxlen = uint64_t(bs.max_iter) + 1;
+  BeamResult res;
+  if (!bs.dynamic) {
+    auto o = br.run(uint32_t(bs.beam_size), maxlen);
+    res.bound = std::min(o.len, upper);
+    res.word = o.word;
+    return res;
+  }
+  // Dynamic sizing (beam.hpp:66-89): a cheap run, then a wider one sized from
+  // the estimated cost of the exact search.
+  using cost_t = long double;
+  const unsigned min_beam = static_cast<unsigned>(bs.min_beam);
+  const unsigned max_beam = static_cast<unsigned>(bs.max_beam);
+  auto o1 = br.run(min_beam, maxlen);
+  unsigned bound = static_cast<unsigned>(o1.len);
+  res.bound = bound;
+  res.word = o1.word;
+  cost_t exact_cost = 2 * std::pow(static_cast<cost_t>(k), (bound + 1) / 2);
+  cost_t proposed = std::round(exact_cost * bs.ratio / bound);
+  unsigned size = static_cast<unsigned>(std::clamp<cost_t>(proposed, min_beam, max_beam));
+  if (size > min_beam) {
+    auto o2 = br.run(size, bound);
+    if (o2.len < bound) {
+      res.bound = o2.len;
+      res.word = o2.word;
+    }
+  }
+  return res;
+}
+
+BeamResult beam(uint32_t n, uint32_t k, const uint32_t* d, uint64_t upper, const BeamSettings& bs) {
+  switch ((n + 63) / 64) {
+    case 1: return beam_w<1>(n, k, d, upper, bs);
+    case 2: return beam_w<2>(n, k, d, upper, bs);
+    case 3: return beam_w<3>(n, k, d, upper, bs);
+    case 4: return beam_w<4>(n, k, d, upper, bs);
+    case 5: return beam_w<5>(n, k, d, upper, bs);
+    case 6: return beam_w<6>(n, k, d, upper, bs);
+    case 7: return beam_w<7>(n, k, d, upper, bs);
+    case 8: return beam_w<8>(n, k, d, upper, bs);
+  }
+  throw Error{SW_ERR_UNSUPPORTED, "n above 512"};
+}
+
+const char* kDefaultPlan = R"PLAN({
+  "upper_bound": "1ULL * (AUT_N - 1) * (AUT_N - 1) + 1",
+  "threads": 1,
+  "gpu": true,
+  "algorithms": [
+    {"name": "Brute", "config": {"max_n": "15"}},
+    {"name": "Eppstein", "config": {"transition_tables": false, "find_word": false}},
+    {"name": "Beam", "config": {"presort": "indeg", "dynamic": true, "min_beam_size": "AUT_N * std::log2(AUT_N)",
+      "max_beam_size": "AUT_N * AUT_N * std::log2(AUT_N)", "beam_exact_ratio": 0.01}},
+    {"name": "Exact", "config": {"dfs_min_list_size": "8192", "bfs_small_list_size": "AUT_N * AUT_K",
+      "dfs": true, "dfs_shortcut": true, "max_memory_mb": "12 * 1024"}}
+  ]
+})PLAN";
+
+}  // namespace
+
+// The reference's run() over a plan (jit/jitmain.cpp:11-49): bounds start at
+// [0, upper_bound] and every algorithm may only tighten them; an algorithm
+// whose bounds already meet returns at once (each plugin's early exit).
+void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char* plan_json, sw_result* out,
+    std::vector<uint32_t>& word) {
+  const std::string text = plan_json ? plan_json : kDefaultPlan;
+  JsonParser jp{text.data(), text.data() + text.size()};
+  Json plan = jp.parse();
+  SW_REQUIRE(plan.kind == Json::OBJ, "plan must be a JSON object");
+  const Json* algos = plan.get("algorithms");
+  SW_REQUIRE(algos && algos->kind == Json::ARR, "plan needs an \"algorithms\" list");
+  SW_REQUIRE(algos->arr.size() <= 8, "at most 8 algorithms in a plan");
+  std::memset(out, 0, sizeof(*out));
+  uint64_t lower = 0;
+  long double ub = eval_num(plan.get("upper_bound"), "1ULL * AUT_N * AUT_N * AUT_N / 6", n, k);
+  uint64_t upper = ub >= 1.8e19L ? UINT64_MAX : uint64_t(ub);
+  bool non_synchro = false;
+  word.clear();
+  int idx = 0;
+  for (const Json& a : algos->arr) {
+    const Json* nm = a.get("name");
+    SW_REQUIRE(nm && nm->kind == Json::STR, "algorithm without a name");
+    const Json empty;
+    const Json* cfg = a.get("config") ? a.get("config") : &empty;
+    const std::string name = nm->str;
+    auto t0 = std::chrono::steady_clock::now();
+    const bool open = !non_synchro && lower != upper;
+    if (name == "Brute") {
+      const long long max_n = (long long)eval_num(cfg->get("max_n"), "20", n, k);
+      SW_REQUIRE(max_n <= 32, "max_n can not be greater than 32");
+      if (open && n <= uint64_t(max_n)) {
+        long r = brute(n, k, d);
+        if (r < 0) non_synchro = true;
+        else lower = upper = uint64_t(r);
+      }
+    } else if (name == "Eppstein") {
+      const bool find_word = eval_bool(cfg->get("find_word"), false, n, k);
+      if (open) {
+        EppOut e = eppstein(n, k, d, upper);
+        if (e.non_synchro) {
+          non_synchro = true;
+        } else if (e.improved) {
+          upper = e.bound;
+          if (find_word) word = e.word;
+        }
+        out->eppstein_bound = upper;
+      }
+    } else if (name == "Beam") {
+      if (open) {
+        BeamSettings bs;
+        bs.beam_size = eval_num(cfg->get("beam_size"), "std::log2(AUT_N)", n, k);
+        bs.dynamic = eval_bool(cfg->get("dynamic"), false, n, k);
+        bs.min_beam = eval_num(cfg->get("min_beam_size"), "AUT_N * std::log2(AUT_N)", n, k);
+        bs.max_beam = eval_num(cfg->get("max_beam_size"), "AUT_N * AUT_N * std::log2(AUT_N)", n, k);
+        bs.ratio = eval_num(cfg->get("beam_exact_ratio"), "0.01", n, k);
+        bs.max_iter = (long long)eval_num(cfg->get("max_iter"), "-1", n, k);
+        const Json* ps = cfg->get("presort");
+        std::string presort = ps && ps->kind == Json::STR ? ps->str : "none";
+        SW_REQUIRE(presort == "none" || presort == "indeg", "presort value not in [none, indeg]");
+        bs.presort_indeg = presort == "indeg";
+        if (n == 1) {
+          lower = upper = 0;
+        } else {
+          BeamResult b = beam(n, k, d, upper, bs);
+          if (b.bound < upper) {
+            upper = b.bound;
+            word = b.word;
+          }
+        }
+        out->beam_bound = upper;
+      }
+    } else if (name == "Exact") {
+      if (open) {
+        sw_exact_config ec{};
+        ec.dfs = eval_bool(cfg->get("dfs"), true, n, k) ? 1 : 0;
+        const Json* mm = cfg->get("gpu_max_memory_mb");
+        if (mm) ec.max_memory_mb = uint64_t(eval_num(mm, "0", n, k));
+        const Json* vb = cfg->get("verbose");
+        ec.verbose = vb ? eval_bool(vb, false, n, k) : 0;
+        auto r = exact_search(ctx, n, k, d, lower, upper, ec);
+        out->exact = r.r;
+        if (r.r.found) {
+          lower = upper = r.r.length;
+          if (!r.r.bound_word) word = r.word;
+          else if (word.size() != upper) word.clear();
+        } else {
+          lower = std::max(lower, r.r.length);
+        }
+      }
+    } else {
+      throw Error{SW_ERR_UNSUPPORTED, "algorithm not supported by this library: " + name};
+    }
+    out->ms[idx] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::snprintf(out->names[idx], sizeof(out->names[idx]), "%s", name.c_str());
+    ++idx;
+  }
+  out->n_algorithms = idx;
+  out->non_synchro = non_synchro;
+  out->lower = lower;
+  out->upper = upper;
+  if (non_synchro) word.clear();
+  out->word_len = word.size();
+}
+
+}  // namespace sw
+
+extern "C" int sw_pipeline_solve(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, const char* plan_json,
+    sw_result* out, uint32_t* word, uint64_t word_cap) {
+  try {
+    SW_REQUIRE(ctx && delta && out, "NULL argument");
+    SW_REQUIRE(n >= 1 && k >= 1 && k <= 255, "n must be >= 1 and k in [1, 255]");
+    if (n > 512) throw sw::Error{SW_ERR_UNSUPPORTED, "n above 512 states is not compiled"};
+    for (size_t i = 0; i < size_t(n) * k; ++i) SW_REQUIRE(delta[i] < n, "transition out of range");
+    std::vector<uint32_t> w;
+    sw::pipeline(ctx, n, k, delta, plan_json, out, w);
+    if (word && w.size() <= word_cap) std::copy(w.begin(), w.end(), word);
+    return SW_OK;
+  } catch (const sw::Error& e) {
+    sw::set_error(e.msg);
+    return e.code;
+  } catch (const std::exception& e) {
+    sw::set_error(e.what());
+    return SW_ERR_INVALID;
+  }
+}

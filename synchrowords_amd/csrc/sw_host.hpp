@@ -1,0 +1,103 @@
+// Host-side internals: context, device buffers, error reporting.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/synchrowords_amd.h"
+
+namespace sw {
+
+void set_error(const std::string& msg);
+
+// Thrown inside the library and turned into a status code at the C ABI.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define SW_CHECK_HIP(expr)                                                                  \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      throw ::sw::Error{e_ == hipErrorOutOfMemory ? SW_ERR_NOMEM : SW_ERR_HIP,              \
+          std::string(#expr) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + ":" +        \
+              std::to_string(__LINE__) + ")"};                                              \
+  } while (0)
+
+#define SW_REQUIRE(cond, msg) \
+  do {                        \
+    if (!(cond)) throw ::sw::Error{SW_ERR_INVALID, (msg)}; \
+  } while (0)
+
+// Growable device allocation (contents kept on growth only if asked).
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  // Ensure at least `bytes`; when growing, keep the first `keep` bytes.
+  void ensure(size_t bytes, hipStream_t st = nullptr, size_t keep = 0) {
+    if (bytes <= cap) return;
+    size_t nc = cap ? cap : 4096;
+    while (nc < bytes) nc = nc * 3 / 2 + 4096;
+    void* np = nullptr;
+    SW_CHECK_HIP(hipMalloc(&np, nc));
+    if (p && keep) {
+      SW_CHECK_HIP(hipMemcpyAsync(np, p, keep < cap ? keep : cap, hipMemcpyDeviceToDevice, st));
+      SW_CHECK_HIP(hipStreamSynchronize(st));
+    }
+    if (p) SW_CHECK_HIP(hipFree(p));
+    p = np;
+    cap = nc;
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+}  // namespace sw
+
+struct sw_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string name;
+  int cus = 0;
+  uint64_t mem = 0;
+  // Scratch of the kernel-level entry points.
+  sw::DevBuf a, b, c, d, e;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+};
+
+namespace sw {
+
+// Make the context's device current.
+void use(sw_ctx* ctx);
+
+// Reference-layout slice tables for every letter: [k][ceil(n/8)][256][W]
+// words; dir 0 = image, 1 = preimage.
+std::vector<uint64_t> build_tables(uint32_t n, uint32_t k, const uint32_t* delta, int dir);
+
+// States of the sink component (exact.cpp).
+std::vector<char> sink_states(uint32_t n, uint32_t k, const uint32_t* delta);
+
+// Result of a search (exact.cpp).
+struct ExactOutcome {
+  sw_exact_result r{};
+  std::vector<uint32_t> word;
+};
+ExactOutcome exact_search(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint64_t lower, uint64_t upper,
+    const sw_exact_config& cfg);
+
+}  // namespace sw

@@ -1,0 +1,85 @@
+// Launchers of the HIP kernels (host-callable, stream-ordered, no syncs).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace sw {
+
+enum VsubMode { VSUB_COUNT = 0, VSUB_ANY = 1, VSUB_FLAGS = 2, VSUB_SCOUNT = 3 };
+
+// Parameters of the list-vs-list subset kernel (k_subset.hip).
+struct VsubParams {
+  const uint64_t* stream = nullptr;  // S_i, row-major W words
+  uint64_t s_count = 0;
+  const uint64_t* tile = nullptr;    // T_j, row-major W words
+  uint64_t t_count = 0;
+  uint32_t n = 0;                    // number of states
+  int complement_tile = 0;           // test S_i against the complement of T_j
+  int mode = VSUB_COUNT;
+  int early_exit = 1;                // stop a stream set once no lane can hit
+  int proper_by_size = 0;            // skip pairs of equal cardinality (S == T)
+  int64_t diag = INT64_MIN;          // skip pairs with i - j == diag
+  uint64_t s_chunk = 0;              // stream sets per workgroup row (0: auto)
+  unsigned long long* count_out = nullptr;  // COUNT: += number of pairs
+  int* found = nullptr;                     // ANY: set to 1 on a hit
+  unsigned long long* witness = nullptr;    // ANY: min of (i << 32) | j
+  void* flags = nullptr;  // FLAGS: bitmap over tile sets (bit j set on a hit)
+  uint32_t* scount = nullptr;  // SCOUNT: += pairs per stream set
+};
+
+// Tile sets per workgroup and LDS bytes of the subset kernel for W words.
+size_t vsub_tile_sets(int W);
+size_t vsub_lds_bytes(uint32_t n, int W, bool proper_by_size);
+// Bytes of the FLAGS bitmap for t tile sets: whole mask words.
+inline size_t vsub_flag_bytes(int W, uint64_t t) {
+  const uint64_t mb = W <= 4 ? 64 : 32;
+  return size_t((t + mb - 1) / mb) * (mb / 8);
+}
+hipError_t launch_vsub(int W, const VsubParams& p, hipStream_t st);
+
+// out[a*m + i] = image of parents[i] under letter a, from 8-state slice
+// tables tables[a][s][256][W] (the reference's PreprocessedTransition
+// layout).  valid[a*m + i] = 0 when the child is to be dropped:
+// drop_mode 1 drops children of >= n-1 states (complemented sets of at most
+// one state).  valid may be NULL (nothing dropped).
+hipError_t launch_expand(int W, const uint64_t* parents, uint64_t m, const uint64_t* tables, uint32_t slices,
+    uint32_t letters, const uint32_t* letter_list, uint32_t n, int drop_mode, uint64_t* out, uint32_t* valid,
+    hipStream_t st);
+
+// Open-addressing hash set over rows of an arena.  Slots hold an arena index,
+// or kCand | candidate index while a step is being merged.
+constexpr uint32_t kEmpty = 0xffffffffu;
+constexpr uint32_t kCand = 0x80000000u;
+hipError_t launch_hash_insert(int W, const uint64_t* cand, const uint32_t* valid, uint64_t count, uint32_t* table,
+    uint64_t mask, const uint64_t* arena, uint32_t* slot_of, unsigned long long* probes, hipStream_t st);
+// keep[i] = 1 iff candidate i owns its slot (smallest index among equal sets
+// that are new).
+hipError_t launch_hash_resolve(const uint32_t* slot_of, const uint32_t* table, uint64_t count, uint32_t* keep,
+    hipStream_t st);
+// Exclusive prefix sum of keep into pos; total into *total (device).
+hipError_t launch_scan(const uint32_t* keep, uint32_t* pos, uint64_t count, uint32_t* total, void* temp,
+    size_t* temp_bytes, hipStream_t st);
+// Append kept candidates to the arena at `base` (+pos[i]); record parent
+// (parent_idx[i % m]) and letter (letter_list[i / m]); point the hash slot
+// at the new row.
+// (parent_idx NULL: parent = parent_base + i % m.)
+hipError_t launch_commit(int W, const uint64_t* cand, const uint32_t* keep, const uint32_t* pos,
+    const uint32_t* slot_of, uint64_t count, uint64_t m, const uint32_t* parent_idx, uint64_t parent_base,
+    const uint32_t* letter_list, uint64_t base, uint64_t* arena, uint32_t* parent, uint8_t* letter, uint32_t* table,
+    hipStream_t st);
+// Insert arena rows [0, rows) (all distinct) into an empty table.
+hipError_t launch_hash_rebuild(int W, const uint64_t* arena, uint64_t rows, uint32_t* table, uint64_t mask,
+    hipStream_t st);
+// keep[i] = !(bit i of dead) for i < count.
+hipError_t launch_alive(int W, const void* dead_bits, uint64_t count, uint32_t* keep, hipStream_t st);
+// out[j] = rows[lo + i] for kept i (j = pos[i]); idx_out[j] = lo + i.
+hipError_t launch_select(int W, const uint64_t* rows, uint64_t lo, const uint32_t* keep, const uint32_t* pos,
+    uint64_t count, uint64_t* out, uint32_t* idx_out, hipStream_t st);
+// Per-state frequency over a list (acc must be zeroed): acc[q] += #sets with q.
+hipError_t launch_count_bits(int W, const uint64_t* sets, uint64_t count, uint32_t n, unsigned long long* acc,
+    hipStream_t st);
+// Fill 32-bit words.
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t count, hipStream_t st);
+
+}  // namespace sw

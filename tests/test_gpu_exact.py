@@ -1,0 +1,81 @@
+"""Exact search and plan pipeline on the GPU against the reference's golden
+vectors (tests/golden/, written by the compiled reference) and the oracle."""
+import pytest
+
+from conftest import golden
+from synchrowords_amd import gen, replay
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_word(case, r_len, word):
+    assert word is not None, "no word returned"
+    assert len(word) == r_len
+    assert replay(case["n"], case["k"], case["delta"], word) == 1
+
+
+def _cases(*names):
+    out = []
+    for nm in names:
+        out += [pytest.param(c, id=c["name"]) for c in golden(nm) if not c["non_synchro"]]
+    return out
+
+
+@pytest.mark.parametrize("case", _cases("c1_random48.json", "cerny.json", "hard.json", "slowly_sink.json",
+                                        "readme.json"))
+def test_exact_trivial_bound(ctx, case):
+    """Exact alone, from the trivial bound (n-1)^2+1: finds the reference's length and a word."""
+    n, k = case["n"], case["k"]
+    if n == 1:
+        pytest.skip("n=1 has the empty word")
+    r = ctx.exact(n, k, case["delta"])
+    assert r.found
+    assert r.length == case["upper"]
+    if r.length < (n - 1) ** 2 + 1:
+        _check_word(case, r.length, r.word)
+
+
+@pytest.mark.parametrize("case", _cases("c1_random48.json", "readme.json"))
+def test_exact_with_beam_bound(ctx, case):
+    """Exact given the reference's Beam bound: proves it optimal (or finds shorter)."""
+    n, k = case["n"], case["k"]
+    if n <= 15:
+        pytest.skip("Brute decides n <= 15")
+    r = ctx.exact(n, k, case["delta"], upper=case["beam"])
+    assert r.found and r.length == case["upper"]
+
+
+def _pipeline_cases():
+    out = []
+    for nm in ("c1_random48.json", "readme.json", "cerny.json", "hard.json", "slowly_sink.json",
+               "random_misc.json", "c3_random200.json"):
+        try:
+            out += [pytest.param(c, id=c["name"]) for c in golden(nm)]
+        except FileNotFoundError:
+            pass
+    return out
+
+
+@pytest.mark.parametrize("case", _pipeline_cases())
+def test_pipeline_matches_reference(ctx, case):
+    """configs/exact.json plan: bounds of every stage and the final length."""
+    n, k = case["n"], case["k"]
+    r = ctx.pipeline(n, k, case["delta"])
+    assert r.non_synchro == bool(case["non_synchro"])
+    if case["non_synchro"]:
+        return
+    assert (r.lower, r.upper) == (case["lower"], case["upper"])
+    if n > 15:
+        assert r.eppstein_bound == case["eppstein"]
+        assert r.beam_bound == case["beam"]
+    if r.upper > 0 and r.word is not None:
+        _check_word(case, r.upper, r.word)
+    if n > 15:
+        assert r.word is not None
+
+
+def test_cerny64_word(ctx):
+    d = gen.cerny(64)
+    r = ctx.exact(64, 2, d)
+    assert r.found and r.length == 63 * 63
+    assert replay(64, 2, d, r.word) == 1
