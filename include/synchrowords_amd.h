@@ -167,6 +167,68 @@ typedef struct sw_result {
 int sw_pipeline_solve(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, const char* plan_json,
     sw_result* out, uint32_t* word, uint64_t word_cap);
 
+/* ----------------------------------------------------------- batched search */
+
+typedef struct sw_batch_config {
+  uint32_t grid;       /* workgroups of the persistent launch (0: 4 per CU) */
+  uint32_t cap_rows;   /* sets per side a workgroup may hold (0: 65536) */
+  uint32_t cap_cand;   /* candidates of one step (0: 98304) */
+  uint32_t pad;
+  uint64_t max_pairs;  /* largest goal check done inside a workgroup (0: 2^24) */
+} sw_batch_config;
+
+typedef struct sw_batch_stats {
+  double ms_host_prep, ms_kernel, ms_overflow, ms_total;
+  uint32_t non_synchro, found, bound, overflowed;
+  uint32_t grid;
+  uint32_t pad;
+  uint64_t scratch_bytes;
+} sw_batch_stats;
+
+/* Shortest reset words of `count` automata in one persistent launch.
+ * Instance i: ns[i] states, ks[i] letters, transitions at
+ * deltas + delta_offsets[i] (row-major, n*k entries).  Outputs per instance:
+ * non_synchro[i]; lengths[i] (exact length when synchronizing); word_lens[i]
+ * and the word at words + i*word_cap (both may be NULL).  Instances that do
+ * not fit one workgroup finish on the whole device.  The reference runs such
+ * a file one automaton at a time (app/main.cpp:49-78). */
+int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint32_t* ks, const uint64_t* delta_offsets,
+    const uint32_t* deltas, const sw_batch_config* cfg, int32_t* non_synchro, uint64_t* lengths,
+    uint32_t* word_lens, uint32_t* words, uint32_t word_cap, sw_batch_stats* stats);
+
+/* -------------------------------------------------------- learned upper bound
+ * A BF16 MLP (MFMA, FP32 accumulation) that scores beam candidates by their
+ * estimated forward distance.  No reference counterpart: it replaces the
+ * cardinality order of the reference's Beam (algorithm/beam/beam.hpp:98-188)
+ * with a learned one; the bound stays a real word's length.
+ * Inputs of a sample: the n state bits of S and |delta_a(S)|/n per letter,
+ * zero-padded to in_dim = ceil((n+k)/32)*32.  Network in_dim -> H -> H -> 1,
+ * ReLU.  Parameters flattened as [W1 (H x in_dim), b1, W2 (H x H), b2, w3 (H), b3].
+ */
+typedef struct sw_mlp sw_mlp;
+int sw_mlp_create(sw_ctx* ctx, uint32_t n, uint32_t k, uint32_t hidden, uint64_t seed, sw_mlp** out);
+void sw_mlp_destroy(sw_mlp* m);
+int sw_mlp_shape(sw_mlp* m, uint32_t* in_dim, uint32_t* hidden, uint64_t* nparams);
+int sw_mlp_get_params(sw_mlp* m, float* out, uint64_t cap);
+int sw_mlp_set_params(sw_mlp* m, const float* in, uint64_t count);
+/* BF16 features x[count][in_dim] of host sets for automaton delta. */
+int sw_mlp_features(sw_mlp* m, const uint32_t* delta, const uint64_t* sets, uint64_t count, uint16_t* x);
+/* y[count] = network(x[count][in_dim]). */
+int sw_mlp_forward(sw_mlp* m, const uint16_t* x, uint64_t count, float* y);
+/* Mean squared error of one batch and its gradient (flattened like the parameters). */
+int sw_mlp_gradients(sw_mlp* m, const uint16_t* x, const float* target, uint64_t count, float* grads, float* loss);
+/* Adam, `epochs` passes of shuffled minibatches; loss_curve[e] = mean batch loss of epoch e (may be NULL). */
+int sw_mlp_train(sw_mlp* m, const uint16_t* x, const float* target, uint64_t count, uint32_t epochs, uint32_t batch,
+    float lr, uint64_t seed, float* loss_curve);
+/* Training samples: beam candidate sets labelled with their forward distance
+ * measured on the search's forward levels up to fdepth (fdepth+1 if beyond). */
+int sw_mlp_samples(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t fdepth, uint32_t width,
+    uint32_t bdepth, uint32_t per_depth, uint64_t* sets_out, float* dist_out, uint64_t cap, uint64_t* count);
+/* Beam of `width` scored by the network: *bound = first depth producing Q
+ * (upper if none below it), with that word. */
+int sw_learned_beam(sw_ctx* ctx, sw_mlp* m, const uint32_t* delta, uint32_t width, uint64_t upper, uint64_t* bound,
+    uint32_t* word, uint64_t word_cap, uint64_t* word_len);
+
 #ifdef __cplusplus
 }
 #endif

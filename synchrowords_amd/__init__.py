@@ -59,6 +59,18 @@ class _Result(C.Structure):
                 ("exact", _ExactResult)]
 
 
+class _BatchConfig(C.Structure):
+    _fields_ = [("grid", C.c_uint32), ("cap_rows", C.c_uint32), ("cap_cand", C.c_uint32), ("pad", C.c_uint32),
+                ("max_pairs", C.c_uint64)]
+
+
+class _BatchStats(C.Structure):
+    _fields_ = [("ms_host_prep", C.c_double), ("ms_kernel", C.c_double), ("ms_overflow", C.c_double),
+                ("ms_total", C.c_double), ("non_synchro", C.c_uint32), ("found", C.c_uint32),
+                ("bound", C.c_uint32), ("overflowed", C.c_uint32), ("grid", C.c_uint32), ("pad", C.c_uint32),
+                ("scratch_bytes", C.c_uint64)]
+
+
 _P = C.c_void_p
 _U64P = C.POINTER(C.c_uint64)
 _U32P = C.POINTER(C.c_uint32)
@@ -87,6 +99,20 @@ _SIGS = {
                                  C.POINTER(_ExactConfig), C.POINTER(_ExactResult), _P, C.c_uint64]),
     "sw_pipeline_solve": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_char_p, C.POINTER(_Result), _P,
                                     C.c_uint64]),
+    "sw_batch_solve": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, C.POINTER(_BatchConfig), _P, _P, _P, _P,
+                                 C.c_uint32, C.POINTER(_BatchStats)]),
+    "sw_mlp_create": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(_P)]),
+    "sw_mlp_destroy": (None, [_P]),
+    "sw_mlp_shape": (C.c_int, [_P, _U32P, _U32P, _U64P]),
+    "sw_mlp_get_params": (C.c_int, [_P, _P, C.c_uint64]),
+    "sw_mlp_set_params": (C.c_int, [_P, _P, C.c_uint64]),
+    "sw_mlp_features": (C.c_int, [_P, _P, _P, C.c_uint64, _P]),
+    "sw_mlp_forward": (C.c_int, [_P, _P, C.c_uint64, _P]),
+    "sw_mlp_gradients": (C.c_int, [_P, _P, _P, C.c_uint64, _P, C.POINTER(C.c_float)]),
+    "sw_mlp_train": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _P]),
+    "sw_mlp_samples": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                 _P, _P, C.c_uint64, _U64P]),
+    "sw_learned_beam": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint64, _U64P, _P, C.c_uint64, _U64P]),
 }
 
 
@@ -234,6 +260,88 @@ class Leaf:
             pass
 
 
+class Mlp:
+    """The learned scorer (sw_mlp): BF16 MLP on MFMA, FP32 accumulation."""
+
+    def __init__(self, ctx: "Context", n: int, k: int, hidden: int = 512, seed: int = 1):
+        self.ctx, self.n, self.k = ctx, n, k
+        h = C.c_void_p()
+        _check(load().sw_mlp_create(ctx.handle, n, k, hidden, seed, C.byref(h)))
+        self.handle = h
+        d, hh, npar = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        _check(load().sw_mlp_shape(h, C.byref(d), C.byref(hh), C.byref(npar)))
+        self.in_dim, self.hidden, self.nparams = d.value, hh.value, npar.value
+
+    def close(self):
+        if self.handle:
+            load().sw_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.nparams, dtype=np.float32)
+        _check(load().sw_mlp_get_params(self.handle, _ptr(out), out.size))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(load().sw_mlp_set_params(self.handle, _ptr(p), p.size))
+
+    def features(self, delta, sets) -> np.ndarray:
+        s = _sets(sets, self.n)
+        d = _delta(self.n, self.k, delta)
+        x = np.zeros((s.shape[0], self.in_dim), dtype=np.uint16)
+        _check(load().sw_mlp_features(self.handle, _ptr(d), _ptr(s), s.shape[0], _ptr(x)))
+        return x
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.uint16)
+        y = np.zeros(x.shape[0], dtype=np.float32)
+        _check(load().sw_mlp_forward(self.handle, _ptr(x), x.shape[0], _ptr(y)))
+        return y
+
+    def gradients(self, x, target):
+        x = np.ascontiguousarray(x, dtype=np.uint16)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        loss = C.c_float()
+        _check(load().sw_mlp_gradients(self.handle, _ptr(x), _ptr(t), x.shape[0], _ptr(g), C.byref(loss)))
+        return g, float(loss.value)
+
+    def train(self, x, target, epochs=10, batch=512, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(x, dtype=np.uint16)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        curve = np.zeros(epochs, dtype=np.float32)
+        _check(load().sw_mlp_train(self.handle, _ptr(x), _ptr(t), x.shape[0], epochs, batch, lr, seed, _ptr(curve)))
+        return curve
+
+    def beam(self, delta, width: int, upper: int):
+        d = _delta(self.n, self.k, delta)
+        cap = max(int(upper), 1)
+        word = np.zeros(min(cap, 1 << 20), dtype=np.uint32)
+        b, wl = C.c_uint64(), C.c_uint64()
+        _check(load().sw_learned_beam(self.ctx.handle, self.handle, _ptr(d), width, upper, C.byref(b), _ptr(word),
+                                      word.size, C.byref(wl)))
+        return int(b.value), [int(v) for v in word[: wl.value]]
+
+
+def mlp_samples(ctx: "Context", n: int, k: int, delta, fdepth: int = 14, width: int = 256, bdepth: int = 60,
+                per_depth: int = 32, cap: int = 1 << 16):
+    """(sets, distances) labelled on the search's forward levels."""
+    d = _delta(n, k, delta)
+    sets = np.zeros((cap, words(n)), dtype=np.uint64)
+    dist = np.zeros(cap, dtype=np.float32)
+    cnt = C.c_uint64()
+    _check(load().sw_mlp_samples(ctx.handle, n, k, _ptr(d), fdepth, width, bdepth, per_depth, _ptr(sets), _ptr(dist),
+                                 cap, C.byref(cnt)))
+    return sets[: cnt.value], dist[: cnt.value]
+
+
 class Context:
     """One device, one stream (sw_ctx)."""
 
@@ -334,6 +442,40 @@ class Context:
         w = [int(x) for x in word[: r.word_len]] if r.word_len else None
         stats = {f: getattr(r, f) for f, _ in _ExactResult._fields_}
         return ExactResult(bool(r.found), int(r.length), w, bool(r.bound_word), stats)
+
+    def batch(self, instances, grid: int = 0, cap_rows: int = 0, cap_cand: int = 0, max_pairs: int = 0,
+              word_cap: int = 512):
+        """Exact lengths of many automata in one persistent launch.
+
+        instances: list of (n, k, delta).  Returns (results, stats): results[i]
+        = {"non_synchro", "length", "word"}.
+        """
+        cnt = len(instances)
+        ns = np.array([i[0] for i in instances], dtype=np.uint32)
+        ks = np.array([i[1] for i in instances], dtype=np.uint32)
+        offs = np.zeros(cnt, dtype=np.uint64)
+        parts = []
+        o = 0
+        for j, (n, k, d) in enumerate(instances):
+            offs[j] = o
+            dd = _delta(n, k, d)
+            parts.append(dd)
+            o += dd.size
+        deltas = np.concatenate(parts) if parts else np.zeros(1, dtype=np.uint32)
+        nsync = np.zeros(cnt, dtype=np.int32)
+        lengths = np.zeros(cnt, dtype=np.uint64)
+        wl = np.zeros(cnt, dtype=np.uint32)
+        words_buf = np.zeros(cnt * word_cap, dtype=np.uint32)
+        cfg = _BatchConfig(grid, cap_rows, cap_cand, 0, max_pairs)
+        st = _BatchStats()
+        _check(load().sw_batch_solve(self.handle, cnt, _ptr(ns), _ptr(ks), _ptr(offs), _ptr(deltas), C.byref(cfg),
+                                     _ptr(nsync), _ptr(lengths), _ptr(wl), _ptr(words_buf), word_cap, C.byref(st)))
+        res = []
+        for j in range(cnt):
+            w = [int(x) for x in words_buf[j * word_cap: j * word_cap + wl[j]]] if wl[j] else None
+            res.append({"non_synchro": bool(nsync[j]), "length": int(lengths[j]), "word": w})
+        stats = {f: getattr(st, f) for f, _ in _BatchStats._fields_ if f != "pad"}
+        return res, stats
 
     def pipeline(self, n: int, k: int, delta, plan: dict | str | None = None) -> PipelineResult:
         d = _delta(n, k, delta)

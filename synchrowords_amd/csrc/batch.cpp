@@ -1,0 +1,253 @@
+// Batched exact search, host side: one persistent kernel launch over a
+// cost-ordered instance queue (k_batch.hip), then the whole-device search
+// (exact.cpp) for the instances that outgrew a workgroup.
+//
+// The reference solves a file of automata one at a time, re-entering its JIT
+// per (N, K) (app/main.cpp:49-78, app/jit.hpp:25-56); here one call takes the
+// whole batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "sw_batch.hpp"
+#include "sw_host.hpp"
+
+namespace sw {
+
+// Every pair of states merges under some word (the pair-graph test of the
+// reference's Eppstein, pairs_tree.hpp:25-65,99-104): BFS over unordered pairs
+// backwards from the pairs merged by one letter.
+bool pairs_synchronizing(uint32_t n, uint32_t k, const uint32_t* d) {
+  if (n <= 1) return true;
+  std::vector<uint32_t> beg(size_t(n) * k + 1, 0), edges(size_t(n) * k);
+  for (uint32_t q = 0; q < n; ++q)
+    for (uint32_t a = 0; a < k; ++a) beg[size_t(a) * n + d[size_t(q) * k + a] + 1]++;
+  for (size_t i = 0; i < size_t(n) * k; ++i) beg[i + 1] += beg[i];
+  std::vector<uint32_t> fill(beg.begin(), beg.end() - 1);
+  for (uint32_t a = 0; a < k; ++a)
+    for (uint32_t q = 0; q < n; ++q) edges[fill[size_t(a) * n + d[size_t(q) * k + a]]++] = q;
+  std::vector<uint8_t> seen(size_t(n) * n, 0);
+  std::vector<std::pair<uint32_t, uint32_t>> queue;
+  queue.reserve(size_t(n) * (n - 1) / 2);
+  for (uint32_t q = 0; q < n; ++q)
+    for (uint32_t a = 0; a < k; ++a) {
+      const size_t t = size_t(a) * n + q;
+      for (uint32_t i = beg[t]; i < beg[t + 1]; ++i)
+        for (uint32_t j = i + 1; j < beg[t + 1]; ++j) {
+          uint32_t x = edges[i], y = edges[j];
+          if (seen[size_t(x) * n + y]) continue;
+          seen[size_t(x) * n + y] = seen[size_t(y) * n + x] = 1;
+          queue.emplace_back(x, y);
+        }
+    }
+  const size_t all = size_t(n) * (n - 1) / 2;
+  for (size_t bottom = 0; bottom < queue.size() && queue.size() < all; ++bottom) {
+    auto [v1, v2] = queue[bottom];
+    for (uint32_t a = 0; a < k; ++a) {
+      const size_t t1 = size_t(a) * n + v1, t2 = size_t(a) * n + v2;
+      for (uint32_t i = beg[t1]; i < beg[t1 + 1]; ++i)
+        for (uint32_t j = beg[t2]; j < beg[t2 + 1]; ++j) {
+          uint32_t x = edges[i], y = edges[j];
+          if (seen[size_t(x) * n + y]) continue;
+          seen[size_t(x) * n + y] = seen[size_t(y) * n + x] = 1;
+          queue.emplace_back(x, y);
+        }
+    }
+  }
+  return queue.size() == all;
+}
+
+}  // namespace sw
+
+extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint32_t* ks,
+    const uint64_t* delta_offsets, const uint32_t* deltas, const sw_batch_config* cfg_in, int32_t* non_synchro,
+    uint64_t* lengths, uint32_t* word_lens, uint32_t* words, uint32_t word_cap, sw_batch_stats* stats) {
+  using namespace sw;
+  try {
+    SW_REQUIRE(ctx && ns && ks && delta_offsets && deltas && non_synchro && lengths, "NULL argument");
+    sw_batch_config cfg{};
+    if (cfg_in) cfg = *cfg_in;
+    use(ctx);
+    auto t0 = std::chrono::steady_clock::now();
+    sw_batch_stats st{};
+
+    // ---- host preparation: validation, synchronizability, sink singletons
+    struct Prep {
+      bool ns = false, device = false;
+      uint32_t wmax = 0;
+      uint64_t b0[kBatchMaxW] = {0, 0, 0};
+    };
+    std::vector<Prep> prep(count);
+    uint32_t wmax = 1;
+    for (uint32_t i = 0; i < count; ++i) {
+      SW_REQUIRE(ns[i] >= 1 && ks[i] >= 1 && ks[i] <= 255, "instance with n < 1 or k outside [1, 255]");
+      if (ns[i] > 512) throw Error{SW_ERR_UNSUPPORTED, "n above 512 states is not compiled"};
+      const uint32_t* d = deltas + delta_offsets[i];
+      for (size_t j = 0; j < size_t(ns[i]) * ks[i]; ++j) SW_REQUIRE(d[j] < ns[i], "transition out of range");
+    }
+    {
+      std::atomic<uint32_t> next{0};
+      const unsigned nt = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+      std::vector<std::thread> pool;
+      for (unsigned t = 0; t < nt; ++t)
+        pool.emplace_back([&] {
+          for (uint32_t i; (i = next.fetch_add(1)) < count;) {
+            const uint32_t n = ns[i], k = ks[i];
+            const uint32_t* d = deltas + delta_offsets[i];
+            Prep& p = prep[i];
+            p.ns = !pairs_synchronizing(n, k, d);
+            if (p.ns || n == 1) continue;
+            p.device = n <= kBatchMaxW * 64 && k <= kBatchMaxK;
+            if (!p.device) continue;
+            std::vector<char> sink = sink_states(n, k, d);
+            std::vector<uint32_t> indeg(size_t(n) * k, 0);
+            for (uint32_t q = 0; q < n; ++q)
+              for (uint32_t a = 0; a < k; ++a) indeg[size_t(d[size_t(q) * k + a]) * k + a]++;
+            for (uint32_t q = 0; q < n; ++q) {
+              if (!sink[q]) continue;
+              bool ok = false;
+              for (uint32_t a = 0; a < k; ++a) ok |= indeg[size_t(q) * k + a] > 1;
+              if (ok) p.b0[q / 64] |= 1ull << (q % 64);
+            }
+            p.wmax = (n + 63) / 64;
+          }
+        });
+      for (auto& th : pool) th.join();
+    }
+    std::vector<BatchInstance> inst;
+    std::vector<uint32_t> map;  // kernel slot -> instance
+    std::vector<uint8_t> bytes;
+    for (uint32_t i = 0; i < count; ++i) {
+      non_synchro[i] = prep[i].ns;
+      lengths[i] = 0;
+      if (word_lens) word_lens[i] = 0;
+      if (prep[i].ns) {
+        st.non_synchro++;
+        continue;
+      }
+      if (ns[i] == 1) {
+        st.found++;
+        continue;
+      }
+      if (!prep[i].device) continue;
+      BatchInstance b{};
+      b.n = ns[i];
+      b.k = ks[i];
+      b.delta_off = uint32_t(bytes.size());
+      const uint64_t ub = uint64_t(ns[i] - 1) * (ns[i] - 1) + 1;
+      b.upper = uint32_t(std::min<uint64_t>(ub, 0xffffffffu));
+      std::memcpy(b.b0, prep[i].b0, sizeof(b.b0));
+      const uint32_t* d = deltas + delta_offsets[i];
+      for (size_t j = 0; j < size_t(ns[i]) * ks[i]; ++j) bytes.push_back(uint8_t(d[j]));
+      SW_REQUIRE(ns[i] <= 256, "byte transitions need n <= 256");
+      wmax = std::max(wmax, prep[i].wmax);
+      inst.push_back(b);
+      map.push_back(i);
+    }
+    // Expensive first: more letters, then more states.
+    std::vector<uint32_t> order(inst.size());
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+      if (inst[a].k != inst[b].k) return inst[a].k > inst[b].k;
+      return inst[a].n > inst[b].n;
+    });
+    auto t1 = std::chrono::steady_clock::now();
+    st.ms_host_prep = std::chrono::duration<double, std::milli>(t1 - t0).count();
+
+    // ---- the persistent launch
+    std::vector<BatchResult> res(inst.size());
+    const uint32_t wc = std::max<uint32_t>(word_cap ? word_cap : 0, 2);
+    const uint32_t kernel_wcap = std::min<uint32_t>(std::max<uint32_t>(wc, 64), 4096);
+    std::vector<uint8_t> wbuf(size_t(inst.size()) * kernel_wcap);
+    if (!inst.empty()) {
+      BatchLimits L;
+      L.cap_rows = cfg.cap_rows ? cfg.cap_rows : (1u << 16);
+      L.cap_cand = cfg.cap_cand ? cfg.cap_cand : 3u * (1u << 15);
+      uint32_t hs = 1024;
+      while (hs < 2 * (L.cap_rows + L.cap_cand)) hs <<= 1;
+      L.hash_slots = hs;
+      L.max_pairs = cfg.max_pairs ? cfg.max_pairs : (1ull << 24);
+      const size_t stride = batch_scratch_bytes(int(wmax), L);
+      uint32_t grid = cfg.grid ? cfg.grid : uint32_t(ctx->cus) * 4;
+      grid = std::min<uint32_t>(grid, uint32_t(inst.size()));
+      DevBuf d_inst, d_bytes, d_order, d_head, d_res, d_words, d_scratch;
+      d_inst.ensure(inst.size() * sizeof(BatchInstance));
+      d_bytes.ensure(bytes.size() + 8);
+      d_order.ensure(order.size() * 4);
+      d_head.ensure(8);
+      d_res.ensure(res.size() * sizeof(BatchResult));
+      d_words.ensure(wbuf.size() + 8);
+      d_scratch.ensure(stride * grid);
+      hipStream_t s = ctx->stream;
+      SW_CHECK_HIP(hipMemcpyAsync(d_inst.p, inst.data(), inst.size() * sizeof(BatchInstance), hipMemcpyHostToDevice, s));
+      SW_CHECK_HIP(hipMemcpyAsync(d_bytes.p, bytes.data(), bytes.size(), hipMemcpyHostToDevice, s));
+      SW_CHECK_HIP(hipMemcpyAsync(d_order.p, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
+      SW_CHECK_HIP(hipMemsetAsync(d_head.p, 0, 8, s));
+      SW_CHECK_HIP(hipMemsetAsync(d_res.p, 0, res.size() * sizeof(BatchResult), s));
+      auto k0 = std::chrono::steady_clock::now();
+      SW_CHECK_HIP(launch_batch(int(wmax), d_inst.as<BatchInstance>(), d_bytes.as<uint8_t>(), d_order.as<uint32_t>(),
+          uint32_t(inst.size()), d_head.as<uint32_t>(), d_res.as<BatchResult>(), d_words.as<uint8_t>(), kernel_wcap,
+          d_scratch.as<uint8_t>(), stride, L, grid, s));
+      SW_CHECK_HIP(hipStreamSynchronize(s));
+      st.ms_kernel = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - k0).count();
+      SW_CHECK_HIP(hipMemcpyAsync(res.data(), d_res.p, res.size() * sizeof(BatchResult), hipMemcpyDeviceToHost, s));
+      SW_CHECK_HIP(hipMemcpyAsync(wbuf.data(), d_words.p, wbuf.size(), hipMemcpyDeviceToHost, s));
+      SW_CHECK_HIP(hipStreamSynchronize(s));
+      st.grid = grid;
+      st.scratch_bytes = stride * grid;
+    }
+
+    // ---- results; overflowed (and out-of-range) instances on the whole device
+    auto t2 = std::chrono::steady_clock::now();
+    std::vector<char> done(count, 0);
+    for (size_t j = 0; j < inst.size(); ++j) {
+      const uint32_t i = map[j];
+      const BatchResult& r = res[j];
+      if (r.status == kBatchFound || r.status == kBatchBound) {
+        lengths[i] = r.length;
+        if (r.status == kBatchFound) {
+          st.found++;
+          if (words && word_lens && r.word_len <= word_cap) {
+            word_lens[i] = r.word_len;
+            for (uint32_t c = 0; c < r.word_len; ++c) words[size_t(i) * word_cap + c] = wbuf[j * kernel_wcap + c];
+          }
+        } else {
+          st.bound++;
+        }
+        done[i] = 1;
+      }
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+      if (done[i] || prep[i].ns || ns[i] == 1) continue;
+      const uint32_t* d = deltas + delta_offsets[i];
+      sw_exact_config ec{};
+      ec.dfs = 1;
+      const uint64_t ub = uint64_t(ns[i] - 1) * (ns[i] - 1) + 1;
+      auto r = exact_search(ctx, ns[i], ks[i], d, 0, ub, ec);
+      st.overflowed++;
+      SW_REQUIRE(r.r.found, "exact search ended without a length (memory budget)");
+      lengths[i] = r.r.length;
+      if (words && word_lens && r.word.size() <= word_cap) {
+        word_lens[i] = uint32_t(r.word.size());
+        std::copy(r.word.begin(), r.word.end(), words + size_t(i) * word_cap);
+      }
+      st.found++;
+    }
+    st.ms_overflow = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t2).count();
+    st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+    return SW_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return SW_ERR_INVALID;
+  }
+}

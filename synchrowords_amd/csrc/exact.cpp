@@ -127,6 +127,21 @@ class Search {
 
   void set_memory_error() { throw Error{SW_ERR_NOMEM, "exact search: device memory budget exceeded"}; }
 
+  // Forward levels only (distinct images of Q by depth, dominance-pruned
+  // frontiers), copied to `out`; level_end[d] = rows after depth d.
+  void forward_levels(uint32_t depth, uint64_t cap_rows, DevBuf& out, std::vector<uint64_t>& level_end) {
+    upload_tables();
+    init_forward();
+    for (uint32_t d = 1; d <= depth; ++d) {
+      if (!F_.fcount || F_.rows + F_.fcount * k_ > cap_rows) break;
+      step(F_, false, d);
+    }
+    out.ensure(F_.rows * row_bytes() + 8);
+    SW_CHECK_HIP(hipMemcpyAsync(out.p, F_.arena.p, F_.rows * row_bytes(), hipMemcpyDeviceToDevice, st_));
+    SW_CHECK_HIP(hipStreamSynchronize(st_));
+    level_end = F_.level_end;
+  }
+
  private:
   sw_ctx* ctx_;
   hipStream_t st_;
@@ -441,6 +456,15 @@ std::vector<uint64_t> build_tables(uint32_t n, uint32_t k, const uint32_t* delta
         }
       }
   return t;
+}
+
+void forward_levels(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t depth, uint64_t cap_rows,
+    DevBuf& out, std::vector<uint64_t>& level_end) {
+  use(ctx);
+  sw_exact_config cfg{};
+  cfg.forward_cap = ~0ull;
+  Search s(ctx, n, k, delta, cfg);
+  s.forward_levels(depth, cap_rows, out, level_end);
 }
 
 ExactOutcome exact_search(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint64_t lower, uint64_t upper,

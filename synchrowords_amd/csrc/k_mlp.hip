@@ -1,0 +1,375 @@
+// The learned scorer: a BF16 MLP on MFMA with FP32 accumulation.
+//
+// Inputs of one sample: the state bitset of a set S (n values in {0, 1}) and,
+// per letter a, |delta_a(S)| / n, zero-padded to D (a multiple of 32).
+// Network: D -> H -> H -> 1 with ReLU, weights in BF16 (FP32 masters kept for
+// training), FP32 accumulation everywhere.  Target: the forward distance of S
+// (the length of the shortest word u with delta(Q, u) inside S), as measured
+// by the exact search's own forward levels.
+//
+// GEMM tiles use v_mfma_f32_16x16x16bf16_1k: a workgroup of 4 waves computes a
+// 64x64 tile of C, each wave a 32x32 quadrant of 2x2 MFMA blocks; K advances
+// 32 at a time through LDS.  The network is small (about 0.4 M parameters), so
+// the layers are latency- rather than MFMA-bound at beam batch sizes.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include "sw_mlp.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+namespace sw {
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline uint16_t f2bf(float f) {
+  // Round to nearest even.
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t(u >> 16) | ((u & 0xffffu) ? 0x40 : 0);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+constexpr int TM = 64, TN = 64, TK = 32;
+
+// C[M x N] (+)= op(A)[M x K] * op(B)[K x N].
+//   A: row-major [M][K] (lda) or, with TA, [K][M].
+//   B: row-major [K][N] (ldb) or, with TB, [N][K].
+// Epilogue: + bias[n] (if bias), ReLU (if relu); write f32 (Cf) and/or bf16 (Cb).
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, int lda, const uint16_t* __restrict__ B,
+    int ldb, float* __restrict__ Cf, uint16_t* __restrict__ Cb, int ldc, int M, int N, int K,
+    const float* __restrict__ bias, int relu, int accumulate) {
+  __shared__ uint16_t As[TM][TK + 4];  // [m][k]
+  __shared__ uint16_t Bs[TN][TK + 4];  // [n][k]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = 0; k0 < K; k0 += TK) {
+    // Stage the tiles (zero outside the matrices).
+    for (int e = tid; e < TM * TK; e += 256) {
+      int i, k;
+      if (TA) {
+        k = e / TM;
+        i = e % TM;
+      } else {
+        i = e / TK;
+        k = e % TK;
+      }
+      const int gm = m0 + i, gk = k0 + k;
+      uint16_t v = 0;
+      if (gm < M && gk < K) v = TA ? A[size_t(gk) * lda + gm] : A[size_t(gm) * lda + gk];
+      As[i][k] = v;
+    }
+    for (int e = tid; e < TN * TK; e += 256) {
+      int j, k;
+      if (TB) {
+        j = e / TK;
+        k = e % TK;
+      } else {
+        k = e / TN;
+        j = e % TN;
+      }
+      const int gn = n0 + j, gk = k0 + k;
+      uint16_t v = 0;
+      if (gn < N && gk < K) v = TB ? B[size_t(gn) * ldb + gk] : B[size_t(gk) * ldb + gn];
+      Bs[j][k] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 16) {
+      // 16x16x16 block: lane l holds A[l%16][4*(l/16) .. +3] and B[4*(l/16) .. +3][l%16].
+      const int r = lane & 15, kq = kk + 4 * (lane >> 4);
+      bf16x4 a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const uint16_t* p = &As[wm + i * 16 + r][kq];
+        a[i] = bf16x4{short(p[0]), short(p[1]), short(p[2]), short(p[3])};
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const uint16_t* p = &Bs[wn + j * 16 + r][kq];
+        b[j] = bf16x4{short(p[0]), short(p[1]), short(p[2]), short(p[3])};
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // D block layout: lane l holds D[4*(l/16) + v][l%16], v = 0..3.
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gm = m0 + wm + i * 16 + 4 * (lane >> 4) + v;
+        const int gn = n0 + wn + j * 16 + (lane & 15);
+        if (gm >= M || gn >= N) continue;
+        float x = acc[i][j][v];
+        if (accumulate && Cf) x += Cf[size_t(gm) * ldc + gn];
+        if (bias) x += bias[gn];
+        if (relu) x = x > 0.f ? x : 0.f;
+        if (Cf) Cf[size_t(gm) * ldc + gn] = x;
+        if (Cb) Cb[size_t(gm) * ldc + gn] = f2bf(x);
+      }
+}
+
+// Features of set rows: x[i][q] = bit q, x[i][n + a] = |delta_a(S_i)| / n.
+template <int W>
+__global__ void k_features(const uint64_t* __restrict__ sets, const uint64_t* __restrict__ images, uint64_t count,
+    uint32_t n, uint32_t k, uint32_t D, uint16_t* __restrict__ x) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint16_t* row = x + i * D;
+  const uint64_t* s = sets + i * W;
+  for (uint32_t q = 0; q < n; ++q) row[q] = ((s[q >> 6] >> (q & 63)) & 1) ? 0x3f80 : 0;  // 1.0 / 0.0
+  for (uint32_t a = 0; a < k; ++a) {
+    const uint64_t* im = images + (uint64_t(a) * count + i) * W;
+    int c = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) c += __popcll(im[w]);
+    row[n + a] = f2bf(float(c) / float(n));
+  }
+  for (uint32_t q = n + k; q < D; ++q) row[q] = 0;
+}
+
+// y[i] = h[i] . w3 + b3 (FP32), one wave per row.
+__global__ void k_head(const float* __restrict__ h, const float* __restrict__ w3, const float* __restrict__ b3,
+    uint64_t count, uint32_t H, float* __restrict__ y) {
+  const uint64_t row = uint64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= count) return;
+  float s = 0.f;
+  for (uint32_t j = lane; j < H; j += 64) s += h[row * H + j] * w3[j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) y[row] = s + b3[0];
+}
+
+// Loss gradient of the head and first hidden gradient:
+//   dy = 2 (y - t) / B;  dH2 = dy * w3 (masked by ReLU(H2) > 0), bf16 + f32.
+__global__ void k_head_back(const float* __restrict__ y, const float* __restrict__ t, const float* __restrict__ h2,
+    const float* __restrict__ w3, uint64_t count, uint32_t H, float inv_b, float* __restrict__ dy,
+    uint16_t* __restrict__ dh2, float* __restrict__ loss) {
+  const uint64_t row = blockIdx.x;
+  if (row >= count) return;
+  const float d = 2.f * (y[row] - t[row]) * inv_b;
+  if (threadIdx.x == 0) {
+    dy[row] = d;
+    atomicAdd(loss, (y[row] - t[row]) * (y[row] - t[row]) * inv_b);
+  }
+  for (uint32_t j = threadIdx.x; j < H; j += blockDim.x) {
+    const float g = h2[row * H + j] > 0.f ? d * w3[j] : 0.f;
+    dh2[row * H + j] = f2bf(g);
+  }
+}
+
+// dW3[j] = sum_i dy[i] * H2[i][j]; db3 = sum dy.  One block per 256 columns.
+__global__ void k_head_wgrad(const float* __restrict__ dy, const float* __restrict__ h2, uint64_t count, uint32_t H,
+    float* __restrict__ gw3, float* __restrict__ gb3) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < H) {
+    float s = 0.f;
+    for (uint64_t i = 0; i < count; ++i) s += dy[i] * h2[i * H + j];
+    gw3[j] = s;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    float s = 0.f;
+    for (uint64_t i = 0; i < count; ++i) s += dy[i];
+    gb3[0] = s;
+  }
+}
+
+// g (bf16 in, f32 master) masked by ReLU of the layer output; column sums -> bias grads.
+__global__ void k_relu_mask(const uint16_t* __restrict__ g, const float* __restrict__ h, uint64_t count, uint32_t H,
+    uint16_t* __restrict__ out) {
+  const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= count * H) return;
+  out[e] = h[e] > 0.f ? g[e] : uint16_t(0);
+}
+
+__global__ void k_colsum_bf16(const uint16_t* __restrict__ g, uint64_t count, uint32_t H, float* __restrict__ out) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= H) return;
+  float s = 0.f;
+  for (uint64_t i = 0; i < count; ++i) s += bf2f(g[i * H + j]);
+  out[j] = s;
+}
+
+// Adam on FP32 masters, then refresh the BF16 copy.
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+    uint64_t count, float lr, float b1, float b2, float eps, float c1, float c2, uint16_t* __restrict__ wb) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = f2bf(wi);
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, uint64_t count, uint16_t* __restrict__ out) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = f2bf(x[i]);
+}
+
+inline unsigned blocks(uint64_t n, unsigned bs = 256) { return unsigned((n + bs - 1) / bs); }
+
+}  // namespace
+
+hipError_t launch_gemm(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* Cf,
+    uint16_t* Cb, int ldc, int M, int N, int K, const float* bias, int relu, int accumulate, hipStream_t st) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
+#define L(a, b) hipLaunchKernelGGL((k_gemm<a, b>), grid, dim3(256), 0, st, A, lda, B, ldb, Cf, Cb, ldc, M, N, K, bias, \
+    relu, accumulate)
+  if (!ta && !tb) L(false, false);
+  else if (!ta && tb) L(false, true);
+  else if (ta && !tb) L(true, false);
+  else L(true, true);
+#undef L
+  return hipGetLastError();
+}
+
+hipError_t launch_features(int W, const uint64_t* sets, const uint64_t* images, uint64_t count, uint32_t n, uint32_t k,
+    uint32_t D, uint16_t* x, hipStream_t st) {
+  if (!count) return hipSuccess;
+  switch (W) {
+#define C(w) case w: hipLaunchKernelGGL(k_features<w>, dim3(blocks(count)), dim3(256), 0, st, sets, images, count, n, k, D, x); break;
+    C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8)
+#undef C
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+namespace {
+
+__global__ void k_first_full(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n, uint32_t W,
+    unsigned long long* first) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  int c = 0;
+  for (uint32_t w = 0; w < W; ++w) c += __popcll(sets[i * W + w]);
+  if (uint32_t(c) == n) atomicMin(first, (unsigned long long)i);
+}
+
+__global__ void k_gather_rows(const uint64_t* __restrict__ rows, const uint32_t* __restrict__ idx, uint64_t count,
+    uint32_t W, uint64_t* __restrict__ out) {
+  const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const uint64_t r = idx[j];
+  for (uint32_t w = 0; w < W; ++w) out[j * W + w] = rows[r * W + w];
+}
+
+__global__ void k_iota_u32(uint32_t* p, uint64_t count) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = uint32_t(i);
+}
+
+}  // namespace
+
+hipError_t launch_first_full(const uint64_t* sets, uint64_t count, uint32_t n, uint32_t W, unsigned long long* first,
+    hipStream_t st) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_first_full, dim3(blocks(count)), dim3(256), 0, st, sets, count, n, W, first);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(const uint64_t* rows, const uint32_t* idx, uint64_t count, uint32_t W, uint64_t* out,
+    hipStream_t st) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_rows, dim3(blocks(count)), dim3(256), 0, st, rows, idx, count, W, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_iota(uint32_t* p, uint64_t count, hipStream_t st) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_iota_u32, dim3(blocks(count)), dim3(256), 0, st, p, count);
+  return hipGetLastError();
+}
+
+hipError_t sort_scores(const float* keys_in, float* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
+    uint64_t count, void* temp, size_t* temp_bytes, hipStream_t st) {
+  return hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, keys_in, keys_out, vals_in, vals_out, int(count), 0,
+      32, st);
+}
+
+hipError_t mlp_forward(const MlpDev& p, const uint16_t* x, uint64_t B, MlpWork& w, float* y, hipStream_t st) {
+  hipError_t e;
+  if ((e = launch_gemm(false, true, x, p.D, p.w1b, p.D, w.h1f, w.h1b, p.H, int(B), p.H, p.D, p.b1, 1, 0, st))) return e;
+  if ((e = launch_gemm(false, true, w.h1b, p.H, p.w2b, p.H, w.h2f, w.h2b, p.H, int(B), p.H, p.H, p.b2, 1, 0, st)))
+    return e;
+  hipLaunchKernelGGL(k_head, dim3(unsigned((B + 3) / 4)), dim3(256), 0, st, w.h2f, p.w3, p.b3, B, p.H, y);
+  return hipGetLastError();
+}
+
+hipError_t mlp_backward(const MlpDev& p, const uint16_t* x, uint64_t B, const float* target, MlpWork& w, MlpGrads& g,
+    float* loss, hipStream_t st) {
+  hipError_t e;
+  const float inv_b = 1.f / float(B);
+  hipLaunchKernelGGL(k_head_back, dim3(unsigned(B)), dim3(256), 0, st, w.y, target, w.h2f, p.w3, B, p.H, inv_b, w.dy,
+      w.g2b, loss);
+  hipLaunchKernelGGL(k_head_wgrad, dim3(blocks(p.H)), dim3(256), 0, st, w.dy, w.h2f, B, p.H, g.w3, g.b3);
+  // dW2 = dH2^T H1, db2 = colsum dH2
+  if ((e = launch_gemm(true, false, w.g2b, p.H, w.h1b, p.H, g.w2, nullptr, p.H, p.H, p.H, int(B), nullptr, 0, 0, st)))
+    return e;
+  hipLaunchKernelGGL(k_colsum_bf16, dim3(blocks(p.H)), dim3(256), 0, st, w.g2b, B, p.H, g.b2);
+  // dH1 = (dH2 W2) masked by ReLU(H1)
+  if ((e = launch_gemm(false, false, w.g2b, p.H, p.w2b, p.H, w.g1f, w.g1b, p.H, int(B), p.H, p.H, nullptr, 0, 0, st)))
+    return e;
+  hipLaunchKernelGGL(k_relu_mask, dim3(blocks(B * p.H)), dim3(256), 0, st, w.g1b, w.h1f, B, p.H, w.g1b);
+  hipLaunchKernelGGL(k_colsum_bf16, dim3(blocks(p.H)), dim3(256), 0, st, w.g1b, B, p.H, g.b1);
+  // dW1 = dH1^T X
+  if ((e = launch_gemm(true, false, w.g1b, p.H, x, p.D, g.w1, nullptr, p.D, p.H, p.D, int(B), nullptr, 0, 0, st)))
+    return e;
+  return hipGetLastError();
+}
+
+hipError_t mlp_adam(MlpDev& p, MlpGrads& g, MlpAdam& a, float lr, hipStream_t st) {
+  a.t += 1;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float c1 = 1.f - powf(b1, float(a.t)), c2 = 1.f - powf(b2, float(a.t));
+  struct T {
+    float* w;
+    float* g;
+    float* m;
+    float* v;
+    uint64_t n;
+    uint16_t* wb;
+  } ts[] = {
+      {p.w1, g.w1, a.m + a.off[0], a.v + a.off[0], uint64_t(p.H) * p.D, p.w1b},
+      {p.b1, g.b1, a.m + a.off[1], a.v + a.off[1], uint64_t(p.H), nullptr},
+      {p.w2, g.w2, a.m + a.off[2], a.v + a.off[2], uint64_t(p.H) * p.H, p.w2b},
+      {p.b2, g.b2, a.m + a.off[3], a.v + a.off[3], uint64_t(p.H), nullptr},
+      {p.w3, g.w3, a.m + a.off[4], a.v + a.off[4], uint64_t(p.H), nullptr},
+      {p.b3, g.b3, a.m + a.off[5], a.v + a.off[5], 1, nullptr},
+  };
+  for (auto& t : ts)
+    hipLaunchKernelGGL(k_adam, dim3(blocks(t.n)), dim3(256), 0, st, t.w, t.g, t.m, t.v, t.n, lr, b1, b2, eps, c1, c2,
+        t.wb);
+  return hipGetLastError();
+}
+
+hipError_t mlp_refresh_bf16(MlpDev& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks(uint64_t(p.H) * p.D)), dim3(256), 0, st, p.w1, uint64_t(p.H) * p.D, p.w1b);
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks(uint64_t(p.H) * p.H)), dim3(256), 0, st, p.w2, uint64_t(p.H) * p.H, p.w2b);
+  return hipGetLastError();
+}
+
+}  // namespace sw

@@ -1,0 +1,577 @@
+// The learned upper bound: a BF16 MLP that scores beam candidates.
+//
+// Role (and its limits): the reference's Beam keeps the candidates of largest
+// cardinality (algorithm/beam/beam.hpp:98-188); any scoring keeps the bound
+// valid, because only real preimages are followed and the bound is the depth
+// at which a candidate equals Q.  Here the score is the network's estimate of
+// the forward distance of a candidate S — the length of the shortest word u
+// with delta(Q, u) inside S — learned from distances the exact search itself
+// measures.  The exact search still decides every length.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "sw_host.hpp"
+#include "sw_kernels.hpp"
+#include "sw_mlp.hpp"
+
+struct sw_mlp {
+  sw_ctx* ctx = nullptr;
+  uint32_t n = 0, k = 0;
+  sw::MlpDev p;
+  sw::MlpAdam adam;
+  sw::DevBuf params, bf, grads, moments;   // masters, bf16 copies, grads, Adam m|v
+  sw::DevBuf h1f, h2f, y, dy, g1f, h1b, h2b, g1b, g2b, lossb;
+  uint64_t work_rows = 0;
+  uint64_t nparams = 0;
+};
+
+namespace sw {
+namespace {
+
+uint64_t splitmix(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+double uniform(uint64_t& s) { return double(splitmix(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+void bind(sw_mlp* m) {
+  float* base = m->params.as<float>();
+  const uint64_t D = m->p.D, H = m->p.H;
+  m->p.w1 = base;
+  m->p.b1 = m->p.w1 + H * D;
+  m->p.w2 = m->p.b1 + H;
+  m->p.b2 = m->p.w2 + H * H;
+  m->p.w3 = m->p.b2 + H;
+  m->p.b3 = m->p.w3 + H;
+  m->p.w1b = m->bf.as<uint16_t>();
+  m->p.w2b = m->p.w1b + H * D;
+}
+
+MlpGrads grads_of(sw_mlp* m) {
+  float* base = m->grads.as<float>();
+  const uint64_t D = m->p.D, H = m->p.H;
+  MlpGrads g;
+  g.w1 = base;
+  g.b1 = g.w1 + H * D;
+  g.w2 = g.b1 + H;
+  g.b2 = g.w2 + H * H;
+  g.w3 = g.b2 + H;
+  g.b3 = g.w3 + H;
+  return g;
+}
+
+MlpWork work_of(sw_mlp* m, uint64_t rows) {
+  const uint64_t H = m->p.H;
+  if (rows > m->work_rows) {
+    m->h1f.ensure(rows * H * 4);
+    m->h2f.ensure(rows * H * 4);
+    m->g1f.ensure(rows * H * 4);
+    m->h1b.ensure(rows * H * 2);
+    m->h2b.ensure(rows * H * 2);
+    m->g1b.ensure(rows * H * 2);
+    m->g2b.ensure(rows * H * 2);
+    m->y.ensure(rows * 4 + 4);
+    m->dy.ensure(rows * 4 + 4);
+    m->work_rows = rows;
+  }
+  MlpWork w;
+  w.h1f = m->h1f.as<float>();
+  w.h2f = m->h2f.as<float>();
+  w.g1f = m->g1f.as<float>();
+  w.h1b = m->h1b.as<uint16_t>();
+  w.h2b = m->h2b.as<uint16_t>();
+  w.g1b = m->g1b.as<uint16_t>();
+  w.g2b = m->g2b.as<uint16_t>();
+  w.y = m->y.as<float>();
+  w.dy = m->dy.as<float>();
+  return w;
+}
+
+}  // namespace
+
+// Features of `count` device sets (images under every letter computed here).
+void mlp_features(sw_mlp* m, const uint32_t* delta_dev_tables_unused, const uint64_t* tables_fwd,
+    const uint64_t* sets, uint64_t count, uint16_t* x, DevBuf& scratch, hipStream_t st) {
+  (void)delta_dev_tables_unused;
+  const uint32_t n = m->n, k = m->k, W = (n + 63) / 64;
+  scratch.ensure(count * k * W * 8 + 8);
+  SW_CHECK_HIP(launch_expand(int(W), sets, count, tables_fwd, (n + 7) / 8, k, nullptr, n, 0, scratch.as<uint64_t>(),
+      nullptr, st));
+  SW_CHECK_HIP(launch_features(int(W), sets, scratch.as<uint64_t>(), count, n, k, m->p.D, x, st));
+}
+
+}  // namespace sw
+
+using sw::DevBuf;
+using sw::Error;
+
+namespace {
+
+int fail(const Error& e) {
+  sw::set_error(e.msg);
+  return e.code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sw_mlp_create(sw_ctx* ctx, uint32_t n, uint32_t k, uint32_t hidden, uint64_t seed, sw_mlp** out) {
+  try {
+    SW_REQUIRE(ctx && out, "NULL argument");
+    SW_REQUIRE(n >= 1 && n <= 512 && k >= 1 && k <= 255, "n in [1, 512], k in [1, 255]");
+    SW_REQUIRE(hidden >= 16 && hidden <= 4096, "hidden in [16, 4096]");
+    sw::use(ctx);
+    auto m = std::make_unique<sw_mlp>();
+    m->ctx = ctx;
+    m->n = n;
+    m->k = k;
+    m->p.D = ((n + k) + 31) / 32 * 32;
+    m->p.H = hidden;
+    const uint64_t D = m->p.D, H = hidden;
+    m->nparams = H * D + H + H * H + H + H + 1;
+    m->params.ensure(m->nparams * 4);
+    m->bf.ensure((H * D + H * H) * 2);
+    m->grads.ensure(m->nparams * 4);
+    m->moments.ensure(m->nparams * 8);
+    m->lossb.ensure(16);
+    sw::bind(m.get());
+    // He-uniform initialisation from a splitmix64 stream; zero biases.
+    std::vector<float> h(m->nparams, 0.f);
+    uint64_t s = seed;
+    auto fill = [&](uint64_t off, uint64_t cnt, double fan_in) {
+      const double lim = std::sqrt(6.0 / fan_in);
+      for (uint64_t i = 0; i < cnt; ++i) h[off + i] = float((2.0 * sw::uniform(s) - 1.0) * lim);
+    };
+    fill(0, H * D, double(n + k));
+    fill(H * D + H, H * H, double(H));
+    fill(H * D + H + H * H + H, H, double(H));
+    SW_CHECK_HIP(hipMemcpyAsync(m->params.p, h.data(), h.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    SW_CHECK_HIP(hipMemsetAsync(m->moments.p, 0, m->nparams * 8, ctx->stream));
+    m->adam.m = m->moments.as<float>();
+    m->adam.v = m->adam.m + m->nparams;
+    const uint64_t offs[6] = {0, H * D, H * D + H, H * D + H + H * H, H * D + H + H * H + H, H * D + 2 * H + H * H + H};
+    for (int i = 0; i < 6; ++i) m->adam.off[i] = offs[i];
+    SW_CHECK_HIP(sw::mlp_refresh_bf16(m->p, ctx->stream));
+    SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    *out = m.release();
+    return SW_OK;
+  } catch (const Error& e) {
+    return fail(e);
+  }
+}
+
+void sw_mlp_destroy(sw_mlp* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->ctx->device);
+  (void)hipStreamSynchronize(m->ctx->stream);
+  delete m;
+}
+
+int sw_mlp_shape(sw_mlp* m, uint32_t* in_dim, uint32_t* hidden, uint64_t* nparams) {
+  if (!m) return SW_ERR_INVALID;
+  if (in_dim) *in_dim = m->p.D;
+  if (hidden) *hidden = m->p.H;
+  if (nparams) *nparams = m->nparams;
+  return SW_OK;
+}
+
+int sw_mlp_get_params(sw_mlp* m, float* out, uint64_t cap) {
+  try {
+    SW_REQUIRE(m && out && cap >= m->nparams, "NULL argument or buffer too small");
+    sw::use(m->ctx);
+    SW_CHECK_HIP(hipMemcpyAsync(out, m->params.p, m->nparams * 4, hipMemcpyDeviceToHost, m->ctx->stream));
+    SW_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+    return SW_OK;
+  } catch (const Error& e) {
+    return fail(e);
+  }
+}
+
+int sw_mlp_set_params(sw_mlp* m, const float* in, uint64_t count) {
+  try {
+    SW_REQUIRE(m && in && count == m->nparams, "NULL argument or wrong parameter count");
+    sw::use(m->ctx);
+    SW_CHECK_HIP(hipMemcpyAsync(m->params.p, in, count * 4, hipMemcpyHostToDevice, m->ctx->stream));
+    SW_CHECK_HIP(sw::mlp_refresh_bf16(m->p, m->ctx->stream));
+    SW_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+    return SW_OK;
+  } catch (const Error& e) {
+    return fail(e);
+  }
+}
+
+// Forward pass on raw BF16 features x[count][D] (host): y[count] (host).
+int sw_mlp_forward(sw_mlp* m, const uint16_t* x, uint64_t count, float* y) {
+  try {
+    SW_REQUIRE(m && (x || !count) && (y || !count), "NULL argument");
+    sw::use(m->ctx);
+    if (!count) return SW_OK;
+    hipStream_t st = m->ctx->stream;
+    auto w = sw::work_of(m, count);
+    m->ctx->a.ensure(count * m->p.D * 2);
+    SW_CHECK_HIP(hipMemcpyAsync(m->ctx->a.p, x, count * m->p.D * 2, hipMemcpyHostToDevice, st));
+    SW_CHECK_HIP(sw::mlp_forward(m->p, m->ctx->a.as<uint16_t>(), count, w, w.y, st));
+    SW_CHECK_HIP(hipMemcpyAsync(y, w.y, count * 4, hipMemcpyDeviceToHost, st));
+    SW_CHECK_HIP(hipStreamSynchronize(st));
+    return SW_OK;
+  } catch (const Error& e) {
+    return fail(e);
+  }
+}
+
+// Mean squared error and its gradient with respect to every parameter
+// (flattened as sw_mlp_get_params) on one batch.
+int sw_mlp_gradients(sw_mlp* m, const uint16_t* x, const float* target, uint64_t count, float* grads, float* loss) {
+  try {
+    SW_REQUIRE(m && x && target && grads && count, "NULL argument");
+    sw::use(m->ctx);
+    hipStream_t st = m->ctx->stream;
+    auto w = sw::work_of(m, count);
+    m->ctx->a.ensure(count * m->p.D * 2);
+    m->ctx->b.ensure(count * 4);
+    SW_CHECK_HIP(hipMemcpyAsync(m->ctx->a.p, x, count * m->p.D * 2, hipMemcpyHostToDevice, st));
+    SW_CHECK_HIP(hipMemcpyAsync(m->ctx->b.p, target, count * 4, hipMemcpyHostToDevice, st));
+    SW_CHECK_HIP(hipMemsetAsync(m->lossb.p, 0, 4, st));
+    auto g = sw::grads_of(m);
+    SW_CHECK_HIP(sw::mlp_forward(m->p, m->ctx->a.as<uint16_t>(), count, w, w.y, st));
+    SW_CHECK_HIP(sw::mlp_backward(m->p, m->ctx->a.as<uint16_t>(), count, m->ctx->b.as<float>(), w, g,
+        m->lossb.as<float>(), st));
+    SW_CHECK_HIP(hipMemcpyAsync(grads, m->grads.p, m->nparams * 4, hipMemcpyDeviceToHost, st));
+    if (loss) SW_CHECK_HIP(hipMemcpyAsync(loss, m->lossb.p, 4, hipMemcpyDeviceToHost, st));
+    SW_CHECK_HIP(hipStreamSynchronize(st));
+    return SW_OK;
+  } catch (const Error& e) {
+    return fail(e);
+  }
+}
+
+// Adam training on samples x[count][D], target[count] (host): `epochs` passes
+// in shuffled minibatches; loss_curve[e] = mean loss of epoch e (may be NULL).
+int sw_mlp_train(sw_mlp* m, const uint16_t* x, const float* target, uint64_t count, uint32_t epochs, uint32_t batch,
+    float lr, uint64_t seed, float* loss_curve) {
+  try {
+    SW_REQUIRE(m && x && target && count && batch, "NULL argument");
+    sw::use(m->ctx);
+    hipStream_t st = m->ctx->stream;
+    const uint64_t D = m->p.D;
+    // Shuffle on the host once per epoch; batches gathered on the host side.
+    DevBuf dx, dt;
+    dx.ensure(uint64_t(batch) * D * 2);
+    dt.ensure(uint64_t(batch) * 4);
+    std::vector<uint64_t> idx(count);
+    std::iota(idx.begin(), idx.end(), 0ull);
+    std::vector<uint16_t> bx(uint64_t(batch) * D);
+    std::vector<float> bt(batch);
+    auto w = sw::work_of(m, batch);
+    auto g = sw::grads_of(m);
+    uint64_t s = seed;
+    for (uint32_t e = 0; e < epochs; ++e) {
+      for (uint64_t i = count; i > 1; --i) std::swap(idx[i - 1], idx[sw::splitmix(s) % i]);
+      SW_CHECK_HIP(hipMemsetAsync(m->lossb.p, 0, 4, st));
+      uint64_t steps = 0;
+      for (uint64_t b0 = 0; b0 + batch <= count; b0 += batch, ++steps) {
+        for (uint32_t r = 0; r < batch; ++r) {
+          std::memcpy(&bx[uint64_t(r) * D], x + idx[b0 + r] * D, D * 2);
+          bt[r] = target[idx[b0 + r]];
+        }
+        SW_CHECK_HIP(hipMemcpyAsync(dx.p, bx.data(), bx.size() * 2, hipMemcpyHostToDevice, st));
+        SW_CHECK_HIP(hipMemcpyAsync(dt.p, bt.data(), bt.size() * 4, hipMemcpyHostToDevice, st));
+        SW_CHECK_HIP(sw::mlp_forward(m->p, dx.as<uint16_t>(), batch, w, w.y, st));
+        SW_CHECK_HIP(sw::mlp_backward(m->p, dx.as<uint16_t>(), batch, dt.as<float>(), w, g, m->lossb.as<float>(), st));
+        SW_CHECK_HIP(sw::mlp_adam(m->p, g, m->adam, lr, st));
+        SW_CHECK_HIP(hipStreamSynchronize(st));  // host batch buffers are reused
+      }
+      float l = 0.f;
+      SW_CHECK_HIP(hipMemcpyAsync(&l, m->lossb.p, 4, hipMemcpyDeviceToHost, st));
+      SW_CHECK_HIP(hipStreamSynchronize(st));
+      if (loss_curve) loss_curve[e] = steps ? l / float(steps) : 0.f;
+    }
+    return SW_OK;
+  } catch (const Error& e) {
+    return fail(e);
+  }
+}
+
+// BF16 features of host sets for automaton delta: x[count][D] (host).
+int sw_mlp_features(sw_mlp* m, const uint32_t* delta, const uint64_t* sets, uint64_t count, uint16_t* x) {
+  try {
+    SW_REQUIRE(m && delta && (sets || !count) && (x || !count), "NULL argument");
+    sw::use(m->ctx);
+    if (!count) return SW_OK;
+    hipStream_t st = m->ctx->stream;
+    const uint32_t n = m->n, k = m->k, W = (n + 63) / 64;
+    auto t = sw::build_tables(n, k, delta, 0);
+    DevBuf dtab, dsets, dx, scratch;
+    dtab.ensure(t.size() * 8);
+    dsets.ensure(count * W * 8);
+    dx.ensure(count * m->p.D * 2);
+    SW_CHECK_HIP(hipMemcpyAsync(dtab.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, st));
+    SW_CHECK_HIP(hipMemcpyAsync(dsets.p, sets, count * W * 8, hipMemcpyHostToDevice, st));
+    sw::mlp_features(m, nullptr, dtab.as<uint64_t>(), dsets.as<uint64_t>(), count, dx.as<uint16_t>(), scratch, st);
+    SW_CHECK_HIP(hipMemcpyAsync(x, dx.p, count * m->p.D * 2, hipMemcpyDeviceToHost, st));
+    SW_CHECK_HIP(hipStreamSynchronize(st));
+    return SW_OK;
+  } catch (const Error& e) {
+    return fail(e);
+  }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- samples
+namespace {
+
+// Host beam (cardinality order, as the reference's) whose candidates are the
+// training sets: per depth, `per_depth` children spread over the sorted list.
+std::vector<uint64_t> beam_candidates(uint32_t n, uint32_t k, const uint32_t* d, uint32_t width, uint32_t max_depth,
+    uint32_t per_depth) {
+  const uint32_t W = (n + 63) / 64;
+  auto tab = sw::build_tables(n, k, d, 1);
+  const uint32_t S = (n + 7) / 8;
+  auto apply = [&](const uint64_t* x, uint32_t a, uint64_t* r) {
+    for (uint32_t w = 0; w < W; ++w) r[w] = 0;
+    for (uint32_t s = 0; s < S; ++s) {
+      uint32_t byte = uint32_t(x[s >> 3] >> ((s & 7) * 8)) & 255u;
+      if (!byte) continue;
+      const uint64_t* row = &tab[((size_t(a) * S + s) * 256 + byte) * W];
+      for (uint32_t w = 0; w < W; ++w) r[w] |= row[w];
+    }
+  };
+  auto pop = [&](const uint64_t* x) {
+    int c = 0;
+    for (uint32_t w = 0; w < W; ++w) c += __builtin_popcountll(x[w]);
+    return c;
+  };
+  const std::vector<char> sink = sw::sink_states(n, k, d);
+  std::vector<uint64_t> list, out;
+  for (uint32_t q = 0; q < n; ++q)
+    if (sink[q])
+      for (uint32_t w = 0; w < W; ++w) list.push_back(w == q / 64 ? (1ull << (q % 64)) : 0ull);
+  for (uint32_t depth = 1; depth <= max_depth && !list.empty(); ++depth) {
+    const size_t m = list.size() / W;
+    std::vector<uint64_t> next(m * k * W);
+    for (uint32_t a = 0; a < k; ++a)
+      for (size_t i = 0; i < m; ++i) apply(&list[i * W], a, &next[(a * m + i) * W]);
+    std::vector<uint32_t> ord(m * k);
+    std::iota(ord.begin(), ord.end(), 0u);
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) {
+      int px = pop(&next[size_t(x) * W]), py = pop(&next[size_t(y) * W]);
+      if (px != py) return px > py;
+      return std::lexicographical_compare(&next[size_t(x) * W], &next[size_t(x) * W + W], &next[size_t(y) * W],
+          &next[size_t(y) * W + W]);
+    });
+    std::vector<uint64_t> uniq;
+    for (size_t j = 0; j < ord.size(); ++j) {
+      const uint64_t* r = &next[size_t(ord[j]) * W];
+      if (!uniq.empty() && std::equal(r, r + W, &uniq[uniq.size() - W])) continue;
+      uniq.insert(uniq.end(), r, r + W);
+    }
+    const size_t u = uniq.size() / W;
+    const size_t take = std::min<size_t>(per_depth, u);
+    for (size_t t = 0; t < take; ++t) {
+      const size_t j = take == 1 ? 0 : t * (u - 1) / (take - 1);
+      out.insert(out.end(), &uniq[j * W], &uniq[j * W + W]);
+    }
+    if (pop(&uniq[0]) == int(n)) break;
+    if (u > width) uniq.resize(size_t(width) * W);
+    list.swap(uniq);
+  }
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Training samples from the exact search's own forward levels: candidate
+// sets S of a cardinality beam, each labelled with its forward distance
+// min{d : some image of Q at depth d lies inside S}, or fdepth + 1 when none
+// up to fdepth.  Writes up to `cap` sets (W words each) and distances.
+int sw_mlp_samples(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t fdepth, uint32_t width,
+    uint32_t bdepth, uint32_t per_depth, uint64_t* sets_out, float* dist_out, uint64_t cap, uint64_t* count) {
+  try {
+    SW_REQUIRE(ctx && delta && sets_out && dist_out && count, "NULL argument");
+    SW_REQUIRE(n >= 2 && n <= 512 && k >= 1 && k <= 255, "n in [2, 512], k in [1, 255]");
+    sw::use(ctx);
+    hipStream_t st = ctx->stream;
+    const uint32_t W = (n + 63) / 64;
+    std::vector<uint64_t> cand = beam_candidates(n, k, delta, width, bdepth, per_depth);
+    uint64_t c = std::min<uint64_t>(cand.size() / W, cap);
+    DevBuf levels, dcand, flags;
+    std::vector<uint64_t> level_end;
+    sw::forward_levels(ctx, n, k, delta, fdepth, 1ull << 24, levels, level_end);
+    dcand.ensure(c * W * 8 + 8);
+    SW_CHECK_HIP(hipMemcpyAsync(dcand.p, cand.data(), c * W * 8, hipMemcpyHostToDevice, st));
+    std::vector<float> dist(c, float(level_end.size()));
+    std::vector<char> set(c, 0);
+    const size_t fb = sw::vsub_flag_bytes(int(W), c);
+    flags.ensure(fb + 8);
+    std::vector<uint8_t> bits(fb);
+    uint64_t lo = 0;
+    for (size_t d = 0; d < level_end.size(); ++d) {
+      const uint64_t hi = level_end[d];
+      SW_CHECK_HIP(hipMemsetAsync(flags.p, 0, fb, st));
+      sw::VsubParams p;
+      p.stream = levels.as<uint64_t>() + lo * W;
+      p.s_count = hi - lo;
+      p.tile = dcand.as<uint64_t>();
+      p.t_count = c;
+      p.n = n;
+      p.mode = sw::VSUB_FLAGS;
+      p.flags = flags.p;
+      SW_CHECK_HIP(sw::launch_vsub(int(W), p, st));
+      SW_CHECK_HIP(hipMemcpyAsync(bits.data(), flags.p, fb, hipMemcpyDeviceToHost, st));
+      SW_CHECK_HIP(hipStreamSynchronize(st));
+      for (uint64_t i = 0; i < c; ++i)
+        if (!set[i] && ((bits[i >> 3] >> (i & 7)) & 1)) {
+          set[i] = 1;
+          dist[i] = float(d);
+        }
+      lo = hi;
+    }
+    std::copy(cand.begin(), cand.begin() + c * W, sets_out);
+    std::copy(dist.begin(), dist.end(), dist_out);
+    *count = c;
+    return SW_OK;
+  } catch (const Error& e) {
+    return fail(e);
+  }
+}
+
+// Beam search scored by the network: from the sink singletons, expand by
+// preimages, drop duplicates, keep the `width` candidates of smallest
+// predicted forward distance; the first depth producing Q bounds the length.
+// Returns *bound (upper when not improved) and the word of that length.
+int sw_learned_beam(sw_ctx* ctx, sw_mlp* m, const uint32_t* delta, uint32_t width, uint64_t upper, uint64_t* bound,
+    uint32_t* word, uint64_t word_cap, uint64_t* word_len) {
+  try {
+    SW_REQUIRE(ctx && m && delta && bound && width, "NULL argument");
+    sw::use(ctx);
+    hipStream_t st = ctx->stream;
+    const uint32_t n = m->n, k = m->k, W = (n + 63) / 64, S = (n + 7) / 8;
+    *bound = upper;
+    if (word_len) *word_len = 0;
+    auto tf = sw::build_tables(n, k, delta, 0);
+    auto tb = sw::build_tables(n, k, delta, 1);
+    DevBuf dtf, dtb, list, cand, table, slot, keep, pos, uniq, uidx, x, scratch, keys, keys2, vals, vals2, sel, small,
+        scan_tmp, sort_tmp;
+    dtf.ensure(tf.size() * 8);
+    dtb.ensure(tb.size() * 8);
+    SW_CHECK_HIP(hipMemcpyAsync(dtf.p, tf.data(), tf.size() * 8, hipMemcpyHostToDevice, st));
+    SW_CHECK_HIP(hipMemcpyAsync(dtb.p, tb.data(), tb.size() * 8, hipMemcpyHostToDevice, st));
+    const std::vector<char> sink = sw::sink_states(n, k, delta);
+    std::vector<uint64_t> seed;
+    for (uint32_t q = 0; q < n; ++q)
+      if (sink[q])
+        for (uint32_t w = 0; w < W; ++w) seed.push_back(w == q / 64 ? (1ull << (q % 64)) : 0ull);
+    uint64_t m0 = seed.size() / W;
+    list.ensure(seed.size() * 8 + 8);
+    SW_CHECK_HIP(hipMemcpyAsync(list.p, seed.data(), seed.size() * 8, hipMemcpyHostToDevice, st));
+    small.ensure(64);
+    std::vector<std::vector<uint32_t>> hist;  // per depth: candidate index of each kept set
+    std::vector<uint64_t> hist_m;              // parent list size per depth
+    size_t scan_bytes = 0, sort_bytes = 0;
+    for (uint64_t len = 1; len < upper; ++len) {
+      const uint64_t cnt = m0 * k;
+      cand.ensure(cnt * W * 8 + 8);
+      SW_CHECK_HIP(sw::launch_expand(int(W), list.as<uint64_t>(), m0, dtb.as<uint64_t>(), S, k, nullptr, n, 0,
+          cand.as<uint64_t>(), nullptr, st));
+      unsigned long long* first = small.as<unsigned long long>();
+      SW_CHECK_HIP(hipMemsetAsync(first, 0xff, 8, st));
+      SW_CHECK_HIP(sw::launch_first_full(cand.as<uint64_t>(), cnt, n, W, first, st));
+      unsigned long long f = ~0ull;
+      SW_CHECK_HIP(hipMemcpyAsync(&f, first, 8, hipMemcpyDeviceToHost, st));
+      SW_CHECK_HIP(hipStreamSynchronize(st));
+      if (f != ~0ull) {
+        // Word: letter of this candidate, then back through the kept parents.
+        std::vector<uint32_t> w;
+        uint64_t c = f;
+        w.push_back(uint32_t(c / m0));
+        uint64_t parent = c % m0;
+        for (size_t dd = hist.size(); dd-- > 0;) {
+          const uint64_t ci = hist[dd][parent];
+          w.push_back(uint32_t(ci / hist_m[dd]));
+          parent = ci % hist_m[dd];
+        }
+        *bound = len;
+        if (word_len) *word_len = w.size();
+        if (word && w.size() <= word_cap) std::copy(w.begin(), w.end(), word);
+        break;
+      }
+      // Dedup within the step.
+      uint64_t tcap = 1024;
+      while (tcap < 2 * cnt) tcap <<= 1;
+      table.ensure(tcap * 4);
+      slot.ensure(cnt * 4 + 4);
+      keep.ensure(cnt * 4 + 4);
+      pos.ensure(cnt * 4 + 4);
+      SW_CHECK_HIP(sw::launch_fill_u32(table.as<uint32_t>(), sw::kEmpty, tcap, st));
+      SW_CHECK_HIP(sw::launch_hash_insert(int(W), cand.as<uint64_t>(), nullptr, cnt, table.as<uint32_t>(), tcap - 1,
+          cand.as<uint64_t>(), slot.as<uint32_t>(), nullptr, st));
+      SW_CHECK_HIP(sw::launch_hash_resolve(slot.as<uint32_t>(), table.as<uint32_t>(), cnt, keep.as<uint32_t>(), st));
+      size_t need = 0;
+      SW_CHECK_HIP(sw::launch_scan(nullptr, nullptr, cnt, nullptr, nullptr, &need, st));
+      if (need > scan_bytes) {
+        scan_tmp.release();
+        scan_tmp.ensure(need);
+        scan_bytes = need;
+      }
+      uint32_t* total = small.as<uint32_t>() + 4;
+      SW_CHECK_HIP(sw::launch_scan(keep.as<uint32_t>(), pos.as<uint32_t>(), cnt, total, scan_tmp.p, &scan_bytes, st));
+      uniq.ensure(cnt * W * 8 + 8);
+      uidx.ensure(cnt * 4 + 4);
+      SW_CHECK_HIP(sw::launch_select(int(W), cand.as<uint64_t>(), 0, keep.as<uint32_t>(), pos.as<uint32_t>(), cnt,
+          uniq.as<uint64_t>(), uidx.as<uint32_t>(), st));
+      uint32_t nu = 0;
+      SW_CHECK_HIP(hipMemcpyAsync(&nu, total, 4, hipMemcpyDeviceToHost, st));
+      SW_CHECK_HIP(hipStreamSynchronize(st));
+      // Score and keep the best `width`.
+      x.ensure(uint64_t(nu) * m->p.D * 2 + 8);
+      sw::mlp_features(m, nullptr, dtf.as<uint64_t>(), uniq.as<uint64_t>(), nu, x.as<uint16_t>(), scratch, st);
+      auto wk = sw::work_of(m, nu);
+      SW_CHECK_HIP(sw::mlp_forward(m->p, x.as<uint16_t>(), nu, wk, wk.y, st));
+      keys2.ensure(uint64_t(nu) * 4 + 4);
+      vals.ensure(uint64_t(nu) * 4 + 4);
+      vals2.ensure(uint64_t(nu) * 4 + 4);
+      SW_CHECK_HIP(sw::launch_iota(vals.as<uint32_t>(), nu, st));
+      size_t sneed = 0;
+      SW_CHECK_HIP(sw::sort_scores(nullptr, nullptr, nullptr, nullptr, nu, nullptr, &sneed, st));
+      if (sneed > sort_bytes) {
+        sort_tmp.release();
+        sort_tmp.ensure(sneed);
+        sort_bytes = sneed;
+      }
+      SW_CHECK_HIP(sw::sort_scores(wk.y, keys2.as<float>(), vals.as<uint32_t>(), vals2.as<uint32_t>(), nu, sort_tmp.p,
+          &sort_bytes, st));
+      const uint64_t keepn = std::min<uint64_t>(width, nu);
+      list.ensure(keepn * W * 8 + 8);
+      SW_CHECK_HIP(sw::launch_gather_rows(uniq.as<uint64_t>(), vals2.as<uint32_t>(), keepn, W, list.as<uint64_t>(), st));
+      // Candidate index of every kept set (for the word).
+      std::vector<uint32_t> kept(keepn), cidx(nu);
+      SW_CHECK_HIP(hipMemcpyAsync(kept.data(), vals2.p, keepn * 4, hipMemcpyDeviceToHost, st));
+      SW_CHECK_HIP(hipMemcpyAsync(cidx.data(), uidx.p, uint64_t(nu) * 4, hipMemcpyDeviceToHost, st));
+      SW_CHECK_HIP(hipStreamSynchronize(st));
+      std::vector<uint32_t> h(keepn);
+      for (uint64_t j = 0; j < keepn; ++j) h[j] = cidx[kept[j]];
+      hist.push_back(std::move(h));
+      hist_m.push_back(m0);
+      m0 = keepn;
+    }
+    return SW_OK;
+  } catch (const Error& e) {
+    return fail(e);
+  }
+}
+
+}  // extern "C"

@@ -97,6 +97,9 @@ struct ExactOutcome {
   sw_exact_result r{};
   std::vector<uint32_t> word;
 };
+// Forward levels of the exact search (images of Q up to `depth`).
+void forward_levels(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t depth, uint64_t cap_rows,
+    DevBuf& out, std::vector<uint64_t>& level_end);
 ExactOutcome exact_search(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint64_t lower, uint64_t upper,
     const sw_exact_config& cfg);
 

@@ -1,0 +1,51 @@
+// The learned scorer (k_mlp.hip, mlp.cpp): device parameter and work buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sw {
+
+// Parameters on the device: FP32 masters and BF16 copies of the two hidden
+// weight matrices.  W1 is [H][D], W2 is [H][H] (row = output unit).
+struct MlpDev {
+  uint32_t D = 0, H = 0;
+  float *w1 = nullptr, *b1 = nullptr, *w2 = nullptr, *b2 = nullptr, *w3 = nullptr, *b3 = nullptr;
+  uint16_t *w1b = nullptr, *w2b = nullptr;
+};
+
+// Activations of one batch.
+struct MlpWork {
+  float *h1f = nullptr, *h2f = nullptr, *y = nullptr, *dy = nullptr, *g1f = nullptr;
+  uint16_t *h1b = nullptr, *h2b = nullptr, *g1b = nullptr, *g2b = nullptr;
+};
+
+struct MlpGrads {
+  float *w1 = nullptr, *b1 = nullptr, *w2 = nullptr, *b2 = nullptr, *w3 = nullptr, *b3 = nullptr;
+};
+
+struct MlpAdam {
+  float *m = nullptr, *v = nullptr;
+  uint64_t off[6] = {0, 0, 0, 0, 0, 0};
+  int t = 0;
+};
+
+hipError_t launch_gemm(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* Cf,
+    uint16_t* Cb, int ldc, int M, int N, int K, const float* bias, int relu, int accumulate, hipStream_t st);
+hipError_t launch_features(int W, const uint64_t* sets, const uint64_t* images, uint64_t count, uint32_t n, uint32_t k,
+    uint32_t D, uint16_t* x, hipStream_t st);
+hipError_t mlp_forward(const MlpDev& p, const uint16_t* x, uint64_t B, MlpWork& w, float* y, hipStream_t st);
+hipError_t mlp_backward(const MlpDev& p, const uint16_t* x, uint64_t B, const float* target, MlpWork& w, MlpGrads& g,
+    float* loss, hipStream_t st);
+hipError_t mlp_adam(MlpDev& p, MlpGrads& g, MlpAdam& a, float lr, hipStream_t st);
+hipError_t mlp_refresh_bf16(MlpDev& p, hipStream_t st);
+// Smallest index of a set with all n states (first must hold ~0).
+hipError_t launch_first_full(const uint64_t* sets, uint64_t count, uint32_t n, uint32_t W, unsigned long long* first,
+    hipStream_t st);
+hipError_t launch_gather_rows(const uint64_t* rows, const uint32_t* idx, uint64_t count, uint32_t W, uint64_t* out,
+    hipStream_t st);
+hipError_t launch_iota(uint32_t* p, uint64_t count, hipStream_t st);
+// Stable ascending sort of float keys with u32 values (temp == NULL: size query).
+hipError_t sort_scores(const float* keys_in, float* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
+    uint64_t count, void* temp, size_t* temp_bytes, hipStream_t st);
+
+}  // namespace sw

@@ -14,6 +14,7 @@ itself, so nothing at test time depends on the generator or the reference:
 * random_misc.json    random n in [16, 96], k in {2,3,4}, including
                       non-synchronizing ones
 * c5_random300.json   random n=300 k=2, seeds 1000..1004
+* c4_batch.json       the 4096-instance batch (n in [16,160], k in {2,3})
 
 Expected values are the reference binary's [lower, upper] after the full
 configs/exact.json plan and the Eppstein and Beam bounds it produced.
@@ -186,8 +187,23 @@ def make_c5():
     write("c5_random300.json", cases, "random n=300 k=2 held-out seeds 1000..1004")
 
 
+def make_c4():
+    """The 4096-instance batch: specs only (n, k, seed); the automaton is
+    gen.random_dfa(n, k, seed), whose stream tests/test_gen.py pins."""
+    specs = gen.batch_specs(4096, 2025)
+    pairs = sorted({(n, k) for n, k, _ in specs})
+    pmap(lambda nk: build(*nk), pairs, 6)
+    def one(t):
+        i, (n, k, seed) = t
+        r = run_ref(n, k, gen.random_dfa(n, k, seed))
+        return {"i": i, "n": n, "k": k, "seed": seed, "non_synchro": r["non_synchro"], "lower": r["lower"],
+                "upper": r["upper"], "ref_us": r["ref_us"]}
+    cases = pmap(one, list(enumerate(specs)), 8)
+    write("c4_batch.json", cases, "batch_specs(4096, 2025); delta = gen.random_dfa(n, k, seed)")
+
+
 MAKERS = {"c1": make_c1, "c3": make_c3, "cerny": make_cerny, "hard": make_hard, "slowly": make_slowly,
-          "readme": make_readme, "misc": make_misc, "c5": make_c5}
+          "readme": make_readme, "misc": make_misc, "c5": make_c5, "c4": make_c4}
 
 if __name__ == "__main__":
     names = sys.argv[1:] or list(MAKERS)

@@ -1,0 +1,170 @@
+"""The learned scorer on the GPU: BF16 MFMA forward/backward against a CPU
+FP32 reference (numpy), training, labelled samples, and the learned beam."""
+import numpy as np
+import pytest
+
+import synchrowords_amd as sw
+from conftest import golden
+from synchrowords_amd import gen, replay
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(x):
+    """Round float32 to bfloat16 (nearest even), returned as float32."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+def bf16_bits(x):
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def unpack(p, D, H):
+    o = 0
+    w1 = p[o:o + H * D].reshape(H, D); o += H * D
+    b1 = p[o:o + H]; o += H
+    w2 = p[o:o + H * H].reshape(H, H); o += H * H
+    b2 = p[o:o + H]; o += H
+    w3 = p[o:o + H]; o += H
+    b3 = p[o:o + 1]
+    return w1, b1, w2, b2, w3, b3
+
+
+def reference(p, x_bits, t, D, H):
+    """FP32 forward/backward with the GPU's BF16 rounding points."""
+    w1, b1, w2, b2, w3, b3 = unpack(p, D, H)
+    x = (x_bits.astype(np.uint32) << 16).view(np.float32)
+    a1 = x @ bf16(w1).T + b1
+    h1 = np.maximum(a1, 0)
+    h1b = bf16(h1)
+    a2 = h1b @ bf16(w2).T + b2
+    h2 = np.maximum(a2, 0)
+    y = h2 @ w3 + b3[0]
+    B = x.shape[0]
+    dy = 2 * (y - t) / B
+    loss = float(np.sum((y - t) ** 2) / B)
+    gw3 = h2.T @ dy
+    gb3 = np.array([dy.sum()], dtype=np.float32)
+    dh2 = bf16(np.outer(dy, w3) * (h2 > 0))
+    gw2 = dh2.T @ h1b
+    gb2 = dh2.sum(axis=0)
+    dh1 = bf16(dh2 @ bf16(w2)) * (h1 > 0)
+    gw1 = dh1.T @ x
+    gb1 = dh1.sum(axis=0)
+    g = np.concatenate([gw1.ravel(), gb1, gw2.ravel(), gb2, gw3, gb3]).astype(np.float32)
+    return y.astype(np.float32), g, loss
+
+
+def rel_err(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12))
+
+
+@pytest.mark.parametrize("n,k,hidden,batch", [(300, 2, 512, 300), (48, 3, 64, 77), (100, 2, 128, 1024)])
+def test_forward_and_gradients_match_fp32_reference(ctx, n, k, hidden, batch):
+    m = sw.Mlp(ctx, n, k, hidden=hidden, seed=3)
+    D, H = m.in_dim, m.hidden
+    rng = np.random.default_rng(n + k)
+    p = m.params()
+    p[H * D:H * D + H] = rng.normal(0, 0.1, H)           # non-zero biases
+    p[H * D + H + H * H:H * D + 2 * H + H * H] = rng.normal(0, 0.1, H)
+    p[-1] = 0.5
+    m.set_params(p)
+    x = np.zeros((batch, D), dtype=np.float32)
+    x[:, :n] = rng.random((batch, n)) < 0.3
+    x[:, n:n + k] = rng.random((batch, k))
+    xb = bf16_bits(x)
+    t = rng.random(batch).astype(np.float32) * 3
+    y_ref, g_ref, loss_ref = reference(m.params(), xb, t, D, H)
+    y = m.forward(xb)
+    np.testing.assert_allclose(y, y_ref, rtol=2e-2, atol=2e-2)
+    g, loss = m.gradients(xb, t)
+    assert abs(loss - loss_ref) <= 2e-2 * max(1.0, loss_ref)
+    # Per tensor: relative L2 error of the gradient within 3e-2.
+    sizes = [H * D, H, H * H, H, H, 1]
+    o = 0
+    for s in sizes:
+        assert rel_err(g[o:o + s], g_ref[o:o + s]) < 3e-2, (o, s)
+        o += s
+
+
+def test_reference_gradient_by_finite_differences():
+    """The CPU reference's gradient formulas against central differences (float64)."""
+    rng = np.random.default_rng(0)
+    D, H, B = 32, 16, 8
+    n = D * H + H + H * H + H + H + 1
+    p = rng.normal(0, 0.3, n).astype(np.float64)
+    x = rng.random((B, D))
+    t = rng.random(B)
+
+    def loss_grad(p):
+        w1, b1, w2, b2, w3, b3 = unpack(p, D, H)
+        a1 = x @ w1.T + b1
+        h1 = np.maximum(a1, 0)
+        a2 = h1 @ w2.T + b2
+        h2 = np.maximum(a2, 0)
+        y = h2 @ w3 + b3[0]
+        dy = 2 * (y - t) / B
+        dh2 = np.outer(dy, w3) * (h2 > 0)
+        dh1 = (dh2 @ w2) * (h1 > 0)
+        g = np.concatenate([(dh1.T @ x).ravel(), dh1.sum(0), (dh2.T @ h1).ravel(), dh2.sum(0), h2.T @ dy, [dy.sum()]])
+        return float(np.sum((y - t) ** 2) / B), g
+
+    _, g = loss_grad(p)
+    for i in rng.choice(n, 40, replace=False):
+        e = np.zeros(n)
+        e[i] = 1e-6
+        fd = (loss_grad(p + e)[0] - loss_grad(p - e)[0]) / 2e-6
+        assert abs(fd - g[i]) <= 1e-5 + 1e-4 * abs(g[i])
+
+
+def test_training_reduces_loss(ctx):
+    m = sw.Mlp(ctx, 64, 2, hidden=128, seed=5)
+    rng = np.random.default_rng(1)
+    bits = rng.random((4096, 64)) < 0.4
+    x = np.zeros((4096, m.in_dim), dtype=np.float32)
+    x[:, :64] = bits
+    t = (bits[:, :8].sum(axis=1) * 0.5).astype(np.float32)   # a learnable target
+    curve = m.train(bf16_bits(x), t, epochs=8, batch=256, lr=2e-3, seed=1)
+    assert curve[-1] < 0.25 * curve[0], curve
+
+
+def _forward_distance(n, k, d, S, cap):
+    """Length of the shortest u with delta(Q,u) inside S, by BFS over images (host)."""
+    d = np.asarray(d).reshape(n, k)
+    start = frozenset(range(n))
+    seen = {start}
+    level = [start]
+    for dist in range(cap + 1):
+        if any(A <= S for A in level):
+            return dist
+        nxt = []
+        for A in level:
+            for a in range(k):
+                B = frozenset(int(d[q, a]) for q in A)
+                if B not in seen:
+                    seen.add(B)
+                    nxt.append(B)
+        level = nxt
+    return cap + 1
+
+
+def test_samples_are_labelled_with_forward_distances(ctx):
+    n, k = 24, 2
+    d = gen.random_dfa(n, k, 11)
+    sets, dist = sw.mlp_samples(ctx, n, k, d, fdepth=6, width=32, bdepth=12, per_depth=6, cap=200)
+    assert len(sets) > 20
+    for row, dd in zip(sets, dist):
+        S = frozenset(q for q in range(n) if (int(row[0]) >> q) & 1)
+        assert int(dd) == _forward_distance(n, k, d, S, 6)
+
+
+def test_learned_beam_gives_a_valid_bound(ctx):
+    cases = golden("c1_random48.json")[:10]
+    m = sw.Mlp(ctx, 48, 2, hidden=64, seed=2)
+    for c in cases:
+        b, w = m.beam(c["delta"], width=64, upper=48 * 48)
+        assert b >= c["upper"]
+        assert len(w) == b and replay(48, 2, c["delta"], w) == 1
