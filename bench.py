@@ -99,12 +99,96 @@ def time_to_exact(ctx, n: int, seed: int) -> dict:
         line = res.stdout.strip()
         out.update({"cpu_reference_s": round(cpu_s, 4), "cpu_reference_line": line, "cpu_threads": 1,
                     "speedup": round(cpu_s / gpu_s, 2)})
+    ref_mt = ROOT / "oracle" / "_ref" / f"ref_{n}_2_16"
+    if ref_mt.exists():
+        with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as f:
+            f.write(gen.to_text(n, 2, d))
+            path = f.name
+        try:
+            t0 = time.perf_counter()
+            subprocess.run([str(ref_mt), path], capture_output=True, text=True, timeout=300)
+            cpu16 = time.perf_counter() - t0
+        finally:
+            os.unlink(path)
+        out.update({"cpu_reference_16_threads_s": round(cpu16, 4), "speedup_vs_16_threads": round(cpu16 / gpu_s, 2)})
     return out
+
+
+def bench_batch(ctx, count: int) -> dict:
+    """The 4096-instance batch (n in [16,160], k in {2,3}) in one persistent launch."""
+    from synchrowords_amd import gen
+    import synchrowords_amd as sw
+    specs = gen.batch_specs(4096, 2025)[:count]
+    inst = [(n, k, sw.random_dfa(n, k, seed)) for n, k, seed in specs]
+    ctx.batch(inst[:64])  # warm
+    t0 = time.perf_counter()
+    res, st = ctx.batch(inst)
+    wall = time.perf_counter() - t0
+    out = {"workload": f"batch of {count} random DFAs, n in [16,160], k in {{2,3}} (batch_specs(4096, 2025))",
+           "wall_s": round(wall, 4), "instances_per_s": count / wall, "stats": st}
+    fx = ROOT / "tests" / "golden" / "c4_batch.json"
+    if fx.exists():
+        cases = json.loads(fx.read_text())["cases"][:count]
+        bad = [c["i"] for c, r in zip(cases, res)
+               if r["non_synchro"] != bool(c["non_synchro"]) or (not c["non_synchro"] and r["length"] != c["upper"])]
+        out["mismatches_vs_reference"] = len(bad)
+        out["reference_cpu_s_build_container"] = round(sum(c["ref_us"] for c in cases) / 1e6, 2)
+    bad_words = sum(1 for (n, k, d), r in zip(inst, res)
+                    if not r["non_synchro"] and n > 1 and (r["word"] is None or sw.replay(n, k, d, r["word"]) != 1))
+    out["words_not_synchronizing"] = bad_words
+    return out
+
+
+def bench_learned(ctx, scorer: str, seeds, width: int) -> dict:
+    """Learned-bound vs plain-bound time to the exact length (n=300, k=2)."""
+    import synchrowords_amd as sw
+    from synchrowords_amd.train_scorer import load_scorer
+    m = load_scorer(ctx, scorer)
+    n, k = m.n, m.k
+    rows = []
+    plan_no_beam = {"upper_bound": "1ULL * (AUT_N - 1) * (AUT_N - 1) + 1", "algorithms": [
+        {"name": "Brute", "config": {"max_n": "15"}},
+        {"name": "Eppstein", "config": {"find_word": True}}]}
+    for s in seeds:
+        d = sw.random_dfa(n, k, s)
+        # plain: the configs/exact.json plan (host Beam, device Exact)
+        t0 = time.perf_counter()
+        rp = ctx.pipeline(n, k, d)
+        t_plain = time.perf_counter() - t0
+        # learned: Eppstein, device beam scored by the network, device Exact
+        t0 = time.perf_counter()
+        re = ctx.pipeline(n, k, d, plan=plan_no_beam)
+        ub = re.upper
+        lb, lw = m.beam(d, width, ub)
+        ex = ctx.exact(n, k, d, upper=min(lb, ub), upper_word=lw if lb <= ub else None)
+        t_learned = time.perf_counter() - t0
+        cb, _ = ctx.beam(n, k, d, width, ub)  # cardinality beam, same width
+        rows.append({"seed": s, "length_plain": rp.upper, "length_learned": ex.length, "beam_bound": rp.beam_bound,
+                     "learned_bound": lb, "cardinality_bound_same_width": cb, "eppstein": ub,
+                     "plain_s": round(t_plain, 4), "learned_s": round(t_learned, 4)})
+        print(f"[bench] learned seed {s}: {rows[-1]}", file=sys.stderr, flush=True)
+    m.close()
+    same = all(r["length_plain"] == r["length_learned"] for r in rows)
+    return {"workload": f"held-out random DFAs n={n} k={k}, seeds {seeds[0]}..{seeds[-1]}, learned width {width}",
+            "identical_lengths": same,
+            "plain_s_total": round(sum(r["plain_s"] for r in rows), 4),
+            "learned_s_total": round(sum(r["learned_s"] for r in rows), 4),
+            "mean_gap_learned": float(np.mean([r["learned_bound"] - r["length_plain"] for r in rows])),
+            "mean_gap_cardinality_same_width": float(np.mean([r["cardinality_bound_same_width"] - r["length_plain"]
+                                                              for r in rows])),
+            "mean_gap_plain_beam": float(np.mean([r["beam_bound"] - r["length_plain"] for r in rows])),
+            "rows": rows}
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["subset", "batch", "learned"], default="subset",
+                    help="subset: the headline line; batch / learned: the batched and learned-bound reports")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--scorer", type=str, default=str(ROOT / "synchrowords_amd" / "models" / "scorer_n300_k2.npz"))
+    ap.add_argument("--learned-width", type=int, default=256)
+    ap.add_argument("--learned-seeds", type=int, default=20)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--quick", action="store_true", help="smaller lists (2^16) for a fast check")
@@ -117,6 +201,16 @@ def main() -> int:
         return 2
 
     import synchrowords_amd as sw
+
+    if args.workload != "subset":
+        ctx = sw.Context(0)
+        if args.workload == "batch":
+            line = bench_batch(ctx, args.batch_count)
+        else:
+            line = bench_learned(ctx, args.scorer, list(range(1000, 1000 + args.learned_seeds)), args.learned_width)
+        ctx.close()
+        print(json.dumps(line))
+        return 0
 
     n = 256
     count = (1 << 16) if args.quick else (1 << 20)

@@ -46,6 +46,9 @@ typedef struct sw_leaf sw_leaf;
 /* ------------------------------------------------------------ housekeeping */
 const char* sw_last_error(void);
 int sw_version(void);
+/* Seeded random DFA (the generator of synchrowords_amd/gen.py):
+ * out[q*k + a] uniform in [0, n).  Host only. */
+void sw_gen_random_dfa(uint32_t n, uint32_t k, uint64_t seed, uint32_t* out);
 /* Create a context on HIP device `device` (its own stream and scratch). */
 int sw_ctx_create(int device, sw_ctx** out);
 void sw_ctx_destroy(sw_ctx* ctx);
@@ -196,6 +199,16 @@ int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint32
     const uint32_t* deltas, const sw_batch_config* cfg, int32_t* non_synchro, uint64_t* lengths,
     uint32_t* word_lens, uint32_t* words, uint32_t word_cap, sw_batch_stats* stats);
 
+/* --------------------------------------------------------------- autotuner
+ * Time every launch shape of the subset kernel (workgroup size, workgroups
+ * aimed at, early exit) on seeded random lists of the given sizes and
+ * densities on the device; the fastest is kept in the context and used by
+ * later calls of the same size class (exact search, batch, the seams).
+ * No reference counterpart (the reference's launch shapes are fixed,
+ * cuda/subsets_implicit_trie_kernel.cu:8,196). */
+int sw_autotune_subset(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_count, float p_stream, float p_tile,
+    uint32_t reps, uint32_t* threads, uint32_t* target_wgs, uint32_t* early_exit, float* ms);
+
 /* -------------------------------------------------------- learned upper bound
  * A BF16 MLP (MFMA, FP32 accumulation) that scores beam candidates by their
  * estimated forward distance.  No reference counterpart: it replaces the
@@ -224,10 +237,11 @@ int sw_mlp_train(sw_mlp* m, const uint16_t* x, const float* target, uint64_t cou
  * measured on the search's forward levels up to fdepth (fdepth+1 if beyond). */
 int sw_mlp_samples(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t fdepth, uint32_t width,
     uint32_t bdepth, uint32_t per_depth, uint64_t* sets_out, float* dist_out, uint64_t cap, uint64_t* count);
-/* Beam of `width` scored by the network: *bound = first depth producing Q
- * (upper if none below it), with that word. */
-int sw_learned_beam(sw_ctx* ctx, sw_mlp* m, const uint32_t* delta, uint32_t width, uint64_t upper, uint64_t* bound,
-    uint32_t* word, uint64_t word_cap, uint64_t* word_len);
+/* Beam of `width` on the device, scored by the network m, or by cardinality
+ * (the reference's order, beam.hpp:145-162) when m is NULL: *bound = first
+ * depth producing Q (upper if none below it), with that word. */
+int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t width,
+    uint64_t upper, uint64_t* bound, uint32_t* word, uint64_t word_cap, uint64_t* word_len);
 
 #ifdef __cplusplus
 }

@@ -78,6 +78,7 @@ _U32P = C.POINTER(C.c_uint32)
 _SIGS = {
     "sw_last_error": (C.c_char_p, []),
     "sw_version": (C.c_int, []),
+    "sw_gen_random_dfa": (None, [C.c_uint32, C.c_uint32, C.c_uint64, _P]),
     "sw_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "sw_ctx_destroy": (None, [_P]),
     "sw_device_info": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_int), _U64P]),
@@ -112,7 +113,10 @@ _SIGS = {
     "sw_mlp_train": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _P]),
     "sw_mlp_samples": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                  _P, _P, C.c_uint64, _U64P]),
-    "sw_learned_beam": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint64, _U64P, _P, C.c_uint64, _U64P]),
+    "sw_beam_device": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint64, _U64P, _P, C.c_uint64,
+                                 _U64P]),
+    "sw_autotune_subset": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint64, C.c_float, C.c_float, C.c_uint32, _U32P,
+                                     _U32P, _U32P, C.POINTER(C.c_float)]),
 }
 
 
@@ -321,13 +325,8 @@ class Mlp:
         return curve
 
     def beam(self, delta, width: int, upper: int):
-        d = _delta(self.n, self.k, delta)
-        cap = max(int(upper), 1)
-        word = np.zeros(min(cap, 1 << 20), dtype=np.uint32)
-        b, wl = C.c_uint64(), C.c_uint64()
-        _check(load().sw_learned_beam(self.ctx.handle, self.handle, _ptr(d), width, upper, C.byref(b), _ptr(word),
-                                      word.size, C.byref(wl)))
-        return int(b.value), [int(v) for v in word[: wl.value]]
+        """(bound, word) of the beam scored by this network."""
+        return self.ctx.beam(self.n, self.k, delta, width, upper, mlp=self)
 
 
 def mlp_samples(ctx: "Context", n: int, k: int, delta, fdepth: int = 14, width: int = 256, bdepth: int = 60,
@@ -417,6 +416,23 @@ class Context:
                                       None, C.byref(cnt) if count else None, C.byref(anyf)))
         return (cnt.value if count else None), bool(anyf.value)
 
+    def beam(self, n: int, k: int, delta, width: int, upper: int, mlp: "Mlp | None" = None):
+        """(bound, word) of a device beam: cardinality order, or scored by `mlp`."""
+        d = _delta(n, k, delta)
+        cap = max(int(upper), 1)
+        word = np.zeros(min(cap, 1 << 20), dtype=np.uint32)
+        b, wl = C.c_uint64(), C.c_uint64()
+        _check(load().sw_beam_device(self.handle, mlp.handle if mlp else None, n, k, _ptr(d), width, upper,
+                                     C.byref(b), _ptr(word), word.size, C.byref(wl)))
+        return int(b.value), [int(v) for v in word[: wl.value]]
+
+    def autotune_subset(self, n: int, s_count: int, t_count: int, p_stream: float, p_tile: float, reps: int = 3):
+        """Tune the subset kernel's launch shape for this size class (kept in the context)."""
+        th, wg, ee, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_float()
+        _check(load().sw_autotune_subset(self.handle, n, s_count, t_count, p_stream, p_tile, reps, C.byref(th),
+                                         C.byref(wg), C.byref(ee), C.byref(ms)))
+        return {"threads": th.value, "target_wgs": wg.value, "early_exit": ee.value, "ms": ms.value}
+
     def count_bits(self, n: int, sets) -> np.ndarray:
         s = _sets(sets, n)
         acc = np.zeros(n, dtype=np.uint64)
@@ -491,6 +507,13 @@ class Context:
         ex = {f: getattr(r.exact, f) for f, _ in _ExactResult._fields_}
         return PipelineResult(bool(r.non_synchro), int(r.lower), int(r.upper), w, int(r.eppstein_bound),
                               int(r.beam_bound), times, ex)
+
+
+def random_dfa(n: int, k: int, seed: int) -> np.ndarray:
+    """gen.random_dfa computed by the library (same stream, native speed)."""
+    out = np.zeros(n * k, dtype=np.uint32)
+    load().sw_gen_random_dfa(n, k, seed, _ptr(out))
+    return out
 
 
 def replay(n: int, k: int, delta, word) -> int:

@@ -1,0 +1,129 @@
+// Autotuner of the subset kernel's launch shape.
+//
+// For a (W, stream size, tile size) class it times every candidate shape —
+// workgroup size (256/512/1024 threads), workgroups aimed at (256..4096) and
+// the per-word early exit — on random lists of the given densities resident on
+// the device, and records the fastest in the context's table.  Later calls of
+// the same class (exact search, batch labelling, the seams) launch with it.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+
+#include "sw_host.hpp"
+#include "sw_kernels.hpp"
+
+namespace sw {
+
+namespace {
+uint32_t size_class(uint64_t c) {
+  uint32_t k = 0;
+  while (c > 1) {
+    c >>= 1;
+    ++k;
+  }
+  return k;
+}
+uint32_t key(int W, uint64_t s, uint64_t t) { return (uint32_t(W) << 16) | (size_class(s) << 8) | size_class(t); }
+}  // namespace
+
+void apply_tuning(sw_ctx* ctx, int W, VsubParams& p) {
+  if (!ctx || ctx->tune.empty()) return;
+  const uint32_t k = key(W, p.s_count, p.t_count);
+  auto it = ctx->tune.find(k);
+  if (it == ctx->tune.end()) {
+    // Nearest class of the same W.
+    int best = 1 << 30;
+    for (auto jt = ctx->tune.begin(); jt != ctx->tune.end(); ++jt) {
+      if ((jt->first >> 16) != uint32_t(W)) continue;
+      const int ds = int((jt->first >> 8) & 255) - int((k >> 8) & 255);
+      const int dt = int(jt->first & 255) - int(k & 255);
+      const int d = ds * ds + dt * dt;
+      if (d < best) {
+        best = d;
+        it = jt;
+      }
+    }
+    if (it == ctx->tune.end()) return;
+  }
+  p.threads = it->second.threads;
+  p.target_wgs = it->second.target_wgs;
+  p.early_exit = int(it->second.early_exit);
+}
+
+}  // namespace sw
+
+extern "C" {
+
+// Tune the subset kernel for stream/tile sizes and densities; returns the
+// chosen shape.  Lists are generated on the device (seeded) and freed after.
+int sw_autotune_subset(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_count, float p_stream, float p_tile,
+    uint32_t reps, uint32_t* threads, uint32_t* target_wgs, uint32_t* early_exit, float* ms) {
+  using namespace sw;
+  try {
+    SW_REQUIRE(ctx && s_count && t_count, "NULL argument or empty lists");
+    SW_REQUIRE(n >= 1 && n <= 512, "n in [1, 512]");
+    use(ctx);
+    const int W = int((n + 63) / 64);
+    hipStream_t st = ctx->stream;
+    DevBuf S, T, out;
+    S.ensure(s_count * W * 8);
+    T.ensure(t_count * W * 8);
+    out.ensure(64);
+    SW_CHECK_HIP(launch_random_sets(W, S.as<uint64_t>(), s_count, n, p_stream, 0x5eed, st));
+    SW_CHECK_HIP(launch_random_sets(W, T.as<uint64_t>(), t_count, n, p_tile, 0xfeed, st));
+    hipEvent_t e0, e1;
+    SW_CHECK_HIP(hipEventCreate(&e0));
+    SW_CHECK_HIP(hipEventCreate(&e1));
+    sw_ctx::Tune best{1024, 1024, 1, 1e30f};
+    const uint32_t th[] = {256, 512, 1024};
+    const uint32_t wg[] = {256, 512, 1024, 2048, 4096};
+    for (uint32_t threads_i : th)
+      for (uint32_t wgs : wg)
+        for (uint32_t ee = 0; ee < 2; ++ee) {
+          VsubParams p;
+          p.stream = S.as<uint64_t>();
+          p.s_count = s_count;
+          p.tile = T.as<uint64_t>();
+          p.t_count = t_count;
+          p.n = n;
+          p.mode = VSUB_COUNT;
+          p.count_out = out.as<unsigned long long>();
+          p.threads = threads_i;
+          p.target_wgs = wgs;
+          p.early_exit = int(ee);
+          SW_CHECK_HIP(launch_vsub(W, p, st));  // warm
+          SW_CHECK_HIP(hipEventRecord(e0, st));
+          for (uint32_t r = 0; r < (reps ? reps : 3); ++r) SW_CHECK_HIP(launch_vsub(W, p, st));
+          SW_CHECK_HIP(hipEventRecord(e1, st));
+          SW_CHECK_HIP(hipEventSynchronize(e1));
+          float t = 0;
+          SW_CHECK_HIP(hipEventElapsedTime(&t, e0, e1));
+          t /= float(reps ? reps : 3);
+          if (t < best.ms) best = {threads_i, wgs, ee, t};
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->tune[(uint32_t(W) << 16) | ([&] {
+      uint32_t k = 0;
+      for (uint64_t c = s_count; c > 1; c >>= 1) ++k;
+      return k;
+    }() << 8) | [&] {
+      uint32_t k = 0;
+      for (uint64_t c = t_count; c > 1; c >>= 1) ++k;
+      return k;
+    }()] = best;
+    if (threads) *threads = best.threads;
+    if (target_wgs) *target_wgs = best.target_wgs;
+    if (early_exit) *early_exit = best.early_exit;
+    if (ms) *ms = best.ms;
+    return SW_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  }
+}
+
+}  // extern "C"

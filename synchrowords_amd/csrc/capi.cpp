@@ -7,6 +7,7 @@
 #include <memory>
 #include <string>
 
+#include "sw_gen.hpp"
 #include "sw_host.hpp"
 #include "sw_kernels.hpp"
 
@@ -66,6 +67,10 @@ extern "C" {
 const char* sw_last_error(void) { return sw::g_error.c_str(); }
 
 int sw_version(void) { return 10000; }
+
+void sw_gen_random_dfa(uint32_t n, uint32_t k, uint64_t seed, uint32_t* out) {
+  if (n && k && out) sw::random_dfa(n, k, seed, out);
+}
 
 int sw_ctx_create(int device, sw_ctx** out) {
   SW_API_BEGIN
@@ -279,6 +284,7 @@ int sw_leaf_run(sw_leaf* leaf, int proper, uint32_t small_pos, uint32_t small_cn
   p.mode = sw::VSUB_FLAGS;
   p.proper_by_size = proper ? 1 : 0;
   p.flags = leaf->flags.p;
+  sw::apply_tuning(ctx, int(W), p);
   SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
   std::unique_ptr<uint8_t[]> bits(new uint8_t[fb]);
   SW_CHECK_HIP(hipMemcpyAsync(bits.get(), leaf->flags.p, fb, hipMemcpyDeviceToHost, ctx->stream));
@@ -324,7 +330,8 @@ int sw_subset_check(sw_ctx* ctx, uint32_t n, const uint64_t* small, uint64_t sma
     SW_CHECK_HIP(hipMemsetAsync(ctx->d.p, 0, small_cnt * 4, ctx->stream));
     p.mode = sw::VSUB_SCOUNT;
     p.scount = ctx->d.as<uint32_t>();
-    SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
+    sw::apply_tuning(ctx, int(W), p);
+  SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
     std::unique_ptr<uint32_t[]> c(new uint32_t[small_cnt + 1]);
     if (small_cnt)
       SW_CHECK_HIP(hipMemcpyAsync(c.get(), ctx->d.p, small_cnt * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -337,7 +344,8 @@ int sw_subset_check(sw_ctx* ctx, uint32_t n, const uint64_t* small, uint64_t sma
   } else if (pair_count) {
     p.mode = sw::VSUB_COUNT;
     p.count_out = dcount;
-    SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
+    sw::apply_tuning(ctx, int(W), p);
+  SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
     unsigned long long c = 0;
     SW_CHECK_HIP(hipMemcpyAsync(&c, dcount, 8, hipMemcpyDeviceToHost, ctx->stream));
     SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
@@ -347,7 +355,8 @@ int sw_subset_check(sw_ctx* ctx, uint32_t n, const uint64_t* small, uint64_t sma
     p.mode = sw::VSUB_ANY;
     p.found = dfound;
     p.witness = dwit;
-    SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
+    sw::apply_tuning(ctx, int(W), p);
+  SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
     SW_CHECK_HIP(hipMemcpyAsync(&found, dfound, 4, hipMemcpyDeviceToHost, ctx->stream));
     SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   }

@@ -311,6 +311,7 @@ class Search {
       p.mode = VSUB_FLAGS;
       p.diag = int64_t(lo);
       p.flags = flags_.p;
+      apply_tuning(ctx_, W, p);
       SW_CHECK_HIP(launch_vsub(W, p, st_));
       pair_tests_ += S.rows * nnew;
       SW_CHECK_HIP(launch_alive(W, flags_.p, nnew, keep_.as<uint32_t>(), st_));
@@ -344,6 +345,7 @@ class Search {
     p.mode = VSUB_ANY;
     p.found = found;
     p.witness = wit;
+    apply_tuning(ctx_, int(W_), p);
     SW_CHECK_HIP(launch_vsub(int(W_), p, st_));
     pair_tests_ += F_.fcount * B_.fcount;
     int f = 0;

@@ -277,6 +277,14 @@ __global__ void k_gather_rows(const uint64_t* __restrict__ rows, const uint32_t*
   for (uint32_t w = 0; w < W; ++w) out[j * W + w] = rows[r * W + w];
 }
 
+__global__ void k_neg_popcount(const uint64_t* __restrict__ sets, uint64_t count, uint32_t W, float* __restrict__ y) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  int c = 0;
+  for (uint32_t w = 0; w < W; ++w) c += __popcll(sets[i * W + w]);
+  y[i] = -float(c);
+}
+
 __global__ void k_iota_u32(uint32_t* p, uint64_t count) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < count) p[i] = uint32_t(i);
@@ -295,6 +303,12 @@ hipError_t launch_gather_rows(const uint64_t* rows, const uint32_t* idx, uint64_
     hipStream_t st) {
   if (!count) return hipSuccess;
   hipLaunchKernelGGL(k_gather_rows, dim3(blocks(count)), dim3(256), 0, st, rows, idx, count, W, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_neg_popcount(const uint64_t* sets, uint64_t count, uint32_t W, float* y, hipStream_t st) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_neg_popcount, dim3(blocks(count)), dim3(256), 0, st, sets, count, W, y);
   return hipGetLastError();
 }
 

@@ -215,9 +215,11 @@ static hipError_t launch_vsub_w(const VsubParams& p0, hipStream_t st) {
   if (p.t_count == 0 || p.s_count == 0) return hipSuccess;
   const size_t tile = vsub_tile_sets(W);
   const uint64_t gx = (p.t_count + tile - 1) / tile;
+  if (p.threads != 256 && p.threads != 512 && p.threads != 1024) return hipErrorInvalidValue;
   if (p.s_chunk == 0) {
-    // Aim for about 1024 workgroups, at least 64 stream sets per workgroup.
-    uint64_t gy = (1024 + gx - 1) / gx;
+    // Aim for target_wgs workgroups, at least 64 stream sets per workgroup.
+    const uint64_t target = p.target_wgs ? p.target_wgs : 1024;
+    uint64_t gy = (target + gx - 1) / gx;
     uint64_t chunk = (p.s_count + gy - 1) / gy;
     if (chunk < 64) chunk = 64;
     p.s_chunk = chunk;
@@ -232,7 +234,39 @@ static hipError_t launch_vsub_w(const VsubParams& p0, hipStream_t st) {
     if (e != hipSuccess) return e;
     attr_set[W] = true;
   }
-  hipLaunchKernelGGL(k_vsub<W>, dim3(unsigned(gx), unsigned(gy)), dim3(1024), lds, st, p);
+  hipLaunchKernelGGL(k_vsub<W>, dim3(unsigned(gx), unsigned(gy)), dim3(p.threads), lds, st, p);
+  return hipGetLastError();
+}
+
+template <int W>
+__global__ void k_random_sets(uint64_t* out, uint64_t count, uint32_t n, uint32_t thr, uint64_t seed) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  for (int w = 0; w < W; ++w) {
+    uint64_t x = 0;
+    for (int b = 0; b < 64; ++b) {
+      const uint32_t q = w * 64 + b;
+      uint64_t z = seed + (i * (W * 64) + q) * 0x9E3779B97F4A7C15ull;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      if (q < n && uint32_t(z >> 48) < thr) x |= 1ull << b;
+    }
+    out[i * W + w] = x;
+  }
+}
+
+hipError_t launch_random_sets(int W, uint64_t* out, uint64_t count, uint32_t n, float p, uint64_t seed,
+    hipStream_t st) {
+  if (!count) return hipSuccess;
+  const uint32_t thr = uint32_t(p * 65536.0f);
+  const unsigned blocks = unsigned((count + 255) / 256);
+  switch (W) {
+#define C(w) case w: hipLaunchKernelGGL(k_random_sets<w>, dim3(blocks), dim3(256), 0, st, out, count, n, thr, seed); break;
+    C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8)
+#undef C
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -430,6 +430,7 @@ int sw_mlp_samples(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, u
       p.n = n;
       p.mode = sw::VSUB_FLAGS;
       p.flags = flags.p;
+      sw::apply_tuning(ctx, int(W), p);
       SW_CHECK_HIP(sw::launch_vsub(int(W), p, st));
       SW_CHECK_HIP(hipMemcpyAsync(bits.data(), flags.p, fb, hipMemcpyDeviceToHost, st));
       SW_CHECK_HIP(hipStreamSynchronize(st));
@@ -449,17 +450,21 @@ int sw_mlp_samples(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, u
   }
 }
 
-// Beam search scored by the network: from the sink singletons, expand by
-// preimages, drop duplicates, keep the `width` candidates of smallest
-// predicted forward distance; the first depth producing Q bounds the length.
-// Returns *bound (upper when not improved) and the word of that length.
-int sw_learned_beam(sw_ctx* ctx, sw_mlp* m, const uint32_t* delta, uint32_t width, uint64_t upper, uint64_t* bound,
-    uint32_t* word, uint64_t word_cap, uint64_t* word_len) {
+// Beam search on the device: from the sink singletons, expand by
+// preimages, drop duplicates, keep the `width` candidates of smallest score —
+// the network's predicted forward distance, or (m == NULL) minus the
+// cardinality, the reference's order; the first depth producing Q bounds the
+// length.  Returns *bound (upper when not improved) and the word of that
+// length.
+int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t width,
+    uint64_t upper, uint64_t* bound, uint32_t* word, uint64_t word_cap, uint64_t* word_len) {
   try {
-    SW_REQUIRE(ctx && m && delta && bound && width, "NULL argument");
+    SW_REQUIRE(ctx && delta && bound && width, "NULL argument");
+    SW_REQUIRE(!m || (m->n == n && m->k == k), "network built for another (n, k)");
+    SW_REQUIRE(n >= 2 && n <= 512 && k >= 1 && k <= 255, "n in [2, 512], k in [1, 255]");
     sw::use(ctx);
     hipStream_t st = ctx->stream;
-    const uint32_t n = m->n, k = m->k, W = (n + 63) / 64, S = (n + 7) / 8;
+    const uint32_t W = (n + 63) / 64, S = (n + 7) / 8;
     *bound = upper;
     if (word_len) *word_len = 0;
     auto tf = sw::build_tables(n, k, delta, 0);
@@ -537,10 +542,17 @@ int sw_learned_beam(sw_ctx* ctx, sw_mlp* m, const uint32_t* delta, uint32_t widt
       SW_CHECK_HIP(hipMemcpyAsync(&nu, total, 4, hipMemcpyDeviceToHost, st));
       SW_CHECK_HIP(hipStreamSynchronize(st));
       // Score and keep the best `width`.
-      x.ensure(uint64_t(nu) * m->p.D * 2 + 8);
-      sw::mlp_features(m, nullptr, dtf.as<uint64_t>(), uniq.as<uint64_t>(), nu, x.as<uint16_t>(), scratch, st);
-      auto wk = sw::work_of(m, nu);
-      SW_CHECK_HIP(sw::mlp_forward(m->p, x.as<uint16_t>(), nu, wk, wk.y, st));
+      keys.ensure(uint64_t(nu) * 4 + 4);
+      float* score = keys.as<float>();
+      if (m) {
+        x.ensure(uint64_t(nu) * m->p.D * 2 + 8);
+        sw::mlp_features(m, nullptr, dtf.as<uint64_t>(), uniq.as<uint64_t>(), nu, x.as<uint16_t>(), scratch, st);
+        auto wk = sw::work_of(m, nu);
+        SW_CHECK_HIP(sw::mlp_forward(m->p, x.as<uint16_t>(), nu, wk, wk.y, st));
+        score = wk.y;
+      } else {
+        SW_CHECK_HIP(sw::launch_neg_popcount(uniq.as<uint64_t>(), nu, W, score, st));
+      }
       keys2.ensure(uint64_t(nu) * 4 + 4);
       vals.ensure(uint64_t(nu) * 4 + 4);
       vals2.ensure(uint64_t(nu) * 4 + 4);
@@ -552,7 +564,7 @@ int sw_learned_beam(sw_ctx* ctx, sw_mlp* m, const uint32_t* delta, uint32_t widt
         sort_tmp.ensure(sneed);
         sort_bytes = sneed;
       }
-      SW_CHECK_HIP(sw::sort_scores(wk.y, keys2.as<float>(), vals.as<uint32_t>(), vals2.as<uint32_t>(), nu, sort_tmp.p,
+      SW_CHECK_HIP(sw::sort_scores(score, keys2.as<float>(), vals.as<uint32_t>(), vals2.as<uint32_t>(), nu, sort_tmp.p,
           &sort_bytes, st));
       const uint64_t keepn = std::min<uint64_t>(width, nu);
       list.ensure(keepn * W * 8 + 8);

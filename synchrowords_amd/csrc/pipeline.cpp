@@ -609,7 +609,10 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
           non_synchro = true;
         } else if (e.improved) {
           upper = e.bound;
-          if (find_word) word = e.word;
+          // The word is kept whatever find_word says: when Exact proves this
+          // bound optimal, it is the shortest word.
+          word = e.word;
+          (void)find_word;
         }
         out->eppstein_bound = upper;
       }
@@ -630,7 +633,7 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
           lower = upper = 0;
         } else {
           BeamResult b = beam(n, k, d, upper, bs);
-          if (b.bound < upper) {
+          if (b.bound < upper || (b.bound == upper && word.size() != upper && b.word.size() == b.bound)) {
             upper = b.bound;
             word = b.word;
           }

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -78,12 +79,24 @@ struct sw_ctx {
   // Scratch of the kernel-level entry points.
   sw::DevBuf a, b, c, d, e;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  // Subset-kernel launch shapes chosen by the autotuner, keyed by
+  // (W, floor(log2 stream sets), floor(log2 tile sets)).
+  struct Tune {
+    uint32_t threads, target_wgs, early_exit;
+    float ms;
+  };
+  std::map<uint32_t, Tune> tune;
 };
 
 namespace sw {
 
 // Make the context's device current.
 void use(sw_ctx* ctx);
+
+struct VsubParams;
+// Fill the launch shape of a subset-kernel call from the context's tuning
+// table (the nearest tuned size class of the same W), if any.
+void apply_tuning(sw_ctx* ctx, int W, VsubParams& p);
 
 // Reference-layout slice tables for every letter: [k][ceil(n/8)][256][W]
 // words; dir 0 = image, 1 = preimage.

@@ -21,6 +21,8 @@ struct VsubParams {
   int proper_by_size = 0;            // skip pairs of equal cardinality (S == T)
   int64_t diag = INT64_MIN;          // skip pairs with i - j == diag
   uint64_t s_chunk = 0;              // stream sets per workgroup row (0: auto)
+  uint32_t threads = 1024;           // workgroup size: 256, 512 or 1024
+  uint32_t target_wgs = 1024;        // workgroups aimed at when s_chunk is auto
   unsigned long long* count_out = nullptr;  // COUNT: += number of pairs
   int* found = nullptr;                     // ANY: set to 1 on a hit
   unsigned long long* witness = nullptr;    // ANY: min of (i << 32) | j
@@ -37,6 +39,9 @@ inline size_t vsub_flag_bytes(int W, uint64_t t) {
   return size_t((t + mb - 1) / mb) * (mb / 8);
 }
 hipError_t launch_vsub(int W, const VsubParams& p, hipStream_t st);
+// Bernoulli(p) random sets (counter-based hash of seed and position).
+hipError_t launch_random_sets(int W, uint64_t* out, uint64_t count, uint32_t n, float p, uint64_t seed,
+    hipStream_t st);
 
 // out[a*m + i] = image of parents[i] under letter a, from 8-state slice
 // tables tables[a][s][256][W] (the reference's PreprocessedTransition

@@ -44,6 +44,8 @@ hipError_t launch_first_full(const uint64_t* sets, uint64_t count, uint32_t n, u
 hipError_t launch_gather_rows(const uint64_t* rows, const uint32_t* idx, uint64_t count, uint32_t W, uint64_t* out,
     hipStream_t st);
 hipError_t launch_iota(uint32_t* p, uint64_t count, hipStream_t st);
+// y[i] = -|sets[i]| (the cardinality order as an ascending key).
+hipError_t launch_neg_popcount(const uint64_t* sets, uint64_t count, uint32_t W, float* y, hipStream_t st);
 // Stable ascending sort of float keys with u32 values (temp == NULL: size query).
 hipError_t sort_scores(const float* keys_in, float* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
     uint64_t count, void* temp, size_t* temp_bytes, hipStream_t st);

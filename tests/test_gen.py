@@ -35,3 +35,10 @@ def test_cerny_definition():
     d = gen.cerny(5)
     assert [d[2 * q] for q in range(5)] == [1, 2, 3, 4, 0]
     assert [d[2 * q + 1] for q in range(5)] == [1, 1, 2, 3, 4]
+
+
+def test_native_generator_matches_python():
+    import numpy as np
+    import synchrowords_amd as sw
+    for n, k, seed in [(48, 2, 0), (300, 2, 1), (160, 3, 2**63 + 5), (17, 4, 99)]:
+        assert np.array_equal(sw.random_dfa(n, k, seed), np.array(gen.random_dfa(n, k, seed), dtype=np.uint32))
