@@ -189,11 +189,16 @@ def main() -> int:
     ap.add_argument("--scorer", type=str, default=str(ROOT / "synchrowords_amd" / "models" / "scorer_n300_k2.npz"))
     ap.add_argument("--learned-width", type=int, default=256)
     ap.add_argument("--learned-seeds", type=int, default=20)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel over MIB MiB (hardware-counter calibration)")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--quick", action="store_true", help="smaller lists (2^16) for a fast check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-exact", action="store_true", help="skip the time-to-exact leg")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch shape)")
+    ap.add_argument("--shape", type=str, default="", metavar="T,G,E,M",
+                    help="use this launch shape (threads, target workgroups, early exit, mask bits) instead of tuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the search does not shard "
@@ -201,6 +206,20 @@ def main() -> int:
         return 2
 
     import synchrowords_amd as sw
+
+    if args.copy_calibration:
+        ctx = sw.Context(0)
+        nbytes = args.copy_calibration << 20
+        src = sw.DeviceArray(ctx, nbytes)
+        dst = sw.DeviceArray(ctx, nbytes)
+        import ctypes
+        for _ in range(max(1, args.steps)):
+            sw._check(sw.load().sw_dev_copy(ctx.handle, dst.ptr, src.ptr, nbytes))
+        print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": max(1, args.steps)}))
+        src.free()
+        dst.free()
+        ctx.close()
+        return 0
 
     if args.workload != "subset":
         ctx = sw.Context(0)
@@ -224,6 +243,19 @@ def main() -> int:
     ds = sw.DeviceArray.from_host(ctx, small)
     dl = sw.DeviceArray.from_host(ctx, large)
     ctx.sync()
+
+    tuned = None
+    if args.shape:
+        th, wg, ee, mb = (int(v) for v in args.shape.split(","))
+        ctx.set_subset_shape(n, count, count, th, wg, ee, mb)
+        tuned = {"threads": th, "target_wgs": wg, "early_exit": ee, "mask_bits": mb, "source": "--shape"}
+    elif not args.no_tune:
+        # The autotuner times every launch shape on seeded lists of the same
+        # densities (a quarter of the stream list, the full tile list) and
+        # keeps the fastest for this size class in the context.
+        t_tune = time.perf_counter()
+        tuned = ctx.autotune_subset(n, max(count // 4, 1), count, 0.08, 0.75, reps=1)
+        print(f"[bench] autotuned {tuned} in {time.perf_counter() - t_tune:.1f}s", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         cnt, anyf = ctx.subset_check_device(n, ds, count, dl, count, count=True)
@@ -262,7 +294,8 @@ def main() -> int:
         "data": "synthetic (seeded Bernoulli bitsets, planted subsets)",
         "config": {"workload": f"subset-check microbench: {count} x {count} sets, n={n} (W=4), p=0.08/0.75, "
                                f"1 in 1e5 planted",
-                   "pair_count": counts[0], "exists": bool(anyf), "device": info["name"], "cus": info["cus"]},
+                   "pair_count": counts[0], "exists": bool(anyf), "device": info["name"], "cus": info["cus"],
+                   "launch_shape": tuned or "default"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "note": "count of 2^40 pairs: integer VALU/LDS bound, not HBM bound; "

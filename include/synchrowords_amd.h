@@ -60,6 +60,9 @@ int sw_dev_free(sw_ctx* ctx, void* p);
 int sw_memcpy_h2d(sw_ctx* ctx, void* dst, const void* src, size_t bytes);
 int sw_memcpy_d2h(sw_ctx* ctx, void* dst, const void* src, size_t bytes);
 int sw_sync(sw_ctx* ctx);
+/* Device-to-device copy by a plain streaming kernel (bytes % 16 == 0): the
+ * known-size kernel that calibrates hardware traffic counters. */
+int sw_dev_copy(sw_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* HIP events on the context's stream, for timing work launched by later
  * calls: op 0 records the start event; op 1 records the stop event, waits for
  * it and writes the elapsed milliseconds to *ms. */
@@ -176,7 +179,7 @@ typedef struct sw_batch_config {
   uint32_t grid;       /* workgroups of the persistent launch (0: 4 per CU) */
   uint32_t cap_rows;   /* sets per side a workgroup may hold (0: 65536) */
   uint32_t cap_cand;   /* candidates of one step (0: 98304) */
-  uint32_t pad;
+  uint32_t overflow_workers; /* host threads finishing overflowed instances (0: 8) */
   uint64_t max_pairs;  /* largest goal check done inside a workgroup (0: 2^24) */
 } sw_batch_config;
 
@@ -200,14 +203,22 @@ int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint32
     uint32_t* word_lens, uint32_t* words, uint32_t word_cap, sw_batch_stats* stats);
 
 /* --------------------------------------------------------------- autotuner
- * Time every launch shape of the subset kernel (workgroup size, workgroups
- * aimed at, early exit) on seeded random lists of the given sizes and
+ * Time every launch shape of the subset kernel (tile mask width 64/32,
+ * workgroup size, workgroups aimed at, early exit; *early_exit returns
+ * early_exit | mask_bits << 8) on seeded random lists of the given sizes and
  * densities on the device; the fastest is kept in the context and used by
  * later calls of the same size class (exact search, batch, the seams).
  * No reference counterpart (the reference's launch shapes are fixed,
  * cuda/subsets_implicit_trie_kernel.cu:8,196). */
 int sw_autotune_subset(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_count, float p_stream, float p_tile,
     uint32_t reps, uint32_t* threads, uint32_t* target_wgs, uint32_t* early_exit, float* ms);
+
+/* Record a launch shape (as sw_autotune_subset returns it; early_exit packs
+ * early_exit | mask_bits << 8) for the size class of (n, s_count, t_count),
+ * so a shape tuned once can be reused without timing again.  No reference
+ * counterpart. */
+int sw_set_subset_shape(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_count, uint32_t threads,
+    uint32_t target_wgs, uint32_t early_exit);
 
 /* -------------------------------------------------------- learned upper bound
  * A BF16 MLP (MFMA, FP32 accumulation) that scores beam candidates by their

@@ -60,8 +60,8 @@ class _Result(C.Structure):
 
 
 class _BatchConfig(C.Structure):
-    _fields_ = [("grid", C.c_uint32), ("cap_rows", C.c_uint32), ("cap_cand", C.c_uint32), ("pad", C.c_uint32),
-                ("max_pairs", C.c_uint64)]
+    _fields_ = [("grid", C.c_uint32), ("cap_rows", C.c_uint32), ("cap_cand", C.c_uint32),
+                ("overflow_workers", C.c_uint32), ("max_pairs", C.c_uint64)]
 
 
 class _BatchStats(C.Structure):
@@ -88,6 +88,7 @@ _SIGS = {
     "sw_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "sw_sync": (C.c_int, [_P]),
     "sw_stream_timer": (C.c_int, [_P, C.c_int, C.POINTER(C.c_float)]),
+    "sw_dev_copy": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "sw_transition_apply": (C.c_int, [_P, C.c_uint32, _P, _P, _P, C.c_uint64, C.c_int]),
     "sw_expand": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_int, _P, C.c_uint64, _P, C.c_int]),
     "sw_leaf_create": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
@@ -117,6 +118,7 @@ _SIGS = {
                                  _U64P]),
     "sw_autotune_subset": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint64, C.c_float, C.c_float, C.c_uint32, _U32P,
                                      _U32P, _U32P, C.POINTER(C.c_float)]),
+    "sw_set_subset_shape": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
 }
 
 
@@ -253,9 +255,9 @@ class Leaf:
         return ret.astype(bool)
 
     def close(self) -> None:
-        if self.handle:
+        if self.handle and self.ctx.handle:
             load().sw_leaf_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -277,9 +279,11 @@ class Mlp:
         self.in_dim, self.hidden, self.nparams = d.value, hh.value, npar.value
 
     def close(self):
-        if self.handle:
+        # A network must go before its context; once the context is closed
+        # its device memory went with it.
+        if self.handle and self.ctx.handle:
             load().sw_mlp_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -431,7 +435,14 @@ class Context:
         th, wg, ee, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_float()
         _check(load().sw_autotune_subset(self.handle, n, s_count, t_count, p_stream, p_tile, reps, C.byref(th),
                                          C.byref(wg), C.byref(ee), C.byref(ms)))
-        return {"threads": th.value, "target_wgs": wg.value, "early_exit": ee.value, "ms": ms.value}
+        return {"threads": th.value, "target_wgs": wg.value, "early_exit": ee.value & 255,
+                "mask_bits": ee.value >> 8, "ms": ms.value}
+
+    def set_subset_shape(self, n: int, s_count: int, t_count: int, threads: int, target_wgs: int,
+                         early_exit: int, mask_bits: int):
+        """Record a launch shape (as autotune_subset reports it) for this size class."""
+        _check(load().sw_set_subset_shape(self.handle, n, s_count, t_count, threads, target_wgs,
+                                          (early_exit & 1) | (mask_bits << 8)))
 
     def count_bits(self, n: int, sets) -> np.ndarray:
         s = _sets(sets, n)
@@ -460,7 +471,7 @@ class Context:
         return ExactResult(bool(r.found), int(r.length), w, bool(r.bound_word), stats)
 
     def batch(self, instances, grid: int = 0, cap_rows: int = 0, cap_cand: int = 0, max_pairs: int = 0,
-              word_cap: int = 512):
+              word_cap: int = 512, overflow_workers: int = 0):
         """Exact lengths of many automata in one persistent launch.
 
         instances: list of (n, k, delta).  Returns (results, stats): results[i]
@@ -482,7 +493,7 @@ class Context:
         lengths = np.zeros(cnt, dtype=np.uint64)
         wl = np.zeros(cnt, dtype=np.uint32)
         words_buf = np.zeros(cnt * word_cap, dtype=np.uint32)
-        cfg = _BatchConfig(grid, cap_rows, cap_cand, 0, max_pairs)
+        cfg = _BatchConfig(grid, cap_rows, cap_cand, overflow_workers, max_pairs)
         st = _BatchStats()
         _check(load().sw_batch_solve(self.handle, cnt, _ptr(ns), _ptr(ks), _ptr(offs), _ptr(deltas), C.byref(cfg),
                                      _ptr(nsync), _ptr(lengths), _ptr(wl), _ptr(words_buf), word_cap, C.byref(st)))

@@ -51,6 +51,7 @@ void apply_tuning(sw_ctx* ctx, int W, VsubParams& p) {
   p.threads = it->second.threads;
   p.target_wgs = it->second.target_wgs;
   p.early_exit = int(it->second.early_exit);
+  p.mask_bits = int(it->second.mask_bits);
 }
 
 }  // namespace sw
@@ -77,12 +78,15 @@ int sw_autotune_subset(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_cou
     hipEvent_t e0, e1;
     SW_CHECK_HIP(hipEventCreate(&e0));
     SW_CHECK_HIP(hipEventCreate(&e1));
-    sw_ctx::Tune best{1024, 1024, 1, 1e30f};
+    sw_ctx::Tune best{1024, 1024, 1, 64, 1e30f};
     const uint32_t th[] = {256, 512, 1024};
     const uint32_t wg[] = {256, 512, 1024, 2048, 4096};
+    const uint32_t mbs[] = {64, 32};
+    for (uint32_t mb : mbs)
     for (uint32_t threads_i : th)
       for (uint32_t wgs : wg)
         for (uint32_t ee = 0; ee < 2; ++ee) {
+          if (mb == 64 && W > 4) continue;
           VsubParams p;
           p.stream = S.as<uint64_t>();
           p.s_count = s_count;
@@ -94,6 +98,7 @@ int sw_autotune_subset(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_cou
           p.threads = threads_i;
           p.target_wgs = wgs;
           p.early_exit = int(ee);
+          p.mask_bits = int(mb);
           SW_CHECK_HIP(launch_vsub(W, p, st));  // warm
           SW_CHECK_HIP(hipEventRecord(e0, st));
           for (uint32_t r = 0; r < (reps ? reps : 3); ++r) SW_CHECK_HIP(launch_vsub(W, p, st));
@@ -102,23 +107,36 @@ int sw_autotune_subset(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_cou
           float t = 0;
           SW_CHECK_HIP(hipEventElapsedTime(&t, e0, e1));
           t /= float(reps ? reps : 3);
-          if (t < best.ms) best = {threads_i, wgs, ee, t};
+          if (t < best.ms) best = {threads_i, wgs, ee, mb, t};
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    ctx->tune[(uint32_t(W) << 16) | ([&] {
-      uint32_t k = 0;
-      for (uint64_t c = s_count; c > 1; c >>= 1) ++k;
-      return k;
-    }() << 8) | [&] {
-      uint32_t k = 0;
-      for (uint64_t c = t_count; c > 1; c >>= 1) ++k;
-      return k;
-    }()] = best;
+    ctx->tune[key(W, s_count, t_count)] = best;
     if (threads) *threads = best.threads;
     if (target_wgs) *target_wgs = best.target_wgs;
-    if (early_exit) *early_exit = best.early_exit;
+    if (early_exit) *early_exit = best.early_exit | (best.mask_bits << 8);
     if (ms) *ms = best.ms;
+    return SW_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  }
+}
+
+// Record a launch shape for a size class by hand (one an earlier autotune
+// reported), so a later process can reuse it without timing again.
+int sw_set_subset_shape(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_count, uint32_t threads,
+    uint32_t target_wgs, uint32_t early_exit) {
+  using namespace sw;
+  try {
+    SW_REQUIRE(ctx && s_count && t_count, "NULL argument or empty lists");
+    SW_REQUIRE(n >= 1 && n <= 512, "n in [1, 512]");
+    SW_REQUIRE(threads == 256 || threads == 512 || threads == 1024, "threads must be 256, 512 or 1024");
+    SW_REQUIRE(target_wgs >= 1, "target_wgs must be positive");
+    const int W = int((n + 63) / 64);
+    const uint32_t mb = early_exit >> 8;
+    SW_REQUIRE(mb == 32 || (mb == 64 && W <= 4), "mask bits must be 32, or 64 for n <= 256");
+    ctx->tune[key(W, s_count, t_count)] = sw_ctx::Tune{threads, target_wgs, early_exit & 1, mb, 0.f};
     return SW_OK;
   } catch (const Error& e) {
     set_error(e.msg);

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cstring>
 #include <numeric>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -223,21 +224,58 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
         done[i] = 1;
       }
     }
-    for (uint32_t i = 0; i < count; ++i) {
-      if (done[i] || prep[i].ns || ns[i] == 1) continue;
-      const uint32_t* d = deltas + delta_offsets[i];
-      sw_exact_config ec{};
-      ec.dfs = 1;
-      const uint64_t ub = uint64_t(ns[i] - 1) * (ns[i] - 1) + 1;
-      auto r = exact_search(ctx, ns[i], ks[i], d, 0, ub, ec);
-      st.overflowed++;
-      SW_REQUIRE(r.r.found, "exact search ended without a length (memory budget)");
-      lengths[i] = r.r.length;
-      if (words && word_lens && r.word.size() <= word_cap) {
-        word_lens[i] = uint32_t(r.word.size());
-        std::copy(r.word.begin(), r.word.end(), words + size_t(i) * word_cap);
-      }
-      st.found++;
+    // Overflowed instances: several host threads, each with its own context
+    // (own stream, own reusable search buffers), so their many small
+    // launches overlap on the device.
+    std::vector<uint32_t> rest;
+    for (uint32_t i = 0; i < count; ++i)
+      if (!done[i] && !prep[i].ns && ns[i] != 1) rest.push_back(i);
+    std::sort(rest.begin(), rest.end(), [&](uint32_t a, uint32_t b) {
+      if (ks[a] != ks[b]) return ks[a] > ks[b];
+      return ns[a] > ns[b];
+    });
+    st.overflowed = uint32_t(rest.size());
+    if (!rest.empty()) {
+      const uint32_t workers = std::min<uint32_t>(cfg.overflow_workers ? cfg.overflow_workers : 8,
+          uint32_t(rest.size()));
+      std::atomic<uint32_t> next{0};
+      std::vector<std::string> errors(workers);
+      std::vector<std::thread> pool;
+      for (uint32_t t = 0; t < workers; ++t)
+        pool.emplace_back([&, t] {
+          sw_ctx* wctx = nullptr;
+          try {
+            if (t == 0) {
+              wctx = ctx;
+            } else {
+              if (sw_ctx_create(ctx->device, &wctx) != SW_OK) throw Error{SW_ERR_HIP, sw_last_error()};
+              wctx->tune = ctx->tune;
+            }
+            for (uint32_t j; (j = next.fetch_add(1)) < rest.size();) {
+              const uint32_t i = rest[j];
+              const uint32_t* d = deltas + delta_offsets[i];
+              sw_exact_config ec{};
+              ec.dfs = 1;
+              const uint64_t ub = uint64_t(ns[i] - 1) * (ns[i] - 1) + 1;
+              auto r = exact_search(wctx, ns[i], ks[i], d, 0, ub, ec);
+              if (!r.r.found) throw Error{SW_ERR_NOMEM, "exact search ended without a length (memory budget)"};
+              lengths[i] = r.r.length;
+              if (words && word_lens && r.word.size() <= word_cap) {
+                word_lens[i] = uint32_t(r.word.size());
+                std::copy(r.word.begin(), r.word.end(), words + size_t(i) * word_cap);
+              }
+            }
+          } catch (const Error& e) {
+            errors[t] = e.msg;
+          } catch (const std::exception& e) {
+            errors[t] = e.what();
+          }
+          if (wctx && wctx != ctx) sw_ctx_destroy(wctx);
+        });
+      for (auto& th : pool) th.join();
+      for (auto& e : errors)
+        if (!e.empty()) throw Error{SW_ERR_HIP, "batch overflow path: " + e};
+      st.found += uint32_t(rest.size());
     }
     st.ms_overflow = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t2).count();
     st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -100,6 +100,8 @@ void sw_ctx_destroy(sw_ctx* ctx) {
   ctx->c.release();
   ctx->d.release();
   ctx->e.release();
+  if (ctx->exact_ws && ctx->exact_ws_free) ctx->exact_ws_free(ctx->exact_ws);
+  ctx->exact_ws = nullptr;
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
   (void)hipStreamDestroy(ctx->stream);
@@ -155,6 +157,15 @@ int sw_sync(sw_ctx* ctx) {
   SW_API_BEGIN
   SW_REQUIRE(ctx, "ctx is NULL");
   sw::use(ctx);
+  SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  SW_API_END
+}
+
+int sw_dev_copy(sw_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx && dst && src && bytes % 16 == 0, "NULL argument or size not a multiple of 16");
+  sw::use(ctx);
+  SW_CHECK_HIP(sw::launch_copy(src, dst, bytes, ctx->stream));
   SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   SW_API_END
 }
@@ -331,7 +342,7 @@ int sw_subset_check(sw_ctx* ctx, uint32_t n, const uint64_t* small, uint64_t sma
     p.mode = sw::VSUB_SCOUNT;
     p.scount = ctx->d.as<uint32_t>();
     sw::apply_tuning(ctx, int(W), p);
-  SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
+    SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
     std::unique_ptr<uint32_t[]> c(new uint32_t[small_cnt + 1]);
     if (small_cnt)
       SW_CHECK_HIP(hipMemcpyAsync(c.get(), ctx->d.p, small_cnt * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -345,7 +356,7 @@ int sw_subset_check(sw_ctx* ctx, uint32_t n, const uint64_t* small, uint64_t sma
     p.mode = sw::VSUB_COUNT;
     p.count_out = dcount;
     sw::apply_tuning(ctx, int(W), p);
-  SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
+    SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
     unsigned long long c = 0;
     SW_CHECK_HIP(hipMemcpyAsync(&c, dcount, 8, hipMemcpyDeviceToHost, ctx->stream));
     SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
@@ -356,7 +367,7 @@ int sw_subset_check(sw_ctx* ctx, uint32_t n, const uint64_t* small, uint64_t sma
     p.found = dfound;
     p.witness = dwit;
     sw::apply_tuning(ctx, int(W), p);
-  SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
+    SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
     SW_CHECK_HIP(hipMemcpyAsync(&found, dfound, 4, hipMemcpyDeviceToHost, ctx->stream));
     SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   }

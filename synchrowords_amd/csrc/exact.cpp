@@ -55,13 +55,40 @@ struct Side {
   uint64_t fcount = 0, prev_fcount = 0;
   uint64_t steps = 0;
   bool frozen = false;
+  void reset() {
+    rows = tcap = fcount = prev_fcount = steps = 0;
+    level_end.clear();
+    frozen = false;
+  }
 };
+
+// Device buffers of the search, kept by the context between searches so
+// that repeated searches (the batch's overflow path) allocate nothing and
+// never call hipFree, which would synchronise the whole device.
+struct Workspace {
+  DevBuf tables[2];
+  Side F, B;
+  DevBuf cand, valid, slot, keep, pos, small, scan_tmp, flags, probes;
+  size_t scan_bytes = 0;
+};
+
+Workspace& workspace(sw_ctx* ctx) {
+  if (!ctx->exact_ws) {
+    ctx->exact_ws = new Workspace();
+    ctx->exact_ws_free = [](void* p) { delete static_cast<Workspace*>(p); };
+  }
+  return *static_cast<Workspace*>(ctx->exact_ws);
+}
 
 class Search {
  public:
   Search(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, const sw_exact_config& cfg)
       : ctx_(ctx), st_(ctx->stream), n_(n), k_(k), W_((n + 63) / 64), slices_((n + 7) / 8), delta_(delta),
-        cfg_(cfg) {
+        cfg_(cfg), ws_(workspace(ctx)), tables_(ws_.tables), F_(ws_.F), B_(ws_.B), cand_(ws_.cand),
+        valid_(ws_.valid), slot_(ws_.slot), keep_(ws_.keep), pos_(ws_.pos), small_(ws_.small),
+        scan_tmp_(ws_.scan_tmp), flags_(ws_.flags), probes_(ws_.probes), scan_bytes_(ws_.scan_bytes) {
+    F_.reset();
+    B_.reset();
     budget_pairs_ = cfg.dominance_pairs ? cfg.dominance_pairs : 100000000000ull;
     forward_cap_ = cfg.forward_cap ? cfg.forward_cap : (1ull << 22);
     mem_cap_ = (cfg.max_memory_mb ? cfg.max_memory_mb : 65536ull) << 20;
@@ -150,10 +177,11 @@ class Search {
   sw_exact_config cfg_;
   uint64_t budget_pairs_, forward_cap_, mem_cap_;
   uint64_t pair_tests_ = 0;
-  DevBuf tables_[2];
-  Side F_, B_;
-  DevBuf cand_, valid_, slot_, keep_, pos_, small_, scan_tmp_, flags_, probes_;
-  size_t scan_bytes_ = 0;
+  Workspace& ws_;
+  DevBuf (&tables_)[2];
+  Side &F_, &B_;
+  DevBuf &cand_, &valid_, &slot_, &keep_, &pos_, &small_, &scan_tmp_, &flags_, &probes_;
+  size_t& scan_bytes_;
 
   uint64_t row_bytes() const { return uint64_t(W_) * 8; }
 
@@ -189,7 +217,6 @@ class Search {
     uint64_t cap = 1024;
     while (cap < need) cap <<= 1;
     SW_REQUIRE(cap <= (1ull << 32), "exact search: hash table over 2^32 slots");
-    S.table.release();
     S.table.ensure(cap * 4);
     S.tcap = cap;
     SW_CHECK_HIP(launch_fill_u32(S.table.as<uint32_t>(), kEmpty, cap, st_));

@@ -195,6 +195,13 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t count) {
     p[i] = v;
 }
 
+// Plain streaming copy (16-byte vectors): the calibration kernel whose
+// counter readings scale the HBM traffic of the other kernels.
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += uint64_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void k_scan_total(const uint32_t* keep, const uint32_t* pos, uint64_t count, uint32_t* total) {
   *total = count ? pos[count - 1] + keep[count - 1] : 0u;
 }
@@ -328,6 +335,16 @@ static hipError_t count_bits_w(const uint64_t* sets, uint64_t count, uint32_t n,
 hipError_t launch_count_bits(int W, const uint64_t* sets, uint64_t count, uint32_t n, unsigned long long* acc,
     hipStream_t st) {
   SW_DISPATCH_W(W, count_bits_w<w_>(sets, count, n, acc, st))
+}
+
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t st) {
+  const uint64_t n16 = bytes / 16;
+  if (!n16) return hipSuccess;
+  unsigned blocks = unsigned((n16 + 255) / 256);
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, static_cast<const uint4*>(src), static_cast<uint4*>(dst),
+      n16);
+  return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t count, hipStream_t st) {

@@ -18,42 +18,25 @@
 #include "sw_device.hpp"
 #include "sw_kernels.hpp"
 
+#include <atomic>
 #include <climits>
 
 namespace sw {
 
-template <int W>
-struct VsubMask {
-  using type = uint64_t;
-};
-template <>
-struct VsubMask<5> {
-  using type = uint32_t;
-};
-template <>
-struct VsubMask<6> {
-  using type = uint32_t;
-};
-template <>
-struct VsubMask<7> {
-  using type = uint32_t;
-};
-template <>
-struct VsubMask<8> {
-  using type = uint32_t;
-};
-
-template <int W>
-constexpr int vsub_mask_bits() {
-  return sizeof(typename VsubMask<W>::type) * 8;
+// Mask width of the tile: 64 tile sets per lane (N <= 256, 1 workgroup per
+// CU at N=256) or 32 (any N <= 512, half the LDS, 2 workgroups per CU).
+int vsub_mask_bits(int W, int requested) {
+  if (requested == 32) return 32;
+  if (requested == 64) return W <= 4 ? 64 : 32;
+  return W <= 4 ? 64 : 32;
 }
 
-size_t vsub_tile_sets(int W) { return W <= 4 ? 64 * 64 : 64 * 32; }
+size_t vsub_tile_sets(int W, int mask_bits) { return size_t(64) * vsub_mask_bits(W, mask_bits); }
 
-size_t vsub_lds_bytes(uint32_t n, int W, bool proper_by_size) {
-  const size_t mb = W <= 4 ? 8 : 4;
-  size_t b = size_t(n + 1) * 64 * mb;
-  if (proper_by_size) b += vsub_tile_sets(W) * sizeof(uint16_t);
+size_t vsub_lds_bytes(uint32_t n, int W, int mask_bits, bool proper_by_size) {
+  const int mb = vsub_mask_bits(W, mask_bits);
+  size_t b = size_t(n + 1) * 64 * (mb / 8);
+  if (proper_by_size) b += vsub_tile_sets(W, mask_bits) * sizeof(uint16_t);
   return b;
 }
 
@@ -62,9 +45,8 @@ __device__ inline int mpopc(M x) {
   return sizeof(M) == 8 ? __popcll(uint64_t(x)) : __popc(uint32_t(x));
 }
 
-template <int W>
+template <int W, typename M>
 __global__ __launch_bounds__(1024) void k_vsub(VsubParams p) {
-  using M = typename VsubMask<W>::type;
   constexpr int MB = sizeof(M) * 8;
   constexpr int TILE = 64 * MB;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -124,14 +106,25 @@ __global__ __launch_bounds__(1024) void k_vsub(VsubParams p) {
   uint32_t cnt = 0;
   M hits = 0;
   uint32_t iter = 0;
+  // The next stream set is loaded while the current one is walked.
+  uint64_t nx[W];
+  {
+    const uint64_t s0 = s_begin + wave;
+#pragma unroll
+    for (int w = 0; w < W; ++w) nx[w] = s0 < s_end ? p.stream[s0 * W + w] : 0ull;
+  }
   for (uint64_t s = s_begin + wave; s < s_end; s += nwaves, ++iter) {
+    uint64_t aw[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) aw[w] = rfl64(nx[w]) & word_mask(n, w);  // states >= n have no row
+    {
+      const uint64_t sn = s + nwaves;
+#pragma unroll
+      for (int w = 0; w < W; ++w) nx[w] = sn < s_end ? p.stream[sn * W + w] : 0ull;
+    }
     if (p.mode == VSUB_ANY && (iter & 15) == 0) {
       if (rfl(__hip_atomic_load(p.found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
     }
-    const uint64_t* a = p.stream + s * W;
-    uint64_t aw[W];
-#pragma unroll
-    for (int w = 0; w < W; ++w) aw[w] = rfl64(a[w]) & word_mask(n, w);  // states >= n have no row
     M r = valid;
     bool go = true;
 #pragma unroll
@@ -209,11 +202,12 @@ __global__ __launch_bounds__(1024) void k_vsub(VsubParams p) {
   }
 }
 
-template <int W>
-static hipError_t launch_vsub_w(const VsubParams& p0, hipStream_t st) {
+template <int W, typename M>
+static hipError_t launch_vsub_wm(const VsubParams& p0, hipStream_t st) {
   VsubParams p = p0;
   if (p.t_count == 0 || p.s_count == 0) return hipSuccess;
-  const size_t tile = vsub_tile_sets(W);
+  constexpr int MB = sizeof(M) * 8;
+  const size_t tile = size_t(64) * MB;
   const uint64_t gx = (p.t_count + tile - 1) / tile;
   if (p.threads != 256 && p.threads != 512 && p.threads != 1024) return hipErrorInvalidValue;
   if (p.s_chunk == 0) {
@@ -226,16 +220,24 @@ static hipError_t launch_vsub_w(const VsubParams& p0, hipStream_t st) {
   }
   const uint64_t gy = (p.s_count + p.s_chunk - 1) / p.s_chunk;
   if (gx > 0x7fffffffull || gy > 65535ull) return hipErrorInvalidValue;
-  const size_t lds = vsub_lds_bytes(p.n, W, p.proper_by_size != 0);
-  static bool attr_set[kMaxW + 1] = {};
-  if (!attr_set[W]) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vsub<W>),
+  const size_t lds = vsub_lds_bytes(p.n, W, MB, p.proper_by_size != 0);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  // Idempotent, so concurrent first calls from several host threads are harmless.
+  static std::atomic<bool> attr_set{false};
+  if (!attr_set.load(std::memory_order_acquire)) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vsub<W, M>),
         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
-    attr_set[W] = true;
+    attr_set.store(true, std::memory_order_release);
   }
-  hipLaunchKernelGGL(k_vsub<W>, dim3(unsigned(gx), unsigned(gy)), dim3(p.threads), lds, st, p);
+  hipLaunchKernelGGL((k_vsub<W, M>), dim3(unsigned(gx), unsigned(gy)), dim3(p.threads), lds, st, p);
   return hipGetLastError();
+}
+
+template <int W>
+static hipError_t launch_vsub_w(const VsubParams& p, hipStream_t st) {
+  if (W <= 4 && vsub_mask_bits(W, p.mask_bits) == 64) return launch_vsub_wm<W, uint64_t>(p, st);
+  return launch_vsub_wm<W, uint32_t>(p, st);
 }
 
 template <int W>

@@ -82,10 +82,13 @@ struct sw_ctx {
   // Subset-kernel launch shapes chosen by the autotuner, keyed by
   // (W, floor(log2 stream sets), floor(log2 tile sets)).
   struct Tune {
-    uint32_t threads, target_wgs, early_exit;
+    uint32_t threads, target_wgs, early_exit, mask_bits;
     float ms;
   };
   std::map<uint32_t, Tune> tune;
+  // Buffers of the device exact search, reused between searches.
+  void* exact_ws = nullptr;
+  void (*exact_ws_free)(void*) = nullptr;
 };
 
 namespace sw {

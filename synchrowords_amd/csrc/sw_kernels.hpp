@@ -22,6 +22,7 @@ struct VsubParams {
   int64_t diag = INT64_MIN;          // skip pairs with i - j == diag
   uint64_t s_chunk = 0;              // stream sets per workgroup row (0: auto)
   uint32_t threads = 1024;           // workgroup size: 256, 512 or 1024
+  int mask_bits = 0;                 // tile sets per lane: 64 (W <= 4), 32, or 0 for the default
   uint32_t target_wgs = 1024;        // workgroups aimed at when s_chunk is auto
   unsigned long long* count_out = nullptr;  // COUNT: += number of pairs
   int* found = nullptr;                     // ANY: set to 1 on a hit
@@ -30,13 +31,15 @@ struct VsubParams {
   uint32_t* scount = nullptr;  // SCOUNT: += pairs per stream set
 };
 
-// Tile sets per workgroup and LDS bytes of the subset kernel for W words.
-size_t vsub_tile_sets(int W);
-size_t vsub_lds_bytes(uint32_t n, int W, bool proper_by_size);
-// Bytes of the FLAGS bitmap for t tile sets: whole mask words.
+// Mask width, tile sets per workgroup and LDS bytes of the subset kernel.
+int vsub_mask_bits(int W, int requested);
+size_t vsub_tile_sets(int W, int mask_bits);
+size_t vsub_lds_bytes(uint32_t n, int W, int mask_bits, bool proper_by_size);
+// Bytes of the FLAGS bitmap for t tile sets (whole 64-bit words; bit j of
+// the bitmap, read as bytes, is tile set j for either mask width).
 inline size_t vsub_flag_bytes(int W, uint64_t t) {
-  const uint64_t mb = W <= 4 ? 64 : 32;
-  return size_t((t + mb - 1) / mb) * (mb / 8);
+  (void)W;
+  return size_t((t + 63) / 64) * 8;
 }
 hipError_t launch_vsub(int W, const VsubParams& p, hipStream_t st);
 // Bernoulli(p) random sets (counter-based hash of seed and position).
@@ -84,6 +87,8 @@ hipError_t launch_select(int W, const uint64_t* rows, uint64_t lo, const uint32_
 // Per-state frequency over a list (acc must be zeroed): acc[q] += #sets with q.
 hipError_t launch_count_bits(int W, const uint64_t* sets, uint64_t count, uint32_t n, unsigned long long* acc,
     hipStream_t st);
+// Streaming copy of bytes/16 16-byte vectors (traffic calibration).
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t st);
 // Fill 32-bit words.
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t count, hipStream_t st);
 

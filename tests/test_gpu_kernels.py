@@ -99,3 +99,46 @@ def test_count_bits(ctx, n):
     rng = np.random.default_rng(n)
     sets = random_sets(rng, 20000, n, 0.4)
     np.testing.assert_array_equal(ctx.count_bits(n, sets), orc.count_bits(sets, n))
+
+
+@pytest.mark.parametrize("n", [100, 256, 300])
+def test_every_launch_shape_gives_the_same_result(n):
+    """The autotuner may pick any launch shape; each one must count the same pairs."""
+    import synchrowords_amd as sw
+    rng = np.random.default_rng(1000 + n)
+    W = words(n)
+    small = random_sets(rng, 1500, n, 0.07)
+    large = random_sets(rng, 5000, n, 0.75)
+    for i in range(0, 1500, 40):
+        small[i] = small[i] & large[int(rng.integers(0, 5000))]
+    total, cnt, _ = orc.subset_count(small, large, W)
+    c = sw.Context(0)
+    try:
+        for mb in ([32, 64] if n <= 256 else [32]):
+            for threads in (256, 512, 1024):
+                for wgs in (1, 7, 256, 4096):
+                    for ee in (0, 1):
+                        c.set_subset_shape(n, 1500, 5000, threads, wgs, ee, mb)
+                        res, got, anyf = c.subset_check(n, small, large, per_row=True)
+                        assert got == total, (mb, threads, wgs, ee)
+                        np.testing.assert_array_equal(res, cnt >= 1)
+    finally:
+        c.close()
+
+
+def test_autotuned_shape_keeps_results():
+    import synchrowords_amd as sw
+    n = 200
+    rng = np.random.default_rng(5)
+    small = random_sets(rng, 4096, n, 0.08)
+    large = random_sets(rng, 8192, n, 0.75)
+    small[3] = large[17]
+    total, _, _ = orc.subset_count(small, large, words(n))
+    c = sw.Context(0)
+    try:
+        shape = c.autotune_subset(n, 4096, 8192, 0.08, 0.75, reps=1)
+        assert shape["threads"] in (256, 512, 1024) and shape["mask_bits"] in (32, 64) and shape["ms"] > 0
+        _, got, anyf = c.subset_check(n, small, large, count=True)
+        assert got == total and anyf
+    finally:
+        c.close()

@@ -161,10 +161,20 @@ def test_samples_are_labelled_with_forward_distances(ctx):
         assert int(dd) == _forward_distance(n, k, d, S, 6)
 
 
-def test_learned_beam_gives_a_valid_bound(ctx):
+def test_device_beams_give_valid_bounds(ctx):
+    """Cardinality-ordered and network-scored device beams: every bound they
+    report below the cap is the length of a real reset word, never below the
+    exact length."""
     cases = golden("c1_random48.json")[:10]
     m = sw.Mlp(ctx, 48, 2, hidden=64, seed=2)
     for c in cases:
-        b, w = m.beam(c["delta"], width=64, upper=48 * 48)
-        assert b >= c["upper"]
-        assert len(w) == b and replay(48, 2, c["delta"], w) == 1
+        cap = c["eppstein"] + 1
+        b, w = ctx.beam(48, 2, c["delta"], 64, cap)
+        assert c["upper"] <= b <= cap
+        if b < cap:
+            assert len(w) == b and replay(48, 2, c["delta"], w) == 1
+        b, w = m.beam(c["delta"], width=64, upper=cap)
+        assert c["upper"] <= b <= cap
+        if b < cap:
+            assert len(w) == b and replay(48, 2, c["delta"], w) == 1
+    m.close()
