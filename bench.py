@@ -84,7 +84,8 @@ def time_to_exact(ctx, n: int, seed: int) -> dict:
     gpu_s = time.perf_counter() - t0
     out = {"workload": f"random binary DFA n={n} seed={seed}, configs/exact.json plan",
            "length": r.upper, "gpu_s": round(gpu_s, 4),
-           "gpu_stage_ms": {k: round(v, 2) for k, v in r.times_ms}}
+           "gpu_stage_ms": {k: round(v, 2) for k, v in r.times_ms},
+           "exact_search": {k: (round(v, 2) if isinstance(v, float) else v) for k, v in r.exact.items()}}
     ref = ROOT / "oracle" / "_ref" / f"ref_{n}_2_1"
     if ref.exists():
         with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as f:

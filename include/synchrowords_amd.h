@@ -140,6 +140,9 @@ typedef struct sw_exact_result {
   uint64_t pair_tests;  /* subset pairs tested (goal checks and dominance) */
   uint64_t hash_probes; /* probes made by the dedup hash */
   double ms_total;      /* wall time of the search */
+  double ms_expand;     /* expansion, dedup and commit of new levels */
+  double ms_dominance;  /* dominance pruning of new levels */
+  double ms_goal;       /* meet checks between the frontiers */
 } sw_exact_result;
 
 /* Shortest reset word of a synchronizing automaton, given bounds

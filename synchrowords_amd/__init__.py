@@ -49,7 +49,8 @@ class _ExactResult(C.Structure):
     _fields_ = [("found", C.c_int32), ("bound_word", C.c_int32), ("length", C.c_uint64), ("word_len", C.c_uint64),
                 ("steps_forward", C.c_uint64), ("steps_backward", C.c_uint64), ("sets_forward", C.c_uint64),
                 ("sets_backward", C.c_uint64), ("pair_tests", C.c_uint64), ("hash_probes", C.c_uint64),
-                ("ms_total", C.c_double)]
+                ("ms_total", C.c_double), ("ms_expand", C.c_double), ("ms_dominance", C.c_double),
+                ("ms_goal", C.c_double)]
 
 
 class _Result(C.Structure):

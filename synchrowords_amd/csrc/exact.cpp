@@ -119,10 +119,14 @@ class Search {
       ++t;
       step(fwd ? F_ : B_, !fwd, t);
       if (cfg_.verbose)
-        std::fprintf(stderr, "[sw exact] len %llu %s |F| %llu |B| %llu rows F %llu B %llu\n",
+        std::fprintf(stderr, "[sw exact] len %llu %s |F| %llu |B| %llu rows F %llu B %llu ms %.2f %.2f %.2f\n",
             (unsigned long long)t, fwd ? "F" : "B", (unsigned long long)F_.fcount,
-            (unsigned long long)B_.fcount, (unsigned long long)F_.rows, (unsigned long long)B_.rows);
-      if (goal_check(&wit)) {
+            (unsigned long long)B_.fcount, (unsigned long long)F_.rows, (unsigned long long)B_.rows, ms_expand_,
+            ms_dom_, ms_goal_);
+      const auto g0 = std::chrono::steady_clock::now();
+      const bool met = goal_check(&wit);
+      ms_goal_ += ms_since(g0);
+      if (met) {
         found = true;
         break;
       }
@@ -134,6 +138,9 @@ class Search {
     r.sets_backward = B_.rows;
     r.pair_tests = pair_tests_;
     r.hash_probes = read_probes();
+    r.ms_expand = ms_expand_;
+    r.ms_dominance = ms_dom_;
+    r.ms_goal = ms_goal_;
     if (found) {
       r.found = 1;
       r.length = t;
@@ -177,6 +184,7 @@ class Search {
   sw_exact_config cfg_;
   uint64_t budget_pairs_, forward_cap_, mem_cap_;
   uint64_t pair_tests_ = 0;
+  double ms_expand_ = 0, ms_dom_ = 0, ms_goal_ = 0;  // host time between synchronizations
   Workspace& ws_;
   DevBuf (&tables_)[2];
   Side &F_, &B_;
@@ -184,6 +192,9 @@ class Search {
   size_t& scan_bytes_;
 
   uint64_t row_bytes() const { return uint64_t(W_) * 8; }
+  static double ms_since(std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+  }
 
   void upload_tables() {
     for (int dir = 0; dir < 2; ++dir) {
@@ -291,6 +302,7 @@ class Search {
   // One expansion step of side S (backward: complemented sets).
   void step(Side& S, bool backward, uint64_t t) {
     (void)t;
+    const auto t0 = std::chrono::steady_clock::now();
     const uint64_t m = S.fcount;
     const uint64_t cnt = m * k_;
     SW_REQUIRE(cnt < 0x7fffffffull, "exact search: step over 2^31 candidates");
@@ -324,6 +336,9 @@ class Search {
     S.front_idx.ensure(nnew * 4 + 4);
     const bool dominance = nnew > 0 && S.rows * nnew <= budget_pairs_;
     if (dominance) {
+      SW_CHECK_HIP(hipStreamSynchronize(st_));
+      const auto t1 = std::chrono::steady_clock::now();
+      ms_expand_ += std::chrono::duration<double, std::milli>(t1 - t0).count();
       // Tile: the new level; stream: every row of the side (the new level
       // included; a row is never its own witness).
       const size_t fb = vsub_flag_bytes(W, nnew);
@@ -347,10 +362,13 @@ class Search {
       SW_CHECK_HIP(launch_select(W, S.arena.as<uint64_t>(), lo, keep_.as<uint32_t>(), pos_.as<uint32_t>(), nnew,
           S.front.as<uint64_t>(), S.front_idx.as<uint32_t>(), st_));
       S.fcount = read_u32(total);
+      ms_dom_ += ms_since(t1);
     } else {
       SW_CHECK_HIP(launch_select(W, S.arena.as<uint64_t>(), lo, nullptr, nullptr, nnew, S.front.as<uint64_t>(),
           S.front_idx.as<uint32_t>(), st_));
       S.fcount = nnew;
+      SW_CHECK_HIP(hipStreamSynchronize(st_));
+      ms_expand_ += ms_since(t0);
     }
     if (!backward && S.fcount >= forward_cap_) S.frozen = true;
   }
