@@ -192,6 +192,10 @@ typedef struct sw_batch_stats {
   uint32_t grid;
   uint32_t pad;
   uint64_t scratch_bytes;
+  /* instances that left the workgroup, by reason: candidates of one step,
+   * rows of a side, word buffer, hash table, goal-check pairs */
+  uint32_t over_cand, over_rows, over_word, over_hash, over_pairs, pad2;
+  double ms_overflow_search_sum;  /* summed wall time of the overflowed searches */
 } sw_batch_stats;
 
 /* Shortest reset words of `count` automata in one persistent launch.

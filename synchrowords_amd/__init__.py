@@ -69,7 +69,9 @@ class _BatchStats(C.Structure):
     _fields_ = [("ms_host_prep", C.c_double), ("ms_kernel", C.c_double), ("ms_overflow", C.c_double),
                 ("ms_total", C.c_double), ("non_synchro", C.c_uint32), ("found", C.c_uint32),
                 ("bound", C.c_uint32), ("overflowed", C.c_uint32), ("grid", C.c_uint32), ("pad", C.c_uint32),
-                ("scratch_bytes", C.c_uint64)]
+                ("scratch_bytes", C.c_uint64), ("over_cand", C.c_uint32), ("over_rows", C.c_uint32),
+                ("over_word", C.c_uint32), ("over_hash", C.c_uint32), ("over_pairs", C.c_uint32),
+                ("pad2", C.c_uint32), ("ms_overflow_search_sum", C.c_double)]
 
 
 _P = C.c_void_p
@@ -502,7 +504,7 @@ class Context:
         for j in range(cnt):
             w = [int(x) for x in words_buf[j * word_cap: j * word_cap + wl[j]]] if wl[j] else None
             res.append({"non_synchro": bool(nsync[j]), "length": int(lengths[j]), "word": w})
-        stats = {f: getattr(st, f) for f, _ in _BatchStats._fields_ if f != "pad"}
+        stats = {f: getattr(st, f) for f, _ in _BatchStats._fields_ if not f.startswith("pad")}
         return res, stats
 
     def pipeline(self, n: int, k: int, delta, plan: dict | str | None = None) -> PipelineResult:

@@ -222,6 +222,15 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
           st.bound++;
         }
         done[i] = 1;
+      } else if (r.status == kBatchOverflow) {
+        switch (r.detail) {
+          case kOverflowCand: st.over_cand++; break;
+          case kOverflowRows: st.over_rows++; break;
+          case kOverflowWord: st.over_word++; break;
+          case kOverflowHash: st.over_hash++; break;
+          case kOverflowPairs: st.over_pairs++; break;
+          default: break;
+        }
       }
     }
     // Overflowed instances: several host threads, each with its own context
@@ -240,6 +249,7 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
           uint32_t(rest.size()));
       std::atomic<uint32_t> next{0};
       std::vector<std::string> errors(workers);
+      std::vector<double> busy_ms(workers, 0.0);
       std::vector<std::thread> pool;
       for (uint32_t t = 0; t < workers; ++t)
         pool.emplace_back([&, t] {
@@ -257,7 +267,9 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
               sw_exact_config ec{};
               ec.dfs = 1;
               const uint64_t ub = uint64_t(ns[i] - 1) * (ns[i] - 1) + 1;
+              const auto s0 = std::chrono::steady_clock::now();
               auto r = exact_search(wctx, ns[i], ks[i], d, 0, ub, ec);
+              busy_ms[t] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - s0).count();
               if (!r.r.found) throw Error{SW_ERR_NOMEM, "exact search ended without a length (memory budget)"};
               lengths[i] = r.r.length;
               if (words && word_lens && r.word.size() <= word_cap) {
@@ -273,6 +285,7 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
           if (wctx && wctx != ctx) sw_ctx_destroy(wctx);
         });
       for (auto& th : pool) th.join();
+      for (double b : busy_ms) st.ms_overflow_search_sum += b;
       for (auto& e : errors)
         if (!e.empty()) throw Error{SW_ERR_HIP, "batch overflow path: " + e};
       st.found += uint32_t(rest.size());

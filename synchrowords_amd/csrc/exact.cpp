@@ -70,6 +70,15 @@ struct Workspace {
   Side F, B;
   DevBuf cand, valid, slot, keep, pos, small, scan_tmp, flags, probes;
   size_t scan_bytes = 0;
+  // Pinned landing zone for the few scalars read back per step (a copy to
+  // pageable memory would go through a staging buffer).
+  uint64_t* host = nullptr;
+  Workspace() { SW_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), 64, hipHostMallocDefault)); }
+  ~Workspace() {
+    if (host) (void)hipHostFree(host);
+  }
+  Workspace(const Workspace&) = delete;
+  Workspace& operator=(const Workspace&) = delete;
 };
 
 Workspace& workspace(sw_ctx* ctx) {
@@ -209,17 +218,15 @@ class Search {
   }
 
   uint64_t read_probes() {
-    unsigned long long v = 0;
-    SW_CHECK_HIP(hipMemcpyAsync(&v, probes_.p, 8, hipMemcpyDeviceToHost, st_));
+    SW_CHECK_HIP(hipMemcpyAsync(ws_.host, probes_.p, 8, hipMemcpyDeviceToHost, st_));
     SW_CHECK_HIP(hipStreamSynchronize(st_));
-    return v;
+    return ws_.host[0];
   }
 
   uint32_t read_u32(const void* dev) {
-    uint32_t v = 0;
-    SW_CHECK_HIP(hipMemcpyAsync(&v, dev, 4, hipMemcpyDeviceToHost, st_));
+    SW_CHECK_HIP(hipMemcpyAsync(ws_.host, dev, 4, hipMemcpyDeviceToHost, st_));
     SW_CHECK_HIP(hipStreamSynchronize(st_));
-    return v;
+    return uint32_t(ws_.host[0]);
   }
 
   void ensure_table(Side& S, uint64_t incoming) {
@@ -393,12 +400,11 @@ class Search {
     apply_tuning(ctx_, int(W_), p);
     SW_CHECK_HIP(launch_vsub(int(W_), p, st_));
     pair_tests_ += F_.fcount * B_.fcount;
-    int f = 0;
-    unsigned long long w = 0;
-    SW_CHECK_HIP(hipMemcpyAsync(&f, found, 4, hipMemcpyDeviceToHost, st_));
-    SW_CHECK_HIP(hipMemcpyAsync(&w, wit, 8, hipMemcpyDeviceToHost, st_));
+    // found (4 bytes at +8) and witness (8 bytes at +16) in one copy
+    SW_CHECK_HIP(hipMemcpyAsync(ws_.host, small_.as<char>() + 8, 16, hipMemcpyDeviceToHost, st_));
     SW_CHECK_HIP(hipStreamSynchronize(st_));
-    *witness = w;
+    const int f = int(uint32_t(ws_.host[0]));
+    *witness = ws_.host[1];
     return f != 0;
   }
 

@@ -165,7 +165,7 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchInstance* __restr
     level0_end_b = rows[1];
     __syncthreads();
 
-    uint32_t status = kBatchBound, length = I.upper;
+    uint32_t status = kBatchBound, length = I.upper, detail = kOverflowNone;
     uint32_t t = 0;
     unsigned long long witness = 0;
     bool found = false;
@@ -177,6 +177,7 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchInstance* __restr
       const uint64_t ncand = uint64_t(m) * K;
       if (ncand > L.cap_cand || rows[side] + ncand > L.cap_rows || t + 1 >= word_cap) {
         status = kBatchOverflow;
+        detail = ncand > L.cap_cand ? kOverflowCand : (rows[side] + ncand > L.cap_rows ? kOverflowRows : kOverflowWord);
         break;
       }
       ++t;
@@ -187,6 +188,7 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchInstance* __restr
         while (nc < need) nc <<= 1;
         if (nc > L.hash_slots) {
           status = kBatchOverflow;
+          detail = kOverflowHash;
           break;
         }
         tcap[side] = nc;
@@ -324,6 +326,7 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchInstance* __restr
       const uint32_t nf = hi[0] - lo[0], nbk = hi[1] - lo[1];
       if (uint64_t(nf) * nbk > L.max_pairs) {
         status = kBatchOverflow;
+        detail = kOverflowPairs;
         break;
       }
       if (tid == 0) {
@@ -380,6 +383,7 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchInstance* __restr
       R.length = length;
       R.word_len = 0;
       R.steps = t;
+      R.detail = detail;
       if (found) {
         R.status = kBatchFound;
         R.length = t;

@@ -45,6 +45,23 @@ __device__ inline int mpopc(M x) {
   return sizeof(M) == 8 ? __popcll(uint64_t(x)) : __popc(uint32_t(x));
 }
 
+// a & b & c in one gfx950 three-input bit operation per 32 bits
+// (truth table 0x80: only the input 111 maps to 1).
+__device__ inline uint32_t and3(uint32_t a, uint32_t b, uint32_t c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0x80); }
+__device__ inline uint64_t and3(uint64_t a, uint64_t b, uint64_t c) {
+  const uint32_t lo = and3(uint32_t(a), uint32_t(b), uint32_t(c));
+  const uint32_t hi = and3(uint32_t(a >> 32), uint32_t(b >> 32), uint32_t(c >> 32));
+  return (uint64_t(hi) << 32) | lo;
+}
+
+// Index of the lowest set bit of a wave-uniform word, which is then cleared:
+// s_ff1 and s_bitset0 on the scalar unit (two instructions per bit).
+__device__ inline uint32_t pop_low(uint64_t& bits) {
+  const uint32_t q = __builtin_ctzll(bits);
+  asm("s_bitset0_b64 %0, %1" : "+s"(bits) : "s"(q));
+  return q;
+}
+
 template <int W, typename M>
 __global__ __launch_bounds__(1024) void k_vsub(VsubParams p) {
   constexpr int MB = sizeof(M) * 8;
@@ -131,27 +148,25 @@ __global__ __launch_bounds__(1024) void k_vsub(VsubParams p) {
     for (int w = 0; w < W; ++w) {
       if (!go) continue;
       uint64_t bits = aw[w];
-      const uint32_t base = uint32_t(w) * 64;
-      while (__popcll(bits) >= 4) {
-        const uint32_t q0 = base + __builtin_ctzll(bits);
-        bits &= bits - 1;
-        const uint32_t q1 = base + __builtin_ctzll(bits);
-        bits &= bits - 1;
-        const uint32_t q2 = base + __builtin_ctzll(bits);
-        bits &= bits - 1;
-        const uint32_t q3 = base + __builtin_ctzll(bits);
-        bits &= bits - 1;
-        const M m0 = rows[size_t(q0) * 64 + lane];
-        const M m1 = rows[size_t(q1) * 64 + lane];
-        const M m2 = rows[size_t(q2) * 64 + lane];
-        const M m3 = rows[size_t(q3) * 64 + lane];
-        r &= (m0 & m1) & (m2 & m3);
+      const M* rw = rows + size_t(w) * 64 * 64 + lane;  // row (w*64 + q) of this lane
+      int c = __popcll(bits);
+      for (; c >= 4; c -= 4) {
+        const uint32_t q0 = pop_low(bits);
+        const uint32_t q1 = pop_low(bits);
+        const uint32_t q2 = pop_low(bits);
+        const uint32_t q3 = pop_low(bits);
+        const M m0 = rw[q0 * 64];
+        const M m1 = rw[q1 * 64];
+        const M m2 = rw[q2 * 64];
+        const M m3 = rw[q3 * 64];
+        r = and3(and3(r, m0, m1), m2, m3);
       }
-      while (bits) {
-        const uint32_t q0 = base + __builtin_ctzll(bits);
-        bits &= bits - 1;
-        r &= rows[size_t(q0) * 64 + lane];
+      if (c >= 2) {
+        const uint32_t q0 = pop_low(bits);
+        const uint32_t q1 = pop_low(bits);
+        r = and3(r, rw[q0 * 64], rw[q1 * 64]);
       }
+      if (c & 1) r &= rw[__builtin_ctzll(bits) * 64];
       if (p.early_exit && w + 1 < W) go = __builtin_amdgcn_ballot_w64(r != 0) != 0;
     }
     if (p.diag != INT64_MIN) {

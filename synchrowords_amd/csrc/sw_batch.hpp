@@ -24,8 +24,18 @@ struct BatchInstance {
   uint64_t b0[kBatchMaxW];  // sink states with two preimages under a letter
 };
 
+// Why an instance left the workgroup (BatchResult::detail for kBatchOverflow).
+enum BatchOverflow : uint32_t {
+  kOverflowNone = 0,
+  kOverflowCand = 1,   // one step's candidates above cap_cand
+  kOverflowRows = 2,   // a side above cap_rows
+  kOverflowWord = 3,   // length reached the word buffer
+  kOverflowHash = 4,   // hash table above hash_slots
+  kOverflowPairs = 5,  // goal check above max_pairs
+};
+
 struct BatchResult {
-  uint32_t status, length, word_len, steps;
+  uint32_t status, length, word_len, steps, detail;
 };
 
 // Capacities of one workgroup's scratch.

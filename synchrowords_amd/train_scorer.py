@@ -24,7 +24,7 @@ import numpy as np
 import synchrowords_amd as sw
 
 
-def collect(n, k, seeds, fdepth, width, bdepth, per_depth, threads, log_every=2000):
+def collect(n, k, seeds, fdepth, width, bdepth, per_depth, threads, log_every=2000, raw=None):
     xs, ts = [], []
     lock = threading.Lock()
     it = iter(seeds)
@@ -49,6 +49,8 @@ def collect(n, k, seeds, fdepth, width, bdepth, per_depth, threads, log_every=20
                 with lock:
                     xs.append(x)
                     ts.append(dist)
+                    if raw is not None:
+                        raw.append((s, np.asarray(sets), np.asarray(dist)))
                     done[0] += 1
                     if done[0] % log_every == 0:
                         print(f"[train] sampled {done[0]} automata, {sum(len(t) for t in ts)} samples, "
@@ -81,11 +83,16 @@ def main() -> int:
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--out", type=str, required=True)
+    ap.add_argument("--dump", type=str, default="", help="also save the raw labelled sets (seed, set, distance)")
     a = ap.parse_args()
 
     t0 = time.perf_counter()
+    raw = [] if a.dump else None
     x, t = collect(a.n, a.k, range(a.seed0, a.seed0 + a.dfas), a.fdepth, a.width, a.bdepth, a.per_depth,
-                   a.threads)
+                   a.threads, raw=raw)
+    if raw:
+        np.savez_compressed(a.dump, seed=np.concatenate([np.full(len(d), s, dtype=np.int64) for s, _, d in raw]),
+                            sets=np.concatenate([r for _, r, _ in raw]), dist=np.concatenate([d for _, _, d in raw]))
     t_sample = time.perf_counter() - t0
     print(f"[train] {len(t)} samples from {a.dfas} automata in {t_sample:.1f}s; "
           f"distance mean {t.mean():.2f}, censored {(t > a.fdepth).mean():.3f}", flush=True)
