@@ -141,7 +141,13 @@ def bench_batch(ctx, count: int) -> dict:
 
 
 def bench_learned(ctx, scorer: str, seeds, width: int) -> dict:
-    """Learned-bound vs plain-bound time to the exact length (n=300, k=2)."""
+    """Learned-bound against plain-bound exact search on the GPU (n=300, k=2).
+
+    Both pipelines run Eppstein (host), a device beam of the same width and the
+    device exact search capped by the smaller bound; they differ only in the
+    beam's order: cardinality (the reference's) or the trained network.  The
+    configs/exact.json plan (host Beam of the reference's dynamic width) is
+    timed beside them."""
     import synchrowords_amd as sw
     from synchrowords_amd.train_scorer import load_scorer
     m = load_scorer(ctx, scorer)
@@ -150,34 +156,41 @@ def bench_learned(ctx, scorer: str, seeds, width: int) -> dict:
     plan_no_beam = {"upper_bound": "1ULL * (AUT_N - 1) * (AUT_N - 1) + 1", "algorithms": [
         {"name": "Brute", "config": {"max_n": "15"}},
         {"name": "Eppstein", "config": {"find_word": True}}]}
-    for s in seeds:
-        d = sw.random_dfa(n, k, s)
-        # plain: the configs/exact.json plan (host Beam, device Exact)
-        t0 = time.perf_counter()
-        rp = ctx.pipeline(n, k, d)
-        t_plain = time.perf_counter() - t0
-        # learned: Eppstein, device beam scored by the network, device Exact
+
+    def run(d, mlp):
         t0 = time.perf_counter()
         re = ctx.pipeline(n, k, d, plan=plan_no_beam)
         ub = re.upper
-        lb, lw = m.beam(d, width, ub)
-        ex = ctx.exact(n, k, d, upper=min(lb, ub), upper_word=lw if lb <= ub else None)
-        t_learned = time.perf_counter() - t0
-        cb, _ = ctx.beam(n, k, d, width, ub)  # cardinality beam, same width
-        rows.append({"seed": s, "length_plain": rp.upper, "length_learned": ex.length, "beam_bound": rp.beam_bound,
-                     "learned_bound": lb, "cardinality_bound_same_width": cb, "eppstein": ub,
-                     "plain_s": round(t_plain, 4), "learned_s": round(t_learned, 4)})
+        tb = time.perf_counter()
+        b, w = ctx.beam(n, k, d, width, ub, mlp=mlp)
+        t_beam = time.perf_counter() - tb
+        ex = ctx.exact(n, k, d, upper=min(b, ub), upper_word=w if b <= ub else None)
+        return ex.length, b, time.perf_counter() - t0, t_beam
+
+    run(sw.random_dfa(n, k, seeds[0]), m)  # warm
+    for s in seeds:
+        d = sw.random_dfa(n, k, s)
+        t0 = time.perf_counter()
+        rp = ctx.pipeline(n, k, d)
+        t_plan = time.perf_counter() - t0
+        len_c, b_c, t_c, tb_c = run(d, None)
+        len_l, b_l, t_l, tb_l = run(d, m)
+        rows.append({"seed": s, "length": rp.upper, "length_plain": len_c, "length_learned": len_l,
+                     "bound_cardinality": b_c, "bound_learned": b_l, "bound_plan_beam": rp.beam_bound,
+                     "plain_s": round(t_c, 4), "learned_s": round(t_l, 4), "plan_s": round(t_plan, 4),
+                     "beam_s_cardinality": round(tb_c, 4), "beam_s_learned": round(tb_l, 4)})
         print(f"[bench] learned seed {s}: {rows[-1]}", file=sys.stderr, flush=True)
     m.close()
-    same = all(r["length_plain"] == r["length_learned"] for r in rows)
-    return {"workload": f"held-out random DFAs n={n} k={k}, seeds {seeds[0]}..{seeds[-1]}, learned width {width}",
+    same = all(r["length_plain"] == r["length_learned"] == r["length"] for r in rows)
+    tot = lambda key: round(sum(r[key] for r in rows), 4)
+    gap = lambda key: float(np.mean([r[key] - r["length"] for r in rows]))
+    return {"workload": f"held-out random DFAs n={n} k={k}, seeds {seeds[0]}..{seeds[-1]}, beam width {width}",
             "identical_lengths": same,
-            "plain_s_total": round(sum(r["plain_s"] for r in rows), 4),
-            "learned_s_total": round(sum(r["learned_s"] for r in rows), 4),
-            "mean_gap_learned": float(np.mean([r["learned_bound"] - r["length_plain"] for r in rows])),
-            "mean_gap_cardinality_same_width": float(np.mean([r["cardinality_bound_same_width"] - r["length_plain"]
-                                                              for r in rows])),
-            "mean_gap_plain_beam": float(np.mean([r["beam_bound"] - r["length_plain"] for r in rows])),
+            "plain_s_total": tot("plain_s"), "learned_s_total": tot("learned_s"), "plan_s_total": tot("plan_s"),
+            "mean_gap_cardinality": gap("bound_cardinality"), "mean_gap_learned": gap("bound_learned"),
+            "mean_gap_plan_beam": gap("bound_plan_beam"),
+            "exact_bound_cardinality": sum(r["bound_cardinality"] == r["length"] for r in rows),
+            "exact_bound_learned": sum(r["bound_learned"] == r["length"] for r in rows),
             "rows": rows}
 
 
@@ -188,7 +201,7 @@ def main() -> int:
                     help="subset: the headline line; batch / learned: the batched and learned-bound reports")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--scorer", type=str, default=str(ROOT / "synchrowords_amd" / "models" / "scorer_n300_k2.npz"))
-    ap.add_argument("--learned-width", type=int, default=256)
+    ap.add_argument("--learned-width", type=int, default=64)
     ap.add_argument("--learned-seeds", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (hardware-counter calibration)")

@@ -228,13 +228,20 @@ int sw_set_subset_shape(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_co
     uint32_t target_wgs, uint32_t early_exit);
 
 /* -------------------------------------------------------- learned upper bound
- * A BF16 MLP (MFMA, FP32 accumulation) that scores beam candidates by their
- * estimated forward distance.  No reference counterpart: it replaces the
- * cardinality order of the reference's Beam (algorithm/beam/beam.hpp:98-188)
- * with a learned one; the bound stays a real word's length.
- * Inputs of a sample: the n state bits of S and |delta_a(S)|/n per letter,
- * zero-padded to in_dim = ceil((n+k)/32)*32.  Network in_dim -> H -> H -> 1,
- * ReLU.  Parameters flattened as [W1 (H x in_dim), b1, W2 (H x H), b2, w3 (H), b3].
+ * A BF16 MLP (MFMA, FP32 accumulation) that scores beam candidates; lower
+ * scores are kept.  No reference counterpart: it replaces the cardinality
+ * order of the reference's Beam (algorithm/beam/beam.hpp:98-188) with a
+ * learned one; the bound stays a real word's length.
+ * Inputs of a set S of an automaton (in_dim = ceil((n+k+5)/32)*32 columns):
+ *   column perm[q] = 1 for each state q in S, where perm is the automaton's
+ *     canonical state order: states by how often they occur among the
+ *     distinct images of Q first reached at depth 8 (breadth-first), most
+ *     frequent first, ties by state number;
+ *   column n+a = |delta_a(S)|/n per letter;
+ *   column n+k+j = min |A \ S| / n over the images A of Q of depth <= 2(j+1), j < 4;
+ *   column n+k+4 = |S|/n; zero elsewhere.
+ * Network in_dim -> H -> H -> 1, ReLU.  Parameters flattened as
+ * [W1 (H x in_dim), b1, W2 (H x H), b2, w3 (H), b3].
  */
 typedef struct sw_mlp sw_mlp;
 int sw_mlp_create(sw_ctx* ctx, uint32_t n, uint32_t k, uint32_t hidden, uint64_t seed, sw_mlp** out);
@@ -246,13 +253,20 @@ int sw_mlp_set_params(sw_mlp* m, const float* in, uint64_t count);
 int sw_mlp_features(sw_mlp* m, const uint32_t* delta, const uint64_t* sets, uint64_t count, uint16_t* x);
 /* y[count] = network(x[count][in_dim]). */
 int sw_mlp_forward(sw_mlp* m, const uint16_t* x, uint64_t count, float* y);
-/* Mean squared error of one batch and its gradient (flattened like the parameters). */
+/* Loss of sw_mlp_gradients / sw_mlp_train: rank_group 0 = mean squared error
+ * against the targets (default); G in [2, 16] = pairwise ranking within
+ * consecutive groups of G samples: sum over pairs (a, b) of a group with
+ * t_a < t_b of softplus(y_a - y_b), divided by groups * G(G-1)/2. */
+int sw_mlp_set_loss(sw_mlp* m, uint32_t rank_group);
+/* Loss of one batch and its gradient (flattened like the parameters). */
 int sw_mlp_gradients(sw_mlp* m, const uint16_t* x, const float* target, uint64_t count, float* grads, float* loss);
 /* Adam, `epochs` passes of shuffled minibatches; loss_curve[e] = mean batch loss of epoch e (may be NULL). */
 int sw_mlp_train(sw_mlp* m, const uint16_t* x, const float* target, uint64_t count, uint32_t epochs, uint32_t batch,
     float lr, uint64_t seed, float* loss_curve);
 /* Training samples: beam candidate sets labelled with their forward distance
- * measured on the search's forward levels up to fdepth (fdepth+1 if beyond). */
+ * measured on the search's forward levels up to fdepth (fdepth+1 if beyond),
+ * in groups of per_depth candidates of one beam depth (depths with fewer
+ * candidates contribute none). */
 int sw_mlp_samples(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t fdepth, uint32_t width,
     uint32_t bdepth, uint32_t per_depth, uint64_t* sets_out, float* dist_out, uint64_t cap, uint64_t* count);
 /* Beam of `width` on the device, scored by the network m, or by cardinality

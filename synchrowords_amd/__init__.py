@@ -114,6 +114,7 @@ _SIGS = {
     "sw_mlp_features": (C.c_int, [_P, _P, _P, C.c_uint64, _P]),
     "sw_mlp_forward": (C.c_int, [_P, _P, C.c_uint64, _P]),
     "sw_mlp_gradients": (C.c_int, [_P, _P, _P, C.c_uint64, _P, C.POINTER(C.c_float)]),
+    "sw_mlp_set_loss": (C.c_int, [_P, C.c_uint32]),
     "sw_mlp_train": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _P]),
     "sw_mlp_samples": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                  _P, _P, C.c_uint64, _U64P]),
@@ -315,6 +316,10 @@ class Mlp:
         y = np.zeros(x.shape[0], dtype=np.float32)
         _check(load().sw_mlp_forward(self.handle, _ptr(x), x.shape[0], _ptr(y)))
         return y
+
+    def set_loss(self, rank_group: int = 0) -> None:
+        """0: squared error on the targets; G: pairwise ranking within groups of G samples."""
+        _check(load().sw_mlp_set_loss(self.handle, rank_group))
 
     def gradients(self, x, target):
         x = np.ascontiguousarray(x, dtype=np.uint16)

@@ -127,15 +127,35 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, in
       }
 }
 
-// Features of set rows: x[i][q] = bit q, x[i][n + a] = |delta_a(S_i)| / n.
+// Features of set rows S_i (D columns, BF16):
+//   x[perm[q]]      = 1 if state q is in S_i (states in the automaton's canonical order),
+//   x[n + a]        = |delta_a(S_i)| / n for each letter a,
+//   x[n + k + j]    = min over images A of Q of depth <= 2(j+1) of |A \ S_i|, / n (j < 4),
+//   x[n + k + 4]    = |S_i| / n,
+// zero elsewhere.
 template <int W>
 __global__ void k_features(const uint64_t* __restrict__ sets, const uint64_t* __restrict__ images, uint64_t count,
-    uint32_t n, uint32_t k, uint32_t D, uint16_t* __restrict__ x) {
+    uint32_t n, uint32_t k, uint32_t D, FeatureGuide g, uint16_t* __restrict__ x) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= count) return;
   uint16_t* row = x + i * D;
-  const uint64_t* s = sets + i * W;
-  for (uint32_t q = 0; q < n; ++q) row[q] = ((s[q >> 6] >> (q & 63)) & 1) ? 0x3f80 : 0;  // 1.0 / 0.0
+  uint64_t s[W];
+  int card = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    s[w] = sets[i * W + w];
+    card += __popcll(s[w]);
+  }
+  for (uint32_t q = 0; q < D; ++q) row[q] = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    uint64_t b = s[w];
+    while (b) {
+      const uint32_t q = uint32_t(w) * 64 + __builtin_ctzll(b);
+      b &= b - 1;
+      row[g.perm[q]] = 0x3f80;  // 1.0
+    }
+  }
   for (uint32_t a = 0; a < k; ++a) {
     const uint64_t* im = images + (uint64_t(a) * count + i) * W;
     int c = 0;
@@ -143,7 +163,17 @@ __global__ void k_features(const uint64_t* __restrict__ sets, const uint64_t* __
     for (int w = 0; w < W; ++w) c += __popcll(im[w]);
     row[n + a] = f2bf(float(c) / float(n));
   }
-  for (uint32_t q = n + k; q < D; ++q) row[q] = 0;
+  uint32_t best = n;
+  int j = 0;
+  for (uint32_t r = 0; r < g.ends[kGuideDepths - 1]; ++r) {
+    while (j < kGuideDepths && r == g.ends[j]) row[n + k + j++] = f2bf(float(best) / float(n));
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) c += __popcll(g.rows[uint64_t(r) * W + w] & ~s[w]);
+    best = c < best ? c : best;
+  }
+  while (j < kGuideDepths) row[n + k + j++] = f2bf(float(best) / float(n));
+  row[n + k + kGuideDepths] = f2bf(float(card) / float(n));
 }
 
 // y[i] = h[i] . w3 + b3 (FP32), one wave per row.
@@ -160,20 +190,51 @@ __global__ void k_head(const float* __restrict__ h, const float* __restrict__ w3
 
 // Loss gradient of the head and first hidden gradient:
 //   dy = 2 (y - t) / B;  dH2 = dy * w3 (masked by ReLU(H2) > 0), bf16 + f32.
+// With dy_given, dy already holds the loss gradient (the ranking loss).
 __global__ void k_head_back(const float* __restrict__ y, const float* __restrict__ t, const float* __restrict__ h2,
-    const float* __restrict__ w3, uint64_t count, uint32_t H, float inv_b, float* __restrict__ dy,
+    const float* __restrict__ w3, uint64_t count, uint32_t H, float inv_b, int dy_given, float* __restrict__ dy,
     uint16_t* __restrict__ dh2, float* __restrict__ loss) {
   const uint64_t row = blockIdx.x;
   if (row >= count) return;
-  const float d = 2.f * (y[row] - t[row]) * inv_b;
-  if (threadIdx.x == 0) {
-    dy[row] = d;
-    atomicAdd(loss, (y[row] - t[row]) * (y[row] - t[row]) * inv_b);
+  float d;
+  if (dy_given) {
+    d = dy[row];
+  } else {
+    d = 2.f * (y[row] - t[row]) * inv_b;
+    if (threadIdx.x == 0) {
+      dy[row] = d;
+      atomicAdd(loss, (y[row] - t[row]) * (y[row] - t[row]) * inv_b);
+    }
   }
   for (uint32_t j = threadIdx.x; j < H; j += blockDim.x) {
     const float g = h2[row * H + j] > 0.f ? d * w3[j] : 0.f;
     dh2[row * H + j] = f2bf(g);
   }
+}
+
+// Pairwise ranking loss over groups of G consecutive rows (one thread per
+// group): for every pair (a, b) of a group with t_a < t_b (a is closer to Q),
+// loss += softplus(y_a - y_b), so a learns to score below b.  Normalised by
+// the number of ordered pairs the batch could hold, groups * G(G-1)/2.
+__global__ void k_rank_dy(const float* __restrict__ y, const float* __restrict__ t, uint64_t groups, uint32_t G,
+    float inv_pairs, float* __restrict__ dy, float* __restrict__ loss) {
+  const uint64_t gi = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gi >= groups) return;
+  const uint64_t base = gi * G;
+  float d[kRankMaxGroup];
+  for (uint32_t a = 0; a < G; ++a) d[a] = 0.f;
+  float l = 0.f;
+  for (uint32_t a = 0; a < G; ++a)
+    for (uint32_t b = 0; b < G; ++b) {
+      if (!(t[base + a] < t[base + b])) continue;
+      const float z = y[base + a] - y[base + b];
+      const float sg = 1.f / (1.f + expf(-z));
+      l += (z > 0.f ? z : 0.f) + log1pf(expf(-fabsf(z)));
+      d[a] += sg;
+      d[b] -= sg;
+    }
+  for (uint32_t a = 0; a < G; ++a) dy[base + a] = d[a] * inv_pairs;
+  atomicAdd(loss, l * inv_pairs);
 }
 
 // dW3[j] = sum_i dy[i] * H2[i][j]; db3 = sum dy.  One block per 256 columns.
@@ -247,10 +308,10 @@ hipError_t launch_gemm(bool ta, bool tb, const uint16_t* A, int lda, const uint1
 }
 
 hipError_t launch_features(int W, const uint64_t* sets, const uint64_t* images, uint64_t count, uint32_t n, uint32_t k,
-    uint32_t D, uint16_t* x, hipStream_t st) {
+    uint32_t D, const FeatureGuide& g, uint16_t* x, hipStream_t st) {
   if (!count) return hipSuccess;
   switch (W) {
-#define C(w) case w: hipLaunchKernelGGL(k_features<w>, dim3(blocks(count)), dim3(256), 0, st, sets, images, count, n, k, D, x); break;
+#define C(w) case w: hipLaunchKernelGGL(k_features<w>, dim3(blocks(count)), dim3(256), 0, st, sets, images, count, n, k, D, g, x); break;
     C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8)
 #undef C
     default: return hipErrorInvalidValue;
@@ -334,11 +395,18 @@ hipError_t mlp_forward(const MlpDev& p, const uint16_t* x, uint64_t B, MlpWork& 
 }
 
 hipError_t mlp_backward(const MlpDev& p, const uint16_t* x, uint64_t B, const float* target, MlpWork& w, MlpGrads& g,
-    float* loss, hipStream_t st) {
+    float* loss, uint32_t rank_group, hipStream_t st) {
   hipError_t e;
   const float inv_b = 1.f / float(B);
-  hipLaunchKernelGGL(k_head_back, dim3(unsigned(B)), dim3(256), 0, st, w.y, target, w.h2f, p.w3, B, p.H, inv_b, w.dy,
-      w.g2b, loss);
+  if (rank_group) {
+    if (rank_group < 2 || rank_group > kRankMaxGroup || B % rank_group) return hipErrorInvalidValue;
+    const uint64_t groups = B / rank_group;
+    const float inv_pairs = 1.f / (float(groups) * float(rank_group * (rank_group - 1) / 2));
+    hipLaunchKernelGGL(k_rank_dy, dim3(blocks(groups)), dim3(256), 0, st, w.y, target, groups, rank_group, inv_pairs,
+        w.dy, loss);
+  }
+  hipLaunchKernelGGL(k_head_back, dim3(unsigned(B)), dim3(256), 0, st, w.y, target, w.h2f, p.w3, B, p.H, inv_b,
+      rank_group ? 1 : 0, w.dy, w.g2b, loss);
   hipLaunchKernelGGL(k_head_wgrad, dim3(blocks(p.H)), dim3(256), 0, st, w.dy, w.h2f, B, p.H, g.w3, g.b3);
   // dW2 = dH2^T H1, db2 = colsum dH2
   if ((e = launch_gemm(true, false, w.g2b, p.H, w.h1b, p.H, g.w2, nullptr, p.H, p.H, p.H, int(B), nullptr, 0, 0, st)))

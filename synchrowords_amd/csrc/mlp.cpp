@@ -15,6 +15,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <set>
 #include <vector>
 
 #include "sw_host.hpp"
@@ -30,6 +31,7 @@ struct sw_mlp {
   sw::DevBuf h1f, h2f, y, dy, g1f, h1b, h2b, g1b, g2b, lossb;
   uint64_t work_rows = 0;
   uint64_t nparams = 0;
+  uint32_t rank_group = 0;  // 0: squared error; G: ranking within groups of G
 };
 
 namespace sw {
@@ -99,15 +101,70 @@ MlpWork work_of(sw_mlp* m, uint64_t rows) {
 
 }  // namespace
 
+// The per-automaton side of the features: the images of Q by breadth-first
+// search up to depth 2*kGuideDepths (distinct images, each at the first depth
+// it appears), and the canonical state order — states by how often they
+// occur in the deepest level's images, most frequent first, ties by state.
+// Random automata share no state names; this order makes a feature column
+// mean the same thing ("the r-th most often needed state") in every one.
+struct Guide {
+  DevBuf perm, rows;
+  FeatureGuide g;
+};
+
+void build_guide(uint32_t n, uint32_t k, const uint32_t* delta, Guide& out, hipStream_t st) {
+  const uint32_t W = (n + 63) / 64;
+  std::set<std::vector<uint64_t>> seen;
+  std::vector<std::vector<uint64_t>> level(1, std::vector<uint64_t>(W, 0));
+  for (uint32_t q = 0; q < n; ++q) level[0][q >> 6] |= 1ull << (q & 63);
+  seen.insert(level[0]);
+  std::vector<uint64_t> rows(level[0]);
+  std::vector<std::vector<uint64_t>> last = level;
+  uint32_t ends[kGuideDepths];
+  const uint32_t depth = 2 * kGuideDepths;
+  for (uint32_t d = 1; d <= depth; ++d) {
+    std::vector<std::vector<uint64_t>> next;
+    for (const auto& A : level)
+      for (uint32_t a = 0; a < k; ++a) {
+        std::vector<uint64_t> B(W, 0);
+        for (uint32_t q = 0; q < n; ++q)
+          if ((A[q >> 6] >> (q & 63)) & 1) {
+            const uint32_t t = delta[size_t(q) * k + a];
+            B[t >> 6] |= 1ull << (t & 63);
+          }
+        if (seen.insert(B).second) next.push_back(std::move(B));
+      }
+    for (const auto& B : next) rows.insert(rows.end(), B.begin(), B.end());
+    if (!next.empty()) last = next;
+    level.swap(next);
+    if (d % 2 == 0) ends[d / 2 - 1] = uint32_t(rows.size() / W);
+  }
+  std::vector<uint32_t> freq(n, 0);
+  for (const auto& A : last)
+    for (uint32_t q = 0; q < n; ++q) freq[q] += (A[q >> 6] >> (q & 63)) & 1;
+  std::vector<uint32_t> order(n);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return freq[a] > freq[b]; });
+  std::vector<uint16_t> perm(n);
+  for (uint32_t r = 0; r < n; ++r) perm[order[r]] = uint16_t(r);
+  out.perm.ensure(size_t(n) * 2 + 8);
+  out.rows.ensure(rows.size() * 8 + 8);
+  SW_CHECK_HIP(hipMemcpyAsync(out.perm.p, perm.data(), size_t(n) * 2, hipMemcpyHostToDevice, st));
+  SW_CHECK_HIP(hipMemcpyAsync(out.rows.p, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, st));
+  SW_CHECK_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
+  out.g.perm = out.perm.as<uint16_t>();
+  out.g.rows = out.rows.as<uint64_t>();
+  for (int j = 0; j < kGuideDepths; ++j) out.g.ends[j] = ends[j];
+}
+
 // Features of `count` device sets (images under every letter computed here).
-void mlp_features(sw_mlp* m, const uint32_t* delta_dev_tables_unused, const uint64_t* tables_fwd,
-    const uint64_t* sets, uint64_t count, uint16_t* x, DevBuf& scratch, hipStream_t st) {
-  (void)delta_dev_tables_unused;
+void mlp_features(sw_mlp* m, const Guide& guide, const uint64_t* tables_fwd, const uint64_t* sets, uint64_t count,
+    uint16_t* x, DevBuf& scratch, hipStream_t st) {
   const uint32_t n = m->n, k = m->k, W = (n + 63) / 64;
   scratch.ensure(count * k * W * 8 + 8);
   SW_CHECK_HIP(launch_expand(int(W), sets, count, tables_fwd, (n + 7) / 8, k, nullptr, n, 0, scratch.as<uint64_t>(),
       nullptr, st));
-  SW_CHECK_HIP(launch_features(int(W), sets, scratch.as<uint64_t>(), count, n, k, m->p.D, x, st));
+  SW_CHECK_HIP(launch_features(int(W), sets, scratch.as<uint64_t>(), count, n, k, m->p.D, guide.g, x, st));
 }
 
 }  // namespace sw
@@ -136,7 +193,7 @@ int sw_mlp_create(sw_ctx* ctx, uint32_t n, uint32_t k, uint32_t hidden, uint64_t
     m->ctx = ctx;
     m->n = n;
     m->k = k;
-    m->p.D = ((n + k) + 31) / 32 * 32;
+    m->p.D = ((n + k + sw::kGuideFeatures) + 31) / 32 * 32;
     m->p.H = hidden;
     const uint64_t D = m->p.D, H = hidden;
     m->nparams = H * D + H + H * H + H + H + 1;
@@ -245,11 +302,23 @@ int sw_mlp_gradients(sw_mlp* m, const uint16_t* x, const float* target, uint64_t
     SW_CHECK_HIP(hipMemsetAsync(m->lossb.p, 0, 4, st));
     auto g = sw::grads_of(m);
     SW_CHECK_HIP(sw::mlp_forward(m->p, m->ctx->a.as<uint16_t>(), count, w, w.y, st));
+    SW_REQUIRE(!m->rank_group || count % m->rank_group == 0, "count must be a multiple of the ranking group");
     SW_CHECK_HIP(sw::mlp_backward(m->p, m->ctx->a.as<uint16_t>(), count, m->ctx->b.as<float>(), w, g,
-        m->lossb.as<float>(), st));
+        m->lossb.as<float>(), m->rank_group, st));
     SW_CHECK_HIP(hipMemcpyAsync(grads, m->grads.p, m->nparams * 4, hipMemcpyDeviceToHost, st));
     if (loss) SW_CHECK_HIP(hipMemcpyAsync(loss, m->lossb.p, 4, hipMemcpyDeviceToHost, st));
     SW_CHECK_HIP(hipStreamSynchronize(st));
+    return SW_OK;
+  } catch (const Error& e) {
+    return fail(e);
+  }
+}
+
+int sw_mlp_set_loss(sw_mlp* m, uint32_t rank_group) {
+  try {
+    SW_REQUIRE(m, "NULL argument");
+    SW_REQUIRE(rank_group == 0 || (rank_group >= 2 && rank_group <= sw::kRankMaxGroup), "rank_group 0 or 2..16");
+    m->rank_group = rank_group;
     return SW_OK;
   } catch (const Error& e) {
     return fail(e);
@@ -265,30 +334,38 @@ int sw_mlp_train(sw_mlp* m, const uint16_t* x, const float* target, uint64_t cou
     sw::use(m->ctx);
     hipStream_t st = m->ctx->stream;
     const uint64_t D = m->p.D;
-    // Shuffle on the host once per epoch; batches gathered on the host side.
+    // Shuffle on the host once per epoch (whole groups under the ranking
+    // loss); batches gathered on the host side.
+    const uint32_t G = m->rank_group ? m->rank_group : 1;
+    SW_REQUIRE(count % G == 0, "sample count must be a multiple of the ranking group");
+    batch = batch / G * G;
+    SW_REQUIRE(batch >= G, "batch smaller than one ranking group");
     DevBuf dx, dt;
     dx.ensure(uint64_t(batch) * D * 2);
     dt.ensure(uint64_t(batch) * 4);
-    std::vector<uint64_t> idx(count);
+    std::vector<uint64_t> idx(count / G);
     std::iota(idx.begin(), idx.end(), 0ull);
     std::vector<uint16_t> bx(uint64_t(batch) * D);
     std::vector<float> bt(batch);
     auto w = sw::work_of(m, batch);
     auto g = sw::grads_of(m);
     uint64_t s = seed;
+    const uint64_t units = count / G, per = batch / G;
     for (uint32_t e = 0; e < epochs; ++e) {
-      for (uint64_t i = count; i > 1; --i) std::swap(idx[i - 1], idx[sw::splitmix(s) % i]);
+      for (uint64_t i = units; i > 1; --i) std::swap(idx[i - 1], idx[sw::splitmix(s) % i]);
       SW_CHECK_HIP(hipMemsetAsync(m->lossb.p, 0, 4, st));
       uint64_t steps = 0;
-      for (uint64_t b0 = 0; b0 + batch <= count; b0 += batch, ++steps) {
+      for (uint64_t b0 = 0; b0 + per <= units; b0 += per, ++steps) {
         for (uint32_t r = 0; r < batch; ++r) {
-          std::memcpy(&bx[uint64_t(r) * D], x + idx[b0 + r] * D, D * 2);
-          bt[r] = target[idx[b0 + r]];
+          const uint64_t src = idx[b0 + r / G] * G + r % G;
+          std::memcpy(&bx[uint64_t(r) * D], x + src * D, D * 2);
+          bt[r] = target[src];
         }
         SW_CHECK_HIP(hipMemcpyAsync(dx.p, bx.data(), bx.size() * 2, hipMemcpyHostToDevice, st));
         SW_CHECK_HIP(hipMemcpyAsync(dt.p, bt.data(), bt.size() * 4, hipMemcpyHostToDevice, st));
         SW_CHECK_HIP(sw::mlp_forward(m->p, dx.as<uint16_t>(), batch, w, w.y, st));
-        SW_CHECK_HIP(sw::mlp_backward(m->p, dx.as<uint16_t>(), batch, dt.as<float>(), w, g, m->lossb.as<float>(), st));
+        SW_CHECK_HIP(sw::mlp_backward(m->p, dx.as<uint16_t>(), batch, dt.as<float>(), w, g, m->lossb.as<float>(),
+            m->rank_group, st));
         SW_CHECK_HIP(sw::mlp_adam(m->p, g, m->adam, lr, st));
         SW_CHECK_HIP(hipStreamSynchronize(st));  // host batch buffers are reused
       }
@@ -312,13 +389,15 @@ int sw_mlp_features(sw_mlp* m, const uint32_t* delta, const uint64_t* sets, uint
     hipStream_t st = m->ctx->stream;
     const uint32_t n = m->n, k = m->k, W = (n + 63) / 64;
     auto t = sw::build_tables(n, k, delta, 0);
+    sw::Guide guide;
+    sw::build_guide(n, k, delta, guide, st);
     DevBuf dtab, dsets, dx, scratch;
     dtab.ensure(t.size() * 8);
     dsets.ensure(count * W * 8);
     dx.ensure(count * m->p.D * 2);
     SW_CHECK_HIP(hipMemcpyAsync(dtab.p, t.data(), t.size() * 8, hipMemcpyHostToDevice, st));
     SW_CHECK_HIP(hipMemcpyAsync(dsets.p, sets, count * W * 8, hipMemcpyHostToDevice, st));
-    sw::mlp_features(m, nullptr, dtab.as<uint64_t>(), dsets.as<uint64_t>(), count, dx.as<uint16_t>(), scratch, st);
+    sw::mlp_features(m, guide, dtab.as<uint64_t>(), dsets.as<uint64_t>(), count, dx.as<uint16_t>(), scratch, st);
     SW_CHECK_HIP(hipMemcpyAsync(x, dx.p, count * m->p.D * 2, hipMemcpyDeviceToHost, st));
     SW_CHECK_HIP(hipStreamSynchronize(st));
     return SW_OK;
@@ -378,7 +457,8 @@ std::vector<uint64_t> beam_candidates(uint32_t n, uint32_t k, const uint32_t* d,
       uniq.insert(uniq.end(), r, r + W);
     }
     const size_t u = uniq.size() / W;
-    const size_t take = std::min<size_t>(per_depth, u);
+    // Whole groups only: a depth with fewer candidates than per_depth adds none.
+    const size_t take = u >= per_depth ? per_depth : 0;
     for (size_t t = 0; t < take; ++t) {
       const size_t j = take == 1 ? 0 : t * (u - 1) / (take - 1);
       out.insert(out.end(), &uniq[j * W], &uniq[j * W + W]);
@@ -407,7 +487,8 @@ int sw_mlp_samples(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, u
     hipStream_t st = ctx->stream;
     const uint32_t W = (n + 63) / 64;
     std::vector<uint64_t> cand = beam_candidates(n, k, delta, width, bdepth, per_depth);
-    uint64_t c = std::min<uint64_t>(cand.size() / W, cap);
+    SW_REQUIRE(per_depth >= 1, "per_depth >= 1");
+    uint64_t c = std::min<uint64_t>(cand.size() / W, cap / per_depth * per_depth);  // whole groups
     DevBuf levels, dcand, flags;
     std::vector<uint64_t> level_end;
     sw::forward_levels(ctx, n, k, delta, fdepth, 1ull << 24, levels, level_end);
@@ -469,6 +550,8 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
     if (word_len) *word_len = 0;
     auto tf = sw::build_tables(n, k, delta, 0);
     auto tb = sw::build_tables(n, k, delta, 1);
+    sw::Guide guide;
+    if (m) sw::build_guide(n, k, delta, guide, st);
     DevBuf dtf, dtb, list, cand, table, slot, keep, pos, uniq, uidx, x, scratch, keys, keys2, vals, vals2, sel, small,
         scan_tmp, sort_tmp;
     dtf.ensure(tf.size() * 8);
@@ -546,7 +629,7 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
       float* score = keys.as<float>();
       if (m) {
         x.ensure(uint64_t(nu) * m->p.D * 2 + 8);
-        sw::mlp_features(m, nullptr, dtf.as<uint64_t>(), uniq.as<uint64_t>(), nu, x.as<uint16_t>(), scratch, st);
+        sw::mlp_features(m, guide, dtf.as<uint64_t>(), uniq.as<uint64_t>(), nu, x.as<uint16_t>(), scratch, st);
         auto wk = sw::work_of(m, nu);
         SW_CHECK_HIP(sw::mlp_forward(m->p, x.as<uint16_t>(), nu, wk, wk.y, st));
         score = wk.y;

@@ -29,13 +29,26 @@ struct MlpAdam {
   int t = 0;
 };
 
+// Per-automaton inputs of the feature kernel: the canonical order of the
+// states and the images of Q up to depth 2*kGuideDepths.
+constexpr int kGuideDepths = 4;
+constexpr uint32_t kGuideFeatures = kGuideDepths + 1;  // min-missing counts + cardinality
+struct FeatureGuide {
+  const uint16_t* perm = nullptr;  // perm[q]: feature column of state q
+  const uint64_t* rows = nullptr;  // images of Q, by depth (W words each)
+  uint32_t ends[kGuideDepths] = {0, 0, 0, 0};  // rows of depth <= 2, 4, 6, 8
+};
+constexpr uint32_t kRankMaxGroup = 16;
+
 hipError_t launch_gemm(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* Cf,
     uint16_t* Cb, int ldc, int M, int N, int K, const float* bias, int relu, int accumulate, hipStream_t st);
 hipError_t launch_features(int W, const uint64_t* sets, const uint64_t* images, uint64_t count, uint32_t n, uint32_t k,
-    uint32_t D, uint16_t* x, hipStream_t st);
+    uint32_t D, const FeatureGuide& g, uint16_t* x, hipStream_t st);
 hipError_t mlp_forward(const MlpDev& p, const uint16_t* x, uint64_t B, MlpWork& w, float* y, hipStream_t st);
+// rank_group 0: mean squared error against target; G >= 2: pairwise ranking
+// loss within consecutive groups of G rows (B a multiple of G).
 hipError_t mlp_backward(const MlpDev& p, const uint16_t* x, uint64_t B, const float* target, MlpWork& w, MlpGrads& g,
-    float* loss, hipStream_t st);
+    float* loss, uint32_t rank_group, hipStream_t st);
 hipError_t mlp_adam(MlpDev& p, MlpGrads& g, MlpAdam& a, float lr, hipStream_t st);
 hipError_t mlp_refresh_bf16(MlpDev& p, hipStream_t st);
 // Smallest index of a set with all n states (first must hold ~0).

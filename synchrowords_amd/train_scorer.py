@@ -4,12 +4,16 @@
         --out synchrowords_amd/models/scorer_n300_k2.npz
 
 Samples come from the exact search itself: for each seeded random DFA
-(seeds from --seed0 on) a cardinality beam proposes candidate sets and the
-search's forward levels label each with its forward distance
-(sw_mlp_samples).  Features are built on the device, and the network
-(n + k -> H -> H -> 1, BF16 on MFMA with FP32 accumulation) trains with Adam on
-the device.  Several host threads each drive a context, so sampling overlaps.
-Writes the FP32 master parameters and the training record.
+(seeds from --seed0 on) a cardinality beam proposes candidate sets, --per-depth
+of them per beam depth, and the search's forward levels label each with its
+forward distance (sw_mlp_samples).  Features are built on the device, and the
+network (in_dim -> H -> H -> 1, BF16 on MFMA with FP32 accumulation) trains
+with Adam on the device.  The default loss ranks the candidates of one beam
+depth by their distance (pairwise, within each group): the beam only ever
+compares siblings, and distances beyond the labelled depth are censored, so
+a ranking is what the labels support.  Several host threads each drive a
+context, so sampling overlaps.  Writes the FP32 master parameters and the
+training record.
 """
 from __future__ import annotations
 
@@ -82,6 +86,7 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--threads", type=int, default=8)
+    ap.add_argument("--loss", choices=["rank", "mse"], default="rank")
     ap.add_argument("--out", type=str, required=True)
     ap.add_argument("--dump", type=str, default="", help="also save the raw labelled sets (seed, set, distance)")
     a = ap.parse_args()
@@ -98,13 +103,15 @@ def main() -> int:
           f"distance mean {t.mean():.2f}, censored {(t > a.fdepth).mean():.3f}", flush=True)
     ctx = sw.Context(0)
     m = sw.Mlp(ctx, a.n, a.k, hidden=a.hidden, seed=7)
+    if a.loss == "rank":
+        m.set_loss(a.per_depth)
     t1 = time.perf_counter()
     curve = m.train(x, t, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=11)
     t_train = time.perf_counter() - t1
     print(f"[train] loss per epoch {[round(float(c), 4) for c in curve]} in {t_train:.1f}s", flush=True)
     out = Path(a.out)
     out.parent.mkdir(parents=True, exist_ok=True)
-    meta = {"n": a.n, "k": a.k, "hidden": a.hidden, "dfas": a.dfas, "seed0": a.seed0, "fdepth": a.fdepth,
+    meta = {"n": a.n, "k": a.k, "hidden": a.hidden, "dfas": a.dfas, "loss": a.loss, "seed0": a.seed0, "fdepth": a.fdepth,
             "width": a.width, "bdepth": a.bdepth, "per_depth": a.per_depth, "samples": int(len(t)),
             "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "loss_curve": [float(c) for c in curve],
             "seconds_sampling": t_sample, "seconds_training": t_train}

@@ -33,7 +33,26 @@ def unpack(p, D, H):
     return w1, b1, w2, b2, w3, b3
 
 
-def reference(p, x_bits, t, D, H):
+def rank_loss(y, t, G):
+    """Pairwise ranking loss within consecutive groups of G rows and its dy."""
+    B = len(y)
+    groups = B // G
+    inv = 1.0 / (groups * (G * (G - 1) // 2))
+    dy = np.zeros(B)
+    loss = 0.0
+    for g in range(groups):
+        for a in range(g * G, g * G + G):
+            for b in range(g * G, g * G + G):
+                if t[a] < t[b]:
+                    z = float(y[a]) - float(y[b])
+                    loss += np.logaddexp(0.0, z) * inv
+                    s = 1.0 / (1.0 + np.exp(-z))
+                    dy[a] += s * inv
+                    dy[b] -= s * inv
+    return loss, dy
+
+
+def reference(p, x_bits, t, D, H, rank_group=0):
     """FP32 forward/backward with the GPU's BF16 rounding points."""
     w1, b1, w2, b2, w3, b3 = unpack(p, D, H)
     x = (x_bits.astype(np.uint32) << 16).view(np.float32)
@@ -44,8 +63,12 @@ def reference(p, x_bits, t, D, H):
     h2 = np.maximum(a2, 0)
     y = h2 @ w3 + b3[0]
     B = x.shape[0]
-    dy = 2 * (y - t) / B
-    loss = float(np.sum((y - t) ** 2) / B)
+    if rank_group:
+        loss, dy = rank_loss(y, t, rank_group)
+        dy = dy.astype(np.float32)
+    else:
+        dy = 2 * (y - t) / B
+        loss = float(np.sum((y - t) ** 2) / B)
     gw3 = h2.T @ dy
     gb3 = np.array([dy.sum()], dtype=np.float32)
     dh2 = bf16(np.outer(dy, w3) * (h2 > 0))
@@ -62,9 +85,11 @@ def rel_err(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12))
 
 
-@pytest.mark.parametrize("n,k,hidden,batch", [(300, 2, 512, 300), (48, 3, 64, 77), (100, 2, 128, 1024)])
-def test_forward_and_gradients_match_fp32_reference(ctx, n, k, hidden, batch):
+@pytest.mark.parametrize("n,k,hidden,batch,rank", [(300, 2, 512, 300, 0), (48, 3, 64, 77, 0), (100, 2, 128, 1024, 0),
+                                                   (300, 2, 512, 256, 8), (64, 2, 64, 96, 4)])
+def test_forward_and_gradients_match_fp32_reference(ctx, n, k, hidden, batch, rank):
     m = sw.Mlp(ctx, n, k, hidden=hidden, seed=3)
+    m.set_loss(rank)
     D, H = m.in_dim, m.hidden
     rng = np.random.default_rng(n + k)
     p = m.params()
@@ -77,7 +102,9 @@ def test_forward_and_gradients_match_fp32_reference(ctx, n, k, hidden, batch):
     x[:, n:n + k] = rng.random((batch, k))
     xb = bf16_bits(x)
     t = rng.random(batch).astype(np.float32) * 3
-    y_ref, g_ref, loss_ref = reference(m.params(), xb, t, D, H)
+    if rank:
+        t = np.floor(t * 2).astype(np.float32)  # ties too: tied pairs carry no loss
+    y_ref, g_ref, loss_ref = reference(m.params(), xb, t, D, H, rank)
     y = m.forward(xb)
     np.testing.assert_allclose(y, y_ref, rtol=2e-2, atol=2e-2)
     g, loss = m.gradients(xb, t)
@@ -178,3 +205,67 @@ def test_device_beams_give_valid_bounds(ctx):
         if b < cap:
             assert len(w) == b and replay(48, 2, c["delta"], w) == 1
     m.close()
+
+
+def guided_features(n, k, delta, sets):
+    """Restatement of the scorer's input features (see sw_mlp_features)."""
+    d = np.asarray(delta).reshape(n, k)
+    full = frozenset(range(n))
+    seen, level, rows, ends, last = {full}, [full], [full], [], [full]
+    for depth in range(1, 9):
+        nxt = []
+        for A in level:
+            for a in range(k):
+                B = frozenset(int(d[q, a]) for q in A)
+                if B not in seen:
+                    seen.add(B)
+                    nxt.append(B)
+        rows += nxt
+        if nxt:
+            last = nxt
+        level = nxt
+        if depth % 2 == 0:
+            ends.append(len(rows))
+    freq = np.zeros(n)
+    for A in last:
+        for q in A:
+            freq[q] += 1
+    order = np.argsort(-freq, kind="stable")
+    perm = np.empty(n, dtype=np.int64)
+    perm[order] = np.arange(n)
+    D = (n + k + 5 + 31) // 32 * 32
+    x = np.zeros((len(sets), D), dtype=np.uint16)
+    for i, row in enumerate(sets):
+        S = {q for q in range(n) if (int(row[q // 64]) >> (q % 64)) & 1}
+        for q in S:
+            x[i, perm[q]] = 0x3F80
+        for a in range(k):
+            x[i, n + a] = bf16_bits(np.float32(len({int(d[q, a]) for q in S})) / np.float32(n))
+        for j in range(4):
+            best = min(len(A - S) for A in rows[:ends[j]])
+            x[i, n + k + j] = bf16_bits(np.float32(best) / np.float32(n))
+        x[i, n + k + 4] = bf16_bits(np.float32(len(S)) / np.float32(n))
+    return x
+
+
+@pytest.mark.parametrize("n,k,seed", [(24, 2, 1), (70, 3, 2), (130, 2, 3)])
+def test_features_match_restatement(ctx, n, k, seed):
+    d = gen.random_dfa(n, k, seed)
+    rng = np.random.default_rng(seed)
+    W = (n + 63) // 64
+    sets = np.zeros((60, W), dtype=np.uint64)
+    for i in range(60):
+        bits = rng.random(n) < rng.random()
+        for q in np.nonzero(bits)[0]:
+            sets[i, q // 64] |= np.uint64(1) << np.uint64(q % 64)
+    m = sw.Mlp(ctx, n, k, hidden=32, seed=0)
+    got = m.features(d, sets)
+    m.close()
+    np.testing.assert_array_equal(got, guided_features(n, k, d, sets))
+
+
+def test_samples_come_in_whole_groups(ctx):
+    n, k = 40, 2
+    d = gen.random_dfa(n, k, 5)
+    sets, dist = sw.mlp_samples(ctx, n, k, d, fdepth=8, width=32, bdepth=30, per_depth=8, cap=1000)
+    assert len(sets) > 0 and len(sets) % 8 == 0
