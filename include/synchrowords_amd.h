@@ -237,9 +237,9 @@ int sw_set_subset_shape(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_co
  *     canonical state order: states by how often they occur among the
  *     distinct images of Q first reached at depth 8 (breadth-first), most
  *     frequent first, ties by state number;
- *   column n+a = |delta_a(S)|/n per letter;
- *   column n+k+j = min |A \ S| / n over the images A of Q of depth <= 2(j+1), j < 4;
- *   column n+k+4 = |S|/n; zero elsewhere.
+ *   column n+a = 8 |delta_a(S)|/n per letter;
+ *   column n+k+j = 8 min |A \ S| / n over the images A of Q of depth <= 2(j+1), j < 4;
+ *   column n+k+4 = 8 |S|/n; zero elsewhere.
  * Network in_dim -> H -> H -> 1, ReLU.  Parameters flattened as
  * [W1 (H x in_dim), b1, W2 (H x H), b2, w3 (H), b3].
  */

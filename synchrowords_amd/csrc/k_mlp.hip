@@ -129,10 +129,11 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, in
 
 // Features of set rows S_i (D columns, BF16):
 //   x[perm[q]]      = 1 if state q is in S_i (states in the automaton's canonical order),
-//   x[n + a]        = |delta_a(S_i)| / n for each letter a,
-//   x[n + k + j]    = min over images A of Q of depth <= 2(j+1) of |A \ S_i|, / n (j < 4),
-//   x[n + k + 4]    = |S_i| / n,
-// zero elsewhere.
+//   x[n + a]        = s |delta_a(S_i)| / n for each letter a,
+//   x[n + k + j]    = s min over images A of Q of depth <= 2(j+1) of |A \ S_i| / n (j < 4),
+//   x[n + k + 4]    = s |S_i| / n,
+// zero elsewhere; s = kFeatureScale puts these few count features on a scale
+// that competes with the n bits.
 template <int W>
 __global__ void k_features(const uint64_t* __restrict__ sets, const uint64_t* __restrict__ images, uint64_t count,
     uint32_t n, uint32_t k, uint32_t D, FeatureGuide g, uint16_t* __restrict__ x) {
@@ -161,19 +162,19 @@ __global__ void k_features(const uint64_t* __restrict__ sets, const uint64_t* __
     int c = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) c += __popcll(im[w]);
-    row[n + a] = f2bf(float(c) / float(n));
+    row[n + a] = f2bf(kFeatureScale * float(c) / float(n));
   }
   uint32_t best = n;
   int j = 0;
   for (uint32_t r = 0; r < g.ends[kGuideDepths - 1]; ++r) {
-    while (j < kGuideDepths && r == g.ends[j]) row[n + k + j++] = f2bf(float(best) / float(n));
+    while (j < kGuideDepths && r == g.ends[j]) row[n + k + j++] = f2bf(kFeatureScale * float(best) / float(n));
     uint32_t c = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) c += __popcll(g.rows[uint64_t(r) * W + w] & ~s[w]);
     best = c < best ? c : best;
   }
-  while (j < kGuideDepths) row[n + k + j++] = f2bf(float(best) / float(n));
-  row[n + k + kGuideDepths] = f2bf(float(card) / float(n));
+  while (j < kGuideDepths) row[n + k + j++] = f2bf(kFeatureScale * float(best) / float(n));
+  row[n + k + kGuideDepths] = f2bf(kFeatureScale * float(card) / float(n));
 }
 
 // y[i] = h[i] . w3 + b3 (FP32), one wave per row.

@@ -39,6 +39,7 @@ struct FeatureGuide {
   uint32_t ends[kGuideDepths] = {0, 0, 0, 0};  // rows of depth <= 2, 4, 6, 8
 };
 constexpr uint32_t kRankMaxGroup = 16;
+constexpr float kFeatureScale = 8.f;  // count features are kFeatureScale * count / n
 
 hipError_t launch_gemm(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* Cf,
     uint16_t* Cb, int ldc, int M, int N, int K, const float* bias, int relu, int accumulate, hipStream_t st);

@@ -240,11 +240,11 @@ def guided_features(n, k, delta, sets):
         for q in S:
             x[i, perm[q]] = 0x3F80
         for a in range(k):
-            x[i, n + a] = bf16_bits(np.float32(len({int(d[q, a]) for q in S})) / np.float32(n))
+            x[i, n + a] = bf16_bits(np.float32(8) * np.float32(len({int(d[q, a]) for q in S})) / np.float32(n))
         for j in range(4):
             best = min(len(A - S) for A in rows[:ends[j]])
-            x[i, n + k + j] = bf16_bits(np.float32(best) / np.float32(n))
-        x[i, n + k + 4] = bf16_bits(np.float32(len(S)) / np.float32(n))
+            x[i, n + k + j] = bf16_bits(np.float32(8) * np.float32(best) / np.float32(n))
+        x[i, n + k + 4] = bf16_bits(np.float32(8) * np.float32(len(S)) / np.float32(n))
     return x
 
 
