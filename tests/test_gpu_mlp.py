@@ -269,3 +269,32 @@ def test_samples_come_in_whole_groups(ctx):
     d = gen.random_dfa(n, k, 5)
     sets, dist = sw.mlp_samples(ctx, n, k, d, fdepth=8, width=32, bdepth=30, per_depth=8, cap=1000)
     assert len(sets) > 0 and len(sets) % 8 == 0
+
+
+def test_trained_scorer_gains_over_cardinality_order(ctx):
+    """What the trained network gains: on the held-out automata of the
+    learned-bound benchmark (n=300, k=2, seeds 1000..1019), a device beam of
+    width 64 ordered by the network ends closer to the shortest length than
+    the same beam ordered by cardinality (the reference's order).  Every
+    bound is the length of a word that replays to a singleton."""
+    from pathlib import Path
+    from synchrowords_amd.train_scorer import load_scorer
+    path = Path(sw.__file__).parent / "models" / "scorer_n300_k2.npz"
+    m = load_scorer(ctx, path)
+    n, k, width, cap = 300, 2, 64, 300
+    learned, cardinality = [], []
+    for s in range(1000, 1020):
+        d = sw.random_dfa(n, k, s)
+        bl, wl = m.beam(d, width, cap)
+        bc, wc = ctx.beam(n, k, d, width, cap)
+        for b, w in ((bl, wl), (bc, wc)):
+            assert b < cap and len(w) == b and replay(n, k, d, w) == 1
+        learned.append(bl)
+        cardinality.append(bc)
+    m.close()
+    known = {c["seed"]: c["upper"] for c in golden("c5_random300.json")}
+    for s, b in zip(range(1000, 1020), learned):
+        if s in known:
+            assert b >= known[s]
+    assert sum(learned) < sum(cardinality), (learned, cardinality)
+    assert sum(l < c for l, c in zip(learned, cardinality)) > sum(l > c for l, c in zip(learned, cardinality))
