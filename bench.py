@@ -197,8 +197,11 @@ def bench_learned(ctx, scorer: str, seeds, width: int) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["subset", "batch", "learned"], default="subset",
-                    help="subset: the headline line; batch / learned: the batched and learned-bound reports")
+    ap.add_argument("--workload", choices=["subset", "batch", "learned", "exact"], default="subset",
+                    help="subset: the headline line; batch / learned: the batched and learned-bound reports; "
+                         "exact: one verbose exact search (--exact-n, --exact-seed)")
+    ap.add_argument("--exact-n", type=int, default=300)
+    ap.add_argument("--exact-seed", type=int, default=1)
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--scorer", type=str, default=str(ROOT / "synchrowords_amd" / "models" / "scorer_n300_k2.npz"))
     ap.add_argument("--learned-width", type=int, default=64)
@@ -239,6 +242,12 @@ def main() -> int:
         ctx = sw.Context(0)
         if args.workload == "batch":
             line = bench_batch(ctx, args.batch_count)
+        elif args.workload == "exact":
+            d = sw.random_dfa(args.exact_n, 2, args.exact_seed)
+            ub = ctx.pipeline(args.exact_n, 2, d).beam_bound  # warm; the plan's Beam bound caps the search
+            r = ctx.exact(args.exact_n, 2, d, upper=ub, cfg=sw.ExactConfig(verbose=True))
+            line = {"workload": f"exact search, random binary DFA n={args.exact_n} seed={args.exact_seed}, "
+                                f"upper bound {ub} (Beam)", "length": r.length, "stats": r.stats}
         else:
             line = bench_learned(ctx, args.scorer, list(range(1000, 1000 + args.learned_seeds)), args.learned_width)
         ctx.close()

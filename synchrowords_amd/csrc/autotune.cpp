@@ -86,7 +86,7 @@ int sw_autotune_subset(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_cou
     for (uint32_t threads_i : th)
       for (uint32_t wgs : wg)
         for (uint32_t ee = 0; ee < 2; ++ee) {
-          if (mb == 64 && W > 4) continue;
+          if (mb == 64 && vsub_mask_bits(n, W, 64, false) != 64) continue;
           VsubParams p;
           p.stream = S.as<uint64_t>();
           p.s_count = s_count;
@@ -135,7 +135,8 @@ int sw_set_subset_shape(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_co
     SW_REQUIRE(target_wgs >= 1, "target_wgs must be positive");
     const int W = int((n + 63) / 64);
     const uint32_t mb = early_exit >> 8;
-    SW_REQUIRE(mb == 32 || (mb == 64 && W <= 4), "mask bits must be 32, or 64 for n <= 256");
+    SW_REQUIRE(mb == 32 || (mb == 64 && vsub_mask_bits(n, W, 64, false) == 64),
+        "mask bits must be 32, or 64 when the tile fits LDS (n <= 318)");
     ctx->tune[key(W, s_count, t_count)] = sw_ctx::Tune{threads, target_wgs, early_exit & 1, mb, 0.f};
     return SW_OK;
   } catch (const Error& e) {

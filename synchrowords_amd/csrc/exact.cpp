@@ -127,14 +127,13 @@ class Search {
       else fwd = F_.fcount <= B_.fcount;
       ++t;
       step(fwd ? F_ : B_, !fwd, t);
-      if (cfg_.verbose)
-        std::fprintf(stderr, "[sw exact] len %llu %s |F| %llu |B| %llu rows F %llu B %llu ms %.2f %.2f %.2f\n",
-            (unsigned long long)t, fwd ? "F" : "B", (unsigned long long)F_.fcount,
-            (unsigned long long)B_.fcount, (unsigned long long)F_.rows, (unsigned long long)B_.rows, ms_expand_,
-            ms_dom_, ms_goal_);
       const auto g0 = std::chrono::steady_clock::now();
       const bool met = goal_check(&wit);
-      ms_goal_ += ms_since(g0);
+      const double gms = ms_since(g0);
+      ms_goal_ += gms;
+      if (cfg_.verbose)
+        std::fprintf(stderr, "[sw exact] len %llu goal |F| %llu x |B| %llu  %.3f ms\n", (unsigned long long)t,
+            (unsigned long long)F_.fcount, (unsigned long long)B_.fcount, gms);
       if (met) {
         found = true;
         break;
@@ -308,7 +307,6 @@ class Search {
 
   // One expansion step of side S (backward: complemented sets).
   void step(Side& S, bool backward, uint64_t t) {
-    (void)t;
     const auto t0 = std::chrono::steady_clock::now();
     const uint64_t m = S.fcount;
     const uint64_t cnt = m * k_;
@@ -378,6 +376,11 @@ class Search {
       ms_expand_ += ms_since(t0);
     }
     if (!backward && S.fcount >= forward_cap_) S.frozen = true;
+    if (cfg_.verbose)
+      std::fprintf(stderr, "[sw exact] len %llu %s parents %llu candidates %llu new %llu kept %llu rows %llu  %.3f ms%s\n",
+          (unsigned long long)t, backward ? "B" : "F", (unsigned long long)m, (unsigned long long)cnt,
+          (unsigned long long)nnew, (unsigned long long)S.fcount, (unsigned long long)S.rows, ms_since(t0),
+          dominance ? "" : " (no dominance)");
   }
 
   // Some forward frontier set disjoint from some backward complement.

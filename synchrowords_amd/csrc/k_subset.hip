@@ -23,20 +23,20 @@
 
 namespace sw {
 
-// Mask width of the tile: 64 tile sets per lane (N <= 256, 1 workgroup per
-// CU at N=256) or 32 (any N <= 512, half the LDS, 2 workgroups per CU).
-int vsub_mask_bits(int W, int requested) {
+// Mask width of the tile: 64 tile sets per lane when the transposed tile
+// ((n + 1) rows of 64 lanes x 8 bytes) fits the 160 KB of LDS (n <= 318; one
+// workgroup per CU near the limit), else 32 (any n <= 512, half the LDS).
+int vsub_mask_bits(uint32_t n, int W, int requested, bool proper_by_size) {
+  const size_t lds64 = size_t(n + 1) * 64 * 8 + (proper_by_size ? size_t(64) * 64 * 2 : 0);
+  const bool fits64 = W <= 5 && lds64 <= 160 * 1024;
   if (requested == 32) return 32;
-  if (requested == 64) return W <= 4 ? 64 : 32;
-  return W <= 4 ? 64 : 32;
+  return fits64 ? 64 : 32;
 }
 
-size_t vsub_tile_sets(int W, int mask_bits) { return size_t(64) * vsub_mask_bits(W, mask_bits); }
-
 size_t vsub_lds_bytes(uint32_t n, int W, int mask_bits, bool proper_by_size) {
-  const int mb = vsub_mask_bits(W, mask_bits);
+  const int mb = vsub_mask_bits(n, W, mask_bits, proper_by_size);
   size_t b = size_t(n + 1) * 64 * (mb / 8);
-  if (proper_by_size) b += vsub_tile_sets(W, mask_bits) * sizeof(uint16_t);
+  if (proper_by_size) b += size_t(64) * mb * sizeof(uint16_t);
   return b;
 }
 
@@ -251,7 +251,8 @@ static hipError_t launch_vsub_wm(const VsubParams& p0, hipStream_t st) {
 
 template <int W>
 static hipError_t launch_vsub_w(const VsubParams& p, hipStream_t st) {
-  if (W <= 4 && vsub_mask_bits(W, p.mask_bits) == 64) return launch_vsub_wm<W, uint64_t>(p, st);
+  if (W <= 5 && vsub_mask_bits(p.n, W, p.mask_bits, p.proper_by_size != 0) == 64)
+    return launch_vsub_wm<W, uint64_t>(p, st);
   return launch_vsub_wm<W, uint32_t>(p, st);
 }
 

@@ -22,7 +22,7 @@ struct VsubParams {
   int64_t diag = INT64_MIN;          // skip pairs with i - j == diag
   uint64_t s_chunk = 0;              // stream sets per workgroup row (0: auto)
   uint32_t threads = 1024;           // workgroup size: 256, 512 or 1024
-  int mask_bits = 0;                 // tile sets per lane: 64 (W <= 4), 32, or 0 for the default
+  int mask_bits = 0;                 // tile sets per lane: 64 (when the tile fits LDS), 32, or 0 (widest)
   uint32_t target_wgs = 1024;        // workgroups aimed at when s_chunk is auto
   unsigned long long* count_out = nullptr;  // COUNT: += number of pairs
   int* found = nullptr;                     // ANY: set to 1 on a hit
@@ -31,9 +31,8 @@ struct VsubParams {
   uint32_t* scount = nullptr;  // SCOUNT: += pairs per stream set
 };
 
-// Mask width, tile sets per workgroup and LDS bytes of the subset kernel.
-int vsub_mask_bits(int W, int requested);
-size_t vsub_tile_sets(int W, int mask_bits);
+// Mask width and LDS bytes of the subset kernel.
+int vsub_mask_bits(uint32_t n, int W, int requested, bool proper_by_size);
 size_t vsub_lds_bytes(uint32_t n, int W, int mask_bits, bool proper_by_size);
 // Bytes of the FLAGS bitmap for t tile sets (whole 64-bit words; bit j of
 // the bitmap, read as bytes, is tile set j for either mask width).

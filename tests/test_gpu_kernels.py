@@ -114,7 +114,7 @@ def test_every_launch_shape_gives_the_same_result(n):
     total, cnt, _ = orc.subset_count(small, large, W)
     c = sw.Context(0)
     try:
-        for mb in ([32, 64] if n <= 256 else [32]):
+        for mb in ([32, 64] if n <= 318 else [32]):
             for threads in (256, 512, 1024):
                 for wgs in (1, 7, 256, 4096):
                     for ee in (0, 1):
