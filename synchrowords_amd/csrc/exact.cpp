@@ -350,14 +350,32 @@ class Search {
       flags_.ensure(fb);
       SW_CHECK_HIP(hipMemsetAsync(flags_.p, 0, fb, st_));
       VsubParams p;
-      p.stream = S.arena.as<uint64_t>();
-      p.s_count = S.rows;
-      p.tile = S.arena.as<uint64_t>() + lo * W_;
-      p.t_count = nnew;
       p.n = n_;
-      p.mode = VSUB_FLAGS;
-      p.diag = int64_t(lo);
       p.flags = flags_.p;
+      if (!backward) {
+        // A new image is dropped when some kept image lies inside it: the
+        // kept images stream (they are small sets, the kernel walks their
+        // states), the new level is the tile, flagged per tile set.
+        p.stream = S.arena.as<uint64_t>();
+        p.s_count = S.rows;
+        p.tile = S.arena.as<uint64_t>() + lo * W_;
+        p.t_count = nnew;
+        p.mode = VSUB_FLAGS;
+        p.diag = int64_t(lo);
+      } else {
+        // A new preimage is dropped when it lies inside a kept one.  Rows are
+        // stored complemented, so both lists are complemented on load; the
+        // new level streams (small sets once complemented), flagged per
+        // stream set, against every row of the side as the tile.
+        p.stream = S.arena.as<uint64_t>() + lo * W_;
+        p.s_count = nnew;
+        p.tile = S.arena.as<uint64_t>();
+        p.t_count = S.rows;
+        p.complement_stream = 1;
+        p.complement_tile = 1;
+        p.mode = VSUB_SFLAGS;
+        p.diag = -int64_t(lo);
+      }
       apply_tuning(ctx_, W, p);
       SW_CHECK_HIP(launch_vsub(W, p, st_));
       pair_tests_ += S.rows * nnew;

@@ -133,7 +133,10 @@ __global__ __launch_bounds__(1024) void k_vsub(VsubParams p) {
   for (uint64_t s = s_begin + wave; s < s_end; s += nwaves, ++iter) {
     uint64_t aw[W];
 #pragma unroll
-    for (int w = 0; w < W; ++w) aw[w] = rfl64(nx[w]) & word_mask(n, w);  // states >= n have no row
+    for (int w = 0; w < W; ++w) {
+      const uint64_t x = rfl64(nx[w]);
+      aw[w] = (p.complement_stream ? ~x : x) & word_mask(n, w);  // states >= n have no row
+    }
     {
       const uint64_t sn = s + nwaves;
 #pragma unroll
@@ -197,6 +200,9 @@ __global__ __launch_bounds__(1024) void k_vsub(VsubParams p) {
       }
     } else if (p.mode == VSUB_FLAGS) {
       hits |= r;
+    } else if (p.mode == VSUB_SFLAGS) {
+      if (__builtin_amdgcn_ballot_w64(r != 0) && lane == 0)
+        atomicOr(reinterpret_cast<unsigned long long*>(p.flags) + (s >> 6), 1ull << (s & 63));
     } else if (r) {
       const int b = sizeof(M) == 8 ? __builtin_ctzll(uint64_t(r)) : __builtin_ctz(uint32_t(r));
       const unsigned long long wv = (s << 32) | (tile_base + uint64_t(lane) * MB + b);

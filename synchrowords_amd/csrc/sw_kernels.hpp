@@ -6,7 +6,10 @@
 
 namespace sw {
 
-enum VsubMode { VSUB_COUNT = 0, VSUB_ANY = 1, VSUB_FLAGS = 2, VSUB_SCOUNT = 3 };
+// COUNT: total pairs; ANY: existence + smallest witness; FLAGS: bitmap over
+// tile sets (contains some stream set); SCOUNT: pairs per stream set; SFLAGS:
+// bitmap over stream sets (inside some tile set).
+enum VsubMode { VSUB_COUNT = 0, VSUB_ANY = 1, VSUB_FLAGS = 2, VSUB_SCOUNT = 3, VSUB_SFLAGS = 4 };
 
 // Parameters of the list-vs-list subset kernel (k_subset.hip).
 struct VsubParams {
@@ -16,6 +19,7 @@ struct VsubParams {
   uint64_t t_count = 0;
   uint32_t n = 0;                    // number of states
   int complement_tile = 0;           // test S_i against the complement of T_j
+  int complement_stream = 0;         // use the complement of S_i
   int mode = VSUB_COUNT;
   int early_exit = 1;                // stop a stream set once no lane can hit
   int proper_by_size = 0;            // skip pairs of equal cardinality (S == T)
@@ -27,7 +31,7 @@ struct VsubParams {
   unsigned long long* count_out = nullptr;  // COUNT: += number of pairs
   int* found = nullptr;                     // ANY: set to 1 on a hit
   unsigned long long* witness = nullptr;    // ANY: min of (i << 32) | j
-  void* flags = nullptr;  // FLAGS: bitmap over tile sets (bit j set on a hit)
+  void* flags = nullptr;  // FLAGS / SFLAGS: bitmap over tile / stream sets (bit j set on a hit)
   uint32_t* scount = nullptr;  // SCOUNT: += pairs per stream set
 };
 

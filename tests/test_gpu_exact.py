@@ -79,3 +79,16 @@ def test_cerny64_word(ctx):
     r = ctx.exact(64, 2, d)
     assert r.found and r.length == 63 * 63
     assert replay(64, 2, d, r.word) == 1
+
+
+@pytest.mark.parametrize("n,k", [(40, 3), (90, 2), (90, 3), (150, 2), (260, 2), (300, 2)])
+def test_dominance_pruning_keeps_the_length(ctx, n, k):
+    """Dropping dominated sets (both sides) never changes the shortest length:
+    the search with pruning against the search with pruning switched off."""
+    import synchrowords_amd as sw
+    for seed in range(3):
+        d = gen.random_dfa(n, k, 7000 + seed)
+        off = ctx.exact(n, k, d, cfg=sw.ExactConfig(dominance_pairs=1))
+        on = ctx.exact(n, k, d)
+        assert on.found and off.found and on.length == off.length
+        assert replay(n, k, d, on.word) == 1 and replay(n, k, d, off.word) == 1
