@@ -183,7 +183,7 @@ typedef struct sw_batch_config {
   uint32_t cap_rows;   /* sets per side a workgroup may hold (0: 65536) */
   uint32_t cap_cand;   /* candidates of one step (0: 98304) */
   uint32_t overflow_workers; /* host threads finishing overflowed instances (0: 8) */
-  uint64_t max_pairs;  /* largest goal check done inside a workgroup (0: 2^24) */
+  uint64_t max_pairs;  /* largest goal check done inside a workgroup (0: 2^26) */
 } sw_batch_config;
 
 typedef struct sw_batch_stats {

@@ -173,7 +173,7 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
       uint32_t hs = 1024;
       while (hs < 2 * (L.cap_rows + L.cap_cand)) hs <<= 1;
       L.hash_slots = hs;
-      L.max_pairs = cfg.max_pairs ? cfg.max_pairs : (1ull << 24);
+      L.max_pairs = cfg.max_pairs ? cfg.max_pairs : (1ull << 26);
       const size_t stride = batch_scratch_bytes(int(wmax), L);
       uint32_t grid = cfg.grid ? cfg.grid : uint32_t(ctx->cus) * 4;
       grid = std::min<uint32_t>(grid, uint32_t(inst.size()));
