@@ -35,6 +35,31 @@ METRIC = "time-to-exact reset length (s) + subset checks/s; learned vs plain bou
 HBM_PEAK_GBS = 8000.0
 
 
+LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12  # 256 CUs x 128 B/clk x 2.4 GHz
+
+
+def lds_roofline(small, tile_count: int, kernel_ms: float) -> dict:
+    bits = int(np.unpackbits(small.view(np.uint8)).sum())
+    lds_bytes = bits * tile_count / 8
+    achieved = lds_bytes / (kernel_ms / 1000.0) / 1e12
+    return {"achieved": achieved, "peak": LDS_PEAK_TBS, "unit": "TB/s", "frac": achieved / LDS_PEAK_TBS,
+            "lds_bytes_per_launch": lds_bytes,
+            "note": "peak = 256 CUs x 128 B/clk x 2.4 GHz (planning figure, not measured)"}
+
+
+def measured_traffic(path: str):
+    """HBM bytes per launch of the subset kernel from a committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy of known size),
+    or None when there is none."""
+    try:
+        for line in Path(path).read_text().splitlines():
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def make_lists(count: int, n: int, p_small: float, p_large: float, seed: int):
     """Bernoulli bitsets; bit q of set i is bit q%64 of word q/64."""
     rng = np.random.default_rng(seed)
@@ -214,6 +239,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-exact", action="store_true", help="skip the time-to-exact leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch shape)")
+    ap.add_argument("--traffic-profile", type=str,
+                    default=str(ROOT / "profiles" / "k_vsub_subset_microbench_pmc_traffic.md"),
+                    help="rocprofv3 --pmc summary holding 'traffic_bytes_per_launch: N' for the roofline")
     ap.add_argument("--shape", type=str, default="", metavar="T,G,E,M",
                     help="use this launch shape (threads, target workgroups, early exit, mask bits) instead of tuning")
     args = ap.parse_args()
@@ -274,10 +302,10 @@ def main() -> int:
         tuned = {"threads": th, "target_wgs": wg, "early_exit": ee, "mask_bits": mb, "source": "--shape"}
     elif not args.no_tune:
         # The autotuner times every launch shape on seeded lists of the same
-        # densities (a quarter of the stream list, the full tile list) and
-        # keeps the fastest for this size class in the context.
+        # sizes and densities and keeps the fastest for this size class in
+        # the context.
         t_tune = time.perf_counter()
-        tuned = ctx.autotune_subset(n, max(count // 4, 1), count, 0.08, 0.75, reps=1)
+        tuned = ctx.autotune_subset(n, count, count, 0.08, 0.75, reps=1)
         print(f"[bench] autotuned {tuned} in {time.perf_counter() - t_tune:.1f}s", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
@@ -320,10 +348,13 @@ def main() -> int:
                    "pair_count": counts[0], "exists": bool(anyf), "device": info["name"], "cus": info["cus"],
                    "launch_shape": tuned or "default"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.traffic_profile),
                      "note": "count of 2^40 pairs: integer VALU/LDS bound, not HBM bound; "
                              "achieved = algorithmic bytes (|X|+|Y|)*32 per launch / kernel time"},
         "pair_tests_per_s_kernel": pairs / (kernel_ms / 1000.0),
+        # The bound that applies: each stream state costs one 64-bit LDS row
+        # read per lane, so one launch reads sum(|S|) * |T| / 8 bytes of LDS.
+        "lds_roofline": lds_roofline(small, count, kernel_ms),
     }
     if not args.no_cpu:
         rows = 256 if args.quick else 4096
