@@ -38,13 +38,16 @@ HBM_PEAK_GBS = 8000.0
 LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12  # 256 CUs x 128 B/clk x 2.4 GHz
 
 
-def lds_roofline(small, tile_count: int, kernel_ms: float) -> dict:
+def lds_roofline(ctx, small, tile_count: int, kernel_ms: float) -> dict:
     bits = int(np.unpackbits(small.view(np.uint8)).sum())
     lds_bytes = bits * tile_count / 8
     achieved = lds_bytes / (kernel_ms / 1000.0) / 1e12
-    return {"achieved": achieved, "peak": LDS_PEAK_TBS, "unit": "TB/s", "frac": achieved / LDS_PEAK_TBS,
+    measured = {b: ctx.calibrate_lds(b) for b in (8, 16)}
+    return {"achieved": achieved, "peak": measured[8], "unit": "TB/s", "frac": achieved / measured[8],
             "lds_bytes_per_launch": lds_bytes,
-            "note": "peak = 256 CUs x 128 B/clk x 2.4 GHz (planning figure, not measured)"}
+            "peak_measured_b64": measured[8], "peak_measured_b128": measured[16], "peak_planning": LDS_PEAK_TBS,
+            "note": "peak = measured LDS read rate with the kernel's own 8-byte row reads (sw_calibrate_lds); "
+                    "planning figure 256 CUs x 128 B/clk x 2.4 GHz"}
 
 
 def measured_traffic(path: str):
@@ -354,7 +357,7 @@ def main() -> int:
         "pair_tests_per_s_kernel": pairs / (kernel_ms / 1000.0),
         # The bound that applies: each stream state costs one 64-bit LDS row
         # read per lane, so one launch reads sum(|S|) * |T| / 8 bytes of LDS.
-        "lds_roofline": lds_roofline(small, count, kernel_ms),
+        "lds_roofline": lds_roofline(ctx, small, count, kernel_ms),
     }
     if not args.no_cpu:
         rows = 256 if args.quick else 4096

@@ -63,6 +63,11 @@ int sw_sync(sw_ctx* ctx);
 /* Device-to-device copy by a plain streaming kernel (bytes % 16 == 0): the
  * known-size kernel that calibrates hardware traffic counters. */
 int sw_dev_copy(sw_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Measured LDS read bandwidth of the device (TB/s): every CU reads whole
+ * 64-lane rows of bytes_per_lane (8 or 16) bytes per lane, eight reads in
+ * flight, for a few milliseconds.  The peak the subset kernel's LDS roofline is
+ * quoted against.  No reference counterpart. */
+int sw_calibrate_lds(sw_ctx* ctx, uint32_t bytes_per_lane, double* tb_per_s);
 /* HIP events on the context's stream, for timing work launched by later
  * calls: op 0 records the start event; op 1 records the stop event, waits for
  * it and writes the elapsed milliseconds to *ms. */

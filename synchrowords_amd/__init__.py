@@ -122,6 +122,7 @@ _SIGS = {
                                  _U64P]),
     "sw_autotune_subset": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint64, C.c_float, C.c_float, C.c_uint32, _U32P,
                                      _U32P, _U32P, C.POINTER(C.c_float)]),
+    "sw_calibrate_lds": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_double)]),
     "sw_set_subset_shape": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
 }
 
@@ -445,6 +446,12 @@ class Context:
                                          C.byref(wg), C.byref(ee), C.byref(ms)))
         return {"threads": th.value, "target_wgs": wg.value, "early_exit": ee.value & 255,
                 "mask_bits": ee.value >> 8, "ms": ms.value}
+
+    def calibrate_lds(self, bytes_per_lane: int = 8) -> float:
+        """Measured LDS read bandwidth of the device, TB/s."""
+        v = C.c_double()
+        _check(load().sw_calibrate_lds(self.handle, bytes_per_lane, C.byref(v)))
+        return v.value
 
     def set_subset_shape(self, n: int, s_count: int, t_count: int, threads: int, target_wgs: int,
                          early_exit: int, mask_bits: int):

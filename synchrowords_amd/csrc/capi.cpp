@@ -170,6 +170,31 @@ int sw_dev_copy(sw_ctx* ctx, void* dst, const void* src, size_t bytes) {
   SW_API_END
 }
 
+int sw_calibrate_lds(sw_ctx* ctx, uint32_t bytes_per_lane, double* tb_per_s) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx && tb_per_s, "NULL argument");
+  SW_REQUIRE(bytes_per_lane == 8 || bytes_per_lane == 16, "bytes_per_lane must be 8 or 16");
+  sw::use(ctx);
+  const uint32_t blocks = uint32_t(ctx->cus) * 2;  // 64 KB each: two workgroups per CU
+  const uint64_t iters = 20000;
+  ctx->d.ensure(size_t(blocks) * 4 + 8);
+  hipEvent_t e0, e1;
+  SW_CHECK_HIP(hipEventCreate(&e0));
+  SW_CHECK_HIP(hipEventCreate(&e1));
+  SW_CHECK_HIP(sw::launch_lds_read(int(bytes_per_lane), iters / 10, blocks, ctx->d.as<uint32_t>(), ctx->stream));
+  SW_CHECK_HIP(hipEventRecord(e0, ctx->stream));
+  SW_CHECK_HIP(sw::launch_lds_read(int(bytes_per_lane), iters, blocks, ctx->d.as<uint32_t>(), ctx->stream));
+  SW_CHECK_HIP(hipEventRecord(e1, ctx->stream));
+  SW_CHECK_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  SW_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const double bytes = double(blocks) * 16.0 /* waves */ * double(iters) * 8.0 * 64.0 * double(bytes_per_lane);
+  *tb_per_s = bytes / (double(ms) * 1e-3) / 1e12;
+  SW_API_END
+}
+
 int sw_stream_timer(sw_ctx* ctx, int op, float* ms) {
   SW_API_BEGIN
   SW_REQUIRE(ctx, "ctx is NULL");

@@ -62,7 +62,9 @@ __device__ inline uint32_t pop_low(uint64_t& bits) {
   return q;
 }
 
-template <int W, typename M>
+// MODE is a template parameter: a runtime mode costs scalar tests and
+// branches on every stream set.
+template <int W, typename M, int MODE>
 __global__ __launch_bounds__(1024) void k_vsub(VsubParams p) {
   constexpr int MB = sizeof(M) * 8;
   constexpr int TILE = 64 * MB;
@@ -123,26 +125,26 @@ __global__ __launch_bounds__(1024) void k_vsub(VsubParams p) {
   uint32_t cnt = 0;
   M hits = 0;
   uint32_t iter = 0;
-  // The next stream set is loaded while the current one is walked.
+  // The next stream set is loaded while the current one is walked; the index
+  // is clamped instead of tested, so the load has no branch.
+  const uint64_t cmask = p.complement_stream ? ~0ull : 0ull;
+  const uint64_t s_last = s_end > s_begin ? s_end - 1 : s_begin;
   uint64_t nx[W];
   {
-    const uint64_t s0 = s_begin + wave;
+    const uint64_t s0 = s_begin + wave < s_last ? s_begin + wave : s_last;
 #pragma unroll
-    for (int w = 0; w < W; ++w) nx[w] = s0 < s_end ? p.stream[s0 * W + w] : 0ull;
+    for (int w = 0; w < W; ++w) nx[w] = p.stream[s0 * W + w];
   }
   for (uint64_t s = s_begin + wave; s < s_end; s += nwaves, ++iter) {
     uint64_t aw[W];
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-      const uint64_t x = rfl64(nx[w]);
-      aw[w] = (p.complement_stream ? ~x : x) & word_mask(n, w);  // states >= n have no row
-    }
+    for (int w = 0; w < W; ++w) aw[w] = (rfl64(nx[w]) ^ cmask) & word_mask(n, w);  // states >= n have no row
     {
-      const uint64_t sn = s + nwaves;
+      const uint64_t sn = s + nwaves < s_last ? s + nwaves : s_last;
 #pragma unroll
-      for (int w = 0; w < W; ++w) nx[w] = sn < s_end ? p.stream[sn * W + w] : 0ull;
+      for (int w = 0; w < W; ++w) nx[w] = p.stream[sn * W + w];
     }
-    if (p.mode == VSUB_ANY && (iter & 15) == 0) {
+    if (MODE == VSUB_ANY && (iter & 15) == 0) {
       if (rfl(__hip_atomic_load(p.found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
     }
     M r = valid;
@@ -189,18 +191,18 @@ __global__ __launch_bounds__(1024) void k_vsub(VsubParams p) {
         if (tsize[lane * MB + b] == c) r &= ~(M(1) << b);
       }
     }
-    if (p.mode == VSUB_COUNT) {
+    if (MODE == VSUB_COUNT) {
       cnt += mpopc(r);
-    } else if (p.mode == VSUB_SCOUNT) {
+    } else if (MODE == VSUB_SCOUNT) {
       // Per stream set: pairs summed over the wave's lanes.
       if (__builtin_amdgcn_ballot_w64(r != 0)) {
         uint32_t c = mpopc(r);
         for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
         if (lane == 0) atomicAdd(&p.scount[s], c);
       }
-    } else if (p.mode == VSUB_FLAGS) {
+    } else if (MODE == VSUB_FLAGS) {
       hits |= r;
-    } else if (p.mode == VSUB_SFLAGS) {
+    } else if (MODE == VSUB_SFLAGS) {
       if (__builtin_amdgcn_ballot_w64(r != 0) && lane == 0)
         atomicOr(reinterpret_cast<unsigned long long*>(p.flags) + (s >> 6), 1ull << (s & 63));
     } else if (r) {
@@ -211,16 +213,30 @@ __global__ __launch_bounds__(1024) void k_vsub(VsubParams p) {
     }
   }
 
-  if (p.mode == VSUB_COUNT) {
+  if (MODE == VSUB_COUNT) {
     uint64_t c = cnt;
     for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
     if (lane == 0 && c) atomicAdd(p.count_out, (unsigned long long)c);
-  } else if (p.mode == VSUB_FLAGS) {
+  } else if (MODE == VSUB_FLAGS) {
     if (hits) {
       M* f = reinterpret_cast<M*>(p.flags);
       atomicOr(&f[tile_base / MB + lane], hits);
     }
   }
+}
+
+template <int W, typename M, int MODE>
+static hipError_t launch_vsub_wmm(const VsubParams& p, uint64_t gx, uint64_t gy, size_t lds, hipStream_t st) {
+  // Idempotent, so concurrent first calls from several host threads are harmless.
+  static std::atomic<bool> attr_set{false};
+  if (!attr_set.load(std::memory_order_acquire)) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vsub<W, M, MODE>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    attr_set.store(true, std::memory_order_release);
+  }
+  hipLaunchKernelGGL((k_vsub<W, M, MODE>), dim3(unsigned(gx), unsigned(gy)), dim3(p.threads), lds, st, p);
+  return hipGetLastError();
 }
 
 template <int W, typename M>
@@ -243,16 +259,14 @@ static hipError_t launch_vsub_wm(const VsubParams& p0, hipStream_t st) {
   if (gx > 0x7fffffffull || gy > 65535ull) return hipErrorInvalidValue;
   const size_t lds = vsub_lds_bytes(p.n, W, MB, p.proper_by_size != 0);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  // Idempotent, so concurrent first calls from several host threads are harmless.
-  static std::atomic<bool> attr_set{false};
-  if (!attr_set.load(std::memory_order_acquire)) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vsub<W, M>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    if (e != hipSuccess) return e;
-    attr_set.store(true, std::memory_order_release);
+  switch (p.mode) {
+    case VSUB_COUNT: return launch_vsub_wmm<W, M, VSUB_COUNT>(p, gx, gy, lds, st);
+    case VSUB_ANY: return launch_vsub_wmm<W, M, VSUB_ANY>(p, gx, gy, lds, st);
+    case VSUB_FLAGS: return launch_vsub_wmm<W, M, VSUB_FLAGS>(p, gx, gy, lds, st);
+    case VSUB_SCOUNT: return launch_vsub_wmm<W, M, VSUB_SCOUNT>(p, gx, gy, lds, st);
+    case VSUB_SFLAGS: return launch_vsub_wmm<W, M, VSUB_SFLAGS>(p, gx, gy, lds, st);
   }
-  hipLaunchKernelGGL((k_vsub<W, M>), dim3(unsigned(gx), unsigned(gy)), dim3(p.threads), lds, st, p);
-  return hipGetLastError();
+  return hipErrorInvalidValue;
 }
 
 template <int W>

@@ -92,6 +92,9 @@ hipError_t launch_count_bits(int W, const uint64_t* sets, uint64_t count, uint32
     hipStream_t st);
 // Streaming copy of bytes/16 16-byte vectors (traffic calibration).
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t st);
+// LDS read calibration: `blocks` workgroups of 1024 threads, each wave reading
+// iters * 8 rows of 64 lanes x bytes_per_lane (8 or 16) bytes.
+hipError_t launch_lds_read(int bytes_per_lane, uint64_t iters, uint32_t blocks, uint32_t* out, hipStream_t st);
 // Fill 32-bit words.
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t count, hipStream_t st);
 
