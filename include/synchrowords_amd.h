@@ -232,6 +232,14 @@ int sw_autotune_subset(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_cou
 int sw_set_subset_shape(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_count, uint32_t threads,
     uint32_t target_wgs, uint32_t early_exit);
 
+/* Which subset kernel the context's later calls launch (exact search, seams):
+ * with indexed != 0 (the default), launches of at least min_pairs pairs
+ * (0: 2^30) list the states of every stream set once and use the indexed
+ * kernel (two stream sets per wave, 128-bit LDS rows); the others, and all
+ * launches when indexed == 0, walk the bits of each stream set per tile.
+ * Both give the same results.  No reference counterpart. */
+int sw_set_subset_kernel(sw_ctx* ctx, int indexed, uint64_t min_pairs);
+
 /* -------------------------------------------------------- learned upper bound
  * A BF16 MLP (MFMA, FP32 accumulation) that scores beam candidates; lower
  * scores are kept.  No reference counterpart: it replaces the cardinality

@@ -122,6 +122,7 @@ _SIGS = {
                                  _U64P]),
     "sw_autotune_subset": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint64, C.c_float, C.c_float, C.c_uint32, _U32P,
                                      _U32P, _U32P, C.POINTER(C.c_float)]),
+    "sw_set_subset_kernel": (C.c_int, [_P, C.c_int, C.c_uint64]),
     "sw_calibrate_lds": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_double)]),
     "sw_set_subset_shape": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
 }
@@ -446,6 +447,10 @@ class Context:
                                          C.byref(wg), C.byref(ee), C.byref(ms)))
         return {"threads": th.value, "target_wgs": wg.value, "early_exit": ee.value & 255,
                 "mask_bits": ee.value >> 8, "ms": ms.value}
+
+    def set_subset_kernel(self, indexed: bool = True, min_pairs: int = 0) -> None:
+        """Indexed subset kernel for launches of at least min_pairs pairs (0: 2^30), or never."""
+        _check(load().sw_set_subset_kernel(self.handle, int(indexed), min_pairs))
 
     def calibrate_lds(self, bytes_per_lane: int = 8) -> float:
         """Measured LDS read bandwidth of the device, TB/s."""

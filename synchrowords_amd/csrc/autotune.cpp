@@ -145,4 +145,20 @@ int sw_set_subset_shape(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_co
   }
 }
 
+// Which subset kernel later launches use: the indexed one (k_vsub_ix) for
+// launches of at least min_pairs pairs when `indexed`, else always the
+// bit-walking one (k_vsub).
+int sw_set_subset_kernel(sw_ctx* ctx, int indexed, uint64_t min_pairs) {
+  using namespace sw;
+  try {
+    SW_REQUIRE(ctx, "NULL argument");
+    ctx->ix_enabled = indexed ? 1 : 0;
+    ctx->ix_min_pairs = min_pairs ? min_pairs : (1ull << 30);
+    return SW_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  }
+}
+
 }  // extern "C"

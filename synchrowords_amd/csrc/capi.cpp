@@ -102,6 +102,8 @@ void sw_ctx_destroy(sw_ctx* ctx) {
   ctx->e.release();
   if (ctx->exact_ws && ctx->exact_ws_free) ctx->exact_ws_free(ctx->exact_ws);
   ctx->exact_ws = nullptr;
+  if (ctx->ix_host) (void)hipHostFree(ctx->ix_host);
+  ctx->ix_host = nullptr;
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
   (void)hipStreamDestroy(ctx->stream);
@@ -367,7 +369,7 @@ int sw_subset_check(sw_ctx* ctx, uint32_t n, const uint64_t* small, uint64_t sma
     p.mode = sw::VSUB_SCOUNT;
     p.scount = ctx->d.as<uint32_t>();
     sw::apply_tuning(ctx, int(W), p);
-    SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
+    sw::run_subset(ctx, int(W), p);
     std::unique_ptr<uint32_t[]> c(new uint32_t[small_cnt + 1]);
     if (small_cnt)
       SW_CHECK_HIP(hipMemcpyAsync(c.get(), ctx->d.p, small_cnt * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -381,7 +383,7 @@ int sw_subset_check(sw_ctx* ctx, uint32_t n, const uint64_t* small, uint64_t sma
     p.mode = sw::VSUB_COUNT;
     p.count_out = dcount;
     sw::apply_tuning(ctx, int(W), p);
-    SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
+    sw::run_subset(ctx, int(W), p);
     unsigned long long c = 0;
     SW_CHECK_HIP(hipMemcpyAsync(&c, dcount, 8, hipMemcpyDeviceToHost, ctx->stream));
     SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
@@ -392,7 +394,7 @@ int sw_subset_check(sw_ctx* ctx, uint32_t n, const uint64_t* small, uint64_t sma
     p.found = dfound;
     p.witness = dwit;
     sw::apply_tuning(ctx, int(W), p);
-    SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
+    sw::run_subset(ctx, int(W), p);
     SW_CHECK_HIP(hipMemcpyAsync(&found, dfound, 4, hipMemcpyDeviceToHost, ctx->stream));
     SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   }

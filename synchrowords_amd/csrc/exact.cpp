@@ -377,7 +377,7 @@ class Search {
         p.diag = -int64_t(lo);
       }
       apply_tuning(ctx_, W, p);
-      SW_CHECK_HIP(launch_vsub(W, p, st_));
+      run_subset(ctx_, W, p);
       pair_tests_ += S.rows * nnew;
       SW_CHECK_HIP(launch_alive(W, flags_.p, nnew, keep_.as<uint32_t>(), st_));
       ensure_scan(nnew);
@@ -419,7 +419,7 @@ class Search {
     p.found = found;
     p.witness = wit;
     apply_tuning(ctx_, int(W_), p);
-    SW_CHECK_HIP(launch_vsub(int(W_), p, st_));
+    run_subset(ctx_, int(W_), p);
     pair_tests_ += F_.fcount * B_.fcount;
     // found (4 bytes at +8) and witness (8 bytes at +16) in one copy
     SW_CHECK_HIP(hipMemcpyAsync(ws_.host, small_.as<char>() + 8, 16, hipMemcpyDeviceToHost, st_));

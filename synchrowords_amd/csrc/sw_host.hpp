@@ -89,6 +89,13 @@ struct sw_ctx {
   // Buffers of the device exact search, reused between searches.
   void* exact_ws = nullptr;
   void (*exact_ws_free)(void*) = nullptr;
+  // Stream index of the indexed subset kernel, reused between launches, and
+  // when it is used: never (0), or for launches of at least ix_min_pairs pairs.
+  sw::DevBuf ix_padded, ix_size, ix_start, ix_list, ix_tmp;
+  size_t ix_tmp_bytes = 0;
+  uint32_t* ix_host = nullptr;  // pinned: the index length read back
+  int ix_enabled = 1;
+  uint64_t ix_min_pairs = 1ull << 30;
 };
 
 namespace sw {
@@ -100,6 +107,11 @@ struct VsubParams;
 // Fill the launch shape of a subset-kernel call from the context's tuning
 // table (the nearest tuned size class of the same W), if any.
 void apply_tuning(sw_ctx* ctx, int W, VsubParams& p);
+
+// Launch a subset test on the context's stream: the indexed kernel when it
+// is enabled, fits and the launch is large enough (building the stream index
+// first), else the bit-walking kernel.
+void run_subset(sw_ctx* ctx, int W, VsubParams& p);
 
 // Reference-layout slice tables for every letter: [k][ceil(n/8)][256][W]
 // words; dir 0 = image, 1 = preimage.
