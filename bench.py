@@ -242,6 +242,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-exact", action="store_true", help="skip the time-to-exact leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch shape)")
+    ap.add_argument("--subset-kernel", choices=["auto", "indexed", "bitwalk"], default="auto",
+                    help="which subset kernel the microbench launches (auto: indexed for large launches)")
     ap.add_argument("--traffic-profile", type=str,
                     default=str(ROOT / "profiles" / "k_vsub_subset_microbench_pmc_traffic.md"),
                     help="rocprofv3 --pmc summary holding 'traffic_bytes_per_launch: N' for the roofline")
@@ -293,6 +295,8 @@ def main() -> int:
     print(f"[bench] generated {count} x {count} sets over n={n} in {t_gen:.1f}s", file=sys.stderr, flush=True)
 
     ctx = sw.Context(0)
+    if args.subset_kernel != "auto":
+        ctx.set_subset_kernel(args.subset_kernel == "indexed", 1)
     info = ctx.info()
     ds = sw.DeviceArray.from_host(ctx, small)
     dl = sw.DeviceArray.from_host(ctx, large)
@@ -349,7 +353,7 @@ def main() -> int:
         "config": {"workload": f"subset-check microbench: {count} x {count} sets, n={n} (W=4), p=0.08/0.75, "
                                f"1 in 1e5 planted",
                    "pair_count": counts[0], "exists": bool(anyf), "device": info["name"], "cus": info["cus"],
-                   "launch_shape": tuned or "default"},
+                   "launch_shape": tuned or "default", "subset_kernel": args.subset_kernel},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.traffic_profile),
                      "note": "count of 2^40 pairs: integer VALU/LDS bound, not HBM bound; "
