@@ -5,20 +5,22 @@
 // mask g says "state q is in tile set 64 g + b".  Two things differ.
 //
 // * The states of every stream set are listed once, by a pre-pass
-//   (k_ix_count, an exclusive scan, k_ix_fill), as the LDS byte offsets of
-//   their rows, padded to a multiple of four with the all-ones padding row.
-//   k_vsub finds the states of each stream set on the scalar unit (s_ff1 and
-//   s_bitset0 per state, plus per-set control), once per tile, and that unit
-//   is its busiest; here the walk is vector loads of offsets and LDS reads.
-// * A wave takes two stream sets at once, one per half-wave.  Lane L of a
-//   half holds 128 tile sets (masks 2L and 2L+1 of the row) and reads its 16
-//   bytes of row q with one ds_read_b128, so one LDS instruction serves both
-//   sets; gfx950 reads 128-bit lanes at twice the rate of 64-bit ones.
+//   (k_ix_pair_count, an exclusive scan, k_ix_pair_fill), as the LDS byte
+//   offsets of their rows.  k_vsub finds the states of each stream set on the
+//   scalar unit (s_ff1 and s_bitset0 per state, plus per-set control), once
+//   per tile, and that unit is its busiest; here the walk is vector loads of
+//   offsets and LDS reads.
+// * A wave takes two stream sets at once — the pair (2i, 2i+1) — one per
+//   half-wave.  Lane L of a half holds 128 tile sets (masks 2L and 2L+1 of the
+//   row) and reads its 16 bytes of row q with one ds_read_b128, so one LDS
+//   instruction serves both sets; gfx950 reads 128-bit lanes at twice the
+//   rate of 64-bit ones.
 //
-// A half whose set has fewer states reads the padding row (all ones) for the
-// rest of the longer set's walk.  Offsets of up to 32 states per set are held
-// in registers and loaded one pair of sets ahead; longer sets load the rest as
-// they go.
+// Index layout, per pair: G groups of 8 offsets, group k = 4 offsets of the
+// first set then 4 of the second, so lane L of half h loads its group as one
+// uint4 at entry 8k + 4h.  G is the larger set's state count rounded up to 4,
+// over 4; the shorter set is padded with the all-ones padding row.  A pair's
+// groups (up to kIxPrefetch) are loaded while the previous pair is walked.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -51,39 +53,53 @@ __device__ inline const uint4& row16(const unsigned char* smem, uint32_t byte) {
   return *reinterpret_cast<const uint4*>(smem + byte);
 }
 
-// Per-set state counts: padded[i] = |S_i| rounded up to a multiple of 4,
-// size[i] = |S_i| (S_i complemented first when cmask is all ones).
+// Per pair i of stream sets (2i, 2i+1; the second may not exist): size[s] =
+// |S_s| (complemented first when cmask is all ones) and entries[i] = 8 G, G =
+// max of the two sizes rounded up to 4, over 4.
 template <int W>
-__global__ void k_ix_count(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n, uint64_t cmask,
-    uint32_t* __restrict__ padded, uint32_t* __restrict__ size) {
+__global__ void k_ix_pair_count(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n, uint64_t cmask,
+    uint32_t* __restrict__ size, uint32_t* __restrict__ entries) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  uint32_t c = 0;
+  const uint64_t pairs = (count + 1) / 2;
+  if (i >= pairs) return;
+  uint32_t c[2] = {0u, 0u};
+  for (int h = 0; h < 2; ++h) {
+    const uint64_t s = 2 * i + h;
+    if (s >= count) break;
 #pragma unroll
-  for (int w = 0; w < W; ++w) c += __popcll((sets[i * W + w] ^ cmask) & word_mask(n, w));
-  padded[i] = (c + 3u) & ~3u;
-  size[i] = c;
+    for (int w = 0; w < W; ++w) c[h] += __popcll((sets[s * W + w] ^ cmask) & word_mask(n, w));
+    size[s] = c[h];
+  }
+  const uint32_t g = (max(c[0], c[1]) + 3u) / 4u;
+  entries[i] = 8u * g;
 }
 
-// list[start[i] ...]: row byte offset q * 512 of every state q of S_i, then
-// the padding row's offset n * 512 up to start[i + 1].
+// list[start[i] + 8k + 4h + j] = row byte offset (q * 512) of the (4k + j)-th
+// state of set 2i + h; the padding row's offset (n * 512) past its end.
 template <int W>
-__global__ void k_ix_fill(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n, uint64_t cmask,
+__global__ void k_ix_pair_fill(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n, uint64_t cmask,
     const uint32_t* __restrict__ start, uint32_t* __restrict__ list) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  uint32_t o = start[i];
-  const uint32_t end = start[i + 1];
+  const uint64_t s = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t pairs = (count + 1) / 2;
+  if (s >= 2 * pairs) return;
+  const uint64_t i = s / 2;
+  const uint32_t h = uint32_t(s & 1);
+  uint32_t* out = list + start[i] + 4 * h;
+  const uint32_t slots = (start[i + 1] - start[i]) / 2;  // 4 G
+  uint32_t j = 0;
+  if (s < count) {
 #pragma unroll
-  for (int w = 0; w < W; ++w) {
-    uint64_t b = (sets[i * W + w] ^ cmask) & word_mask(n, w);
-    while (b) {
-      const uint32_t q = uint32_t(w) * 64 + __builtin_ctzll(b);
-      b &= b - 1;
-      list[o++] = q * 512u;
+    for (int w = 0; w < W; ++w) {
+      uint64_t b = (sets[s * W + w] ^ cmask) & word_mask(n, w);
+      while (b) {
+        const uint32_t q = uint32_t(w) * 64 + __builtin_ctzll(b);
+        b &= b - 1;
+        out[8 * (j >> 2) + (j & 3)] = q * 512u;
+        ++j;
+      }
     }
   }
-  while (o < end) list[o++] = n * 512u;
+  for (; j < slots; ++j) out[8 * (j >> 2) + (j & 3)] = n * 512u;
 }
 
 template <int W, int MODE>
@@ -136,57 +152,39 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
     }
   }
 
-  const uint64_t s_begin = uint64_t(blockIdx.y) * p.s_chunk;
+  const uint64_t s_begin = uint64_t(blockIdx.y) * p.s_chunk;  // even
   uint64_t s_end = s_begin + p.s_chunk;
   if (s_end > p.s_count) s_end = p.s_count;
+  const uint64_t p_end = (s_end + 1) / 2;  // pairs [s_begin / 2, p_end)
   const uint32_t* __restrict__ start = p.ix_start;
   const uint4* __restrict__ list4 = reinterpret_cast<const uint4*>(p.ix_list);
-  const uint32_t padoff = n * 512u;
-  const uint4 pad4 = make_uint4(padoff, padoff, padoff, padoff);
   const uint32_t lane_off = uint32_t(L) * 16u;
 
   uint64_t cnt = 0;
   R4 hits = {{0u, 0u, 0u, 0u}};
   uint32_t iter = 0;
 
-  // Pair k of this wave: stream sets s0 = s_begin + 2 (wave + k nwaves) and s0 + 1.
-  const uint64_t step = 2ull * uint64_t(nwaves);
-  uint64_t s0 = s_begin + 2ull * uint64_t(wave);
-  // Offsets of the current pair (cur) and start/count of the next one.
-  uint4 cur[kIxPrefetch];
-  uint32_t st = 0, c = 0;
-  {
-    const uint64_t sh = s0 + half;
-    if (sh < s_end) {
-      st = start[sh];
-      c = start[sh + 1] - st;
+  // Walk one pair whose first groups are in `cur` (G groups from entry e0),
+  // while the next pair's first groups load into `nxt`.
+  auto walk = [&](uint64_t pi, uint32_t e0, uint32_t G, uint4 (&cur)[kIxPrefetch], uint64_t pn, uint32_t& en,
+                  uint32_t& Gn, uint4 (&nxt)[kIxPrefetch]) {
+    if (pn < p_end) {
+      en = rfl(start[pn]);
+      Gn = (rfl(start[pn + 1]) - en) / 8u;
+    } else {
+      en = 0;
+      Gn = 0;
     }
 #pragma unroll
-    for (int k = 0; k < kIxPrefetch; ++k) cur[k] = uint32_t(4 * k) < c ? list4[(st >> 2) + k] : pad4;
-  }
-  for (; s0 < s_end; s0 += step, ++iter) {
-    const uint64_t sh = s0 + half;
+    for (int k = 0; k < kIxPrefetch; ++k)
+      if (uint32_t(k) < Gn) nxt[k] = list4[(en >> 2) + 2 * k + half];
+    const uint64_t sh = 2 * pi + half;
     const bool live = sh < s_end;
-    // Next pair: start and count, then its first offsets, issued now.
-    const uint64_t shn = sh + step;
-    uint32_t stn = 0, cn = 0;
-    if (shn < s_end) {
-      stn = start[shn];
-      cn = start[shn + 1] - stn;
-    }
-    uint4 nxt[kIxPrefetch];
-#pragma unroll
-    for (int k = 0; k < kIxPrefetch; ++k) nxt[k] = uint32_t(4 * k) < cn ? list4[(stn >> 2) + k] : pad4;
-
-    if (MODE == VSUB_ANY && (iter & 15) == 0) {
-      if (rfl(__hip_atomic_load(p.found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
-    }
-    const uint32_t cmax = max(__builtin_amdgcn_readlane(c, 0), __builtin_amdgcn_readlane(c, 32));
     R4 r = valid;
     if (!live) r = R4{{0u, 0u, 0u, 0u}};
 #pragma unroll
     for (int k = 0; k < kIxPrefetch; ++k) {
-      if (uint32_t(4 * k) >= cmax) break;
+      if (uint32_t(k) >= G) break;
       const uint4 o = cur[k];
       const uint4 m0 = row16(smem, o.x + lane_off);
       const uint4 m1 = row16(smem, o.y + lane_off);
@@ -195,8 +193,8 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
       fold2(r, m0, m1);
       fold2(r, m2, m3);
     }
-    for (uint32_t i = 4 * kIxPrefetch; i < cmax; i += 4) {  // sets of more than 32 states
-      const uint4 o = i < c ? list4[(st + i) >> 2] : pad4;
+    for (uint32_t k = kIxPrefetch; k < G; ++k) {  // sets of more than 4 kIxPrefetch states
+      const uint4 o = list4[(e0 >> 2) + 2 * k + half];
       const uint4 m0 = row16(smem, o.x + lane_off);
       const uint4 m1 = row16(smem, o.y + lane_off);
       const uint4 m2 = row16(smem, o.z + lane_off);
@@ -250,10 +248,29 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
         __hip_atomic_store(p.found, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
+  };
+
+  // Pairs of this wave: s_begin / 2 + wave, then every nwaves-th.  Two
+  // register buffers take turns (no copies).
+  uint4 bufA[kIxPrefetch], bufB[kIxPrefetch];
+  uint64_t pi = s_begin / 2 + uint64_t(wave);
+  uint32_t eA = 0, GA = 0, eB = 0, GB = 0;
+  if (pi < p_end) {
+    eA = rfl(start[pi]);
+    GA = (rfl(start[pi + 1]) - eA) / 8u;
 #pragma unroll
-    for (int k = 0; k < kIxPrefetch; ++k) cur[k] = nxt[k];
-    st = stn;
-    c = cn;
+    for (int k = 0; k < kIxPrefetch; ++k)
+      if (uint32_t(k) < GA) bufA[k] = list4[(eA >> 2) + 2 * k + half];
+  }
+  while (pi < p_end) {
+    if (MODE == VSUB_ANY && (iter++ & 7) == 0) {
+      if (rfl(__hip_atomic_load(p.found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
+    }
+    walk(pi, eA, GA, bufA, pi + nwaves, eB, GB, bufB);
+    pi += nwaves;
+    if (pi >= p_end) break;
+    walk(pi, eB, GB, bufB, pi + nwaves, eA, GA, bufA);
+    pi += nwaves;
   }
 
   if (MODE == VSUB_COUNT) {
@@ -296,8 +313,9 @@ hipError_t launch_ix_w(const VsubParams& p0, hipStream_t st) {
     const uint64_t gyt = (target + gx - 1) / gx;
     uint64_t chunk = (p.s_count + gyt - 1) / gyt;
     if (chunk < 128) chunk = 128;
-    p.s_chunk = (chunk + 1) & ~1ull;  // pairs never straddle two workgroups
+    p.s_chunk = chunk;
   }
+  p.s_chunk = (p.s_chunk + 1) & ~1ull;  // even: pairs never straddle two workgroups
   const uint64_t gy = (p.s_count + p.s_chunk - 1) / p.s_chunk;
   if (gx > 0x7fffffffull || gy > 65535ull) return hipErrorInvalidValue;
   const size_t lds = size_t(p.n + 1) * 512 + (p.proper_by_size ? size_t(kIxTile) * 2 : 0);
@@ -318,13 +336,13 @@ bool vsub_ix_fits(uint32_t n, int W, bool proper_by_size) {
   return W >= 1 && W <= 5 && size_t(n + 1) * 512 + (proper_by_size ? size_t(kIxTile) * 2 : 0) <= 160 * 1024;
 }
 
-hipError_t launch_ix_count(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement, uint32_t* padded,
-    uint32_t* size, hipStream_t st) {
+hipError_t launch_ix_pair_count(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement,
+    uint32_t* size, uint32_t* entries, hipStream_t st) {
   if (!count) return hipSuccess;
   const uint64_t cmask = complement ? ~0ull : 0ull;
-  const unsigned blocks = unsigned((count + 255) / 256);
+  const unsigned blocks = unsigned(((count + 1) / 2 + 255) / 256);
   switch (W) {
-#define C(w) case w: hipLaunchKernelGGL(k_ix_count<w>, dim3(blocks), dim3(256), 0, st, sets, count, n, cmask, padded, size); break;
+#define C(w) case w: hipLaunchKernelGGL(k_ix_pair_count<w>, dim3(blocks), dim3(256), 0, st, sets, count, n, cmask, size, entries); break;
     C(1) C(2) C(3) C(4) C(5)
 #undef C
     default: return hipErrorInvalidValue;
@@ -332,13 +350,13 @@ hipError_t launch_ix_count(int W, const uint64_t* sets, uint64_t count, uint32_t
   return hipGetLastError();
 }
 
-hipError_t launch_ix_fill(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement,
+hipError_t launch_ix_pair_fill(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement,
     const uint32_t* start, uint32_t* list, hipStream_t st) {
   if (!count) return hipSuccess;
   const uint64_t cmask = complement ? ~0ull : 0ull;
-  const unsigned blocks = unsigned((count + 255) / 256);
+  const unsigned blocks = unsigned(((count + 1) / 2 * 2 + 255) / 256);
   switch (W) {
-#define C(w) case w: hipLaunchKernelGGL(k_ix_fill<w>, dim3(blocks), dim3(256), 0, st, sets, count, n, cmask, start, list); break;
+#define C(w) case w: hipLaunchKernelGGL(k_ix_pair_fill<w>, dim3(blocks), dim3(256), 0, st, sets, count, n, cmask, start, list); break;
     C(1) C(2) C(3) C(4) C(5)
 #undef C
     default: return hipErrorInvalidValue;

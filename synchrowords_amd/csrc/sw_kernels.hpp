@@ -33,9 +33,9 @@ struct VsubParams {
   unsigned long long* witness = nullptr;    // ANY: min of (i << 32) | j
   void* flags = nullptr;  // FLAGS / SFLAGS: bitmap over tile / stream sets (bit j set on a hit)
   uint32_t* scount = nullptr;  // SCOUNT: += pairs per stream set
-  // Stream index of the indexed kernel (k_vsub_ix): per set, LDS row offsets
-  // of its states from ix_list[ix_start[i]] to ix_list[ix_start[i + 1]]
-  // (padded to multiples of 4), and its size |S_i| in ix_size[i].
+  // Stream index of the indexed kernel (k_vsub_ix): per pair i of stream
+  // sets, the LDS row offsets of both sets' states, interleaved by four, in
+  // ix_list[ix_start[i] .. ix_start[i + 1]); |S_s| in ix_size[s].
   const uint32_t* ix_start = nullptr;
   const uint32_t* ix_list = nullptr;
   const uint32_t* ix_size = nullptr;
@@ -44,9 +44,9 @@ struct VsubParams {
 // Indexed subset kernel (k_subset_ix.hip): whether its tile fits LDS for n
 // states, the pre-pass kernels that build the stream index, and the launch.
 bool vsub_ix_fits(uint32_t n, int W, bool proper_by_size);
-hipError_t launch_ix_count(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement, uint32_t* padded,
-    uint32_t* size, hipStream_t st);
-hipError_t launch_ix_fill(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement,
+hipError_t launch_ix_pair_count(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement,
+    uint32_t* size, uint32_t* entries, hipStream_t st);
+hipError_t launch_ix_pair_fill(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement,
     const uint32_t* start, uint32_t* list, hipStream_t st);
 hipError_t launch_vsub_ix(int W, const VsubParams& p, hipStream_t st);
 
