@@ -206,7 +206,12 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
     if (live && p.diag != INT64_MIN) {
       // Skip the pair (s, t) with s - t == diag: the same element of one list.
       const int64_t t = int64_t(sh) - p.diag - int64_t(tile_base);
-      if (t >= 0 && t < kIxTile && L == int(t / 128)) r.v[(t % 128) / 32] &= ~(1u << (t % 32));
+      if (t >= 0 && t < kIxTile && L == int(t / 128)) {
+        const int d = int(t % 128) / 32;  // constant indices only: r stays in registers
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (e == d) r.v[e] &= ~(1u << (t % 32));
+      }
     }
     if (live && p.proper_by_size) {
       // A stream set inside a tile set of equal cardinality is that set.
