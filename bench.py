@@ -250,7 +250,7 @@ def main() -> int:
     ap.add_argument("--subset-kernel", choices=["auto", "indexed", "bitwalk"], default="auto",
                     help="which subset kernel the microbench launches (auto: indexed for large launches)")
     ap.add_argument("--traffic-profile", type=str,
-                    default=str(ROOT / "profiles" / "k_vsub_subset_microbench_pmc_traffic.md"),
+                    default=str(ROOT / "profiles" / "k_vsub_ix_subset_microbench_pmc_traffic.md"),
                     help="rocprofv3 --pmc summary holding 'traffic_bytes_per_launch: N' for the roofline")
     ap.add_argument("--shape", type=str, default="", metavar="T,G,E,M",
                     help="use this launch shape (threads, target workgroups, early exit, mask bits) instead of tuning")

@@ -217,9 +217,11 @@ int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint32
 /* --------------------------------------------------------------- autotuner
  * Time every launch shape of the subset kernel (tile mask width 64/32,
  * workgroup size, workgroups aimed at, early exit; *early_exit returns
- * early_exit | mask_bits << 8) on seeded random lists of the given sizes and
- * densities on the device; the fastest is kept in the context and used by
- * later calls of the same size class (exact search, batch, the seams).
+ * early_exit | mask_bits << 8; for a size class that takes the indexed
+ * kernel, see sw_set_subset_kernel, only the workgroups aimed at) on seeded
+ * random lists of the given sizes and densities on the device; the fastest
+ * is kept in the context and used by later calls of the same size class
+ * (exact search, batch, the seams).
  * No reference counterpart (the reference's launch shapes are fixed,
  * cuda/subsets_implicit_trie_kernel.cu:8,196). */
 int sw_autotune_subset(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_count, float p_stream, float p_tile,

@@ -1,10 +1,13 @@
 // Autotuner of the subset kernel's launch shape.
 //
-// For a (W, stream size, tile size) class it times every candidate shape —
-// workgroup size (256/512/1024 threads), workgroups aimed at (256..4096) and
-// the per-word early exit — on random lists of the given densities resident on
-// the device, and records the fastest in the context's table.  Later calls of
-// the same class (exact search, batch labelling, the seams) launch with it.
+// For a (W, stream size, tile size) class it times every candidate shape on
+// random lists of the given densities resident on the device, and records the
+// fastest in the context's table; later calls of the same class (exact
+// search, batch labelling, the seams) launch with it.  For the bit-walking
+// kernel the shape is the mask width (64/32), the workgroup size
+// (256/512/1024 threads), the workgroups aimed at (256..4096) and the
+// per-word early exit; for a class that takes the indexed kernel, only the
+// workgroups aimed at.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -79,11 +82,38 @@ int sw_autotune_subset(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_cou
     SW_CHECK_HIP(hipEventCreate(&e0));
     SW_CHECK_HIP(hipEventCreate(&e1));
     sw_ctx::Tune best{1024, 1024, 1, 64, 1e30f};
+    const bool indexed = ctx->ix_enabled && s_count * t_count >= ctx->ix_min_pairs && vsub_ix_fits(n, W, false);
+    if (indexed) {
+      // Launches of this class take the indexed kernel (1024 threads, 64-bit
+      // masks): only the number of workgroups aimed at is free.  The index is
+      // built by the first (warm) call and reused by the timed ones.
+      for (uint32_t wgs : {256u, 512u, 1024u, 2048u, 4096u}) {
+        VsubParams p;
+        p.stream = S.as<uint64_t>();
+        p.s_count = s_count;
+        p.tile = T.as<uint64_t>();
+        p.t_count = t_count;
+        p.n = n;
+        p.mode = VSUB_COUNT;
+        p.count_out = out.as<unsigned long long>();
+        p.target_wgs = wgs;
+        run_subset(ctx, W, p);  // warm; builds the index
+        SW_CHECK_HIP(hipEventRecord(e0, st));
+        for (uint32_t r = 0; r < (reps ? reps : 3); ++r) SW_CHECK_HIP(launch_vsub_ix(W, p, st));
+        SW_CHECK_HIP(hipEventRecord(e1, st));
+        SW_CHECK_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        SW_CHECK_HIP(hipEventElapsedTime(&t, e0, e1));
+        t /= float(reps ? reps : 3);
+        if (t < best.ms) best = {1024, wgs, 0, 64, t};
+      }
+    }
     const uint32_t th[] = {256, 512, 1024};
     const uint32_t wg[] = {256, 512, 1024, 2048, 4096};
     const uint32_t mbs[] = {64, 32};
     for (uint32_t mb : mbs)
     for (uint32_t threads_i : th)
+      if (!indexed)
       for (uint32_t wgs : wg)
         for (uint32_t ee = 0; ee < 2; ++ee) {
           if (mb == 64 && vsub_mask_bits(n, W, 64, false) != 64) continue;
