@@ -284,9 +284,13 @@ int sw_mlp_train(sw_mlp* m, const uint16_t* x, const float* target, uint64_t cou
  * candidates contribute none). */
 int sw_mlp_samples(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t fdepth, uint32_t width,
     uint32_t bdepth, uint32_t per_depth, uint64_t* sets_out, float* dist_out, uint64_t cap, uint64_t* count);
-/* Beam of `width` on the device, scored by the network m, or by cardinality
- * (the reference's order, beam.hpp:145-162) when m is NULL: *bound = first
- * depth producing Q (upper if none below it), with that word. */
+/* Beam of `width` on the device, from the reference's start (sink-component
+ * singletons with two preimages under a letter, beam.hpp:104-116), ordered by
+ * the network m, or in the reference's order when m is NULL (larger sets
+ * first, then word by word ascending; beam.hpp:145-162): *bound = first depth
+ * producing Q (upper if none below it), with that word.  With m == NULL and
+ * the automaton's states relabelled as the plan's presort does, the bound is
+ * the reference Beam's bound for that width. */
 int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t width,
     uint64_t upper, uint64_t* bound, uint32_t* word, uint64_t word_cap, uint64_t* word_len);
 

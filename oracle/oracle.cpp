@@ -1396,6 +1396,18 @@ uint64_t oracle_beam(uint32_t n, uint32_t k, const uint32_t* delta, uint64_t upp
   return 0;
 }
 
+// One fixed-width run of the Beam on the automaton as given (no presort):
+// the first length reaching Q, or max_len (beam/beam.hpp:98-188).
+uint64_t oracle_beam_width(uint32_t n, uint32_t k, const uint32_t* delta, uint32_t width, uint64_t max_len) {
+  Aut A = make_aut(n, k, delta);
+  switch ((n + 63) / 64) {
+#define CASE(w) case w: return beam_run<w>(A, width, max_len);
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+  }
+  return 0;
+}
+
 // |delta(Q, word)| (the replay check for returned words).
 uint32_t oracle_replay(uint32_t n, uint32_t k, const uint32_t* delta, const uint32_t* word, uint64_t len) {
   std::vector<char> cur(n, 1), nx(n);

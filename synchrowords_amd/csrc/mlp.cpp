@@ -531,12 +531,12 @@ int sw_mlp_samples(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, u
   }
 }
 
-// Beam search on the device: from the sink singletons, expand by
-// preimages, drop duplicates, keep the `width` candidates of smallest score —
-// the network's predicted forward distance, or (m == NULL) minus the
-// cardinality, the reference's order; the first depth producing Q bounds the
-// length.  Returns *bound (upper when not improved) and the word of that
-// length.
+// Beam search on the device: from the sink singletons with two preimages
+// under a letter, expand by preimages, drop duplicates, keep the first
+// `width` candidates — in the network's score order, or (m == NULL) in the
+// reference's Beam order (larger sets first, then by their words,
+// beam.hpp:98-188); the first depth producing Q bounds the length.  Returns
+// *bound (upper when not improved) and the word of that length.
 int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t width,
     uint64_t upper, uint64_t* bound, uint32_t* word, uint64_t word_cap, uint64_t* word_len) {
   try {
@@ -558,11 +558,21 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
     dtb.ensure(tb.size() * 8);
     SW_CHECK_HIP(hipMemcpyAsync(dtf.p, tf.data(), tf.size() * 8, hipMemcpyHostToDevice, st));
     SW_CHECK_HIP(hipMemcpyAsync(dtb.p, tb.data(), tb.size() * 8, hipMemcpyHostToDevice, st));
+    // The reference's start (beam.hpp:104-116): sink-component states with two
+    // preimages under some letter, as singletons.
     const std::vector<char> sink = sw::sink_states(n, k, delta);
-    std::vector<uint64_t> seed;
+    std::vector<uint32_t> indeg(size_t(n) * k, 0);
     for (uint32_t q = 0; q < n; ++q)
-      if (sink[q])
+      for (uint32_t a = 0; a < k; ++a) indeg[size_t(delta[size_t(q) * k + a]) * k + a]++;
+    std::vector<uint64_t> seed;
+    for (uint32_t q = 0; q < n; ++q) {
+      if (!sink[q]) continue;
+      bool two = false;
+      for (uint32_t a = 0; a < k; ++a) two |= indeg[size_t(q) * k + a] > 1;
+      if (two)
         for (uint32_t w = 0; w < W; ++w) seed.push_back(w == q / 64 ? (1ull << (q % 64)) : 0ull);
+    }
+    if (seed.empty()) return SW_OK;  // no state with two preimages: nothing to follow
     uint64_t m0 = seed.size() / W;
     list.ensure(seed.size() * 8 + 8);
     SW_CHECK_HIP(hipMemcpyAsync(list.p, seed.data(), seed.size() * 8, hipMemcpyHostToDevice, st));
@@ -624,37 +634,51 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
       uint32_t nu = 0;
       SW_CHECK_HIP(hipMemcpyAsync(&nu, total, 4, hipMemcpyDeviceToHost, st));
       SW_CHECK_HIP(hipStreamSynchronize(st));
-      // Score and keep the best `width`.
-      keys.ensure(uint64_t(nu) * 4 + 4);
-      float* score = keys.as<float>();
+      // Order the distinct candidates and keep the first `width`: by the
+      // network's score, or (m == NULL) in the reference's Beam order.
+      const uint32_t* order = nullptr;
+      vals.ensure(uint64_t(nu) * 4 + 4);
+      vals2.ensure(uint64_t(nu) * 4 + 4);
       if (m) {
+        keys.ensure(uint64_t(nu) * 4 + 4);
         x.ensure(uint64_t(nu) * m->p.D * 2 + 8);
         sw::mlp_features(m, guide, dtf.as<uint64_t>(), uniq.as<uint64_t>(), nu, x.as<uint16_t>(), scratch, st);
         auto wk = sw::work_of(m, nu);
         SW_CHECK_HIP(sw::mlp_forward(m->p, x.as<uint16_t>(), nu, wk, wk.y, st));
-        score = wk.y;
+        keys2.ensure(uint64_t(nu) * 4 + 4);
+        SW_CHECK_HIP(sw::launch_iota(vals.as<uint32_t>(), nu, st));
+        size_t sneed = 0;
+        SW_CHECK_HIP(sw::sort_scores(nullptr, nullptr, nullptr, nullptr, nu, nullptr, &sneed, st));
+        if (sneed > sort_bytes) {
+          sort_tmp.release();
+          sort_tmp.ensure(sneed);
+          sort_bytes = sneed;
+        }
+        SW_CHECK_HIP(sw::sort_scores(wk.y, keys2.as<float>(), vals.as<uint32_t>(), vals2.as<uint32_t>(), nu,
+            sort_tmp.p, &sort_bytes, st));
+        order = vals2.as<uint32_t>();
       } else {
-        SW_CHECK_HIP(sw::launch_neg_popcount(uniq.as<uint64_t>(), nu, W, score, st));
+        keys.ensure(uint64_t(nu) * 8 + 8);
+        keys2.ensure(uint64_t(nu) * 8 + 8);
+        size_t sneed = 0;
+        uint32_t* res = nullptr;
+        SW_CHECK_HIP(sw::beam_reference_order(uniq.as<uint64_t>(), nu, W, n, vals.as<uint32_t>(), vals2.as<uint32_t>(),
+            keys.as<uint64_t>(), keys2.as<uint64_t>(), nullptr, &sneed, &res, st));
+        if (sneed > sort_bytes) {
+          sort_tmp.release();
+          sort_tmp.ensure(sneed);
+          sort_bytes = sneed;
+        }
+        SW_CHECK_HIP(sw::beam_reference_order(uniq.as<uint64_t>(), nu, W, n, vals.as<uint32_t>(), vals2.as<uint32_t>(),
+            keys.as<uint64_t>(), keys2.as<uint64_t>(), sort_tmp.p, &sort_bytes, &res, st));
+        order = res;
       }
-      keys2.ensure(uint64_t(nu) * 4 + 4);
-      vals.ensure(uint64_t(nu) * 4 + 4);
-      vals2.ensure(uint64_t(nu) * 4 + 4);
-      SW_CHECK_HIP(sw::launch_iota(vals.as<uint32_t>(), nu, st));
-      size_t sneed = 0;
-      SW_CHECK_HIP(sw::sort_scores(nullptr, nullptr, nullptr, nullptr, nu, nullptr, &sneed, st));
-      if (sneed > sort_bytes) {
-        sort_tmp.release();
-        sort_tmp.ensure(sneed);
-        sort_bytes = sneed;
-      }
-      SW_CHECK_HIP(sw::sort_scores(score, keys2.as<float>(), vals.as<uint32_t>(), vals2.as<uint32_t>(), nu, sort_tmp.p,
-          &sort_bytes, st));
       const uint64_t keepn = std::min<uint64_t>(width, nu);
       list.ensure(keepn * W * 8 + 8);
-      SW_CHECK_HIP(sw::launch_gather_rows(uniq.as<uint64_t>(), vals2.as<uint32_t>(), keepn, W, list.as<uint64_t>(), st));
+      SW_CHECK_HIP(sw::launch_gather_rows(uniq.as<uint64_t>(), order, keepn, W, list.as<uint64_t>(), st));
       // Candidate index of every kept set (for the word).
       std::vector<uint32_t> kept(keepn), cidx(nu);
-      SW_CHECK_HIP(hipMemcpyAsync(kept.data(), vals2.p, keepn * 4, hipMemcpyDeviceToHost, st));
+      SW_CHECK_HIP(hipMemcpyAsync(kept.data(), order, keepn * 4, hipMemcpyDeviceToHost, st));
       SW_CHECK_HIP(hipMemcpyAsync(cidx.data(), uidx.p, uint64_t(nu) * 4, hipMemcpyDeviceToHost, st));
       SW_CHECK_HIP(hipStreamSynchronize(st));
       std::vector<uint32_t> h(keepn);

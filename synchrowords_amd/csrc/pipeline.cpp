@@ -374,91 +374,8 @@ EppOut eppstein(uint32_t n, uint32_t k, const uint32_t* d, uint64_t upper) {
 // ------------------------------------------------------------------ Beam
 // Inverse BFS from the sink singletons keeping the `beam` largest preimages
 // (ties by per-word numeric order); the first depth reaching all N states
-// bounds the length (algorithm/beam/beam.hpp:98-188).  Parents are kept so
-// the bound comes with its word.
-template <int W>
-struct BeamRunner {
-  uint32_t n, k;
-  std::vector<uint32_t> d;  // permuted automaton
-  std::vector<uint64_t> tab;
-  uint32_t slices;
-  std::vector<uint64_t> seed;
-
-  struct Out {
-    uint64_t len;
-    std::vector<uint32_t> word;
-  };
-
-  void apply(const uint64_t* x, uint32_t a, uint64_t* r) const {
-    for (int w = 0; w < W; ++w) r[w] = 0;
-    for (uint32_t s = 0; s < slices; ++s) {
-      uint32_t byte = uint32_t(x[s >> 3] >> ((s & 7) * 8)) & 255u;
-      if (!byte) continue;
-      const uint64_t* row = &tab[((size_t(a) * slices + s) * 256 + byte) * W];
-      for (int w = 0; w < W; ++w) r[w] |= row[w];
-    }
-  }
-
-  Out run(uint32_t beam, uint64_t max_len) {
-    struct Node {
-      uint64_t v[W];
-      uint32_t parent;
-      uint32_t letter;
-      uint32_t size;
-    };
-    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> hist;  // per depth: (parent, letter)
-    std::vector<Node> list;
-    for (size_t i = 0; i < seed.size() / W; ++i) {
-      Node nd{};
-      std::memcpy(nd.v, &seed[i * W], 8 * W);
-      list.push_back(nd);
-    }
-    std::vector<Node> next;
-    for (uint64_t len = 1; len < max_len; ++len) {
-      next.resize(size_t(k) * list.size());
-      for (uint32_t a = 0; a < k; ++a)
-        for (size_t i = 0; i < list.size(); ++i) {
-          Node& c = next[a * list.size() + i];
-          apply(list[i].v, a, c.v);
-          c.parent = uint32_t(i);
-          c.letter = a;
-          int sz = 0;
-          for (int w = 0; w < W; ++w) sz += __builtin_popcountll(c.v[w]);
-          c.size = uint32_t(sz);
-        }
-      std::stable_sort(next.begin(), next.end(), [](const Node& x, const Node& y) {
-        if (x.size != y.size) return x.size > y.size;
-        for (int w = 0; w < W; ++w)
-          if (x.v[w] != y.v[w]) return x.v[w] < y.v[w];
-        return false;
-      });
-      std::swap(list, next);
-      std::vector<std::pair<uint32_t, uint32_t>> h;
-      if (list[0].size == n) {
-        Out o;
-        o.len = len;
-        o.word.push_back(list[0].letter);
-        uint32_t p = list[0].parent;
-        for (size_t depth = hist.size(); depth-- > 0;) {
-          o.word.push_back(hist[depth][p].second);
-          p = hist[depth][p].first;
-        }
-        return o;
-      }
-      auto eq = [](const Node& x, const Node& y) {
-        for (int w = 0; w < W; ++w)
-          if (x.v[w] != y.v[w]) return false;
-        return true;
-      };
-      list.erase(std::unique(list.begin(), list.end(), eq), list.end());
-      if (list.size() > beam) list.resize(beam);
-      for (auto& nd : list) h.emplace_back(nd.parent, nd.letter);
-      hist.push_back(std::move(h));
-    }
-    return Out{max_len, {}};
-  }
-};
-
+// bounds the length (algorithm/beam/beam.hpp:98-188).  Each run is the device
+// beam (sw_beam_device in its reference order), which also returns the word.
 struct BeamSettings {
   bool dynamic = true;
   long double beam_size = 0, min_beam = 0, max_beam = 0, ratio = 0.01;
@@ -471,10 +388,27 @@ struct BeamResult {
   std::vector<uint32_t> word;
 };
 
-template <int W>
-BeamResult beam_w(uint32_t n, uint32_t k, const uint32_t* d0, uint64_t upper, const BeamSettings& bs) {
+struct DeviceBeam {
+  sw_ctx* ctx;
+  uint32_t n, k;
+  std::vector<uint32_t> d;  // relabelled automaton
+  BeamResult run(uint64_t width, uint64_t max_len) const {
+    BeamResult r{max_len, {}};
+    if (max_len <= 1 || width == 0) return r;
+    std::vector<uint32_t> w(std::min<uint64_t>(max_len, 1u << 22));
+    uint64_t b = max_len, wl = 0;
+    const uint32_t wd = uint32_t(std::min<uint64_t>(width, 0xffffffffu));
+    if (sw_beam_device(ctx, nullptr, n, k, d.data(), wd, max_len, &b, w.data(), w.size(), &wl) != SW_OK)
+      throw Error{SW_ERR_HIP, std::string("device beam: ") + sw_last_error()};
+    r.bound = b;
+    if (b < max_len && wl == b) r.word.assign(w.begin(), w.begin() + wl);
+    return r;
+  }
+};
+
+BeamResult beam(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d0, uint64_t upper, const BeamSettings& bs) {
   // Presort "indeg": relabel states by ascending in-degree, ties by index
-  // (beam.hpp:351-369).
+  // (beam.hpp:351-369).  The order of equal-size sets depends on the labels.
   std::vector<uint32_t> order(n);
   if (bs.presort_indeg) {
     std::vector<std::pair<double, int>> indeg(n);
@@ -486,31 +420,15 @@ BeamResult beam_w(uint32_t n, uint32_t k, const uint32_t* d0, uint64_t upper, co
   } else {
     for (uint32_t i = 0; i < n; ++i) order[i] = i;
   }
-  BeamRunner<W> br;
-  br.n = n;
-  br.k = k;
-  br.d.assign(size_t(n) * k, 0);
+  DeviceBeam br{ctx, n, k, std::vector<uint32_t>(size_t(n) * k, 0)};
   for (uint32_t q = 0; q < n; ++q)
     for (uint32_t a = 0; a < k; ++a) br.d[size_t(order[q]) * k + a] = order[d0[size_t(q) * k + a]];
-  br.tab = build_tables(n, k, br.d.data(), 1);
-  br.slices = (n + 7) / 8;
-  const std::vector<char> sink = sink_states(n, k, br.d.data());
-  std::vector<uint32_t> indeg(size_t(n) * k, 0);
-  for (uint32_t q = 0; q < n; ++q)
-    for (uint32_t a = 0; a < k; ++a) indeg[size_t(br.d[size_t(q) * k + a]) * k + a]++;
-  for (uint32_t q = 0; q < n; ++q) {
-    if (!sink[q]) continue;
-    bool ok = false;
-    for (uint32_t a = 0; a < k; ++a) ok |= indeg[size_t(q) * k + a] > 1;
-    if (!ok) continue;
-    for (int w = 0; w < W; ++w) br.seed.push_back(uint32_t(w) == q / 64 ? (1ull << (q % 64)) : 0ull);
-  }
   uint64_t maxlen = upper;
   if (bs.max_iter >= 0 && uint64_t(bs.max_iter) + 1 < maxlen) maxlen = uint64_t(bs.max_iter) + 1;
   BeamResult res;
   if (!bs.dynamic) {
-    auto o = br.run(uint32_t(bs.beam_size), maxlen);
-    res.bound = std::min(o.len, upper);
+    auto o = br.run(uint64_t(bs.beam_size), maxlen);
+    res.bound = std::min(o.bound, upper);
     res.word = o.word;
     return res;
   }
@@ -520,7 +438,7 @@ BeamResult beam_w(uint32_t n, uint32_t k, const uint32_t* d0, uint64_t upper, co
   const unsigned min_beam = static_cast<unsigned>(bs.min_beam);
   const unsigned max_beam = static_cast<unsigned>(bs.max_beam);
   auto o1 = br.run(min_beam, maxlen);
-  unsigned bound = static_cast<unsigned>(o1.len);
+  unsigned bound = static_cast<unsigned>(o1.bound);
   res.bound = bound;
   res.word = o1.word;
   cost_t exact_cost = 2 * std::pow(static_cast<cost_t>(k), (bound + 1) / 2);
@@ -528,26 +446,12 @@ BeamResult beam_w(uint32_t n, uint32_t k, const uint32_t* d0, uint64_t upper, co
   unsigned size = static_cast<unsigned>(std::clamp<cost_t>(proposed, min_beam, max_beam));
   if (size > min_beam) {
     auto o2 = br.run(size, bound);
-    if (o2.len < bound) {
-      res.bound = o2.len;
+    if (o2.bound < bound) {
+      res.bound = o2.bound;
       res.word = o2.word;
     }
   }
   return res;
-}
-
-BeamResult beam(uint32_t n, uint32_t k, const uint32_t* d, uint64_t upper, const BeamSettings& bs) {
-  switch ((n + 63) / 64) {
-    case 1: return beam_w<1>(n, k, d, upper, bs);
-    case 2: return beam_w<2>(n, k, d, upper, bs);
-    case 3: return beam_w<3>(n, k, d, upper, bs);
-    case 4: return beam_w<4>(n, k, d, upper, bs);
-    case 5: return beam_w<5>(n, k, d, upper, bs);
-    case 6: return beam_w<6>(n, k, d, upper, bs);
-    case 7: return beam_w<7>(n, k, d, upper, bs);
-    case 8: return beam_w<8>(n, k, d, upper, bs);
-  }
-  throw Error{SW_ERR_UNSUPPORTED, "n above 512"};
 }
 
 const char* kDefaultPlan = R"PLAN({
@@ -632,7 +536,7 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
         if (n == 1) {
           lower = upper = 0;
         } else {
-          BeamResult b = beam(n, k, d, upper, bs);
+          BeamResult b = beam(ctx, n, k, d, upper, bs);
           if (b.bound < upper || (b.bound == upper && word.size() != upper && b.word.size() == b.bound)) {
             upper = b.bound;
             word = b.word;

@@ -60,6 +60,12 @@ hipError_t launch_gather_rows(const uint64_t* rows, const uint32_t* idx, uint64_
 hipError_t launch_iota(uint32_t* p, uint64_t count, hipStream_t st);
 // y[i] = -|sets[i]| (the cardinality order as an ascending key).
 hipError_t launch_neg_popcount(const uint64_t* sets, uint64_t count, uint32_t W, float* y, hipStream_t st);
+// The reference's Beam order of `count` distinct sets: larger first, then by
+// word 0, word 1, ... ascending (beam.hpp's comparator).  Leaves in *result
+// (perm_a or perm_b) the set indices in that order.  temp == NULL: size query.
+hipError_t beam_reference_order(const uint64_t* sets, uint64_t count, uint32_t W, uint32_t n, uint32_t* perm_a,
+    uint32_t* perm_b, uint64_t* key_a, uint64_t* key_b, void* temp, size_t* temp_bytes, uint32_t** result,
+    hipStream_t st);
 // Stable ascending sort of float keys with u32 values (temp == NULL: size query).
 hipError_t sort_scores(const float* keys_in, float* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
     uint64_t count, void* temp, size_t* temp_bytes, hipStream_t st);

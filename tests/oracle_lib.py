@@ -31,6 +31,7 @@ def lib():
         _lib.oracle_solve.restype = C.c_int
         _lib.oracle_brute.restype = C.c_long
         _lib.oracle_beam.restype = C.c_uint64
+        _lib.oracle_beam_width.restype = C.c_uint64
         _lib.oracle_replay.restype = C.c_uint32
         _lib.oracle_subset_count.restype = C.c_uint64
     return _lib
@@ -79,6 +80,12 @@ def eppstein(n, k, delta, upper):
 def beam(n, k, delta, upper) -> int:
     d = np.ascontiguousarray(delta, dtype=np.uint32)
     return int(lib().oracle_beam(n, k, _p(d), C.c_uint64(upper)))
+
+
+def beam_width(n, k, delta, width, max_len) -> int:
+    """One fixed-width Beam run on the automaton as given (no presort)."""
+    d = np.ascontiguousarray(delta, dtype=np.uint32)
+    return int(lib().oracle_beam_width(n, k, _p(d), C.c_uint32(width), C.c_uint64(max_len)))
 
 
 def replay(n, k, delta, word) -> int:

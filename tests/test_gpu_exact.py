@@ -92,3 +92,19 @@ def test_dominance_pruning_keeps_the_length(ctx, n, k):
         on = ctx.exact(n, k, d)
         assert on.found and off.found and on.length == off.length
         assert replay(n, k, d, on.word) == 1 and replay(n, k, d, off.word) == 1
+
+
+@pytest.mark.parametrize("n,k", [(20, 2), (48, 2), (64, 3), (100, 2), (200, 2), (300, 2), (130, 4)])
+@pytest.mark.parametrize("width", [1, 7, 64, 500])
+def test_device_beam_matches_reference_order(ctx, n, k, width):
+    """The device beam in the reference's order gives the oracle's Beam bound
+    (restated from beam.hpp:98-188) at the same width, and its word."""
+    import oracle_lib as orc
+    for seed in range(2):
+        d = gen.random_dfa(n, k, 300 + 7 * seed + width)
+        cap = (n - 1) ** 2 + 1
+        want = orc.beam_width(n, k, d, width, cap)
+        b, w = ctx.beam(n, k, d, width, cap)
+        assert b == want, (seed, b, want)
+        if b < cap:
+            assert len(w) == b and replay(n, k, d, w) == 1
