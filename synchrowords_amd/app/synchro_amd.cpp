@@ -1,0 +1,238 @@
+// synchro_amd — the reference's command-line driver (app/main.cpp:10-78,
+// app/args.hpp, app/io.hpp) on the device library: the same input and output
+// files, so that scripts written for the reference's `synchro` run unchanged.
+//
+//   synchro_amd -f automata.txt -c config.json [-o results.txt [--continue]]
+//               [-v | -q | -d] [-b suffix] [-h]
+//
+// Input: automata one after another, each "K N" then K*N transitions,
+// delta(q, a) at position q*K + a (app/io.hpp:50-77).  Output: one line per
+// automaton, "ID: NON SYNCHRO" or "ID: [lower, upper] ((name, ms), ...)"
+// followed by " {w_0 w_1 ...}" when a reset word of length `upper` is known
+// (app/io.hpp:95-143).  With --continue the non-empty lines already in the
+// output file are counted and that many automata are skipped
+// (app/main.cpp:26-46).  -b is accepted for compatibility (there is no JIT
+// build here).  Exit codes follow the reference: 1 arguments or output file,
+// 2 config, 3 input, 4 a failed run.
+#include <chrono>
+#include <cctype>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <iostream>
+#include <optional>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "synchrowords_amd.h"
+
+namespace {
+
+int g_level = 1;  // 0 quiet, 1 info, 2 verbose, 3 debug
+
+void log(int level, const char* tag, const std::string& msg) {
+  if (level > g_level && level != 0) return;
+  const auto now = std::chrono::system_clock::now();
+  const std::time_t t = std::chrono::system_clock::to_time_t(now);
+  const long ms = long(std::chrono::duration_cast<std::chrono::milliseconds>(now.time_since_epoch()).count() % 1000);
+  std::tm tm{};
+  localtime_r(&t, &tm);
+  std::fprintf(stderr, "[%02d:%02d:%02d.%03ld] [%s] %s\n", tm.tm_hour, tm.tm_min, tm.tm_sec, ms, tag, msg.c_str());
+}
+void info(const std::string& m) { log(1, "INFO", m); }
+[[noreturn]] void fail(int code, const std::string& m) {
+  log(0, "ERROR", m);
+  std::exit(code);
+}
+
+struct Args {
+  std::string input, config, build_suffix;
+  std::optional<std::string> output;
+  bool verbose = false, quiet = false, debug = false, cont = false;
+};
+
+void usage() {
+  std::cout << "Find synchronizing word of an automaton\n"
+               "Usage:\n  synchro_amd [OPTION...]\n\n"
+               "  -f, --file arg          Path to the input file\n"
+               "  -c, --config arg        Path to the config file\n"
+               "  -o, --output arg        Path to the output file\n"
+               "  -b, --build-suffix arg  Suffix of the build folder (default: \"\")\n"
+               "      --continue          Do not overwrite the output file and run algorithms only for remaining\n"
+               "                          automata\n"
+               "  -v, --verbose           Verbose output\n"
+               "  -q, --quiet             Quiet output (only warnings and errors)\n"
+               "  -d, --debug             Debug output (all messages and timers)\n"
+               "  -h, --help              Print usage\n";
+}
+
+Args parse(int argc, char** argv) {
+  Args a;
+  bool have_f = false, have_c = false;
+  for (int i = 1; i < argc; ++i) {
+    std::string s = argv[i];
+    auto value = [&](const char* name) -> std::string {
+      const size_t eq = s.find('=');
+      if (s.rfind("--", 0) == 0 && eq != std::string::npos) return s.substr(eq + 1);
+      if (i + 1 >= argc) fail(1, std::string("Option '") + name + "' is missing an argument");
+      return argv[++i];
+    };
+    auto is = [&](const char* shortf, const char* longf) {
+      const std::string l = longf;
+      return (shortf && s == shortf) || s == l || s.rfind(l + "=", 0) == 0;
+    };
+    if (is("-h", "--help")) {
+      usage();
+      std::exit(0);
+    } else if (is("-f", "--file")) {
+      a.input = value("file");
+      have_f = true;
+    } else if (is("-c", "--config")) {
+      a.config = value("config");
+      have_c = true;
+    } else if (is("-o", "--output")) {
+      a.output = value("output");
+    } else if (is("-b", "--build-suffix")) {
+      a.build_suffix = value("build-suffix");
+    } else if (s == "--continue") {
+      a.cont = true;
+    } else if (is("-v", "--verbose")) {
+      a.verbose = true;
+    } else if (is("-q", "--quiet")) {
+      a.quiet = true;
+    } else if (is("-d", "--debug")) {
+      a.debug = true;
+    } else {
+      fail(1, "Option '" + s + "' does not exist");
+    }
+  }
+  if (!have_f) fail(1, "Option 'file' is required");
+  if (!have_c) fail(1, "Option 'config' is required");
+  if (int(a.verbose) + int(a.quiet) + int(a.debug) > 1)
+    fail(1, "Only one of [--verbose, --quiet, --debug] can be enabled at one time");
+  if (a.cont && !a.output) fail(1, "--continue can be enabled only if an output file is specified");
+  return a;
+}
+
+struct Automaton {
+  uint32_t n, k;
+  std::vector<uint32_t> delta;
+};
+
+std::vector<Automaton> read_automata(const std::string& path) {
+  std::ifstream in(path);
+  if (!in) fail(3, "Cannot open the input file " + path);
+  std::vector<Automaton> out;
+  long long K, N;
+  while (in >> K >> N) {
+    if (N <= 0 || K <= 0) fail(3, "N and K must be greater than 0");
+    Automaton a{uint32_t(N), uint32_t(K), std::vector<uint32_t>(size_t(N) * size_t(K))};
+    for (size_t i = 0; i < a.delta.size(); ++i) {
+      long long x;
+      if (!(in >> x)) fail(3, "Expected " + std::to_string(a.delta.size()) + " integers, found " + std::to_string(i));
+      if (x < 0 || x >= N)
+        fail(3, "Expected integer in range [0, " + std::to_string(N - 1) + "], found " + std::to_string(x));
+      a.delta[i] = uint32_t(x);
+    }
+    out.push_back(std::move(a));
+  }
+  info("Read " + std::to_string(out.size()) + " automata");
+  return out;
+}
+
+std::string read_config(const std::string& path) {
+  std::ifstream in(path);
+  if (!in) fail(2, "Cannot open the config file " + path);
+  std::stringstream ss;
+  ss << in.rdbuf();
+  std::string s = ss.str();
+  size_t i = 0;
+  while (i < s.size() && std::isspace(static_cast<unsigned char>(s[i]))) ++i;
+  if (i == s.size() || s[i] != '{') fail(2, "The config file is not a JSON object: " + path);
+  return s;
+}
+
+size_t count_nonempty_lines(const std::string& path) {
+  std::ifstream in(path);
+  size_t c = 0;
+  std::string line;
+  while (std::getline(in, line)) {
+    for (char ch : line)
+      if (!std::isspace(static_cast<unsigned char>(ch))) {
+        ++c;
+        break;
+      }
+  }
+  return c;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Args args = parse(argc, argv);
+  if (args.verbose) g_level = 2;
+  if (args.debug) g_level = 3;
+  if (args.quiet) g_level = 0;
+
+  const std::vector<Automaton> auts = read_automata(args.input);
+  const std::string plan = read_config(args.config);
+
+  size_t skip = 0;
+  std::ofstream out;
+  if (args.output) {
+    if (args.cont) skip = count_nonempty_lines(*args.output);
+    out.open(*args.output, args.cont ? std::ios_base::app : std::ios_base::trunc);
+    if (!out) fail(1, "Error while opening the output file");
+  }
+  if (skip) info("Skipping " + std::to_string(skip) + " automata");
+
+  sw_ctx* ctx = nullptr;
+  if (skip < auts.size() && sw_ctx_create(0, &ctx) != SW_OK) fail(4, std::string("device: ") + sw_last_error());
+
+  std::vector<uint32_t> word;
+  for (size_t index = skip; index < auts.size(); ++index) {
+    const Automaton& a = auts[index];
+    const uint64_t cap = uint64_t(a.n) * a.n * a.n / 6 + uint64_t(a.n) * a.n + 8;  // longest bound a plan may give
+    word.assign(size_t(std::min<uint64_t>(cap, 1ull << 26)), 0);
+    sw_result r{};
+    if (sw_pipeline_solve(ctx, a.n, a.k, a.delta.data(), plan.c_str(), &r, word.data(), word.size()) != SW_OK) {
+      sw_ctx_destroy(ctx);
+      fail(4, "automaton " + std::to_string(index) + ": " + sw_last_error());
+    }
+    if (r.non_synchro) {
+      info("NON SYNCHRO");
+    } else {
+      info("Minimum synchronizing word length: [" + std::to_string(r.lower) + ", " + std::to_string(r.upper) + "]");
+    }
+    if (!args.output) {
+      if (r.word_len) info("Found synchronizing word of length " + std::to_string(r.word_len) +
+                           " (use the -o flag to save it)");
+      continue;
+    }
+    out << index << ": ";
+    if (r.non_synchro) {
+      out << "NON SYNCHRO\n";
+      out.flush();
+      continue;
+    }
+    out << "[" << r.lower << ", " << r.upper << "] (";
+    for (int i = 0; i < r.n_algorithms; ++i) {
+      if (i) out << ", ";
+      out << "(" << r.names[i] << ", " << static_cast<long long>(r.ms[i]) << ")";
+    }
+    out << ")";
+    if (r.word_len) {
+      info("Saving synchronizing word of length " + std::to_string(r.word_len));
+      out << " {";
+      for (uint64_t i = 0; i < r.word_len; ++i) out << (i ? " " : "") << word[i];
+      out << "}";
+    }
+    out << "\n";
+    out.flush();
+  }
+  if (ctx) sw_ctx_destroy(ctx);
+  return 0;
+}

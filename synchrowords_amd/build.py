@@ -2,7 +2,8 @@
 
 Kernels (``csrc/*.hip``) are compiled by hipcc for gfx950; host sources
 (``csrc/*.cpp``) by g++ against the HIP runtime; the result is
-``synchrowords_amd/libsynchrowords_amd.so``.
+``synchrowords_amd/libsynchrowords_amd.so``, and the command-line driver
+``synchrowords_amd/synchro_amd`` (``app/synchro_amd.cpp``) linked against it.
 """
 from __future__ import annotations
 
@@ -16,6 +17,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "libsynchrowords_amd.so"
+CLI = HERE / "synchro_amd"
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("SW_OFFLOAD_ARCH", "gfx950")
 
@@ -53,8 +55,14 @@ def build(verbose: bool = False) -> Path:
     tmp = OUT.with_suffix(".so.tmp")
     _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-o", str(tmp), *objs, f"-L{ROCM}/lib", "-lamdhip64"])
     os.replace(tmp, OUT)
+    # The command-line driver, linked against the library beside it.
+    cli_tmp = CLI.with_suffix(".tmp")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{HERE.parent / 'include'}", str(HERE / "app" / "synchro_amd.cpp"),
+          "-o", str(cli_tmp), f"-L{HERE}", "-lsynchrowords_amd", "-Wl,-rpath,$ORIGIN", f"-L{ROCM}/lib",
+          "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"])
+    os.replace(cli_tmp, CLI)
     if verbose:
-        print(f"built {OUT}")
+        print(f"built {OUT} and {CLI}")
     return OUT
 
 

@@ -562,6 +562,16 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
           lower = std::max(lower, r.r.length);
         }
       }
+    } else if (name == "Reduce") {
+      // The reference's Reduce (algorithm/reduce/reduce.hpp:31-103) runs a
+      // short forward search and, when the states reachable from its list
+      // are fewer than N, restarts the plan on that smaller automaton (a new
+      // JIT build) so that Exact fits the host's memory.  It never changes a
+      // bound Exact computes.  The device search runs on the whole automaton,
+      // so the stage is accepted, checked and recorded, and leaves the
+      // bounds to the stages after it.
+      (void)eval_num(cfg->get("min_n"), "80", n, k);
+      (void)eval_num(cfg->get("list_size_threshold"), "AUT_N * AUT_K * 8", n, k);
     } else {
       throw Error{SW_ERR_UNSUPPORTED, "algorithm not supported by this library: " + name};
     }
