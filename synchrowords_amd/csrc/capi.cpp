@@ -102,6 +102,8 @@ void sw_ctx_destroy(sw_ctx* ctx) {
   ctx->e.release();
   if (ctx->exact_ws && ctx->exact_ws_free) ctx->exact_ws_free(ctx->exact_ws);
   ctx->exact_ws = nullptr;
+  if (ctx->beam_ws && ctx->beam_ws_free) ctx->beam_ws_free(ctx->beam_ws);
+  ctx->beam_ws = nullptr;
   if (ctx->ix_host) (void)hipHostFree(ctx->ix_host);
   ctx->ix_host = nullptr;
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);

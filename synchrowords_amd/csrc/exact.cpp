@@ -341,7 +341,9 @@ class Search {
     S.front_idx.ensure(nnew * 4 + 4);
     const bool dominance = nnew > 0 && S.rows * nnew <= budget_pairs_;
     if (dominance) {
-      SW_CHECK_HIP(hipStreamSynchronize(st_));
+      // Host time up to here (the new-level size was read back, so the
+      // expansion and dedup are done) counts as expansion; the commit still
+      // in flight counts as dominance.
       const auto t1 = std::chrono::steady_clock::now();
       ms_expand_ += std::chrono::duration<double, std::milli>(t1 - t0).count();
       // Tile: the new level; stream: every row of the side (the new level
@@ -390,7 +392,6 @@ class Search {
       SW_CHECK_HIP(launch_select(W, S.arena.as<uint64_t>(), lo, nullptr, nullptr, nnew, S.front.as<uint64_t>(),
           S.front_idx.as<uint32_t>(), st_));
       S.fcount = nnew;
-      SW_CHECK_HIP(hipStreamSynchronize(st_));
       ms_expand_ += ms_since(t0);
     }
     if (!backward && S.fcount >= forward_cap_) S.frozen = true;

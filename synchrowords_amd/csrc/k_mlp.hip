@@ -341,6 +341,12 @@ __global__ void k_gather_rows(const uint64_t* __restrict__ rows, const uint32_t*
   for (uint32_t w = 0; w < W; ++w) out[j * W + w] = rows[r * W + w];
 }
 
+__global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t count,
+    uint32_t* __restrict__ out) {
+  const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < count) out[j] = src[idx[j]];
+}
+
 __global__ void k_neg_popcount(const uint64_t* __restrict__ sets, uint64_t count, uint32_t W, float* __restrict__ y) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= count) return;
@@ -384,6 +390,13 @@ hipError_t launch_gather_rows(const uint64_t* rows, const uint32_t* idx, uint64_
     hipStream_t st) {
   if (!count) return hipSuccess;
   hipLaunchKernelGGL(k_gather_rows, dim3(blocks(count)), dim3(256), 0, st, rows, idx, count, W, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint64_t count, uint32_t* out,
+    hipStream_t st) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_u32, dim3(blocks(count)), dim3(256), 0, st, src, idx, count, out);
   return hipGetLastError();
 }
 

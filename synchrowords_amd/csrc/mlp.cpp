@@ -157,6 +157,28 @@ void build_guide(uint32_t n, uint32_t k, const uint32_t* delta, Guide& out, hipS
   for (int j = 0; j < kGuideDepths; ++j) out.g.ends[j] = ends[j];
 }
 
+// Device buffers of the beam, kept by the context between calls.
+struct BeamWs {
+  DevBuf dtf, dtb, list, cand, table, slot, keep, pos, uniq, uidx, x, scratch, keys, keys2, vals, vals2, small,
+      scan_tmp, sort_tmp, hist;
+  size_t scan_bytes = 0, sort_bytes = 0;
+  uint64_t* host = nullptr;  // pinned read-back
+  BeamWs() { SW_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), 64, hipHostMallocDefault)); }
+  ~BeamWs() {
+    if (host) (void)hipHostFree(host);
+  }
+  BeamWs(const BeamWs&) = delete;
+  BeamWs& operator=(const BeamWs&) = delete;
+};
+
+BeamWs& beam_workspace(sw_ctx* ctx) {
+  if (!ctx->beam_ws) {
+    ctx->beam_ws = new BeamWs();
+    ctx->beam_ws_free = [](void* p) { delete static_cast<BeamWs*>(p); };
+  }
+  return *static_cast<BeamWs*>(ctx->beam_ws);
+}
+
 // Features of `count` device sets (images under every letter computed here).
 void mlp_features(sw_mlp* m, const Guide& guide, const uint64_t* tables_fwd, const uint64_t* sets, uint64_t count,
     uint16_t* x, DevBuf& scratch, hipStream_t st) {
@@ -552,8 +574,12 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
     auto tb = sw::build_tables(n, k, delta, 1);
     sw::Guide guide;
     if (m) sw::build_guide(n, k, delta, guide, st);
-    DevBuf dtf, dtb, list, cand, table, slot, keep, pos, uniq, uidx, x, scratch, keys, keys2, vals, vals2, sel, small,
-        scan_tmp, sort_tmp;
+    sw::BeamWs& ws = sw::beam_workspace(ctx);
+    DevBuf &dtf = ws.dtf, &dtb = ws.dtb, &list = ws.list, &cand = ws.cand, &table = ws.table, &slot = ws.slot,
+           &keep = ws.keep, &pos = ws.pos, &uniq = ws.uniq, &uidx = ws.uidx, &x = ws.x, &scratch = ws.scratch,
+           &keys = ws.keys, &keys2 = ws.keys2, &vals = ws.vals, &vals2 = ws.vals2, &small = ws.small,
+           &scan_tmp = ws.scan_tmp, &sort_tmp = ws.sort_tmp, &histd = ws.hist;
+    size_t &scan_bytes = ws.scan_bytes, &sort_bytes = ws.sort_bytes;
     dtf.ensure(tf.size() * 8);
     dtb.ensure(tb.size() * 8);
     SW_CHECK_HIP(hipMemcpyAsync(dtf.p, tf.data(), tf.size() * 8, hipMemcpyHostToDevice, st));
@@ -577,9 +603,10 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
     list.ensure(seed.size() * 8 + 8);
     SW_CHECK_HIP(hipMemcpyAsync(list.p, seed.data(), seed.size() * 8, hipMemcpyHostToDevice, st));
     small.ensure(64);
-    std::vector<std::vector<uint32_t>> hist;  // per depth: candidate index of each kept set
-    std::vector<uint64_t> hist_m;              // parent list size per depth
-    size_t scan_bytes = 0, sort_bytes = 0;
+    // Word history on the device: per depth, the candidate index of every
+    // kept set (its parent is index % parents, its letter index / parents).
+    std::vector<uint64_t> hist_off, hist_m;
+    uint64_t hist_len = 0;
     for (uint64_t len = 1; len < upper; ++len) {
       const uint64_t cnt = m0 * k;
       cand.ensure(cnt * W * 8 + 8);
@@ -588,26 +615,8 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
       unsigned long long* first = small.as<unsigned long long>();
       SW_CHECK_HIP(hipMemsetAsync(first, 0xff, 8, st));
       SW_CHECK_HIP(sw::launch_first_full(cand.as<uint64_t>(), cnt, n, W, first, st));
-      unsigned long long f = ~0ull;
-      SW_CHECK_HIP(hipMemcpyAsync(&f, first, 8, hipMemcpyDeviceToHost, st));
-      SW_CHECK_HIP(hipStreamSynchronize(st));
-      if (f != ~0ull) {
-        // Word: letter of this candidate, then back through the kept parents.
-        std::vector<uint32_t> w;
-        uint64_t c = f;
-        w.push_back(uint32_t(c / m0));
-        uint64_t parent = c % m0;
-        for (size_t dd = hist.size(); dd-- > 0;) {
-          const uint64_t ci = hist[dd][parent];
-          w.push_back(uint32_t(ci / hist_m[dd]));
-          parent = ci % hist_m[dd];
-        }
-        *bound = len;
-        if (word_len) *word_len = w.size();
-        if (word && w.size() <= word_cap) std::copy(w.begin(), w.end(), word);
-        break;
-      }
-      // Dedup within the step.
+      // Dedup within the step (the smallest candidate index of equal sets
+      // survives), launched before the one read-back of the step.
       uint64_t tcap = 1024;
       while (tcap < 2 * cnt) tcap <<= 1;
       table.ensure(tcap * 4);
@@ -631,9 +640,32 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
       uidx.ensure(cnt * 4 + 4);
       SW_CHECK_HIP(sw::launch_select(int(W), cand.as<uint64_t>(), 0, keep.as<uint32_t>(), pos.as<uint32_t>(), cnt,
           uniq.as<uint64_t>(), uidx.as<uint32_t>(), st));
-      uint32_t nu = 0;
-      SW_CHECK_HIP(hipMemcpyAsync(&nu, total, 4, hipMemcpyDeviceToHost, st));
+      // first (8 bytes) and total (4 bytes at +16) in one copy
+      SW_CHECK_HIP(hipMemcpyAsync(ws.host, small.p, 24, hipMemcpyDeviceToHost, st));
       SW_CHECK_HIP(hipStreamSynchronize(st));
+      const unsigned long long f = ws.host[0];
+      const uint32_t nu = uint32_t(ws.host[2]);
+      if (f != ~0ull) {
+        // Word: letter of this candidate, then back through the kept parents.
+        std::vector<uint32_t> h(hist_len);
+        if (hist_len) {
+          SW_CHECK_HIP(hipMemcpyAsync(h.data(), histd.p, hist_len * 4, hipMemcpyDeviceToHost, st));
+          SW_CHECK_HIP(hipStreamSynchronize(st));
+        }
+        std::vector<uint32_t> w;
+        uint64_t c = f;
+        w.push_back(uint32_t(c / m0));
+        uint64_t parent = c % m0;
+        for (size_t dd = hist_off.size(); dd-- > 0;) {
+          const uint64_t ci = h[hist_off[dd] + parent];
+          w.push_back(uint32_t(ci / hist_m[dd]));
+          parent = ci % hist_m[dd];
+        }
+        *bound = len;
+        if (word_len) *word_len = w.size();
+        if (word && w.size() <= word_cap) std::copy(w.begin(), w.end(), word);
+        break;
+      }
       // Order the distinct candidates and keep the first `width`: by the
       // network's score, or (m == NULL) in the reference's Beam order.
       const uint32_t* order = nullptr;
@@ -676,16 +708,13 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
       const uint64_t keepn = std::min<uint64_t>(width, nu);
       list.ensure(keepn * W * 8 + 8);
       SW_CHECK_HIP(sw::launch_gather_rows(uniq.as<uint64_t>(), order, keepn, W, list.as<uint64_t>(), st));
-      // Candidate index of every kept set (for the word).
-      std::vector<uint32_t> kept(keepn), cidx(nu);
-      SW_CHECK_HIP(hipMemcpyAsync(kept.data(), order, keepn * 4, hipMemcpyDeviceToHost, st));
-      SW_CHECK_HIP(hipMemcpyAsync(cidx.data(), uidx.p, uint64_t(nu) * 4, hipMemcpyDeviceToHost, st));
-      SW_CHECK_HIP(hipStreamSynchronize(st));
-      std::vector<uint32_t> h(keepn);
-      for (uint64_t j = 0; j < keepn; ++j) h[j] = cidx[kept[j]];
-      hist.push_back(std::move(h));
+      histd.ensure((hist_len + keepn) * 4 + 4, st, hist_len * 4);
+      SW_CHECK_HIP(sw::launch_gather_u32(uidx.as<uint32_t>(), order, keepn, histd.as<uint32_t>() + hist_len, st));
+      hist_off.push_back(hist_len);
       hist_m.push_back(m0);
+      hist_len += keepn;
       m0 = keepn;
+      if (keepn == 0) break;  // every continuation was empty
     }
     return SW_OK;
   } catch (const Error& e) {

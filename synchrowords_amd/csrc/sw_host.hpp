@@ -89,6 +89,9 @@ struct sw_ctx {
   // Buffers of the device exact search, reused between searches.
   void* exact_ws = nullptr;
   void (*exact_ws_free)(void*) = nullptr;
+  // Buffers of the device beam, reused between calls.
+  void* beam_ws = nullptr;
+  void (*beam_ws_free)(void*) = nullptr;
   // Stream index of the indexed subset kernel, reused between launches, and
   // when it is used: never (0), or for launches of at least ix_min_pairs pairs.
   sw::DevBuf ix_padded, ix_size, ix_start, ix_list, ix_tmp;

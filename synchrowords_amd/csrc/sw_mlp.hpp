@@ -58,6 +58,9 @@ hipError_t launch_first_full(const uint64_t* sets, uint64_t count, uint32_t n, u
 hipError_t launch_gather_rows(const uint64_t* rows, const uint32_t* idx, uint64_t count, uint32_t W, uint64_t* out,
     hipStream_t st);
 hipError_t launch_iota(uint32_t* p, uint64_t count, hipStream_t st);
+// out[j] = src[idx[j]].
+hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint64_t count, uint32_t* out,
+    hipStream_t st);
 // y[i] = -|sets[i]| (the cardinality order as an ascending key).
 hipError_t launch_neg_popcount(const uint64_t* sets, uint64_t count, uint32_t W, float* y, hipStream_t st);
 // The reference's Beam order of `count` distinct sets: larger first, then by
