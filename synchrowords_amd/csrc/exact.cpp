@@ -324,9 +324,9 @@ class Search {
     ensure_table(S, cnt);
     SW_CHECK_HIP(launch_hash_insert(W, cand_.as<uint64_t>(), valid_.as<uint32_t>(), cnt, S.table.as<uint32_t>(),
         S.tcap - 1, S.arena.as<uint64_t>(), slot_.as<uint32_t>(), probes_.as<unsigned long long>(), st_));
-    SW_CHECK_HIP(launch_hash_resolve(slot_.as<uint32_t>(), S.table.as<uint32_t>(), cnt, keep_.as<uint32_t>(), st_));
     uint32_t* total = small_.as<uint32_t>();
-    SW_CHECK_HIP(launch_scan(keep_.as<uint32_t>(), pos_.as<uint32_t>(), cnt, total, scan_tmp_.p, &scan_bytes_, st_));
+    SW_CHECK_HIP(launch_keep_scan(1, slot_.as<uint32_t>(), S.table.as<uint32_t>(), nullptr, cnt, keep_.as<uint32_t>(),
+        pos_.as<uint32_t>(), total, scan_tmp_.p, &scan_bytes_, st_));
     const uint64_t nnew = read_u32(total);
     ensure_rows(S, S.rows + nnew);
     const uint64_t lo = S.rows;
@@ -381,9 +381,9 @@ class Search {
       apply_tuning(ctx_, W, p);
       run_subset(ctx_, W, p);
       pair_tests_ += S.rows * nnew;
-      SW_CHECK_HIP(launch_alive(W, flags_.p, nnew, keep_.as<uint32_t>(), st_));
       ensure_scan(nnew);
-      SW_CHECK_HIP(launch_scan(keep_.as<uint32_t>(), pos_.as<uint32_t>(), nnew, total, scan_tmp_.p, &scan_bytes_, st_));
+      SW_CHECK_HIP(launch_keep_scan(2, nullptr, nullptr, flags_.p, nnew, keep_.as<uint32_t>(), pos_.as<uint32_t>(),
+          total, scan_tmp_.p, &scan_bytes_, st_));
       SW_CHECK_HIP(launch_select(W, S.arena.as<uint64_t>(), lo, keep_.as<uint32_t>(), pos_.as<uint32_t>(), nnew,
           S.front.as<uint64_t>(), S.front_idx.as<uint32_t>(), st_));
       S.fcount = read_u32(total);
@@ -407,8 +407,7 @@ class Search {
     if (!F_.fcount || !B_.fcount) return false;
     int* found = small_.as<int>() + 2;
     unsigned long long* wit = reinterpret_cast<unsigned long long*>(small_.as<char>() + 16);
-    SW_CHECK_HIP(hipMemsetAsync(found, 0, 4, st_));
-    SW_CHECK_HIP(hipMemsetAsync(wit, 0xff, 8, st_));
+    SW_CHECK_HIP(launch_goal_reset(found, wit, st_));
     VsubParams p;
     p.stream = F_.front.as<uint64_t>();
     p.s_count = F_.fcount;

@@ -202,6 +202,60 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
     dst[i] = src[i];
 }
 
+// Keep flags and their exclusive scan in one workgroup, for short lists
+// (the many small steps of a search are bound by launches, not by work):
+//   mode 0: keep[i] given;
+//   mode 1: keep[i] = candidate i owns its hash slot (as k_hash_resolve);
+//   mode 2: keep[i] = bit i of `dead` is clear (as k_alive).
+// pos[i] = sum of keep[j], j < i; *total = the sum of all.
+constexpr uint32_t kSmallScanThreads = 1024;
+constexpr uint64_t kSmallScanMax = 1ull << 17;
+
+__global__ __launch_bounds__(kSmallScanThreads) void k_keep_scan_small(int mode, const uint32_t* __restrict__ slot_of,
+    const uint32_t* __restrict__ table, const uint8_t* __restrict__ dead, uint64_t count, uint32_t* __restrict__ keep,
+    uint32_t* __restrict__ pos, uint32_t* __restrict__ total) {
+  __shared__ uint32_t part[kSmallScanThreads];
+  const uint32_t t = threadIdx.x;
+  const uint64_t per = (count + kSmallScanThreads - 1) / kSmallScanThreads;
+  const uint64_t b = uint64_t(t) * per;
+  const uint64_t e = b + per < count ? b + per : count;
+  uint32_t sum = 0;
+  for (uint64_t i = b; i < e; ++i) {
+    uint32_t k;
+    if (mode == 1) {
+      const uint32_t s = slot_of[i];
+      k = (s != kEmpty && table[s] == (kCand | uint32_t(i))) ? 1u : 0u;
+      keep[i] = k;
+    } else if (mode == 2) {
+      k = ((dead[i >> 3] >> (i & 7)) & 1) ? 0u : 1u;
+      keep[i] = k;
+    } else {
+      k = keep[i];
+    }
+    sum += k;
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (uint32_t off = 1; off < kSmallScanThreads; off <<= 1) {  // inclusive Hillis-Steele scan
+    const uint32_t x = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += x;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - sum;
+  for (uint64_t i = b; i < e; ++i) {
+    pos[i] = run;
+    run += keep[i];
+  }
+  if (t == kSmallScanThreads - 1) *total = part[t];
+}
+
+// Resets the goal flag and witness of a meet check.
+__global__ void k_goal_reset(int* found, unsigned long long* witness) {
+  *found = 0;
+  *witness = ~0ull;
+}
+
 __global__ void k_scan_total(const uint32_t* keep, const uint32_t* pos, uint64_t count, uint32_t* total) {
   *total = count ? pos[count - 1] + keep[count - 1] : 0u;
 }
@@ -246,6 +300,25 @@ hipError_t launch_hash_resolve(const uint32_t* slot_of, const uint32_t* table, u
     hipStream_t st) {
   if (!count) return hipSuccess;
   hipLaunchKernelGGL(k_hash_resolve, dim3(blocks_for(count)), dim3(256), 0, st, slot_of, table, count, keep);
+  return hipGetLastError();
+}
+
+hipError_t launch_keep_scan(int mode, const uint32_t* slot_of, const uint32_t* table, const void* dead_bits,
+    uint64_t count, uint32_t* keep, uint32_t* pos, uint32_t* total, void* temp, size_t* temp_bytes, hipStream_t st) {
+  if (count <= kSmallScanMax) {
+    hipLaunchKernelGGL(k_keep_scan_small, dim3(1), dim3(kSmallScanThreads), 0, st, mode, slot_of, table,
+        static_cast<const uint8_t*>(dead_bits), count, keep, pos, total);
+    return hipGetLastError();
+  }
+  hipError_t e = hipSuccess;
+  if (mode == 1) e = launch_hash_resolve(slot_of, table, count, keep, st);
+  else if (mode == 2) e = launch_alive(1, dead_bits, count, keep, st);
+  if (e != hipSuccess) return e;
+  return launch_scan(keep, pos, count, total, temp, temp_bytes, st);
+}
+
+hipError_t launch_goal_reset(int* found, unsigned long long* witness, hipStream_t st) {
+  hipLaunchKernelGGL(k_goal_reset, dim3(1), dim3(1), 0, st, found, witness);
   return hipGetLastError();
 }
 

@@ -99,6 +99,14 @@ hipError_t launch_hash_rebuild(int W, const uint64_t* arena, uint64_t rows, uint
     hipStream_t st);
 // keep[i] = !(bit i of dead) for i < count.
 hipError_t launch_alive(int W, const void* dead_bits, uint64_t count, uint32_t* keep, hipStream_t st);
+// Keep flags (mode 0 given, 1 = hash-slot owners as launch_hash_resolve,
+// 2 = clear bits of dead_bits as launch_alive) and their exclusive scan with
+// the total, in one launch for short lists and with the scan above
+// otherwise (temp sized as for launch_scan).
+hipError_t launch_keep_scan(int mode, const uint32_t* slot_of, const uint32_t* table, const void* dead_bits,
+    uint64_t count, uint32_t* keep, uint32_t* pos, uint32_t* total, void* temp, size_t* temp_bytes, hipStream_t st);
+// *found = 0, *witness = ~0 (before a meet check).
+hipError_t launch_goal_reset(int* found, unsigned long long* witness, hipStream_t st);
 // out[j] = rows[lo + i] for kept i (j = pos[i]); idx_out[j] = lo + i.
 hipError_t launch_select(int W, const uint64_t* rows, uint64_t lo, const uint32_t* keep, const uint32_t* pos,
     uint64_t count, uint64_t* out, uint32_t* idx_out, hipStream_t st);
