@@ -218,7 +218,8 @@ int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint32
  * Time every launch shape of the subset kernel (tile mask width 64/32,
  * workgroup size, workgroups aimed at, early exit; *early_exit returns
  * early_exit | mask_bits << 8; for a size class that takes the indexed
- * kernel, see sw_set_subset_kernel, only the workgroups aimed at) on seeded
+ * kernel, see sw_set_subset_kernel, the workgroups aimed at and the early
+ * exit) on seeded
  * random lists of the given sizes and densities on the device; the fastest
  * is kept in the context and used by later calls of the same size class
  * (exact search, batch, the seams).
@@ -241,6 +242,16 @@ int sw_set_subset_shape(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_co
  * launches when indexed == 0, walk the bits of each stream set per tile.
  * Both give the same results.  No reference counterpart. */
 int sw_set_subset_kernel(sw_ctx* ctx, int indexed, uint64_t min_pairs);
+
+/* How the indexed kernel walks the states of each stream set.  walk = 0 (the
+ * default): as each launch asks — the exact search's goal checks and
+ * dominance passes list every stream set's states rarest first (fewest sets
+ * of the other list hold them) and drop a stream set as soon as no set of the
+ * other list can still hold it; other launches take the tuned shape.  1: set
+ * order, no early exit, for every launch.  2: rarest first with early exit,
+ * for every launch.  Results are the same in every mode.  No reference
+ * counterpart. */
+int sw_set_subset_walk(sw_ctx* ctx, int walk);
 
 /* -------------------------------------------------------- learned upper bound
  * A BF16 MLP (MFMA, FP32 accumulation) that scores beam candidates; lower

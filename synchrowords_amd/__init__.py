@@ -123,6 +123,7 @@ _SIGS = {
     "sw_autotune_subset": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint64, C.c_float, C.c_float, C.c_uint32, _U32P,
                                      _U32P, _U32P, C.POINTER(C.c_float)]),
     "sw_set_subset_kernel": (C.c_int, [_P, C.c_int, C.c_uint64]),
+    "sw_set_subset_walk": (C.c_int, [_P, C.c_int]),
     "sw_calibrate_lds": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_double)]),
     "sw_set_subset_shape": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
 }
@@ -451,6 +452,11 @@ class Context:
     def set_subset_kernel(self, indexed: bool = True, min_pairs: int = 0) -> None:
         """Indexed subset kernel for launches of at least min_pairs pairs (0: 2^30), or never."""
         _check(load().sw_set_subset_kernel(self.handle, int(indexed), min_pairs))
+
+    def set_subset_walk(self, walk: int = 0) -> None:
+        """How the indexed kernel walks stream sets: as each launch asks (0),
+        set order without early exit (1), rarest first with early exit (2)."""
+        _check(load().sw_set_subset_walk(self.handle, walk))
 
     def calibrate_lds(self, bytes_per_lane: int = 8) -> float:
         """Measured LDS read bandwidth of the device, TB/s."""

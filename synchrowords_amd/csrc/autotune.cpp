@@ -6,8 +6,8 @@
 // search, batch labelling, the seams) launch with it.  For the bit-walking
 // kernel the shape is the mask width (64/32), the workgroup size
 // (256/512/1024 threads), the workgroups aimed at (256..4096) and the
-// per-word early exit; for a class that takes the indexed kernel, only the
-// workgroups aimed at.
+// per-word early exit; for a class that takes the indexed kernel, the
+// workgroups aimed at and the early exit.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -85,8 +85,10 @@ int sw_autotune_subset(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_cou
     const bool indexed = ctx->ix_enabled && s_count * t_count >= ctx->ix_min_pairs && vsub_ix_fits(n, W, false);
     if (indexed) {
       // Launches of this class take the indexed kernel (1024 threads, 64-bit
-      // masks): only the number of workgroups aimed at is free.  The index is
-      // built by the first (warm) call and reused by the timed ones.
+      // masks): the number of workgroups aimed at and the early exit are
+      // free.  The index is built by each warm call and reused by the timed
+      // ones.
+      for (uint32_t ee = 0; ee < 2; ++ee)
       for (uint32_t wgs : {256u, 512u, 1024u, 2048u, 4096u}) {
         VsubParams p;
         p.stream = S.as<uint64_t>();
@@ -97,6 +99,7 @@ int sw_autotune_subset(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_cou
         p.mode = VSUB_COUNT;
         p.count_out = out.as<unsigned long long>();
         p.target_wgs = wgs;
+        p.early_exit = int(ee);
         run_subset(ctx, W, p);  // warm; builds the index
         SW_CHECK_HIP(hipEventRecord(e0, st));
         for (uint32_t r = 0; r < (reps ? reps : 3); ++r) SW_CHECK_HIP(launch_vsub_ix(W, p, st));
@@ -105,7 +108,7 @@ int sw_autotune_subset(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_cou
         float t = 0;
         SW_CHECK_HIP(hipEventElapsedTime(&t, e0, e1));
         t /= float(reps ? reps : 3);
-        if (t < best.ms) best = {1024, wgs, 0, 64, t};
+        if (t < best.ms) best = {1024, wgs, ee, 64, t};
       }
     }
     const uint32_t th[] = {256, 512, 1024};
@@ -184,6 +187,23 @@ int sw_set_subset_kernel(sw_ctx* ctx, int indexed, uint64_t min_pairs) {
     SW_REQUIRE(ctx, "NULL argument");
     ctx->ix_enabled = indexed ? 1 : 0;
     ctx->ix_min_pairs = min_pairs ? min_pairs : (1ull << 30);
+    return SW_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  }
+}
+
+// How the indexed kernel walks the states of a stream set: as each launch
+// asks (0: the exact search asks for 2, other launches take the tuned or
+// default shape), in set order without early exit (1), or rarest-in-tile
+// first with early exit (2).
+int sw_set_subset_walk(sw_ctx* ctx, int walk) {
+  using namespace sw;
+  try {
+    SW_REQUIRE(ctx, "NULL argument");
+    SW_REQUIRE(walk >= 0 && walk <= 2, "walk must be 0, 1 or 2");
+    ctx->ix_walk = walk;
     return SW_OK;
   } catch (const Error& e) {
     set_error(e.msg);

@@ -39,6 +39,7 @@
 #include <chrono>
 #include <cstdio>
 #include <climits>
+#include <string>
 
 #include "sw_host.hpp"
 #include "sw_kernels.hpp"
@@ -131,9 +132,14 @@ class Search {
       const bool met = goal_check(&wit);
       const double gms = ms_since(g0);
       ms_goal_ += gms;
-      if (cfg_.verbose)
+      if (cfg_.verbose) {
         std::fprintf(stderr, "[sw exact] len %llu goal |F| %llu x |B| %llu  %.3f ms\n", (unsigned long long)t,
             (unsigned long long)F_.fcount, (unsigned long long)B_.fcount, gms);
+        if (W_ <= 5) {
+          std::fprintf(stderr, "[sw exact]   forward sizes %s\n", size_quantiles(F_, false).c_str());
+          std::fprintf(stderr, "[sw exact]   backward sizes %s\n", size_quantiles(B_, true).c_str());
+        }
+      }
       if (met) {
         found = true;
         break;
@@ -200,6 +206,42 @@ class Search {
   size_t& scan_bytes_;
 
   uint64_t row_bytes() const { return uint64_t(W_) * 8; }
+
+  // The search's lists meet rarely: a stream set usually lies in no tile set
+  // and is dropped after its first few states, rarest in the tile first; and
+  // sorted by size, a stream set is tested only against tiles with a set at
+  // least as large.
+  // Cardinalities of a frontier (backward: of the preimages, i.e. the rows
+  // complemented): min, deciles 1/5/9, max.  Verbose log only.
+  std::string size_quantiles(Side& S, bool complement) {
+    DevBuf h;
+    h.ensure(size_t(n_ + 1) * 4);
+    SW_CHECK_HIP(launch_fill_u32(h.as<uint32_t>(), 0u, n_ + 1, st_));
+    SW_CHECK_HIP(launch_size_hist(int(W_), S.front.as<uint64_t>(), S.fcount, n_, complement ? 1 : 0,
+        h.as<uint32_t>(), st_));
+    std::vector<uint32_t> hist(n_ + 1);
+    SW_CHECK_HIP(hipMemcpyAsync(hist.data(), h.p, hist.size() * 4, hipMemcpyDeviceToHost, st_));
+    SW_CHECK_HIP(hipStreamSynchronize(st_));
+    const double qs[] = {0.0, 0.1, 0.5, 0.9, 1.0};
+    std::string out;
+    for (double q : qs) {
+      const uint64_t want = std::min<uint64_t>(S.fcount - 1, uint64_t(q * double(S.fcount)));
+      uint64_t acc = 0;
+      uint32_t c = 0;
+      for (; c <= n_; ++c) {
+        acc += hist[c];
+        if (acc > want) break;
+      }
+      out += std::to_string(c) + " ";
+    }
+    return out;
+  }
+
+  static void search_walk(VsubParams& p) {
+    p.early_exit = 1;
+    p.order_states = 1;
+    p.sort_by_size = 1;
+  }
   static double ms_since(std::chrono::steady_clock::time_point t) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
   }
@@ -379,6 +421,7 @@ class Search {
         p.diag = -int64_t(lo);
       }
       apply_tuning(ctx_, W, p);
+      search_walk(p);
       run_subset(ctx_, W, p);
       pair_tests_ += S.rows * nnew;
       ensure_scan(nnew);
@@ -419,6 +462,7 @@ class Search {
     p.found = found;
     p.witness = wit;
     apply_tuning(ctx_, int(W_), p);
+    search_walk(p);
     run_subset(ctx_, int(W_), p);
     pair_tests_ += F_.fcount * B_.fcount;
     // found (4 bytes at +8) and witness (8 bytes at +16) in one copy

@@ -35,6 +35,22 @@ namespace {
 
 constexpr int kIxTile = 4096;
 constexpr int kIxPrefetch = 8;  // uint4 groups of offsets (32 states) kept in registers
+// With early exit: groups per pair kept at a fixed stride (ix_head), and
+// pairs whose head groups are loaded ahead per wave.
+#ifndef SW_IX_HEAD
+#define SW_IX_HEAD 2
+#endif
+#ifndef SW_IX_RING
+#define SW_IX_RING 8
+#endif
+#ifndef SW_IX_REV
+#define SW_IX_REV 1
+#endif
+#ifndef SW_IX_EE_WGS
+#define SW_IX_EE_WGS 1024
+#endif
+constexpr int kIxHead = SW_IX_HEAD;
+constexpr int kIxRing = SW_IX_RING;
 
 __device__ inline uint32_t and3u(uint32_t a, uint32_t b, uint32_t c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0x80); }
 
@@ -57,8 +73,8 @@ __device__ inline const uint4& row16(const unsigned char* smem, uint32_t byte) {
 // |S_s| (complemented first when cmask is all ones) and entries[i] = 8 G, G =
 // max of the two sizes rounded up to 4, over 4.
 template <int W>
-__global__ void k_ix_pair_count(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n, uint64_t cmask,
-    uint32_t* __restrict__ size, uint32_t* __restrict__ entries) {
+__global__ void k_ix_pair_count(const uint64_t* __restrict__ sets, const uint32_t* __restrict__ perm, uint64_t count,
+    uint32_t n, uint64_t cmask, uint32_t* __restrict__ size, uint32_t* __restrict__ entries) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint64_t pairs = (count + 1) / 2;
   if (i >= pairs) return;
@@ -66,8 +82,9 @@ __global__ void k_ix_pair_count(const uint64_t* __restrict__ sets, uint64_t coun
   for (int h = 0; h < 2; ++h) {
     const uint64_t s = 2 * i + h;
     if (s >= count) break;
+    const uint64_t* set = sets + (perm ? uint64_t(perm[s]) : s) * W;
 #pragma unroll
-    for (int w = 0; w < W; ++w) c[h] += __popcll((sets[s * W + w] ^ cmask) & word_mask(n, w));
+    for (int w = 0; w < W; ++w) c[h] += __popcll((set[w] ^ cmask) & word_mask(n, w));
     size[s] = c[h];
   }
   const uint32_t g = (max(c[0], c[1]) + 3u) / 4u;
@@ -76,9 +93,15 @@ __global__ void k_ix_pair_count(const uint64_t* __restrict__ sets, uint64_t coun
 
 // list[start[i] + 8k + 4h + j] = row byte offset (q * 512) of the (4k + j)-th
 // state of set 2i + h; the padding row's offset (n * 512) past its end.
-template <int W>
-__global__ void k_ix_pair_fill(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n, uint64_t cmask,
-    const uint32_t* __restrict__ start, uint32_t* __restrict__ list) {
+// States are listed in ascending q, or, when RANKED, in ascending rank[q]
+// (the set is first re-spelled over ranks, then walked; order[r] is the row
+// offset of the state of rank r).
+// When head != NULL, the first kIxHead groups of every pair are also
+// written at a fixed stride: head[((i kIxHead + g) 2 + h) 4 + j], padded.
+template <int W, bool RANKED>
+__global__ void k_ix_pair_fill(const uint64_t* __restrict__ sets, const uint32_t* __restrict__ perm, uint64_t count,
+    uint32_t n, uint64_t cmask, const uint32_t* __restrict__ start, const uint32_t* __restrict__ rank,
+    const uint32_t* __restrict__ order, uint32_t* __restrict__ list, uint32_t* __restrict__ head) {
   const uint64_t s = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint64_t pairs = (count + 1) / 2;
   if (s >= 2 * pairs) return;
@@ -86,25 +109,165 @@ __global__ void k_ix_pair_fill(const uint64_t* __restrict__ sets, uint64_t count
   const uint32_t h = uint32_t(s & 1);
   uint32_t* out = list + start[i] + 4 * h;
   const uint32_t slots = (start[i + 1] - start[i]) / 2;  // 4 G
+  auto emit = [&](uint32_t j, uint32_t v) {
+    if (j < slots) out[8 * (j >> 2) + (j & 3)] = v;
+    if (head && j < 4u * kIxHead) head[((i * kIxHead + (j >> 2)) * 2 + h) * 4 + (j & 3)] = v;
+  };
   uint32_t j = 0;
   if (s < count) {
+    const uint64_t* set = sets + (perm ? uint64_t(perm[s]) : s) * W;
+    uint64_t x[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) x[w] = (set[w] ^ cmask) & word_mask(n, w);
+    if (RANKED) {
+      uint64_t y[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) y[w] = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        uint64_t b = x[w];
+        while (b) {
+          const uint32_t r = rank[uint32_t(w) * 64 + __builtin_ctzll(b)];
+          b &= b - 1;
+#pragma unroll
+          for (int v = 0; v < W; ++v)  // constant indices only: y stays in registers
+            if (int(r >> 6) == v) y[v] |= 1ull << (r & 63);
+        }
+      }
+#pragma unroll
+      for (int w = 0; w < W; ++w) x[w] = y[w];
+    }
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      uint64_t b = (sets[s * W + w] ^ cmask) & word_mask(n, w);
+      uint64_t b = x[w];
       while (b) {
         const uint32_t q = uint32_t(w) * 64 + __builtin_ctzll(b);
         b &= b - 1;
-        out[8 * (j >> 2) + (j & 3)] = q * 512u;
+        emit(j, RANKED ? order[q] : q * 512u);
         ++j;
       }
     }
   }
-  for (; j < slots; ++j) out[8 * (j >> 2) + (j & 3)] = n * 512u;
+  const uint32_t pad_to = head ? max(slots, 4u * kIxHead) : slots;
+  for (; j < pad_to; ++j) emit(j, n * 512u);
 }
 
-template <int W, int MODE>
+// freq[q] += sets holding state q.  A wave takes 64 sets at a time; ballot b
+// of word w says which of them hold state 64 w + b, and lane b keeps the
+// count of that state.
+template <int W>
+__global__ __launch_bounds__(256) void k_state_freq(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n,
+    uint64_t cmask, uint32_t* __restrict__ freq) {
+  __shared__ uint32_t acc[W * 64];
+  for (int t = threadIdx.x; t < W * 64; t += blockDim.x) acc[t] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const uint64_t waves = uint64_t(gridDim.x) * (blockDim.x >> 6);
+  uint32_t c[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) c[w] = 0;
+  for (uint64_t g = uint64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); g * 64 < count; g += waves) {
+    const uint64_t t = g * 64 + lane;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const uint64_t x = t < count ? ((sets[t * W + w] ^ cmask) & word_mask(n, w)) : 0ull;
+      for (int b = 0; b < 64; ++b) {
+        const uint32_t pc = __popcll(__builtin_amdgcn_ballot_w64((x >> b) & 1ull));
+        c[w] += (lane == b) ? pc : 0u;
+      }
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    if (c[w]) atomicAdd(&acc[w * 64 + lane], c[w]);
+  __syncthreads();
+  for (int t = threadIdx.x; t < W * 64; t += blockDim.x)
+    if (uint32_t(t) < n && acc[t]) atomicAdd(&freq[t], acc[t]);
+}
+
+// hist[c] += sets of cardinality c (0 <= c <= n); hist must start at zero.
+template <int W>
+__global__ __launch_bounds__(256) void k_size_hist(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n,
+    uint64_t cmask, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[W * 64 + 1];
+  for (int t = threadIdx.x; t <= W * 64; t += blockDim.x) h[t] = 0;
+  __syncthreads();
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) c += __popcll((sets[i * W + w] ^ cmask) & word_mask(n, w));
+    atomicAdd(&h[c], 1u);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t <= W * 64; t += blockDim.x)
+    if (uint32_t(t) <= n && h[t]) atomicAdd(&hist[t], h[t]);
+}
+
+// One thread: exclusive (cursor) and inclusive (bound) prefix sums of hist[0..n].
+__global__ void k_size_offsets(const uint32_t* __restrict__ hist, uint32_t n, uint32_t* __restrict__ cursor,
+    uint32_t* __restrict__ bound) {
+  if (threadIdx.x != 0) return;
+  uint32_t acc = 0;
+  for (uint32_t c = 0; c <= n; ++c) {
+    cursor[c] = acc;
+    acc += hist[c];
+    if (bound) bound[c] = acc;
+  }
+}
+
+// A workgroup takes kScatterPer sets per thread, counts them per size in
+// LDS, reserves one range per size with a single global atomic, then writes.
+constexpr int kScatterPer = 8;
+
+template <int W>
+__global__ __launch_bounds__(256) void k_size_scatter(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n,
+    uint64_t cmask, uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm) {
+  __shared__ uint32_t cnt[W * 64 + 1], base[W * 64 + 1];
+  for (int t = threadIdx.x; t <= W * 64; t += blockDim.x) cnt[t] = 0;
+  __syncthreads();
+  const uint64_t i0 = uint64_t(blockIdx.x) * blockDim.x * kScatterPer + threadIdx.x;
+  uint32_t c[kScatterPer], local[kScatterPer];
+#pragma unroll
+  for (int e = 0; e < kScatterPer; ++e) {
+    const uint64_t i = i0 + uint64_t(e) * blockDim.x;
+    c[e] = 0;
+    local[e] = 0;
+    if (i < count) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) c[e] += __popcll((sets[i * W + w] ^ cmask) & word_mask(n, w));
+      local[e] = atomicAdd(&cnt[c[e]], 1u);
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t <= W * 64; t += blockDim.x)
+    if (uint32_t(t) <= n && cnt[t]) base[t] = atomicAdd(&cursor[t], cnt[t]);
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < kScatterPer; ++e) {
+    const uint64_t i = i0 + uint64_t(e) * blockDim.x;
+    if (i < count) perm[base[c[e]] + local[e]] = uint32_t(i);
+  }
+}
+
+// One workgroup: rank[q] = #{q' : (freq[q'], q') < (freq[q], q)}.
+__global__ __launch_bounds__(512) void k_state_rank(const uint32_t* __restrict__ freq, uint32_t n,
+    uint32_t* __restrict__ rank, uint32_t* __restrict__ order) {
+  __shared__ uint32_t f[512];
+  for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) f[t] = freq[t];
+  __syncthreads();
+  for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) {
+    uint32_t r = 0;
+    const uint32_t fq = f[q];
+    for (uint32_t u = 0; u < n; ++u) r += (f[u] < fq || (f[u] == fq && u < q)) ? 1u : 0u;
+    rank[q] = r;
+    order[r] = q * 512u;
+  }
+}
+
+template <int W, int MODE, bool EE>
 __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t tile_max;  // largest tile set (sorted lists)
   uint64_t* rows = reinterpret_cast<uint64_t*>(smem);  // [(n + 1) rows][64 masks]
   const uint32_t n = p.n;
   uint16_t* tsize = reinterpret_cast<uint16_t*>(smem + size_t(n + 1) * 512);
@@ -112,19 +275,25 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
   const int half = lane >> 5, L = lane & 31;
   const int wave = rfl(threadIdx.x >> 6);
   const int nwaves = blockDim.x >> 6;
-  const uint64_t tile_base = uint64_t(blockIdx.x) * kIxTile;
+  // Lists sorted by size: the tiles of the largest sets, whose workgroups
+  // walk the most stream sets, are dispatched first.
+  const uint32_t tile_ix = (SW_IX_REV && p.t_perm) ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
+  const uint64_t tile_base = uint64_t(tile_ix) * kIxTile;
   const uint64_t tile_left = p.t_count - tile_base;  // > 0 by grid construction
 
   // ---- transpose the tile (as k_vsub): mask g of row q = ballot over tile
   // sets [64 g, 64 g + 64) of "q is in the set".
+  if (threadIdx.x == 0) tile_max = 0;
+  __syncthreads();
   for (int g = wave; g < kIxTile / 64; g += nwaves) {
     const uint64_t t = uint64_t(g) * 64 + lane;
     const bool valid = t < tile_left;
+    const uint64_t* set = p.tile + (valid && p.t_perm ? uint64_t(p.t_perm[tile_base + t]) : tile_base + t) * W;
     int pc = 0;
     for (int w = 0; w < W; ++w) {
       uint64_t x = 0;
       if (valid) {
-        x = p.tile[(tile_base + t) * W + w];
+        x = set[w];
         if (p.complement_tile) x = ~x & word_mask(n, w);
       }
       pc += __popcll(x);
@@ -137,6 +306,7 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
       if (q < n) rows[size_t(q) * 64 + g] = kept;
     }
     if (p.proper_by_size) tsize[t] = uint16_t(valid ? pc : 0xffff);
+    if (p.s_bound && valid) atomicMax(&tile_max, uint32_t(pc));
   }
   if (wave == 0) rows[size_t(n) * 64 + lane] = ~0ull;  // padding row
   __syncthreads();
@@ -155,7 +325,13 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
   const uint64_t s_begin = uint64_t(blockIdx.y) * p.s_chunk;  // even
   uint64_t s_end = s_begin + p.s_chunk;
   if (s_end > p.s_count) s_end = p.s_count;
-  const uint64_t p_end = (s_end + 1) / 2;  // pairs [s_begin / 2, p_end)
+  uint64_t p_end = (s_end + 1) / 2;  // pairs [s_begin / 2, p_end)
+  if (p.s_bound) {
+    // Stream sorted by size: no set past the first s_bound[tile_max] fits in
+    // a tile set; the pair holding the last that may is walked whole.
+    const uint64_t lim = (uint64_t(p.s_bound[tile_max]) + 1) / 2;
+    if (lim < p_end) p_end = lim;
+  }
   const uint32_t* __restrict__ start = p.ix_start;
   const uint4* __restrict__ list4 = reinterpret_cast<const uint4*>(p.ix_list);
   const uint32_t lane_off = uint32_t(L) * 16u;
@@ -164,45 +340,20 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
   R4 hits = {{0u, 0u, 0u, 0u}};
   uint32_t iter = 0;
 
-  // Walk one pair whose first groups are in `cur` (G groups from entry e0),
-  // while the next pair's first groups load into `nxt`.
-  auto walk = [&](uint64_t pi, uint32_t e0, uint32_t G, uint4 (&cur)[kIxPrefetch], uint64_t pn, uint32_t& en,
-                  uint32_t& Gn, uint4 (&nxt)[kIxPrefetch]) {
-    if (pn < p_end) {
-      en = rfl(start[pn]);
-      Gn = (rfl(start[pn + 1]) - en) / 8u;
-    } else {
-      en = 0;
-      Gn = 0;
-    }
-#pragma unroll
-    for (int k = 0; k < kIxPrefetch; ++k)
-      if (uint32_t(k) < Gn) nxt[k] = list4[(en >> 2) + 2 * k + half];
-    const uint64_t sh = 2 * pi + half;
-    const bool live = sh < s_end;
-    R4 r = valid;
-    if (!live) r = R4{{0u, 0u, 0u, 0u}};
-#pragma unroll
-    for (int k = 0; k < kIxPrefetch; ++k) {
-      if (uint32_t(k) >= G) break;
-      const uint4 o = cur[k];
-      const uint4 m0 = row16(smem, o.x + lane_off);
-      const uint4 m1 = row16(smem, o.y + lane_off);
-      const uint4 m2 = row16(smem, o.z + lane_off);
-      const uint4 m3 = row16(smem, o.w + lane_off);
-      fold2(r, m0, m1);
-      fold2(r, m2, m3);
-    }
-    for (uint32_t k = kIxPrefetch; k < G; ++k) {  // sets of more than 4 kIxPrefetch states
-      const uint4 o = list4[(e0 >> 2) + 2 * k + half];
-      const uint4 m0 = row16(smem, o.x + lane_off);
-      const uint4 m1 = row16(smem, o.y + lane_off);
-      const uint4 m2 = row16(smem, o.z + lane_off);
-      const uint4 m3 = row16(smem, o.w + lane_off);
-      fold2(r, m0, m1);
-      fold2(r, m2, m3);
-    }
+  // Fold one group of four states per half into r; false once neither half
+  // has a tile set left (the accumulator only loses bits).
+  auto fold_group = [&](R4& r, const uint4& o) {
+    const uint4 m0 = row16(smem, o.x + lane_off);
+    const uint4 m1 = row16(smem, o.y + lane_off);
+    const uint4 m2 = row16(smem, o.z + lane_off);
+    const uint4 m3 = row16(smem, o.w + lane_off);
+    fold2(r, m0, m1);
+    fold2(r, m2, m3);
+    return __builtin_amdgcn_ballot_w64((r.v[0] | r.v[1] | r.v[2] | r.v[3]) != 0) != 0;
+  };
 
+  // The pair's result for stream set sh (this half), r = its tile sets.
+  auto finish = [&](uint64_t sh, bool live, R4& r) {
     if (live && p.diag != INT64_MIN) {
       // Skip the pair (s, t) with s - t == diag: the same element of one list.
       const int64_t t = int64_t(sh) - p.diag - int64_t(tile_base);
@@ -232,14 +383,16 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
     } else if (MODE == VSUB_SCOUNT) {
       uint32_t pc = __popc(r.v[0]) + __popc(r.v[1]) + __popc(r.v[2]) + __popc(r.v[3]);
       for (int off = 16; off > 0; off >>= 1) pc += __shfl_xor(pc, off, 64);  // within the half
-      if (L == 0 && live && pc) atomicAdd(&p.scount[sh], pc);
+      if (L == 0 && live && pc) atomicAdd(&p.scount[p.s_perm ? uint64_t(p.s_perm[sh]) : sh], pc);
     } else if (MODE == VSUB_FLAGS) {
 #pragma unroll
       for (int d = 0; d < 4; ++d) hits.v[d] |= r.v[d];
     } else if (MODE == VSUB_SFLAGS) {
       const uint64_t bal = __builtin_amdgcn_ballot_w64(any);
-      if (L == 0 && live && ((bal >> (32 * half)) & 0xffffffffull))
-        atomicOr(reinterpret_cast<unsigned long long*>(p.flags) + (sh >> 6), 1ull << (sh & 63));
+      if (L == 0 && live && ((bal >> (32 * half)) & 0xffffffffull)) {
+        const uint64_t si = p.s_perm ? uint64_t(p.s_perm[sh]) : sh;
+        atomicOr(reinterpret_cast<unsigned long long*>(p.flags) + (si >> 6), 1ull << (si & 63));
+      }
     } else if (MODE == VSUB_ANY) {
       if (any) {
         int b = 0;
@@ -248,34 +401,142 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
             b = d * 32 + __builtin_ctz(r.v[d]);
             break;
           }
-        const unsigned long long wv = (sh << 32) | (tile_base + uint64_t(L) * 128 + uint64_t(b));
+        const uint64_t tp = tile_base + uint64_t(L) * 128 + uint64_t(b);
+        const uint64_t si = p.s_perm ? uint64_t(p.s_perm[sh]) : sh;
+        const uint64_t ti = p.t_perm ? uint64_t(p.t_perm[tp]) : tp;
+        const unsigned long long wv = (si << 32) | ti;
         atomicMin(p.witness, wv);
         __hip_atomic_store(p.found, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
   };
 
-  // Pairs of this wave: s_begin / 2 + wave, then every nwaves-th.  Two
-  // register buffers take turns (no copies).
-  uint4 bufA[kIxPrefetch], bufB[kIxPrefetch];
-  uint64_t pi = s_begin / 2 + uint64_t(wave);
-  uint32_t eA = 0, GA = 0, eB = 0, GB = 0;
-  if (pi < p_end) {
-    eA = rfl(start[pi]);
-    GA = (rfl(start[pi + 1]) - eA) / 8u;
+  const uint64_t pi0 = s_begin / 2 + uint64_t(wave);  // pairs of this wave: pi0, then every nwaves-th
+  if (EE) {
+    // Most stream sets are dropped within the first groups, so a pair's cost
+    // is the latency of finding its states: the first kIxHead groups of
+    // every pair sit at a fixed stride (ix_head), loaded kIxRing pairs ahead
+    // without a lookup; only a pair still alive after them reads its start.
+    // Every load of the ring is unconditional (past the last pair it reads
+    // the last pair again), so the compiler can count the loads in flight and
+    // wait for exactly the one it needs.
+    const uint4* __restrict__ head4 = reinterpret_cast<const uint4*>(p.ix_head);
+    const uint32_t pe = uint32_t(p_end);  // pairs < 2^30
+    const uint32_t last = pe ? pe - 1u : 0u;
+    const uint32_t stride = uint32_t(kIxRing) * uint32_t(nwaves);
+    auto head_at = [&](uint32_t pk, int g) {  // the list has at least one pair
+      const uint32_t pc = pk < pe ? pk : last;
+      return head4[(uint64_t(pc) * kIxHead + g) * 2 + half];
+    };
+    uint4 ring[kIxRing][kIxHead];
+    const uint32_t p0 = uint32_t(pi0);
+    // Issued in slot order (the scheduler may not reorder them), as the loop
+    // reloads them, so the loop's first slot waits for its own loads only.
 #pragma unroll
-    for (int k = 0; k < kIxPrefetch; ++k)
-      if (uint32_t(k) < GA) bufA[k] = list4[(eA >> 2) + 2 * k + half];
-  }
-  while (pi < p_end) {
-    if (MODE == VSUB_ANY && (iter++ & 7) == 0) {
-      if (rfl(__hip_atomic_load(p.found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
+    for (int k = 0; k < kIxRing; ++k) {
+#pragma unroll
+      for (int g = 0; g < kIxHead; ++g) ring[k][g] = head_at(p0 + uint32_t(k) * nwaves, g);
+      __builtin_amdgcn_sched_barrier(0);
     }
-    walk(pi, eA, GA, bufA, pi + nwaves, eB, GB, bufB);
-    pi += nwaves;
-    if (pi >= p_end) break;
-    walk(pi, eB, GB, bufB, pi + nwaves, eA, GA, bufA);
-    pi += nwaves;
+    for (uint32_t base = p0; base < pe; base += stride) {
+      int stop = 0;
+      if (MODE == VSUB_ANY) stop = __hip_atomic_load(p.found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+      for (int k = 0; k < kIxRing; ++k) {
+        const uint32_t pk = base + uint32_t(k) * nwaves;
+        // Past the wave's last pair nothing is folded, but the ring is still
+        // reloaded, so every path issues the same loads.
+        // A half past the end of the list holds padding only and stays alive
+        // with its valid mask; it is cleared before its result is taken.
+        R4 r = valid;
+        bool alive = pk < pe;
+        if (alive) {
+#pragma unroll
+          for (int g = 0; g < kIxHead; ++g) {
+            if (!fold_group(r, ring[k][g])) {
+              alive = false;
+              break;
+            }
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < kIxHead; ++g) ring[k][g] = head_at(pk + stride, g);
+        if (alive) {  // else r is zero in both halves: no result to take
+          const uint32_t e0 = rfl(start[pk]);
+          const uint32_t G = (rfl(start[pk + 1]) - e0) / 8u;
+          for (uint32_t g = kIxHead; g < G; ++g)
+            if (!fold_group(r, list4[(e0 >> 2) + 2 * g + half])) break;
+          const uint64_t sh = 2 * uint64_t(pk) + half;
+          const bool live = sh < s_end;
+          if (!live) r = R4{{0u, 0u, 0u, 0u}};
+          finish(sh, live, r);
+        }
+      }
+      if (MODE == VSUB_ANY && rfl(stop)) break;
+    }
+  } else {
+    // Walk one pair whose first groups are in `cur` (G groups from entry e0),
+    // while the next pair's first groups load into `nxt`.
+    auto walk = [&](uint64_t pi, uint32_t e0, uint32_t G, uint4 (&cur)[kIxPrefetch], uint64_t pn, uint32_t& en,
+                    uint32_t& Gn, uint4 (&nxt)[kIxPrefetch]) {
+      if (pn < p_end) {
+        en = rfl(start[pn]);
+        Gn = (rfl(start[pn + 1]) - en) / 8u;
+      } else {
+        en = 0;
+        Gn = 0;
+      }
+#pragma unroll
+      for (int k = 0; k < kIxPrefetch; ++k)
+        if (uint32_t(k) < Gn) nxt[k] = list4[(en >> 2) + 2 * k + half];
+      const uint64_t sh = 2 * pi + half;
+      const bool live = sh < s_end;
+      R4 r = valid;
+      if (!live) r = R4{{0u, 0u, 0u, 0u}};
+#pragma unroll
+      for (int k = 0; k < kIxPrefetch; ++k) {
+        if (uint32_t(k) >= G) break;
+        const uint4 o = cur[k];
+        const uint4 m0 = row16(smem, o.x + lane_off);
+        const uint4 m1 = row16(smem, o.y + lane_off);
+        const uint4 m2 = row16(smem, o.z + lane_off);
+        const uint4 m3 = row16(smem, o.w + lane_off);
+        fold2(r, m0, m1);
+        fold2(r, m2, m3);
+      }
+      for (uint32_t k = kIxPrefetch; k < G; ++k) {  // sets of more than 4 kIxPrefetch states
+        const uint4 o = list4[(e0 >> 2) + 2 * k + half];
+        const uint4 m0 = row16(smem, o.x + lane_off);
+        const uint4 m1 = row16(smem, o.y + lane_off);
+        const uint4 m2 = row16(smem, o.z + lane_off);
+        const uint4 m3 = row16(smem, o.w + lane_off);
+        fold2(r, m0, m1);
+        fold2(r, m2, m3);
+      }
+      finish(sh, live, r);
+    };
+
+    // Two register buffers take turns (no copies).
+    uint4 bufA[kIxPrefetch], bufB[kIxPrefetch];
+    uint64_t pi = pi0;
+    uint32_t eA = 0, GA = 0, eB = 0, GB = 0;
+    if (pi < p_end) {
+      eA = rfl(start[pi]);
+      GA = (rfl(start[pi + 1]) - eA) / 8u;
+#pragma unroll
+      for (int k = 0; k < kIxPrefetch; ++k)
+        if (uint32_t(k) < GA) bufA[k] = list4[(eA >> 2) + 2 * k + half];
+    }
+    while (pi < p_end) {
+      if (MODE == VSUB_ANY && (iter++ & 7) == 0) {
+        if (rfl(__hip_atomic_load(p.found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
+      }
+      walk(pi, eA, GA, bufA, pi + nwaves, eB, GB, bufB);
+      pi += nwaves;
+      if (pi >= p_end) break;
+      walk(pi, eB, GB, bufB, pi + nwaves, eA, GA, bufA);
+      pi += nwaves;
+    }
   }
 
   if (MODE == VSUB_COUNT) {
@@ -295,17 +556,23 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
   }
 }
 
-template <int W, int MODE>
-hipError_t launch_ix_mode(const VsubParams& p, uint64_t gx, uint64_t gy, size_t lds, hipStream_t st) {
+template <int W, int MODE, bool EE>
+hipError_t launch_ix_kernel(const VsubParams& p, uint64_t gx, uint64_t gy, size_t lds, hipStream_t st) {
   static std::atomic<bool> attr_set{false};
   if (!attr_set.load(std::memory_order_acquire)) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vsub_ix<W, MODE>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vsub_ix<W, MODE, EE>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 16);  // 16: static tile_max
     if (e != hipSuccess) return e;
     attr_set.store(true, std::memory_order_release);
   }
-  hipLaunchKernelGGL((k_vsub_ix<W, MODE>), dim3(unsigned(gx), unsigned(gy)), dim3(1024), lds, st, p);
+  hipLaunchKernelGGL((k_vsub_ix<W, MODE, EE>), dim3(unsigned(gx), unsigned(gy)), dim3(1024), lds, st, p);
   return hipGetLastError();
+}
+
+template <int W, int MODE>
+hipError_t launch_ix_mode(const VsubParams& p, uint64_t gx, uint64_t gy, size_t lds, hipStream_t st) {
+  return p.early_exit ? launch_ix_kernel<W, MODE, true>(p, gx, gy, lds, st)
+                      : launch_ix_kernel<W, MODE, false>(p, gx, gy, lds, st);
 }
 
 template <int W>
@@ -314,7 +581,8 @@ hipError_t launch_ix_w(const VsubParams& p0, hipStream_t st) {
   if (p.t_count == 0 || p.s_count == 0) return hipSuccess;
   const uint64_t gx = (p.t_count + kIxTile - 1) / kIxTile;
   if (p.s_chunk == 0) {
-    const uint64_t target = p.target_wgs ? p.target_wgs : 1024;
+    uint64_t target = p.target_wgs ? p.target_wgs : 1024;
+    if (p.early_exit && target < SW_IX_EE_WGS) target = SW_IX_EE_WGS;
     const uint64_t gyt = (target + gx - 1) / gx;
     uint64_t chunk = (p.s_count + gyt - 1) / gyt;
     if (chunk < 128) chunk = 128;
@@ -324,7 +592,9 @@ hipError_t launch_ix_w(const VsubParams& p0, hipStream_t st) {
   const uint64_t gy = (p.s_count + p.s_chunk - 1) / p.s_chunk;
   if (gx > 0x7fffffffull || gy > 65535ull) return hipErrorInvalidValue;
   const size_t lds = size_t(p.n + 1) * 512 + (p.proper_by_size ? size_t(kIxTile) * 2 : 0);
-  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  if (lds + 16 > 160 * 1024) return hipErrorInvalidValue;  // 16: static tile_max
+  if (p.s_bound && (!p.s_perm || !p.t_perm || p.diag != INT64_MIN || p.mode == VSUB_FLAGS))
+    return hipErrorInvalidValue;
   switch (p.mode) {
     case VSUB_COUNT: return launch_ix_mode<W, VSUB_COUNT>(p, gx, gy, lds, st);
     case VSUB_ANY: return launch_ix_mode<W, VSUB_ANY>(p, gx, gy, lds, st);
@@ -338,16 +608,17 @@ hipError_t launch_ix_w(const VsubParams& p0, hipStream_t st) {
 }  // namespace
 
 bool vsub_ix_fits(uint32_t n, int W, bool proper_by_size) {
-  return W >= 1 && W <= 5 && size_t(n + 1) * 512 + (proper_by_size ? size_t(kIxTile) * 2 : 0) <= 160 * 1024;
+  // 16 bytes of static LDS (tile_max) besides the rows and sizes.
+  return W >= 1 && W <= 5 && size_t(n + 1) * 512 + (proper_by_size ? size_t(kIxTile) * 2 : 0) + 16 <= 160 * 1024;
 }
 
-hipError_t launch_ix_pair_count(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement,
-    uint32_t* size, uint32_t* entries, hipStream_t st) {
+hipError_t launch_ix_pair_count(int W, const uint64_t* sets, const uint32_t* perm, uint64_t count, uint32_t n,
+    int complement, uint32_t* size, uint32_t* entries, hipStream_t st) {
   if (!count) return hipSuccess;
   const uint64_t cmask = complement ? ~0ull : 0ull;
   const unsigned blocks = unsigned(((count + 1) / 2 + 255) / 256);
   switch (W) {
-#define C(w) case w: hipLaunchKernelGGL(k_ix_pair_count<w>, dim3(blocks), dim3(256), 0, st, sets, count, n, cmask, size, entries); break;
+#define C(w) case w: hipLaunchKernelGGL(k_ix_pair_count<w>, dim3(blocks), dim3(256), 0, st, sets, perm, count, n, cmask, size, entries); break;
     C(1) C(2) C(3) C(4) C(5)
 #undef C
     default: return hipErrorInvalidValue;
@@ -355,23 +626,92 @@ hipError_t launch_ix_pair_count(int W, const uint64_t* sets, uint64_t count, uin
   return hipGetLastError();
 }
 
-hipError_t launch_ix_pair_fill(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement,
-    const uint32_t* start, uint32_t* list, hipStream_t st) {
+int ix_head_groups() { return kIxHead; }
+
+hipError_t launch_ix_pair_fill(int W, const uint64_t* sets, const uint32_t* perm, uint64_t count, uint32_t n,
+    int complement, const uint32_t* start, const uint32_t* rank, const uint32_t* order, uint32_t* list,
+    uint32_t* head, hipStream_t st) {
   if (!count) return hipSuccess;
+  if ((rank == nullptr) != (order == nullptr)) return hipErrorInvalidValue;
   const uint64_t cmask = complement ? ~0ull : 0ull;
   const unsigned blocks = unsigned(((count + 1) / 2 * 2 + 255) / 256);
   switch (W) {
-#define C(w) case w: hipLaunchKernelGGL(k_ix_pair_fill<w>, dim3(blocks), dim3(256), 0, st, sets, count, n, cmask, start, list); break;
+#define C(w)                                                                                                   \
+  case w:                                                                                                      \
+    if (rank)                                                                                                  \
+      hipLaunchKernelGGL((k_ix_pair_fill<w, true>), dim3(blocks), dim3(256), 0, st, sets, perm, count, n,     \
+          cmask, start, rank, order, list, head);                                                              \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_ix_pair_fill<w, false>), dim3(blocks), dim3(256), 0, st, sets, perm, count, n,    \
+          cmask, start, rank, order, list, head);                                                              \
+    break;
     C(1) C(2) C(3) C(4) C(5)
 #undef C
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_state_freq(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement, uint32_t* freq,
+    hipStream_t st) {
+  if (!count) return hipSuccess;
+  const uint64_t cmask = complement ? ~0ull : 0ull;
+  uint64_t blocks = (count + 255) / 256;  // four waves of 64 sets each
+  if (blocks > 2048) blocks = 2048;
+  switch (W) {
+#define C(w) case w: hipLaunchKernelGGL(k_state_freq<w>, dim3(unsigned(blocks)), dim3(256), 0, st, sets, count, n, cmask, freq); break;
+    C(1) C(2) C(3) C(4) C(5)
+#undef C
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_size_hist(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement, uint32_t* hist,
+    hipStream_t st) {
+  if (!count) return hipSuccess;
+  const uint64_t cmask = complement ? ~0ull : 0ull;
+  uint64_t blocks = (count + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  switch (W) {
+#define C(w) case w: hipLaunchKernelGGL(k_size_hist<w>, dim3(unsigned(blocks)), dim3(256), 0, st, sets, count, n, cmask, hist); break;
+    C(1) C(2) C(3) C(4) C(5)
+#undef C
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_size_offsets(const uint32_t* hist, uint32_t n, uint32_t* cursor, uint32_t* bound, hipStream_t st) {
+  hipLaunchKernelGGL(k_size_offsets, dim3(1), dim3(64), 0, st, hist, n, cursor, bound);
+  return hipGetLastError();
+}
+
+hipError_t launch_size_scatter(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement,
+    uint32_t* cursor, uint32_t* perm, hipStream_t st) {
+  if (!count) return hipSuccess;
+  if (count > 0xffffffffull) return hipErrorInvalidValue;
+  const uint64_t cmask = complement ? ~0ull : 0ull;
+  const unsigned blocks = unsigned((count + 256 * kScatterPer - 1) / (256 * kScatterPer));
+  switch (W) {
+#define C(w) case w: hipLaunchKernelGGL(k_size_scatter<w>, dim3(blocks), dim3(256), 0, st, sets, count, n, cmask, cursor, perm); break;
+    C(1) C(2) C(3) C(4) C(5)
+#undef C
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_state_rank(const uint32_t* freq, uint32_t n, uint32_t* rank, uint32_t* order, hipStream_t st) {
+  if (n == 0 || n > 512) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_state_rank, dim3(1), dim3(512), 0, st, freq, n, rank, order);
   return hipGetLastError();
 }
 
 hipError_t launch_vsub_ix(int W, const VsubParams& p, hipStream_t st) {
   if (!p.ix_start || !p.ix_list) return hipErrorInvalidValue;
   if (p.mode == VSUB_ANY && (!p.found || !p.witness)) return hipErrorInvalidValue;
+  if (p.early_exit && !p.ix_head) return hipErrorInvalidValue;
   switch (W) {
     case 1: return launch_ix_w<1>(p, st);
     case 2: return launch_ix_w<2>(p, st);

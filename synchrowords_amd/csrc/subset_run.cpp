@@ -18,12 +18,47 @@ void run_subset(sw_ctx* ctx, int W, VsubParams& p) {
     SW_CHECK_HIP(launch_vsub(W, p, st));
     return;
   }
+  if (ctx->ix_walk == 1) {
+    p.early_exit = 0;
+    p.order_states = 0;
+    p.sort_by_size = 0;
+  } else if (ctx->ix_walk == 2) {
+    p.early_exit = 1;
+    p.order_states = 1;
+    p.sort_by_size = 1;
+  }
   const uint64_t s = p.s_count, pairs = (s + 1) / 2;
+  // Sorting costs two passes over each list and a few launches: worth it
+  // for launches of more than about a millisecond (unless forced).
+  const bool sorted = p.sort_by_size && p.diag == INT64_MIN && p.mode != VSUB_FLAGS &&
+      (ctx->ix_walk == 2 || tests >= (1ull << 34)) && p.t_count < (1ull << 32);
+  p.s_perm = p.t_perm = p.s_bound = nullptr;
+  if (sorted) {
+    const uint32_t n1 = p.n + 1;
+    ctx->ix_sort.ensure(size_t(4) * n1 * 4);
+    uint32_t* hs = ctx->ix_sort.as<uint32_t>();  // stream histogram, then cursor
+    uint32_t* ht = hs + n1;                      // tile histogram, then cursor
+    uint32_t* cs = ht + n1;
+    uint32_t* bs = cs + n1;
+    ctx->ix_sperm.ensure(s * 4 + 16);
+    ctx->ix_tperm.ensure(p.t_count * 4 + 16);
+    SW_CHECK_HIP(launch_fill_u32(hs, 0u, 2 * n1, st));
+    SW_CHECK_HIP(launch_size_hist(W, p.stream, s, p.n, p.complement_stream, hs, st));
+    SW_CHECK_HIP(launch_size_hist(W, p.tile, p.t_count, p.n, p.complement_tile, ht, st));
+    SW_CHECK_HIP(launch_size_offsets(hs, p.n, cs, bs, st));
+    SW_CHECK_HIP(launch_size_offsets(ht, p.n, hs, nullptr, st));  // tile cursor over the stream histogram
+    SW_CHECK_HIP(launch_size_scatter(W, p.stream, s, p.n, p.complement_stream, cs, ctx->ix_sperm.as<uint32_t>(), st));
+    SW_CHECK_HIP(launch_size_scatter(W, p.tile, p.t_count, p.n, p.complement_tile, hs, ctx->ix_tperm.as<uint32_t>(),
+        st));
+    p.s_perm = ctx->ix_sperm.as<uint32_t>();
+    p.t_perm = ctx->ix_tperm.as<uint32_t>();
+    p.s_bound = bs;
+  }
   ctx->ix_padded.ensure(pairs * 4 + 16);  // entries per pair
   ctx->ix_size.ensure(s * 4 + 16);
   ctx->ix_start.ensure((pairs + 1) * 4 + 16);
   if (!ctx->ix_host) SW_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->ix_host), 16, hipHostMallocDefault));
-  SW_CHECK_HIP(launch_ix_pair_count(W, p.stream, s, p.n, p.complement_stream, ctx->ix_size.as<uint32_t>(),
+  SW_CHECK_HIP(launch_ix_pair_count(W, p.stream, p.s_perm, s, p.n, p.complement_stream, ctx->ix_size.as<uint32_t>(),
       ctx->ix_padded.as<uint32_t>(), st));
   size_t need = 0;
   SW_CHECK_HIP(launch_scan(nullptr, nullptr, pairs, nullptr, nullptr, &need, st));
@@ -39,7 +74,25 @@ void run_subset(sw_ctx* ctx, int W, VsubParams& p) {
   SW_CHECK_HIP(hipStreamSynchronize(st));
   const uint64_t total = ctx->ix_host[0];
   ctx->ix_list.ensure(total * 4 + 64);
-  SW_CHECK_HIP(launch_ix_pair_fill(W, p.stream, s, p.n, p.complement_stream, start, ctx->ix_list.as<uint32_t>(), st));
+  const uint32_t* rank = nullptr;
+  const uint32_t* order = nullptr;
+  if (p.order_states) {
+    ctx->ix_order.ensure(3 * 512 * 4);
+    uint32_t* f = ctx->ix_order.as<uint32_t>();
+    SW_CHECK_HIP(launch_fill_u32(f, 0u, 512, st));
+    SW_CHECK_HIP(launch_state_freq(W, p.tile, p.t_count, p.n, p.complement_tile, f, st));
+    SW_CHECK_HIP(launch_state_rank(f, p.n, f + 512, f + 1024, st));
+    rank = f + 512;
+    order = f + 1024;
+  }
+  uint32_t* head = nullptr;
+  if (p.early_exit) {
+    ctx->ix_head.ensure(pairs * size_t(ix_head_groups()) * 32 + 64);
+    head = ctx->ix_head.as<uint32_t>();
+  }
+  SW_CHECK_HIP(launch_ix_pair_fill(W, p.stream, p.s_perm, s, p.n, p.complement_stream, start, rank, order,
+      ctx->ix_list.as<uint32_t>(), head, st));
+  p.ix_head = head;
   p.ix_start = start;
   p.ix_list = ctx->ix_list.as<uint32_t>();
   p.ix_size = ctx->ix_size.as<uint32_t>();

@@ -95,6 +95,11 @@ struct sw_ctx {
   // Stream index of the indexed subset kernel, reused between launches, and
   // when it is used: never (0), or for launches of at least ix_min_pairs pairs.
   sw::DevBuf ix_padded, ix_size, ix_start, ix_list, ix_tmp;
+  sw::DevBuf ix_order;  // state frequencies in the tile, ranks and rank order (3 x 512 words)
+  sw::DevBuf ix_sort, ix_sperm, ix_tperm, ix_head;  // sort by size: histograms and cursors, both permutations
+  // How the indexed kernel walks stream sets: as each launch asks (0), always
+  // in set order without early exit (1), always rarest first with early exit (2).
+  int ix_walk = 0;
   size_t ix_tmp_bytes = 0;
   uint32_t* ix_host = nullptr;  // pinned: the index length read back
   int ix_enabled = 1;

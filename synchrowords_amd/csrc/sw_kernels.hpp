@@ -39,15 +39,54 @@ struct VsubParams {
   const uint32_t* ix_start = nullptr;
   const uint32_t* ix_list = nullptr;
   const uint32_t* ix_size = nullptr;
+  const uint32_t* ix_head = nullptr;  // with early_exit: each pair's first groups, fixed stride
+  // Indexed kernel: list each stream set's states rarest-in-tile first
+  // (fewest tile sets hold them), so that with early_exit a stream set that
+  // lies in no tile set is dropped after few states.
+  int order_states = 0;
+  // Indexed kernel, ANY/COUNT/SCOUNT/SFLAGS without diag: sort both lists by
+  // cardinality first (a stream set can only lie in a tile set at least as
+  // large), so each workgroup stops at the first stream set larger than its
+  // largest tile set, and tiles hold sets of like size.
+  int sort_by_size = 0;
+  // Set by run_subset when sorted: sorted position -> list index of each
+  // list, and s_bound[c] = stream sets of at most c states.
+  const uint32_t* s_perm = nullptr;
+  const uint32_t* t_perm = nullptr;
+  const uint32_t* s_bound = nullptr;
 };
 
 // Indexed subset kernel (k_subset_ix.hip): whether its tile fits LDS for n
 // states, the pre-pass kernels that build the stream index, and the launch.
+// The pre-pass kernels read set s of the list as sets[perm[s]] when perm is
+// given (a list sorted by size), else sets[s].
 bool vsub_ix_fits(uint32_t n, int W, bool proper_by_size);
-hipError_t launch_ix_pair_count(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement,
-    uint32_t* size, uint32_t* entries, hipStream_t st);
-hipError_t launch_ix_pair_fill(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement,
-    const uint32_t* start, uint32_t* list, hipStream_t st);
+hipError_t launch_ix_pair_count(int W, const uint64_t* sets, const uint32_t* perm, uint64_t count, uint32_t n,
+    int complement, uint32_t* size, uint32_t* entries, hipStream_t st);
+hipError_t launch_ix_pair_fill(int W, const uint64_t* sets, const uint32_t* perm, uint64_t count, uint32_t n,
+    int complement, const uint32_t* start, const uint32_t* rank, const uint32_t* order, uint32_t* list,
+    uint32_t* head, hipStream_t st);
+// Groups of four states per half kept at a fixed stride per pair (ix_head):
+// ix_head holds pairs x ix_head_groups() x 32 bytes.
+int ix_head_groups();
+// Sort by cardinality (counting sort; order within one cardinality is
+// arbitrary): from hist (launch_size_hist), cursor[c] = sets of fewer than c
+// states and, when bound != NULL, bound[c] = sets of at most c states; then
+// perm[cursor[|S_i|]++] = i.
+hipError_t launch_size_offsets(const uint32_t* hist, uint32_t n, uint32_t* cursor, uint32_t* bound, hipStream_t st);
+hipError_t launch_size_scatter(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement,
+    uint32_t* cursor, uint32_t* perm, hipStream_t st);
+// Per state q < n, freq[q] += number of sets (complemented when `complement`)
+// holding q; freq must start at zero.
+hipError_t launch_state_freq(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement, uint32_t* freq,
+    hipStream_t st);
+// hist[c] += number of sets (complemented when `complement`) of c states,
+// 0 <= c <= n; hist must start at zero.  W <= 5.
+hipError_t launch_size_hist(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement, uint32_t* hist,
+    hipStream_t st);
+// rank[q] = position of state q in ascending (freq, q) order; order[r] = the
+// LDS row byte offset of the state at position r.
+hipError_t launch_state_rank(const uint32_t* freq, uint32_t n, uint32_t* rank, uint32_t* order, hipStream_t st);
 hipError_t launch_vsub_ix(int W, const VsubParams& p, hipStream_t st);
 
 // Mask width and LDS bytes of the subset kernel.

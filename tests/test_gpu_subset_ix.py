@@ -95,3 +95,59 @@ def test_indexed_and_bitwalk_searches_agree(ixctx, n, k):
             assert a.stats["sets_backward"] == b.stats["sets_backward"]
     finally:
         bw.close()
+
+
+@pytest.mark.parametrize("walk", [1, 2])
+@pytest.mark.parametrize("n", [7, 64, 130, 256, 300])
+def test_walk_orders_give_the_same_results(ixctx, walk, n):
+    """Set-order walk and rarest-first walk with early exit, on lists where
+    most stream sets lie in no tile set (early exit taken) and on lists with
+    many pairs (early exit not taken): counts, per-row counts and the flag
+    equal the oracle's."""
+    rng = np.random.default_rng(700 + n + walk)
+
+    def mixed(count, ps):
+        """Sets of widely different sizes, shuffled (sorting by size matters)."""
+        parts = [random_sets(rng, count // len(ps) + 1, n, q) for q in ps]
+        out = np.concatenate(parts)[:count]
+        return out[rng.permutation(count)]
+
+    ixctx.set_subset_walk(walk)
+    try:
+        for p_small, p_large in ((0.3, 0.3), (0.08, 0.75), (0.02, 0.95), ((0.01, 0.1, 0.4), (0.05, 0.3, 0.9))):
+            if isinstance(p_small, tuple):
+                small, large = mixed(1201, p_small), mixed(4500, p_large)
+            else:
+                small = random_sets(rng, 1201, n, p_small)
+                large = random_sets(rng, 4500, n, p_large)
+            for i in range(0, 1201, 53):
+                small[i] &= large[int(rng.integers(0, 4500))]
+            total, cnt, _ = orc.subset_count(small, large, words(n))
+            res, c, anyf = ixctx.subset_check(n, small, large, per_row=True)
+            assert c == total and anyf == (total > 0)
+            np.testing.assert_array_equal(res, cnt >= 1)
+            _, c2, _ = ixctx.subset_check(n, small, large, count=True)
+            assert c2 == total
+            _, _, any3 = ixctx.subset_check(n, small, large, count=False)
+            assert any3 == (total > 0)
+    finally:
+        ixctx.set_subset_walk(0)
+
+
+def test_walk_orders_prune_the_same_sets(ixctx):
+    """The exact search with either walk: same length, same sets kept (the
+    dominance flags do not depend on the order states are walked in)."""
+    n, k = 300, 2
+    out = {}
+    try:
+        for walk in (1, 2):
+            ixctx.set_subset_walk(walk)
+            d = gen.random_dfa(n, k, 8200)
+            out[walk] = ixctx.exact(n, k, d)
+    finally:
+        ixctx.set_subset_walk(0)
+    a, b = out[1], out[2]
+    assert a.found and b.found and a.length == b.length
+    assert a.stats["sets_forward"] == b.stats["sets_forward"]
+    assert a.stats["sets_backward"] == b.stats["sets_backward"]
+    assert replay(n, k, d, b.word) == 1
