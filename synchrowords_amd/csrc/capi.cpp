@@ -325,7 +325,7 @@ int sw_leaf_run(sw_leaf* leaf, int proper, uint32_t small_pos, uint32_t small_cn
   p.proper_by_size = proper ? 1 : 0;
   p.flags = leaf->flags.p;
   sw::apply_tuning(ctx, int(W), p);
-  SW_CHECK_HIP(sw::launch_vsub(int(W), p, ctx->stream));
+  sw::run_subset(ctx, int(W), p);
   std::unique_ptr<uint8_t[]> bits(new uint8_t[fb]);
   SW_CHECK_HIP(hipMemcpyAsync(bits.get(), leaf->flags.p, fb, hipMemcpyDeviceToHost, ctx->stream));
   SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));

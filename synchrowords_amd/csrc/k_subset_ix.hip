@@ -546,12 +546,25 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
     // Both halves cover the same tile sets: merge, then lane L writes masks 2L, 2L+1.
 #pragma unroll
     for (int d = 0; d < 4; ++d) hits.v[d] |= __shfl_xor(hits.v[d], 32, 64);
-    if (half == 0) {
+    if (half == 0 && !p.t_perm) {
       unsigned long long* f = reinterpret_cast<unsigned long long*>(p.flags) + tile_base / 64 + 2 * L;
       const uint64_t lo = uint64_t(hits.v[0]) | (uint64_t(hits.v[1]) << 32);
       const uint64_t hi = uint64_t(hits.v[2]) | (uint64_t(hits.v[3]) << 32);
       if (lo) atomicOr(f, lo);
       if (hi) atomicOr(f + 1, hi);
+    } else if (half == 0) {
+      // Sorted tile: each hit is flagged at its set's place in the list.
+      unsigned long long* f = reinterpret_cast<unsigned long long*>(p.flags);
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        uint32_t m = hits.v[d];
+        while (m) {
+          const int b = __builtin_ctz(m);
+          m &= m - 1;
+          const uint64_t t = p.t_perm[tile_base + uint64_t(L) * 128 + uint64_t(d) * 32 + uint64_t(b)];
+          atomicOr(f + (t >> 6), 1ull << (t & 63));
+        }
+      }
     }
   }
 }
@@ -593,7 +606,7 @@ hipError_t launch_ix_w(const VsubParams& p0, hipStream_t st) {
   if (gx > 0x7fffffffull || gy > 65535ull) return hipErrorInvalidValue;
   const size_t lds = size_t(p.n + 1) * 512 + (p.proper_by_size ? size_t(kIxTile) * 2 : 0);
   if (lds + 16 > 160 * 1024) return hipErrorInvalidValue;  // 16: static tile_max
-  if (p.s_bound && (!p.s_perm || !p.t_perm || p.diag != INT64_MIN || p.mode == VSUB_FLAGS))
+  if (p.s_bound && (!p.s_perm || !p.t_perm || p.diag != INT64_MIN))
     return hipErrorInvalidValue;
   switch (p.mode) {
     case VSUB_COUNT: return launch_ix_mode<W, VSUB_COUNT>(p, gx, gy, lds, st);

@@ -30,8 +30,13 @@ void run_subset(sw_ctx* ctx, int W, VsubParams& p) {
   const uint64_t s = p.s_count, pairs = (s + 1) / 2;
   // Sorting costs two passes over each list and a few launches: worth it
   // for launches of more than about a millisecond (unless forced).
-  const bool sorted = p.sort_by_size && p.diag == INT64_MIN && p.mode != VSUB_FLAGS &&
-      (ctx->ix_walk == 2 || tests >= (1ull << 34)) && p.t_count < (1ull << 32);
+  const bool want_sort =
+      p.sort_by_size && (ctx->ix_walk == 2 || tests >= (1ull << 34)) && p.t_count < (1ull << 32);
+  if (want_sort && p.diag != INT64_MIN && p.distinct_sets && vsub_ix_fits(p.n, W, true)) {
+    p.diag = INT64_MIN;
+    p.proper_by_size = 1;
+  }
+  const bool sorted = want_sort && p.diag == INT64_MIN;
   p.s_perm = p.t_perm = p.s_bound = nullptr;
   if (sorted) {
     const uint32_t n1 = p.n + 1;

@@ -44,11 +44,15 @@ struct VsubParams {
   // (fewest tile sets hold them), so that with early_exit a stream set that
   // lies in no tile set is dropped after few states.
   int order_states = 0;
-  // Indexed kernel, ANY/COUNT/SCOUNT/SFLAGS without diag: sort both lists by
+  // Indexed kernel, without diag: sort both lists by
   // cardinality first (a stream set can only lie in a tile set at least as
   // large), so each workgroup stops at the first stream set larger than its
   // largest tile set, and tiles hold sets of like size.
   int sort_by_size = 0;
+  // The lists hold no equal sets other than the pairs diag names (both are
+  // one deduplicated list): the indexed kernel may then skip equal sets by
+  // cardinality (proper_by_size) instead of by diag, which sorting allows.
+  int distinct_sets = 0;
   // Set by run_subset when sorted: sorted position -> list index of each
   // list, and s_bound[c] = stream sets of at most c states.
   const uint32_t* s_perm = nullptr;

@@ -151,3 +151,32 @@ def test_walk_orders_prune_the_same_sets(ixctx):
     assert a.stats["sets_forward"] == b.stats["sets_forward"]
     assert a.stats["sets_backward"] == b.stats["sets_backward"]
     assert replay(n, k, d, b.word) == 1
+
+
+@pytest.mark.parametrize("walk", [1, 2])
+@pytest.mark.parametrize("proper", [False, True])
+@pytest.mark.parametrize("n", [64, 300])
+def test_leaf_flags_with_either_walk(ixctx, walk, proper, n):
+    """Per-tile-set flags (the leaf checker) through the indexed kernel; with
+    the rarest-first walk both lists are sorted by size and each flag is
+    written at its set's place in the unsorted list."""
+    rng = np.random.default_rng(40 + n + 2 * walk + proper)
+    W = words(n)
+    small = np.concatenate([random_sets(rng, 200, n, 0.05), random_sets(rng, 201, n, 0.2)])
+    small = small[rng.permutation(401)]
+    large = np.concatenate([random_sets(rng, 2500, n, 0.3), random_sets(rng, 2500, n, 0.8)])
+    large = large[rng.permutation(5000)]
+    large[:40] = small[100:140]  # equal sets: contained but not properly
+    ixctx.set_subset_walk(walk)
+    leaf = ixctx.leaf(n, small, 5000)
+    try:
+        for pos, cnt in [(0, 401), (100, 40), (7, 1)]:
+            got = leaf.run(pos, cnt, large, proper=proper)
+            want = np.zeros(5000, dtype=bool)
+            for j in range(5000):
+                _, cc, _ = orc.subset_count(small[pos:pos + cnt], large[j:j + 1], W, proper=proper)
+                want[j] = cc.sum() > 0
+            np.testing.assert_array_equal(got, want)
+    finally:
+        leaf.close()
+        ixctx.set_subset_walk(0)
