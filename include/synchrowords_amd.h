@@ -163,6 +163,21 @@ int sw_exact_solve(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, u
     const uint32_t* upper_word, uint64_t upper_word_len, const sw_exact_config* cfg, sw_exact_result* out,
     uint32_t* word, uint64_t word_cap);
 
+/* Shortest reset length by breadth-first search over the subsets of Q,
+ * the reference's Brute<N,K>::run (synchrolib/algorithm/brute/brute.hpp:28-122;
+ * its BFS, :88-122): *length = the length, or -1 when the automaton is not
+ * synchronizing.  n <= 32 (k <= 8 once the 2^n-bit visited set no longer fits
+ * LDS, above about 20 states).  Unlike the reference, whose distances are
+ * 8-bit, lengths of 255 and more are found.  Host arrays. */
+int sw_brute(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, int64_t* length);
+
+/* The same for `count` automata (automaton i: ns[i] states, ks[i] letters,
+ * transitions at deltas + delta_offsets[i]); small ones run one workgroup
+ * each in one launch.  No reference counterpart for the batch form (the
+ * reference runs one automaton per plan). */
+int sw_brute_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint32_t* ks, const uint64_t* delta_offsets,
+    const uint32_t* deltas, int64_t* lengths);
+
 /* Result of a plan run: the reference's AlgoResult
  * (synchrolib/algorithm/algorithm.hpp:19-42). */
 typedef struct sw_result {

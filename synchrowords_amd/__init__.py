@@ -124,6 +124,8 @@ _SIGS = {
                                      _U32P, _U32P, C.POINTER(C.c_float)]),
     "sw_set_subset_kernel": (C.c_int, [_P, C.c_int, C.c_uint64]),
     "sw_set_subset_walk": (C.c_int, [_P, C.c_int]),
+    "sw_brute": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
+    "sw_brute_batch": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, _P]),
     "sw_calibrate_lds": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_double)]),
     "sw_set_subset_shape": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
 }
@@ -495,6 +497,31 @@ class Context:
         w = [int(x) for x in word[: r.word_len]] if r.word_len else None
         stats = {f: getattr(r, f) for f, _ in _ExactResult._fields_}
         return ExactResult(bool(r.found), int(r.length), w, bool(r.bound_word), stats)
+
+    def brute(self, n: int, k: int, delta) -> int:
+        """Shortest reset length by power-set BFS (n <= 32), -1 if not synchronizing."""
+        d = _delta(n, k, delta)
+        out = np.zeros(1, dtype=np.int64)
+        _check(load().sw_brute(self.handle, n, k, _ptr(d), _ptr(out)))
+        return int(out[0])
+
+    def brute_batch(self, instances) -> np.ndarray:
+        """Power-set BFS lengths of many small automata (list of (n, k, delta))."""
+        cnt = len(instances)
+        ns = np.array([i[0] for i in instances], dtype=np.uint32)
+        ks = np.array([i[1] for i in instances], dtype=np.uint32)
+        offs = np.zeros(cnt, dtype=np.uint64)
+        parts = []
+        o = 0
+        for j, (n, k, d) in enumerate(instances):
+            offs[j] = o
+            dd = _delta(n, k, d)
+            parts.append(dd)
+            o += dd.size
+        deltas = np.concatenate(parts) if parts else np.zeros(1, dtype=np.uint32)
+        out = np.zeros(max(cnt, 1), dtype=np.int64)
+        _check(load().sw_brute_batch(self.handle, cnt, _ptr(ns), _ptr(ks), _ptr(offs), _ptr(deltas), _ptr(out)))
+        return out[:cnt]
 
     def batch(self, instances, grid: int = 0, cap_rows: int = 0, cap_cand: int = 0, max_pairs: int = 0,
               word_cap: int = 512, overflow_workers: int = 0):

@@ -46,6 +46,9 @@ constexpr int kIxPrefetch = 8;  // uint4 groups of offsets (32 states) kept in r
 #ifndef SW_IX_REV
 #define SW_IX_REV 1
 #endif
+#ifndef SW_IX_EE_CHUNK
+#define SW_IX_EE_CHUNK 131072
+#endif
 #ifndef SW_IX_EE_WGS
 #define SW_IX_EE_WGS 1024
 #endif
@@ -267,7 +270,6 @@ __global__ __launch_bounds__(512) void k_state_rank(const uint32_t* __restrict__
 template <int W, int MODE, bool EE>
 __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ uint32_t tile_max;  // largest tile set (sorted lists)
   uint64_t* rows = reinterpret_cast<uint64_t*>(smem);  // [(n + 1) rows][64 masks]
   const uint32_t n = p.n;
   uint16_t* tsize = reinterpret_cast<uint16_t*>(smem + size_t(n + 1) * 512);
@@ -281,10 +283,26 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
   const uint64_t tile_base = uint64_t(tile_ix) * kIxTile;
   const uint64_t tile_left = p.t_count - tile_base;  // > 0 by grid construction
 
+  // Lists sorted by size: the tile's largest set has tmax states, the
+  // smallest c with t_bound[c] past the tile's last set; no stream set past
+  // the first s_bound[tmax] fits in a tile set (the pair holding the last
+  // that may is walked whole).  A workgroup with no such pair in its chunk
+  // leaves before loading its tile.
+  uint64_t p_bound = UINT64_MAX;
+  if (p.s_bound) {
+    const uint64_t last = tile_base + (tile_left < uint64_t(kIxTile) ? tile_left : uint64_t(kIxTile)) - 1;
+    uint32_t lo = 0, hi = n;  // t_bound[n] = t_count > last
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (p.t_bound[mid] > last) hi = mid;
+      else lo = mid + 1;
+    }
+    p_bound = (uint64_t(p.s_bound[lo]) + 1) / 2;
+    if (uint64_t(blockIdx.y) * p.s_chunk / 2 >= p_bound) return;  // the whole workgroup, before any barrier
+  }
+
   // ---- transpose the tile (as k_vsub): mask g of row q = ballot over tile
   // sets [64 g, 64 g + 64) of "q is in the set".
-  if (threadIdx.x == 0) tile_max = 0;
-  __syncthreads();
   for (int g = wave; g < kIxTile / 64; g += nwaves) {
     const uint64_t t = uint64_t(g) * 64 + lane;
     const bool valid = t < tile_left;
@@ -306,7 +324,6 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
       if (q < n) rows[size_t(q) * 64 + g] = kept;
     }
     if (p.proper_by_size) tsize[t] = uint16_t(valid ? pc : 0xffff);
-    if (p.s_bound && valid) atomicMax(&tile_max, uint32_t(pc));
   }
   if (wave == 0) rows[size_t(n) * 64 + lane] = ~0ull;  // padding row
   __syncthreads();
@@ -326,12 +343,7 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
   uint64_t s_end = s_begin + p.s_chunk;
   if (s_end > p.s_count) s_end = p.s_count;
   uint64_t p_end = (s_end + 1) / 2;  // pairs [s_begin / 2, p_end)
-  if (p.s_bound) {
-    // Stream sorted by size: no set past the first s_bound[tile_max] fits in
-    // a tile set; the pair holding the last that may is walked whole.
-    const uint64_t lim = (uint64_t(p.s_bound[tile_max]) + 1) / 2;
-    if (lim < p_end) p_end = lim;
-  }
+  if (p_bound < p_end) p_end = p_bound;
   const uint32_t* __restrict__ start = p.ix_start;
   const uint4* __restrict__ list4 = reinterpret_cast<const uint4*>(p.ix_list);
   const uint32_t lane_off = uint32_t(L) * 16u;
@@ -574,7 +586,7 @@ hipError_t launch_ix_kernel(const VsubParams& p, uint64_t gx, uint64_t gy, size_
   static std::atomic<bool> attr_set{false};
   if (!attr_set.load(std::memory_order_acquire)) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vsub_ix<W, MODE, EE>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 16);  // 16: static tile_max
+        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
     attr_set.store(true, std::memory_order_release);
   }
@@ -601,12 +613,17 @@ hipError_t launch_ix_w(const VsubParams& p0, hipStream_t st) {
     if (chunk < 128) chunk = 128;
     p.s_chunk = chunk;
   }
+  if (p.early_exit && p.s_bound && p0.s_chunk == 0) {
+    // Sorted lists: chunks of the stream small enough that the workgroups
+    // running at once (consecutive tiles, one chunk) share its index in L2.
+    p.s_chunk = SW_IX_EE_CHUNK;
+  }
   p.s_chunk = (p.s_chunk + 1) & ~1ull;  // even: pairs never straddle two workgroups
   const uint64_t gy = (p.s_count + p.s_chunk - 1) / p.s_chunk;
   if (gx > 0x7fffffffull || gy > 65535ull) return hipErrorInvalidValue;
   const size_t lds = size_t(p.n + 1) * 512 + (p.proper_by_size ? size_t(kIxTile) * 2 : 0);
-  if (lds + 16 > 160 * 1024) return hipErrorInvalidValue;  // 16: static tile_max
-  if (p.s_bound && (!p.s_perm || !p.t_perm || p.diag != INT64_MIN))
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  if (p.s_bound && (!p.s_perm || !p.t_perm || !p.t_bound || p.diag != INT64_MIN))
     return hipErrorInvalidValue;
   switch (p.mode) {
     case VSUB_COUNT: return launch_ix_mode<W, VSUB_COUNT>(p, gx, gy, lds, st);
@@ -621,8 +638,7 @@ hipError_t launch_ix_w(const VsubParams& p0, hipStream_t st) {
 }  // namespace
 
 bool vsub_ix_fits(uint32_t n, int W, bool proper_by_size) {
-  // 16 bytes of static LDS (tile_max) besides the rows and sizes.
-  return W >= 1 && W <= 5 && size_t(n + 1) * 512 + (proper_by_size ? size_t(kIxTile) * 2 : 0) + 16 <= 160 * 1024;
+  return W >= 1 && W <= 5 && size_t(n + 1) * 512 + (proper_by_size ? size_t(kIxTile) * 2 : 0) <= 160 * 1024;
 }
 
 hipError_t launch_ix_pair_count(int W, const uint64_t* sets, const uint32_t* perm, uint64_t count, uint32_t n,

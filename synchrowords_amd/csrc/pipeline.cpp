@@ -250,31 +250,6 @@ bool eval_bool(const Json* j, bool dflt, uint32_t n, uint32_t k) {
   return eval_num(j, "0", n, k) != 0;
 }
 
-// ----------------------------------------------------------------- Brute
-// Power-set BFS (algorithm/brute/brute.hpp:88-122).  -1: not synchronizing.
-long brute(uint32_t n, uint32_t k, const uint32_t* d) {
-  if (n == 1) return 0;
-  const uint32_t full = n == 32 ? 0xffffffffu : ((1u << n) - 1);
-  std::vector<uint8_t> dist(size_t(1) << n, 255);
-  std::queue<uint32_t> q;
-  dist[full] = 0;
-  q.push(full);
-  while (!q.empty()) {
-    uint32_t m = q.front();
-    q.pop();
-    for (uint32_t a = 0; a < k; ++a) {
-      uint32_t v = 0;
-      for (uint32_t x = m; x; x &= x - 1) v |= 1u << d[size_t(__builtin_ctz(x)) * k + a];
-      if (dist[v] == 255) {
-        dist[v] = uint8_t(dist[m] + 1);
-        q.push(v);
-        if (__builtin_popcount(v) == 1) return dist[v];
-      }
-    }
-  }
-  return -1;
-}
-
 // -------------------------------------------------------------- Eppstein
 // Pair graph BFS backwards from merged pairs, then greedy merging of the
 // closest pair (algorithm/eppstein/eppstein.hpp:25-114, pairs_tree.hpp:25-65).
@@ -501,7 +476,9 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
       const long long max_n = (long long)eval_num(cfg->get("max_n"), "20", n, k);
       SW_REQUIRE(max_n <= 32, "max_n can not be greater than 32");
       if (open && n <= uint64_t(max_n)) {
-        long r = brute(n, k, d);
+        int64_t r = 0;
+        const uint64_t off = 0;
+        brute_batch(ctx, 1, &n, &k, &off, d, &r);
         if (r < 0) non_synchro = true;
         else lower = upper = uint64_t(r);
       }

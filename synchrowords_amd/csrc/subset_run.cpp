@@ -37,27 +37,29 @@ void run_subset(sw_ctx* ctx, int W, VsubParams& p) {
     p.proper_by_size = 1;
   }
   const bool sorted = want_sort && p.diag == INT64_MIN;
-  p.s_perm = p.t_perm = p.s_bound = nullptr;
+  p.s_perm = p.t_perm = p.s_bound = p.t_bound = nullptr;
   if (sorted) {
     const uint32_t n1 = p.n + 1;
-    ctx->ix_sort.ensure(size_t(4) * n1 * 4);
-    uint32_t* hs = ctx->ix_sort.as<uint32_t>();  // stream histogram, then cursor
-    uint32_t* ht = hs + n1;                      // tile histogram, then cursor
-    uint32_t* cs = ht + n1;
-    uint32_t* bs = cs + n1;
+    ctx->ix_sort.ensure(size_t(5) * n1 * 4);
+    uint32_t* hs = ctx->ix_sort.as<uint32_t>();  // stream histogram, then tile cursor
+    uint32_t* ht = hs + n1;                      // tile histogram
+    uint32_t* cs = ht + n1;                      // stream cursor
+    uint32_t* bs = cs + n1;                      // stream sets of at most c states
+    uint32_t* bt = bs + n1;                      // tile sets of at most c states
     ctx->ix_sperm.ensure(s * 4 + 16);
     ctx->ix_tperm.ensure(p.t_count * 4 + 16);
     SW_CHECK_HIP(launch_fill_u32(hs, 0u, 2 * n1, st));
     SW_CHECK_HIP(launch_size_hist(W, p.stream, s, p.n, p.complement_stream, hs, st));
     SW_CHECK_HIP(launch_size_hist(W, p.tile, p.t_count, p.n, p.complement_tile, ht, st));
     SW_CHECK_HIP(launch_size_offsets(hs, p.n, cs, bs, st));
-    SW_CHECK_HIP(launch_size_offsets(ht, p.n, hs, nullptr, st));  // tile cursor over the stream histogram
+    SW_CHECK_HIP(launch_size_offsets(ht, p.n, hs, bt, st));  // tile cursor over the stream histogram
     SW_CHECK_HIP(launch_size_scatter(W, p.stream, s, p.n, p.complement_stream, cs, ctx->ix_sperm.as<uint32_t>(), st));
     SW_CHECK_HIP(launch_size_scatter(W, p.tile, p.t_count, p.n, p.complement_tile, hs, ctx->ix_tperm.as<uint32_t>(),
         st));
     p.s_perm = ctx->ix_sperm.as<uint32_t>();
     p.t_perm = ctx->ix_tperm.as<uint32_t>();
     p.s_bound = bs;
+    p.t_bound = bt;
   }
   ctx->ix_padded.ensure(pairs * 4 + 16);  // entries per pair
   ctx->ix_size.ensure(s * 4 + 16);

@@ -96,7 +96,9 @@ struct sw_ctx {
   // when it is used: never (0), or for launches of at least ix_min_pairs pairs.
   sw::DevBuf ix_padded, ix_size, ix_start, ix_list, ix_tmp;
   sw::DevBuf ix_order;  // state frequencies in the tile, ranks and rank order (3 x 512 words)
-  sw::DevBuf ix_sort, ix_sperm, ix_tperm, ix_head;  // sort by size: histograms and cursors, both permutations
+  sw::DevBuf ix_sort, ix_sperm, ix_tperm, ix_head;
+  // Power-set BFS batches: jobs, packed automata, frontiers, lengths.
+  sw::DevBuf brute_jobs, brute_delta, brute_frontier, brute_out;  // sort by size: histograms and cursors, both permutations
   // How the indexed kernel walks stream sets: as each launch asks (0), always
   // in set order without early exit (1), always rarest first with early exit (2).
   int ix_walk = 0;
@@ -119,6 +121,9 @@ void apply_tuning(sw_ctx* ctx, int W, VsubParams& p);
 // Launch a subset test on the context's stream: the indexed kernel when it
 // is enabled, fits and the launch is large enough (building the stream index
 // first), else the bit-walking kernel.
+// Power-set BFS lengths (-1: not synchronizing) for a batch of automata.
+void brute_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint32_t* ks, const uint64_t* delta_offsets,
+    const uint32_t* deltas, int64_t* lengths);
 void run_subset(sw_ctx* ctx, int W, VsubParams& p);
 
 // Reference-layout slice tables for every letter: [k][ceil(n/8)][256][W]

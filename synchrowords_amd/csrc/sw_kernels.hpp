@@ -58,6 +58,7 @@ struct VsubParams {
   const uint32_t* s_perm = nullptr;
   const uint32_t* t_perm = nullptr;
   const uint32_t* s_bound = nullptr;
+  const uint32_t* t_bound = nullptr;  // t_bound[c] = tile sets of at most c states
 };
 
 // Indexed subset kernel (k_subset_ix.hip): whether its tile fits LDS for n
@@ -91,6 +92,24 @@ hipError_t launch_size_hist(int W, const uint64_t* sets, uint64_t count, uint32_
 // rank[q] = position of state q in ascending (freq, q) order; order[r] = the
 // LDS row byte offset of the state at position r.
 hipError_t launch_state_rank(const uint32_t* freq, uint32_t n, uint32_t* rank, uint32_t* order, hipStream_t st);
+
+// Power-set BFS (k_brute.hip).  One automaton per workgroup with its bitmap
+// in LDS (brute_lds_bytes(n, k) <= kBruteMaxLds): frontier_off
+// indexes two buffers of 2^n subsets each.  Or one level over the device
+// (k <= kBruteLevelMaxK): state[0] += new subsets, state[1] = 1 on a
+// singleton.
+struct BruteJob {
+  uint32_t n, k;
+  uint64_t delta_off;     // into deltas, n * k entries
+  uint64_t frontier_off;  // into frontier, 2 * 2^n entries
+};
+constexpr int kBruteLevelMaxK = 8;
+constexpr size_t kBruteMaxLds = 160 * 1024 - 64;  // dynamic LDS of k_brute_wg (64 bytes static)
+int brute_lds_bytes(uint32_t n, uint32_t k);
+hipError_t launch_brute_wg(const BruteJob* jobs, uint32_t count, const uint32_t* deltas, uint32_t* frontier,
+    int64_t* out, size_t lds, hipStream_t st);
+hipError_t launch_brute_level(uint32_t n, uint32_t k, const uint32_t* delta, const uint32_t* cur, const uint32_t* fc,
+    uint32_t fc_host, uint32_t* nxt, uint32_t* map, uint32_t* state, hipStream_t st);
 hipError_t launch_vsub_ix(int W, const VsubParams& p, hipStream_t st);
 
 // Mask width and LDS bytes of the subset kernel.
