@@ -173,6 +173,82 @@ def bench_batch(ctx, count: int) -> dict:
     return out
 
 
+def bench_brute(ctx, count: int, no_cpu: bool) -> dict:
+    """Power-set BFS: a batch of small automata in one launch (one workgroup
+    each, visited set in LDS) and single automata of 24 states (per-level
+    launches), beside the oracle's BFS on one host core (a sample)."""
+    import synchrowords_amd as sw
+    rng = np.random.default_rng(2026)
+    inst = []
+    for j in range(count):
+        n = int(rng.integers(8, 21))
+        k = int(rng.integers(2, 4))
+        inst.append((n, k, sw.random_dfa(n, k, 60000 + j)))
+    ctx.brute_batch(inst[:64])  # warm
+    t0 = time.perf_counter()
+    got = ctx.brute_batch(inst)
+    wall = time.perf_counter() - t0
+    out = {"workload": f"batch of {count} random DFAs, n in [8,20], k in {{2,3}} (seeded), power-set BFS",
+           "wall_s": round(wall, 4), "instances_per_s": count / wall}
+    # Slowly synchronizing ones visit far more subsets: the hard family of
+    # 14..20 states (lengths 49..82), states relabelled at random.
+    from synchrowords_amd import gen
+
+    def relabel(n, k, d, perm):
+        inv = np.argsort(perm)
+        return [int(perm[d[int(inv[q]) * k + a]]) for q in range(n) for a in range(k)]
+
+    hard = []
+    for j in range(min(count, 1024)):
+        n = int(rng.integers(14, 21))
+        hard.append((n, 2, relabel(n, 2, gen.hard(n), rng.permutation(n))))
+    ctx.brute_batch(hard[:16])
+    t0 = time.perf_counter()
+    got_hard = ctx.brute_batch(hard)
+    wall_h = time.perf_counter() - t0
+    out["hard_batch"] = {"workload": f"{len(hard)} relabelled hard DFAs (gen.hard), n in [14,20], k=2",
+                         "wall_s": round(wall_h, 4), "instances_per_s": len(hard) / wall_h}
+    big = [(24, 2, sw.random_dfa(24, 2, 70000 + s)) for s in range(3)]
+    ctx.brute(*big[0])  # warm
+    t0 = time.perf_counter()
+    big_len = [ctx.brute(*b) for b in big]
+    out["n24_k2_s_each"] = round((time.perf_counter() - t0) / len(big), 4)
+    out["n24_k2_lengths"] = big_len
+    if not no_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib as orc
+        t0 = time.perf_counter()
+        done = 0
+        mism = 0
+        while done < count and time.perf_counter() - t0 < 15.0:
+            n, k, d = inst[done]
+            mism += int(orc.brute(n, k, d) != got[done])
+            done += 1
+        cpu = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        big_cpu = [orc.brute(*b) for b in big]
+        big_cpu_s = (time.perf_counter() - t1) / len(big)
+        out["cpu_baseline"] = {"value": done / cpu, "unit": "instances/s", "cores": 1, "kind": "port",
+                               "sample": f"first {done} automata of the batch, oracle power-set BFS"}
+        out["mismatches_vs_oracle_sample"] = mism
+        out["speedup_batch"] = round(out["instances_per_s"] / (done / cpu), 1)
+        out["n24_k2_cpu_s_each"] = round(big_cpu_s, 4)
+        t2 = time.perf_counter()
+        done_h = 0
+        mism_h = 0
+        while done_h < len(hard) and time.perf_counter() - t2 < 15.0:
+            n, k, d = hard[done_h]
+            mism_h += int(orc.brute(n, k, d) != got_hard[done_h])
+            done_h += 1
+        cpu_h = time.perf_counter() - t2
+        out["hard_batch"]["cpu_baseline"] = {"value": done_h / cpu_h, "unit": "instances/s", "cores": 1,
+                                             "kind": "port", "sample": f"first {done_h} of the hard batch"}
+        out["hard_batch"]["mismatches_vs_oracle_sample"] = mism_h
+        out["hard_batch"]["speedup"] = round(out["hard_batch"]["instances_per_s"] / (done_h / cpu_h), 1)
+        out["n24_mismatches"] = sum(int(a != b) for a, b in zip(big_len, big_cpu))
+    return out
+
+
 def bench_learned(ctx, scorer: str, seeds, width: int) -> dict:
     """Learned-bound against plain-bound exact search on the GPU (n=300, k=2).
 
@@ -230,9 +306,9 @@ def bench_learned(ctx, scorer: str, seeds, width: int) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["subset", "batch", "learned", "exact"], default="subset",
+    ap.add_argument("--workload", choices=["subset", "batch", "learned", "exact", "brute"], default="subset",
                     help="subset: the headline line; batch / learned: the batched and learned-bound reports; "
-                         "exact: one verbose exact search (--exact-n, --exact-seed)")
+                         "exact: one verbose exact search (--exact-n, --exact-seed); brute: power-set BFS")
     ap.add_argument("--exact-n", type=int, default=300)
     ap.add_argument("--exact-seed", type=int, default=1)
     ap.add_argument("--batch-count", type=int, default=4096)
@@ -280,6 +356,8 @@ def main() -> int:
         ctx = sw.Context(0)
         if args.workload == "batch":
             line = bench_batch(ctx, args.batch_count)
+        elif args.workload == "brute":
+            line = bench_brute(ctx, args.batch_count, args.no_cpu)
         elif args.workload == "exact":
             d = sw.random_dfa(args.exact_n, 2, args.exact_seed)
             ub = ctx.pipeline(args.exact_n, 2, d).beam_bound  # warm; the plan's Beam bound caps the search

@@ -178,6 +178,25 @@ int sw_brute(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, int64_t
 int sw_brute_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint32_t* ks, const uint64_t* delta_offsets,
     const uint32_t* deltas, int64_t* lengths);
 
+/* Eppstein's greedy bound, the reference's Eppstein<N,K>::run
+ * (synchrolib/algorithm/eppstein/eppstein.hpp:25-114) over its pair tree
+ * (data_structures/pairs_tree.hpp:25-65), on the device: *non_synchro = 1
+ * when some pair of states never merges; otherwise, when the greedy word is
+ * no longer than `upper`, *bound = its length and word[0 .. *word_len) = the
+ * word (up to word_cap letters), else *bound = upper and *word_len = 0.
+ * Same queue order and first letters as the reference, hence the same word.
+ * n <= 512; host arrays; word and word_len may be NULL. */
+int sw_eppstein(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint64_t upper, int32_t* non_synchro,
+    uint64_t* bound, uint32_t* word, uint64_t word_cap, uint64_t* word_len);
+
+/* The same for `count` automata in one launch (one workgroup each):
+ * automaton i at deltas + delta_offsets[i] with bound uppers[i]; its word at
+ * words + i * word_cap.  words and word_lens may be NULL.  No reference
+ * counterpart for the batch form. */
+int sw_eppstein_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint32_t* ks,
+    const uint64_t* delta_offsets, const uint32_t* deltas, const uint64_t* uppers, uint32_t word_cap,
+    int32_t* non_synchro, uint64_t* bounds, uint64_t* word_lens, uint32_t* words);
+
 /* Result of a plan run: the reference's AlgoResult
  * (synchrolib/algorithm/algorithm.hpp:19-42). */
 typedef struct sw_result {

@@ -126,6 +126,8 @@ _SIGS = {
     "sw_set_subset_walk": (C.c_int, [_P, C.c_int]),
     "sw_brute": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     "sw_brute_batch": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, _P]),
+    "sw_eppstein": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint64, _P, _P, _P, C.c_uint64, _P]),
+    "sw_eppstein_batch": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, _P, C.c_uint32, _P, _P, _P, _P]),
     "sw_calibrate_lds": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_double)]),
     "sw_set_subset_shape": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
 }
@@ -522,6 +524,46 @@ class Context:
         out = np.zeros(max(cnt, 1), dtype=np.int64)
         _check(load().sw_brute_batch(self.handle, cnt, _ptr(ns), _ptr(ks), _ptr(offs), _ptr(deltas), _ptr(out)))
         return out[:cnt]
+
+    def eppstein(self, n: int, k: int, delta, upper: int | None = None):
+        """Eppstein's greedy bound on the device: (non_synchro, bound, word or None)."""
+        d = _delta(n, k, delta)
+        up = (n - 1) * (n - 1) + 1 if upper is None else upper
+        cap = min(up, 1 << 22)
+        ns = np.zeros(1, dtype=np.int32)
+        b = np.zeros(1, dtype=np.uint64)
+        wl = np.zeros(1, dtype=np.uint64)
+        w = np.zeros(max(cap, 1), dtype=np.uint32)
+        _check(load().sw_eppstein(self.handle, n, k, _ptr(d), up, _ptr(ns), _ptr(b), _ptr(w), cap, _ptr(wl)))
+        word = [int(x) for x in w[:int(wl[0])]] if wl[0] else None
+        return bool(ns[0]), int(b[0]), word
+
+    def eppstein_batch(self, instances, uppers=None, word_cap: int = 0):
+        """Eppstein bounds of many automata in one launch: (non_synchro, bounds, words)."""
+        cnt = len(instances)
+        ns = np.array([i[0] for i in instances], dtype=np.uint32)
+        ks = np.array([i[1] for i in instances], dtype=np.uint32)
+        offs = np.zeros(cnt, dtype=np.uint64)
+        parts = []
+        o = 0
+        for j, (n, k, d) in enumerate(instances):
+            offs[j] = o
+            dd = _delta(n, k, d)
+            parts.append(dd)
+            o += dd.size
+        deltas = np.concatenate(parts) if parts else np.zeros(1, dtype=np.uint32)
+        if uppers is None:
+            uppers = [(int(n) - 1) ** 2 + 1 for n in ns]
+        up = np.array(uppers, dtype=np.uint64) if cnt else np.zeros(1, dtype=np.uint64)
+        nsync = np.zeros(max(cnt, 1), dtype=np.int32)
+        b = np.zeros(max(cnt, 1), dtype=np.uint64)
+        wl = np.zeros(max(cnt, 1), dtype=np.uint64)
+        w = np.zeros(max(cnt * word_cap, 1), dtype=np.uint32)
+        _check(load().sw_eppstein_batch(self.handle, cnt, _ptr(ns), _ptr(ks), _ptr(offs), _ptr(deltas), _ptr(up),
+                                        word_cap, _ptr(nsync), _ptr(b), _ptr(wl), _ptr(w)))
+        words = [[int(x) for x in w[j * word_cap: j * word_cap + min(int(wl[j]), word_cap)]] if wl[j] else None
+                 for j in range(cnt)]
+        return nsync[:cnt].astype(bool), b[:cnt], words
 
     def batch(self, instances, grid: int = 0, cap_rows: int = 0, cap_cand: int = 0, max_pairs: int = 0,
               word_cap: int = 512, overflow_workers: int = 0):

@@ -1,0 +1,271 @@
+// Eppstein's greedy bound on the device, one workgroup per automaton (the
+// reference's Eppstein, synchrolib/algorithm/eppstein/eppstein.hpp:25-114,
+// over PairsTree::calculate_tree, data_structures/pairs_tree.hpp:25-65).
+//
+// Pair BFS.  Backwards from the pairs one letter merges, over unordered state
+// pairs, recording per pair the length of a shortest merging word and its
+// first letter.  The reference's queue is first-in first-out and a pair keeps
+// the letter of its first discovery, so both the queue order and the letters
+// depend on the order candidates are met: (parent's place in the queue,
+// letter, place of each state in its preimage list).  Here a level is
+// expanded in parallel, one parent per thread, in three passes:
+//   1. every candidate {x, y} not reached before proposes its key, packed in
+//      that order, by atomicMin on the pair;
+//   2. each parent counts its candidates whose key won; a scan places them;
+//   3. each parent writes its winners, in its own order, after the earlier
+//      parents' winners, which is the reference's queue order.
+// The first level's parents are (state, letter) in the reference's loop order.
+//
+// Greedy.  While the current set has two states, take the pair of the set
+// with the shortest merging word (ties: lowest first state, then lowest
+// second, the reference's scan order) and apply its word letter by letter.
+// The bound is the sum of those lengths; past `upper` the bound is unchanged.
+#include <hip/hip_runtime.h>
+
+#include "sw_kernels.hpp"
+
+namespace sw {
+
+namespace {
+
+constexpr uint64_t kNoKey = ~0ull;
+constexpr int kEppThreads = 1024;
+
+__device__ inline uint32_t pair_ix(uint32_t x, uint32_t y, uint32_t n) {
+  return x < y ? x * n + y : y * n + x;
+}
+
+// Exclusive block scan of one value per thread (1024 threads); returns the
+// prefix and sets *total.
+__device__ inline uint32_t block_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= uint32_t(off)) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t s = lane < (blockDim.x >> 6) ? wsum[lane] : 0u;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(s, off, 64);
+      if (lane >= uint32_t(off)) s += y;
+    }
+    if (lane < (blockDim.x >> 6)) wsum[lane] = s;  // inclusive over waves
+  }
+  __syncthreads();
+  const uint32_t before = wave ? wsum[wave - 1] : 0u;
+  *total = wsum[(blockDim.x >> 6) - 1];
+  __syncthreads();  // wsum is reused by the next scan
+  return before + x - v;
+}
+
+// Candidates of parent p (level 0: p = q k + a, the pairs of one preimage
+// list; later: p = place in the level, the preimage pairs of queue[p] under
+// every letter), in the reference's order.  f(x, y, key, letter) per candidate.
+template <typename F>
+__device__ inline void candidates(uint32_t level, uint32_t p, uint32_t n, uint32_t k, const uint32_t* __restrict__ beg,
+    const uint16_t* __restrict__ edges, const uint32_t* __restrict__ lvl, F&& f) {
+  if (level == 0) {
+    const uint32_t q = p / k, a = p % k;
+    const uint32_t b = beg[a * n + q], e = beg[a * n + q + 1];
+    for (uint32_t i1 = b; i1 < e; ++i1)
+      for (uint32_t i2 = i1 + 1; i2 < e; ++i2)
+        f(uint32_t(edges[i1]), uint32_t(edges[i2]),
+          (uint64_t(p) << 20) | (uint64_t(i1 - b) << 10) | uint64_t(i2 - b), a);
+    return;
+  }
+  const uint32_t v = lvl[p];
+  const uint32_t v1 = v & 0xffffu, v2 = v >> 16;
+  for (uint32_t a = 0; a < k; ++a) {
+    const uint32_t b1 = beg[a * n + v1], e1 = beg[a * n + v1 + 1];
+    const uint32_t b2 = beg[a * n + v2], e2 = beg[a * n + v2 + 1];
+    for (uint32_t i1 = b1; i1 < e1; ++i1)
+      for (uint32_t i2 = b2; i2 < e2; ++i2)
+        f(uint32_t(edges[i1]), uint32_t(edges[i2]),
+          (uint64_t(p) << 28) | (uint64_t(a) << 20) | (uint64_t(i1 - b1) << 10) | uint64_t(i2 - b2), a);
+  }
+}
+
+__global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __restrict__ jobs,
+    const uint32_t* __restrict__ deltas, unsigned char* __restrict__ scratch, uint32_t* __restrict__ words,
+    EppOut* __restrict__ outs) {
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t s_total;
+  __shared__ uint64_t s_best[16];
+  __shared__ unsigned char cur[kEppMaxN], nxt[kEppMaxN];
+  const EppJob job = jobs[blockIdx.x];
+  const uint32_t n = job.n, k = job.k;
+  const uint32_t* d = deltas + job.delta_off;
+  EppOut* out = outs + blockIdx.x;
+  const uint32_t tid = threadIdx.x, nt = blockDim.x;
+  if (n <= 1) {  // nothing to merge; the bound is left as it was
+    if (tid == 0) *out = EppOut{0, 0, job.upper, 0};
+    return;
+  }
+  // Scratch regions (epp_layout): scratch_off is a multiple of 256, every
+  // region a multiple of 8.
+  unsigned char* base = scratch + job.scratch_off;
+  const EppLayout lay = epp_layout(n, k);
+  uint32_t* beg = reinterpret_cast<uint32_t*>(base);
+  uint16_t* edges = reinterpret_cast<uint16_t*>(base + lay.edges);
+  uint32_t* info = reinterpret_cast<uint32_t*>(base + lay.info);
+  unsigned long long* keys = reinterpret_cast<unsigned long long*>(base + lay.keys);
+  const uint32_t all = n * (n - 1) / 2;
+  uint32_t* queue = reinterpret_cast<uint32_t*>(base + lay.queue);
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(base + lay.cnt);
+
+  // Preimage lists, sources ascending: list (a, t) at beg[a n + t].
+  for (uint32_t at = tid; at < k * n; at += nt) {
+    const uint32_t a = at / n, t = at % n;
+    uint32_t c = 0;
+    for (uint32_t q = 0; q < n; ++q) c += d[size_t(q) * k + a] == t ? 1u : 0u;
+    cnt[at] = c;
+  }
+  for (uint32_t i = tid; i < n * n; i += nt) {
+    info[i] = 0;
+    keys[i] = kNoKey;
+  }
+  __syncthreads();
+  {
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < k * n; b0 += nt) {
+      const uint32_t i = b0 + tid;
+      const uint32_t v = i < k * n ? cnt[i] : 0u;
+      uint32_t tot;
+      const uint32_t ex = block_scan(v, wsum, &tot);
+      if (i < k * n) beg[i] = carry + ex;
+      carry += tot;
+    }
+    if (tid == 0) beg[k * n] = carry;
+  }
+  __syncthreads();
+  for (uint32_t at = tid; at < k * n; at += nt) {
+    const uint32_t a = at / n, t = at % n;
+    uint32_t o = beg[at];
+    for (uint32_t q = 0; q < n; ++q)
+      if (d[size_t(q) * k + a] == t) edges[o++] = uint16_t(q);
+  }
+  __syncthreads();
+
+  // ---- pair BFS
+  uint32_t head = 0, tail = 0;  // this level's parents: queue[head, tail) (level 0: k n list pairs)
+  uint32_t found = 0;
+  for (uint32_t level = 0;; ++level) {
+    const uint32_t parents = level == 0 ? k * n : tail - head;
+    const uint32_t* lvl = queue + head;
+    // 1. proposals
+    for (uint32_t p = tid; p < parents; p += nt)
+      candidates(level, p, n, k, beg, edges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
+        const uint32_t ix = pair_ix(x, y, n);
+        if (info[ix] == 0) atomicMin(&keys[ix], (unsigned long long)key);
+      });
+    __syncthreads();
+    // 2. winners per parent, placed by a scan
+    uint32_t carry = 0;
+    const uint32_t wbase = tail;
+    for (uint32_t b0 = 0; b0 < parents; b0 += nt) {
+      const uint32_t p = b0 + tid;
+      uint32_t c = 0;
+      if (p < parents)
+        candidates(level, p, n, k, beg, edges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
+          const uint32_t ix = pair_ix(x, y, n);
+          c += (info[ix] == 0 && keys[ix] == key) ? 1u : 0u;
+        });
+      uint32_t tot;
+      const uint32_t ex = block_scan(c, wsum, &tot);
+      if (p < parents) cnt[p] = carry + ex;
+      carry += tot;
+    }
+    __syncthreads();
+    // 3. winners written in order; their length and first letter recorded
+    for (uint32_t p = tid; p < parents; p += nt) {
+      uint32_t o = wbase + cnt[p];
+      candidates(level, p, n, k, beg, edges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t a) {
+        const uint32_t ix = pair_ix(x, y, n);
+        if (info[ix] == 0 && keys[ix] == key) {
+          info[ix] = ((level + 1) << 8) | a;
+          queue[o++] = x | (y << 16);
+        }
+      });
+    }
+    __syncthreads();
+    found += carry;
+    head = wbase;
+    tail = wbase + carry;
+    if (carry == 0 || found >= all) break;
+  }
+  if (found < all) {
+    if (tid == 0) *out = EppOut{1, 0, job.upper, 0};
+    return;
+  }
+
+  // ---- greedy merging of the closest pair
+  for (uint32_t q = tid; q < n; q += nt) cur[q] = 1;
+  __syncthreads();
+  uint32_t size = n;
+  uint64_t total = 0, wl = 0;
+  uint32_t* word = words + job.word_off;
+  while (size > 1) {
+    uint64_t best = ~0ull;  // (length, first, second)
+    for (uint32_t i = tid; i < n * n; i += nt) {
+      const uint32_t a = i / n, b = i % n;
+      if (a < b && cur[a] && cur[b]) {
+        const uint64_t key = (uint64_t(info[i] >> 8) << 32) | (uint64_t(a) << 16) | b;
+        best = key < best ? key : best;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint64_t o = __shfl_xor(best, off, 64);
+      best = o < best ? o : best;
+    }
+    if ((tid & 63u) == 0) s_best[tid >> 6] = best;
+    __syncthreads();
+    best = s_best[0];
+    for (uint32_t w = 1; w < (nt >> 6); ++w) best = s_best[w] < best ? s_best[w] : best;
+    __syncthreads();
+    total += best >> 32;
+    if (total > job.upper) {
+      if (tid == 0) *out = EppOut{0, 0, job.upper, 0};
+      return;
+    }
+    uint32_t n1 = uint32_t(best >> 16) & 0xffffu, n2 = uint32_t(best) & 0xffffu;
+    while (n1 != n2) {
+      const uint32_t x = info[pair_ix(n1, n2, n)] & 255u;
+      if (tid == 0 && wl < job.word_cap) word[wl] = x;
+      ++wl;
+      for (uint32_t q = tid; q < n; q += nt) nxt[q] = 0;
+      __syncthreads();
+      for (uint32_t q = tid; q < n; q += nt)
+        if (cur[q]) nxt[d[size_t(q) * k + x]] = 1;
+      if (tid == 0) s_total = 0;
+      __syncthreads();
+      uint32_t c = 0;
+      for (uint32_t q = tid; q < n; q += nt) {
+        cur[q] = nxt[q];
+        c += nxt[q];
+      }
+      if (c) atomicAdd(&s_total, c);
+      __syncthreads();
+      size = s_total;
+      n1 = d[size_t(n1) * k + x];
+      n2 = d[size_t(n2) * k + x];
+      __syncthreads();  // s_total is reset by the next letter
+    }
+  }
+  if (tid == 0) *out = EppOut{0, 1, total, wl};
+}
+
+}  // namespace
+
+size_t eppstein_scratch_bytes(uint32_t n, uint32_t k) { return epp_layout(n, k).total; }
+
+hipError_t launch_eppstein_wg(const EppJob* jobs, uint32_t count, const uint32_t* deltas, unsigned char* scratch,
+    uint32_t* words, EppOut* outs, hipStream_t st) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_eppstein_wg, dim3(count), dim3(kEppThreads), 0, st, jobs, deltas, scratch, words, outs);
+  return hipGetLastError();
+}
+
+}  // namespace sw

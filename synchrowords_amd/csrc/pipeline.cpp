@@ -250,102 +250,6 @@ bool eval_bool(const Json* j, bool dflt, uint32_t n, uint32_t k) {
   return eval_num(j, "0", n, k) != 0;
 }
 
-// -------------------------------------------------------------- Eppstein
-// Pair graph BFS backwards from merged pairs, then greedy merging of the
-// closest pair (algorithm/eppstein/eppstein.hpp:25-114, pairs_tree.hpp:25-65).
-struct EppOut {
-  bool non_synchro = false;
-  bool improved = false;
-  uint64_t bound = 0;
-  std::vector<uint32_t> word;
-};
-
-EppOut eppstein(uint32_t n, uint32_t k, const uint32_t* d, uint64_t upper) {
-  EppOut o;
-  o.bound = upper;
-  std::vector<uint32_t> beg(size_t(n) * k + 1), edges(size_t(n) * k);
-  // CSR of preimages per (state, letter), sources ascending.
-  std::vector<uint32_t> cnt(size_t(n) * k, 0);
-  for (uint32_t q = 0; q < n; ++q)
-    for (uint32_t a = 0; a < k; ++a) cnt[size_t(a) * n + d[size_t(q) * k + a]]++;
-  beg[0] = 0;
-  for (size_t i = 0; i < size_t(n) * k; ++i) beg[i + 1] = beg[i] + cnt[i];
-  std::vector<uint32_t> fill(beg.begin(), beg.end() - 1);
-  for (uint32_t a = 0; a < k; ++a)
-    for (uint32_t q = 0; q < n; ++q) {
-      size_t t = size_t(a) * n + d[size_t(q) * k + a];
-      edges[fill[t]++] = q;
-    }
-  auto B = [&](uint32_t q, uint32_t a) { return beg[size_t(a) * n + q]; };
-  auto E = [&](uint32_t q, uint32_t a) { return beg[size_t(a) * n + q + 1]; };
-  std::vector<uint32_t> len(size_t(n) * n, 0), let(size_t(n) * n, 0);
-  std::vector<std::pair<uint32_t, uint32_t>> queue;
-  for (uint32_t q = 0; q < n; ++q)
-    for (uint32_t a = 0; a < k; ++a)
-      for (uint32_t i1 = B(q, a); i1 < E(q, a); ++i1)
-        for (uint32_t i2 = i1 + 1; i2 < E(q, a); ++i2) {
-          uint32_t x = edges[i1], y = edges[i2];
-          if (len[size_t(x) * n + y]) continue;
-          len[size_t(x) * n + y] = len[size_t(y) * n + x] = 1;
-          let[size_t(x) * n + y] = let[size_t(y) * n + x] = a;
-          queue.emplace_back(x, y);
-        }
-  const size_t all = size_t(n) * (n - 1) / 2;
-  size_t bottom = 0;
-  while (queue.size() < all && queue.size() > bottom) {
-    auto [v1, v2] = queue[bottom++];
-    const uint32_t l = len[size_t(v1) * n + v2] + 1;
-    for (uint32_t a = 0; a < k; ++a)
-      for (uint32_t i1 = B(v1, a); i1 < E(v1, a); ++i1)
-        for (uint32_t i2 = B(v2, a); i2 < E(v2, a); ++i2) {
-          uint32_t x = edges[i1], y = edges[i2];
-          if (len[size_t(x) * n + y]) continue;
-          len[size_t(x) * n + y] = len[size_t(y) * n + x] = l;
-          let[size_t(x) * n + y] = let[size_t(y) * n + x] = a;
-          queue.emplace_back(x, y);
-        }
-  }
-  if (queue.size() < all) {
-    o.non_synchro = true;
-    return o;
-  }
-  std::vector<char> cur(n, 1), nx(n);
-  uint32_t size = n;
-  uint64_t total = 0;
-  std::vector<uint32_t> w;
-  while (size > 1) {
-    uint32_t n1 = 0, n2 = 0, best = n * n;
-    for (uint32_t a = 0; a + 1 < n; ++a)
-      if (cur[a])
-        for (uint32_t b = a + 1; b < n; ++b)
-          if (cur[b] && len[size_t(a) * n + b] < best) {
-            n1 = a;
-            n2 = b;
-            best = len[size_t(a) * n + b];
-          }
-    total += best;
-    if (total > upper) return o;
-    while (n1 != n2) {
-      const uint32_t x = let[size_t(n1) * n + n2];
-      w.push_back(x);
-      std::fill(nx.begin(), nx.end(), 0);
-      size = 0;
-      for (uint32_t q = 0; q < n; ++q)
-        if (cur[q] && !nx[d[size_t(q) * k + x]]) {
-          nx[d[size_t(q) * k + x]] = 1;
-          size++;
-        }
-      cur.swap(nx);
-      n1 = d[size_t(n1) * k + x];
-      n2 = d[size_t(n2) * k + x];
-    }
-  }
-  o.improved = true;
-  o.bound = total;
-  o.word = std::move(w);
-  return o;
-}
-
 // ------------------------------------------------------------------ Beam
 // Inverse BFS from the sink singletons keeping the `beam` largest preimages
 // (ties by per-word numeric order); the first depth reaching all N states
@@ -485,14 +389,20 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
     } else if (name == "Eppstein") {
       const bool find_word = eval_bool(cfg->get("find_word"), false, n, k);
       if (open) {
-        EppOut e = eppstein(n, k, d, upper);
-        if (e.non_synchro) {
+        // Eppstein's greedy bound on the device (k_eppstein.hip).
+        const uint64_t off = 0;
+        int32_t ns = 0, imp = 0;
+        uint64_t bound = upper, wl = 0;
+        const uint64_t cap = std::min<uint64_t>(upper, 1u << 22);
+        std::vector<uint32_t> w(cap ? cap : 1);
+        eppstein_batch(ctx, 1, &n, &k, &off, d, &upper, cap, &ns, &imp, &bound, &wl, w.data());
+        if (ns) {
           non_synchro = true;
-        } else if (e.improved) {
-          upper = e.bound;
+        } else if (imp) {
+          upper = bound;
           // The word is kept whatever find_word says: when Exact proves this
           // bound optimal, it is the shortest word.
-          word = e.word;
+          word.assign(w.begin(), w.begin() + std::min<uint64_t>(wl, cap));
           (void)find_word;
         }
         out->eppstein_bound = upper;

@@ -98,7 +98,9 @@ struct sw_ctx {
   sw::DevBuf ix_order;  // state frequencies in the tile, ranks and rank order (3 x 512 words)
   sw::DevBuf ix_sort, ix_sperm, ix_tperm, ix_head;
   // Power-set BFS batches: jobs, packed automata, frontiers, lengths.
-  sw::DevBuf brute_jobs, brute_delta, brute_frontier, brute_out;  // sort by size: histograms and cursors, both permutations
+  sw::DevBuf brute_jobs, brute_delta, brute_frontier, brute_out;
+  // Eppstein batches: jobs, packed automata, pair-BFS scratch, words, results.
+  sw::DevBuf epp_jobs, epp_delta, epp_scratch, epp_words, epp_out;  // sort by size: histograms and cursors, both permutations
   // How the indexed kernel walks stream sets: as each launch asks (0), always
   // in set order without early exit (1), always rarest first with early exit (2).
   int ix_walk = 0;
@@ -121,6 +123,12 @@ void apply_tuning(sw_ctx* ctx, int W, VsubParams& p);
 // Launch a subset test on the context's stream: the indexed kernel when it
 // is enabled, fits and the launch is large enough (building the stream index
 // first), else the bit-walking kernel.
+// Eppstein's greedy bound for a batch: per automaton non_synchro, improved
+// (greedy length <= uppers[i]), bounds (uppers[i] unless improved), the word
+// length and the word (words + i * word_cap).  Any output may be NULL.
+void eppstein_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint32_t* ks, const uint64_t* delta_offsets,
+    const uint32_t* deltas, const uint64_t* uppers, uint64_t word_cap, int32_t* non_synchro, int32_t* improved,
+    uint64_t* bounds, uint64_t* word_lens, uint32_t* words);
 // Power-set BFS lengths (-1: not synchronizing) for a batch of automata.
 void brute_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint32_t* ks, const uint64_t* delta_offsets,
     const uint32_t* deltas, int64_t* lengths);

@@ -112,6 +112,41 @@ hipError_t launch_brute_level(uint32_t n, uint32_t k, const uint32_t* delta, con
     uint32_t fc_host, uint32_t* nxt, uint32_t* map, uint32_t* state, hipStream_t st);
 hipError_t launch_vsub_ix(int W, const VsubParams& p, hipStream_t st);
 
+// Eppstein's greedy bound, one workgroup per automaton (k_eppstein.hip);
+// n <= kEppMaxN.  Scratch per automaton: eppstein_scratch_bytes(n, k) at
+// scratch_off; the word (up to word_cap letters) at words + word_off.
+constexpr uint32_t kEppMaxN = 512;
+struct EppJob {
+  uint32_t n, k;
+  uint64_t delta_off, scratch_off, word_off, word_cap, upper;
+};
+struct EppOut {
+  int32_t non_synchro, improved;  // improved: the greedy length is at most upper
+  uint64_t bound, word_len;       // bound = upper unless improved
+};
+// Scratch layout of one automaton, every region 8-byte aligned (keys take
+// 64-bit atomics, which must be naturally aligned): beg[k n + 1] (u32),
+// edges[k n] (u16), info[n n] (u32), keys[n n] (u64), queue and per-parent
+// counts (n (n - 1) / 2 + k n u32 each).
+struct EppLayout {
+  size_t edges, info, keys, queue, cnt, total;
+};
+__host__ __device__ inline EppLayout epp_layout(uint32_t n, uint32_t k) {
+  const size_t kn = size_t(k) * n, nn = size_t(n) * n, all = size_t(n) * (n - 1) / 2;
+  auto a8 = [](size_t b) { return (b + 7) & ~size_t(7); };
+  EppLayout l;
+  l.edges = a8((kn + 1) * 4);
+  l.info = l.edges + a8(kn * 2);
+  l.keys = l.info + a8(nn * 4);
+  l.queue = l.keys + nn * 8;
+  l.cnt = l.queue + a8((all + kn) * 4);
+  l.total = (l.cnt + (all + kn) * 4 + 255) & ~size_t(255);
+  return l;
+}
+size_t eppstein_scratch_bytes(uint32_t n, uint32_t k);
+hipError_t launch_eppstein_wg(const EppJob* jobs, uint32_t count, const uint32_t* deltas, unsigned char* scratch,
+    uint32_t* words, EppOut* outs, hipStream_t st);
+
 // Mask width and LDS bytes of the subset kernel.
 int vsub_mask_bits(uint32_t n, int W, int requested, bool proper_by_size);
 size_t vsub_lds_bytes(uint32_t n, int W, int mask_bits, bool proper_by_size);
