@@ -21,49 +21,6 @@
 
 namespace sw {
 
-// Every pair of states merges under some word (the pair-graph test of the
-// reference's Eppstein, pairs_tree.hpp:25-65,99-104): BFS over unordered pairs
-// backwards from the pairs merged by one letter.
-bool pairs_synchronizing(uint32_t n, uint32_t k, const uint32_t* d) {
-  if (n <= 1) return true;
-  std::vector<uint32_t> beg(size_t(n) * k + 1, 0), edges(size_t(n) * k);
-  for (uint32_t q = 0; q < n; ++q)
-    for (uint32_t a = 0; a < k; ++a) beg[size_t(a) * n + d[size_t(q) * k + a] + 1]++;
-  for (size_t i = 0; i < size_t(n) * k; ++i) beg[i + 1] += beg[i];
-  std::vector<uint32_t> fill(beg.begin(), beg.end() - 1);
-  for (uint32_t a = 0; a < k; ++a)
-    for (uint32_t q = 0; q < n; ++q) edges[fill[size_t(a) * n + d[size_t(q) * k + a]]++] = q;
-  std::vector<uint8_t> seen(size_t(n) * n, 0);
-  std::vector<std::pair<uint32_t, uint32_t>> queue;
-  queue.reserve(size_t(n) * (n - 1) / 2);
-  for (uint32_t q = 0; q < n; ++q)
-    for (uint32_t a = 0; a < k; ++a) {
-      const size_t t = size_t(a) * n + q;
-      for (uint32_t i = beg[t]; i < beg[t + 1]; ++i)
-        for (uint32_t j = i + 1; j < beg[t + 1]; ++j) {
-          uint32_t x = edges[i], y = edges[j];
-          if (seen[size_t(x) * n + y]) continue;
-          seen[size_t(x) * n + y] = seen[size_t(y) * n + x] = 1;
-          queue.emplace_back(x, y);
-        }
-    }
-  const size_t all = size_t(n) * (n - 1) / 2;
-  for (size_t bottom = 0; bottom < queue.size() && queue.size() < all; ++bottom) {
-    auto [v1, v2] = queue[bottom];
-    for (uint32_t a = 0; a < k; ++a) {
-      const size_t t1 = size_t(a) * n + v1, t2 = size_t(a) * n + v2;
-      for (uint32_t i = beg[t1]; i < beg[t1 + 1]; ++i)
-        for (uint32_t j = beg[t2]; j < beg[t2 + 1]; ++j) {
-          uint32_t x = edges[i], y = edges[j];
-          if (seen[size_t(x) * n + y]) continue;
-          seen[size_t(x) * n + y] = seen[size_t(y) * n + x] = 1;
-          queue.emplace_back(x, y);
-        }
-    }
-  }
-  return queue.size() == all;
-}
-
 }  // namespace sw
 
 extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint32_t* ks,
@@ -78,7 +35,7 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
     auto t0 = std::chrono::steady_clock::now();
     sw_batch_stats st{};
 
-    // ---- host preparation: validation, synchronizability, sink singletons
+    // ---- preparation: validation, synchronizability and upper bounds, sink singletons
     struct Prep {
       bool ns = false, device = false;
       uint32_t wmax = 0;
@@ -92,6 +49,30 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
       const uint32_t* d = deltas + delta_offsets[i];
       for (size_t j = 0; j < size_t(ns[i]) * ks[i]; ++j) SW_REQUIRE(d[j] < ns[i], "transition out of range");
     }
+    // Non-synchronizing automata (some pair never merges) and Eppstein's
+    // greedy bound with its word, for every instance in one launch
+    // (k_eppstein.hip).  A bound whose word fits is the search's upper bound:
+    // when nothing shorter exists, that word is the answer.
+    constexpr uint32_t kEppWordCap = 1024;
+    std::vector<int32_t> e_ns(count), e_imp(count);
+    std::vector<uint64_t> e_up(count), e_bound(count), e_wl(count);
+    std::vector<uint32_t> e_words(size_t(count) * kEppWordCap);
+    for (uint32_t i = 0; i < count; ++i) e_up[i] = uint64_t(ns[i] - 1) * (ns[i] - 1) + 1;
+    eppstein_batch(ctx, count, ns, ks, delta_offsets, deltas, e_up.data(), kEppWordCap, e_ns.data(), e_imp.data(),
+        e_bound.data(), e_wl.data(), e_words.data());
+    std::vector<uint64_t> upper(count);
+    std::vector<char> upper_has_word(count, 0);
+    for (uint32_t i = 0; i < count; ++i) {
+      upper[i] = e_up[i];
+      if (!e_ns[i] && e_imp[i] && e_wl[i] <= kEppWordCap && e_bound[i] < upper[i]) {
+        upper[i] = e_bound[i];
+        upper_has_word[i] = 1;
+      }
+    }
+    auto upper_word = [&](uint32_t i) {
+      return std::vector<uint32_t>(e_words.begin() + size_t(i) * kEppWordCap,
+          e_words.begin() + size_t(i) * kEppWordCap + e_wl[i]);
+    };
     {
       std::atomic<uint32_t> next{0};
       const unsigned nt = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
@@ -102,7 +83,7 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
             const uint32_t n = ns[i], k = ks[i];
             const uint32_t* d = deltas + delta_offsets[i];
             Prep& p = prep[i];
-            p.ns = !pairs_synchronizing(n, k, d);
+            p.ns = e_ns[i] != 0;
             if (p.ns || n == 1) continue;
             p.device = n <= kBatchMaxW * 64 && k <= kBatchMaxK;
             if (!p.device) continue;
@@ -141,8 +122,7 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
       b.n = ns[i];
       b.k = ks[i];
       b.delta_off = uint32_t(bytes.size());
-      const uint64_t ub = uint64_t(ns[i] - 1) * (ns[i] - 1) + 1;
-      b.upper = uint32_t(std::min<uint64_t>(ub, 0xffffffffu));
+      b.upper = uint32_t(std::min<uint64_t>(upper[i], 0xffffffffu));
       std::memcpy(b.b0, prep[i].b0, sizeof(b.b0));
       const uint32_t* d = deltas + delta_offsets[i];
       for (size_t j = 0; j < size_t(ns[i]) * ks[i]; ++j) bytes.push_back(uint8_t(d[j]));
@@ -220,6 +200,12 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
           }
         } else {
           st.bound++;
+          // Nothing shorter than the upper bound: its word, when it has one.
+          if (upper_has_word[i] && words && word_lens && e_wl[i] <= word_cap) {
+            word_lens[i] = uint32_t(e_wl[i]);
+            const std::vector<uint32_t> w = upper_word(i);
+            std::copy(w.begin(), w.end(), words + size_t(i) * word_cap);
+          }
         }
         done[i] = 1;
       } else if (r.status == kBatchOverflow) {
@@ -266,12 +252,12 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
               const uint32_t* d = deltas + delta_offsets[i];
               sw_exact_config ec{};
               ec.dfs = 1;
-              const uint64_t ub = uint64_t(ns[i] - 1) * (ns[i] - 1) + 1;
               const auto s0 = std::chrono::steady_clock::now();
-              auto r = exact_search(wctx, ns[i], ks[i], d, 0, ub, ec);
+              auto r = exact_search(wctx, ns[i], ks[i], d, 0, upper[i], ec);
               busy_ms[t] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - s0).count();
               if (!r.r.found) throw Error{SW_ERR_NOMEM, "exact search ended without a length (memory budget)"};
               lengths[i] = r.r.length;
+              if (r.r.bound_word && upper_has_word[i]) r.word = upper_word(i);
               if (words && word_lens && r.word.size() <= word_cap) {
                 word_lens[i] = uint32_t(r.word.size());
                 std::copy(r.word.begin(), r.word.end(), words + size_t(i) * word_cap);
