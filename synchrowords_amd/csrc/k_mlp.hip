@@ -42,12 +42,15 @@ constexpr int TM = 64, TN = 64, TK = 32;
 //   A: row-major [M][K] (lda) or, with TA, [K][M].
 //   B: row-major [K][N] (ldb) or, with TB, [N][K].
 // Epilogue: + bias[n] (if bias), ReLU (if relu); write f32 (Cf) and/or bf16 (Cb).
-template <bool TA, bool TB>
+// VEC (no TA, TB, K and both leading dimensions multiples of 8, 16-byte
+// aligned rows): both tiles are contiguous along k and stage as one 16-byte
+// load per thread each.
+template <bool TA, bool TB, bool VEC>
 __global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, int lda, const uint16_t* __restrict__ B,
     int ldb, float* __restrict__ Cf, uint16_t* __restrict__ Cb, int ldc, int M, int N, int K,
     const float* __restrict__ bias, int relu, int accumulate) {
-  __shared__ uint16_t As[TM][TK + 4];  // [m][k]
-  __shared__ uint16_t Bs[TN][TK + 4];  // [n][k]
+  __shared__ __align__(16) uint16_t As[TM][TK + 8];  // [m][k], rows of 80 bytes
+  __shared__ __align__(16) uint16_t Bs[TN][TK + 8];  // [n][k]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -59,6 +62,15 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, in
 
   for (int k0 = 0; k0 < K; k0 += TK) {
     // Stage the tiles (zero outside the matrices).
+    if (VEC) {
+      static_assert(TM * TK / 8 == 256 && TN * TK / 8 == 256, "one 16-byte chunk per thread and tile");
+      const int i = tid >> 2, c = (tid & 3) * 8;
+      uint4 va = make_uint4(0u, 0u, 0u, 0u), vb = make_uint4(0u, 0u, 0u, 0u);
+      if (m0 + i < M && k0 + c < K) va = *reinterpret_cast<const uint4*>(A + size_t(m0 + i) * lda + k0 + c);
+      if (n0 + i < N && k0 + c < K) vb = *reinterpret_cast<const uint4*>(B + size_t(n0 + i) * ldb + k0 + c);
+      *reinterpret_cast<uint4*>(&As[i][c]) = va;
+      *reinterpret_cast<uint4*>(&Bs[i][c]) = vb;
+    } else {
     for (int e = tid; e < TM * TK; e += 256) {
       int i, k;
       if (TA) {
@@ -86,6 +98,7 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, in
       uint16_t v = 0;
       if (gn < N && gk < K) v = TB ? B[size_t(gn) * ldb + gk] : B[size_t(gk) * ldb + gn];
       Bs[j][k] = v;
+    }
     }
     __syncthreads();
 #pragma unroll
@@ -135,12 +148,16 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, in
 //   x[n + k + j]    = s min over images A of Q of depth <= 2(j+1) of |A \ S_i| / n (j < 4),
 //   x[n + k + 4]    = s |S_i| / n,
 // zero elsewhere; s = kFeatureScale puts these few count features on a scale
-// that competes with the n bits.
+// that competes with the n bits.  One wave per row: lanes take the columns
+// (column r is state order[r], the inverse of perm) and split the guide
+// images, whose minima meet in a wave reduction.
 template <int W>
-__global__ void k_features(const uint64_t* __restrict__ sets, const uint64_t* __restrict__ images, uint64_t count,
-    uint32_t n, uint32_t k, uint32_t D, FeatureGuide g, uint16_t* __restrict__ x) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+__global__ __launch_bounds__(256) void k_features(const uint64_t* __restrict__ sets,
+    const uint64_t* __restrict__ images, uint64_t count, uint32_t n, uint32_t k, uint32_t D, FeatureGuide g,
+    uint16_t* __restrict__ x) {
+  const uint64_t i = uint64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u;
+  if (i >= count) return;  // whole waves
   uint16_t* row = x + i * D;
   uint64_t s[W];
   int card = 0;
@@ -149,34 +166,44 @@ __global__ void k_features(const uint64_t* __restrict__ sets, const uint64_t* __
     s[w] = sets[i * W + w];
     card += __popcll(s[w]);
   }
-  for (uint32_t q = 0; q < D; ++q) row[q] = 0;
+  for (uint32_t r = lane; r < n; r += 64) {
+    const uint32_t q = g.order[r];
+    uint64_t word = 0;
 #pragma unroll
-  for (int w = 0; w < W; ++w) {
-    uint64_t b = s[w];
-    while (b) {
-      const uint32_t q = uint32_t(w) * 64 + __builtin_ctzll(b);
-      b &= b - 1;
-      row[g.perm[q]] = 0x3f80;  // 1.0
-    }
+    for (int w = 0; w < W; ++w)  // constant indices: s stays in registers
+      if (uint32_t(w) == (q >> 6)) word = s[w];
+    row[r] = ((word >> (q & 63)) & 1ull) ? uint16_t(0x3f80) : uint16_t(0);  // 1.0 or 0
   }
-  for (uint32_t a = 0; a < k; ++a) {
+  for (uint32_t a = lane; a < k; a += 64) {
     const uint64_t* im = images + (uint64_t(a) * count + i) * W;
     int c = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) c += __popcll(im[w]);
     row[n + a] = f2bf(kFeatureScale * float(c) / float(n));
   }
-  uint32_t best = n;
-  int j = 0;
-  for (uint32_t r = 0; r < g.ends[kGuideDepths - 1]; ++r) {
-    while (j < kGuideDepths && r == g.ends[j]) row[n + k + j++] = f2bf(kFeatureScale * float(best) / float(n));
+  uint32_t best[kGuideDepths];
+#pragma unroll
+  for (int j = 0; j < kGuideDepths; ++j) best[j] = n;
+  for (uint32_t r = lane; r < g.ends[kGuideDepths - 1]; r += 64) {
     uint32_t c = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) c += __popcll(g.rows[uint64_t(r) * W + w] & ~s[w]);
-    best = c < best ? c : best;
+#pragma unroll
+    for (int j = 0; j < kGuideDepths; ++j)
+      if (r < g.ends[j]) best[j] = c < best[j] ? c : best[j];
   }
-  while (j < kGuideDepths) row[n + k + j++] = f2bf(kFeatureScale * float(best) / float(n));
-  row[n + k + kGuideDepths] = f2bf(kFeatureScale * float(card) / float(n));
+#pragma unroll
+  for (int j = 0; j < kGuideDepths; ++j)
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint32_t o = __shfl_xor(best[j], off, 64);
+      best[j] = o < best[j] ? o : best[j];
+    }
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < kGuideDepths; ++j) row[n + k + j] = f2bf(kFeatureScale * float(best[j]) / float(n));
+    row[n + k + kGuideDepths] = f2bf(kFeatureScale * float(card) / float(n));
+  }
+  for (uint32_t r = n + k + kGuideDepths + 1 + lane; r < D; r += 64) row[r] = 0;
 }
 
 // y[i] = h[i] . w3 + b3 (FP32), one wave per row.
@@ -300,12 +327,15 @@ hipError_t launch_gemm(bool ta, bool tb, const uint16_t* A, int lda, const uint1
     uint16_t* Cb, int ldc, int M, int N, int K, const float* bias, int relu, int accumulate, hipStream_t st) {
   if (M <= 0 || N <= 0) return hipSuccess;
   dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
-#define L(a, b) hipLaunchKernelGGL((k_gemm<a, b>), grid, dim3(256), 0, st, A, lda, B, ldb, Cf, Cb, ldc, M, N, K, bias, \
-    relu, accumulate)
-  if (!ta && !tb) L(false, false);
-  else if (!ta && tb) L(false, true);
-  else if (ta && !tb) L(true, false);
-  else L(true, true);
+#define L(a, b, v) hipLaunchKernelGGL((k_gemm<a, b, v>), grid, dim3(256), 0, st, A, lda, B, ldb, Cf, Cb, ldc, M, N, K, \
+    bias, relu, accumulate)
+  const bool vec = !ta && tb && K % 8 == 0 && lda % 8 == 0 && ldb % 8 == 0 &&
+      (reinterpret_cast<uintptr_t>(A) & 15u) == 0 && (reinterpret_cast<uintptr_t>(B) & 15u) == 0;
+  if (vec) L(false, true, true);
+  else if (!ta && !tb) L(false, false, false);
+  else if (!ta && tb) L(false, true, false);
+  else if (ta && !tb) L(true, false, false);
+  else L(true, true, false);
 #undef L
   return hipGetLastError();
 }
@@ -314,7 +344,7 @@ hipError_t launch_features(int W, const uint64_t* sets, const uint64_t* images, 
     uint32_t D, const FeatureGuide& g, uint16_t* x, hipStream_t st) {
   if (!count) return hipSuccess;
   switch (W) {
-#define C(w) case w: hipLaunchKernelGGL(k_features<w>, dim3(blocks(count)), dim3(256), 0, st, sets, images, count, n, k, D, g, x); break;
+#define C(w) case w: hipLaunchKernelGGL(k_features<w>, dim3(unsigned((count + 3) / 4)), dim3(256), 0, st, sets, images, count, n, k, D, g, x); break;
     C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8)
 #undef C
     default: return hipErrorInvalidValue;

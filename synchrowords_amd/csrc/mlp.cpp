@@ -108,7 +108,7 @@ MlpWork work_of(sw_mlp* m, uint64_t rows) {
 // Random automata share no state names; this order makes a feature column
 // mean the same thing ("the r-th most often needed state") in every one.
 struct Guide {
-  DevBuf perm, rows;
+  DevBuf order, rows;
   FeatureGuide g;
 };
 
@@ -145,14 +145,13 @@ void build_guide(uint32_t n, uint32_t k, const uint32_t* delta, Guide& out, hipS
   std::vector<uint32_t> order(n);
   std::iota(order.begin(), order.end(), 0u);
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return freq[a] > freq[b]; });
-  std::vector<uint16_t> perm(n);
-  for (uint32_t r = 0; r < n; ++r) perm[order[r]] = uint16_t(r);
-  out.perm.ensure(size_t(n) * 2 + 8);
+  std::vector<uint16_t> col_state(order.begin(), order.end());
+  out.order.ensure(size_t(n) * 2 + 8);
   out.rows.ensure(rows.size() * 8 + 8);
-  SW_CHECK_HIP(hipMemcpyAsync(out.perm.p, perm.data(), size_t(n) * 2, hipMemcpyHostToDevice, st));
+  SW_CHECK_HIP(hipMemcpyAsync(out.order.p, col_state.data(), size_t(n) * 2, hipMemcpyHostToDevice, st));
   SW_CHECK_HIP(hipMemcpyAsync(out.rows.p, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, st));
   SW_CHECK_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
-  out.g.perm = out.perm.as<uint16_t>();
+  out.g.order = out.order.as<uint16_t>();
   out.g.rows = out.rows.as<uint64_t>();
   for (int j = 0; j < kGuideDepths; ++j) out.g.ends[j] = ends[j];
 }

@@ -34,7 +34,7 @@ struct MlpAdam {
 constexpr int kGuideDepths = 4;
 constexpr uint32_t kGuideFeatures = kGuideDepths + 1;  // min-missing counts + cardinality
 struct FeatureGuide {
-  const uint16_t* perm = nullptr;  // perm[q]: feature column of state q
+  const uint16_t* order = nullptr;  // order[r]: the state of feature column r (canonical order)
   const uint64_t* rows = nullptr;  // images of Q, by depth (W words each)
   uint32_t ends[kGuideDepths] = {0, 0, 0, 0};  // rows of depth <= 2, 4, 6, 8
 };
