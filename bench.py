@@ -249,6 +249,53 @@ def bench_brute(ctx, count: int, no_cpu: bool) -> dict:
     return out
 
 
+def bench_eppstein(ctx, count: int, no_cpu: bool) -> dict:
+    """Eppstein's bound (pair BFS + greedy word) for the 4096-instance batch in
+    one launch, and for single n=300 automata, beside the oracle's restatement
+    on one host core (a sample)."""
+    import synchrowords_amd as sw
+    from synchrowords_amd import gen
+    specs = gen.batch_specs(4096, 2025)[:count]
+    inst = [(n, k, sw.random_dfa(n, k, seed)) for n, k, seed in specs]
+    ctx.eppstein_batch(inst[:64])  # warm
+    t0 = time.perf_counter()
+    ns, bounds, _ = ctx.eppstein_batch(inst)
+    wall = time.perf_counter() - t0
+    out = {"workload": f"Eppstein bound of {count} random DFAs, n in [16,160], k in {{2,3}} (batch_specs(4096, 2025))",
+           "wall_s": round(wall, 4), "instances_per_s": count / wall, "non_synchro": int(ns.sum())}
+    fx = ROOT / "tests" / "golden" / "c4_batch.json"
+    if fx.exists():
+        cases = json.loads(fx.read_text())["cases"][:count]
+        out["non_synchro_mismatches_vs_reference"] = sum(int(bool(c["non_synchro"]) != bool(a))
+                                                         for c, a in zip(cases, ns))
+    singles = [sw.random_dfa(300, 2, 1000 + s) for s in range(5)]
+    ctx.eppstein(300, 2, singles[0])
+    t0 = time.perf_counter()
+    sb = [ctx.eppstein(300, 2, d)[1] for d in singles]
+    out["n300_s_each"] = round((time.perf_counter() - t0) / len(singles), 5)
+    out["n300_bounds"] = sb
+    if not no_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib as orc
+        t0 = time.perf_counter()
+        done = mism = 0
+        while done < count and time.perf_counter() - t0 < 15.0:
+            n, k, d = inst[done]
+            w = orc.eppstein(n, k, d, (n - 1) ** 2 + 1)
+            mism += int(w[0] != bool(ns[done]) or w[1] != int(bounds[done]))
+            done += 1
+        cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": done / cpu, "unit": "instances/s", "cores": 1, "kind": "port",
+                               "sample": f"first {done} automata of the batch, oracle pair BFS + greedy"}
+        out["mismatches_vs_oracle_sample"] = mism
+        out["speedup_batch"] = round(out["instances_per_s"] / (done / cpu), 1)
+        t1 = time.perf_counter()
+        ob = [orc.eppstein(300, 2, d, 299 * 299 + 1)[1] for d in singles]
+        out["n300_cpu_s_each"] = round((time.perf_counter() - t1) / len(singles), 5)
+        out["n300_mismatches"] = sum(int(a != b) for a, b in zip(sb, ob))
+    return out
+
+
 def bench_learned(ctx, scorer: str, seeds, width: int) -> dict:
     """Learned-bound against plain-bound exact search on the GPU (n=300, k=2).
 
@@ -306,9 +353,11 @@ def bench_learned(ctx, scorer: str, seeds, width: int) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["subset", "batch", "learned", "exact", "brute"], default="subset",
+    ap.add_argument("--workload", choices=["subset", "batch", "learned", "exact", "brute", "eppstein"],
+                    default="subset",
                     help="subset: the headline line; batch / learned: the batched and learned-bound reports; "
-                         "exact: one verbose exact search (--exact-n, --exact-seed); brute: power-set BFS")
+                         "exact: one verbose exact search (--exact-n, --exact-seed); brute: power-set BFS; "
+                         "eppstein: Eppstein's bound")
     ap.add_argument("--exact-n", type=int, default=300)
     ap.add_argument("--exact-seed", type=int, default=1)
     ap.add_argument("--batch-count", type=int, default=4096)
@@ -358,6 +407,8 @@ def main() -> int:
             line = bench_batch(ctx, args.batch_count)
         elif args.workload == "brute":
             line = bench_brute(ctx, args.batch_count, args.no_cpu)
+        elif args.workload == "eppstein":
+            line = bench_eppstein(ctx, args.batch_count, args.no_cpu)
         elif args.workload == "exact":
             d = sw.random_dfa(args.exact_n, 2, args.exact_seed)
             ub = ctx.pipeline(args.exact_n, 2, d).beam_bound  # warm; the plan's Beam bound caps the search

@@ -92,9 +92,9 @@ __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __res
     const uint32_t* __restrict__ deltas, unsigned char* __restrict__ scratch, uint32_t* __restrict__ words,
     EppOut* __restrict__ outs) {
   __shared__ uint32_t wsum[16];
-  __shared__ uint32_t s_total;
   __shared__ uint64_t s_best[16];
   __shared__ unsigned char cur[kEppMaxN], nxt[kEppMaxN];
+  __shared__ uint16_t cs[kEppMaxN];
   const EppJob job = jobs[blockIdx.x];
   const uint32_t n = job.n, k = job.k;
   const uint32_t* d = deltas + job.delta_off;
@@ -204,15 +204,26 @@ __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __res
   // ---- greedy merging of the closest pair
   for (uint32_t q = tid; q < n; q += nt) cur[q] = 1;
   __syncthreads();
-  uint32_t size = n;
+  // The current set's states, ascending, in cs[0 .. m) (n <= 512 < threads:
+  // one flag per thread).
+  auto compact = [&]() {
+    const uint32_t f = tid < n ? uint32_t(cur[tid]) : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_scan(f, wsum, &tot);
+    if (f) cs[ex] = uint16_t(tid);
+    __syncthreads();
+    return tot;
+  };
+  uint32_t m = compact();
   uint64_t total = 0, wl = 0;
   uint32_t* word = words + job.word_off;
-  while (size > 1) {
+  while (m > 1) {
     uint64_t best = ~0ull;  // (length, first, second)
-    for (uint32_t i = tid; i < n * n; i += nt) {
-      const uint32_t a = i / n, b = i % n;
-      if (a < b && cur[a] && cur[b]) {
-        const uint64_t key = (uint64_t(info[i] >> 8) << 32) | (uint64_t(a) << 16) | b;
+    for (uint32_t t = tid; t < m * m; t += nt) {
+      const uint32_t i = t / m, j = t % m;
+      if (i < j) {
+        const uint32_t a = cs[i], b = cs[j];
+        const uint64_t key = (uint64_t(info[a * n + b] >> 8) << 32) | (uint64_t(a) << 16) | b;
         best = key < best ? key : best;
       }
     }
@@ -239,20 +250,13 @@ __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __res
       __syncthreads();
       for (uint32_t q = tid; q < n; q += nt)
         if (cur[q]) nxt[d[size_t(q) * k + x]] = 1;
-      if (tid == 0) s_total = 0;
       __syncthreads();
-      uint32_t c = 0;
-      for (uint32_t q = tid; q < n; q += nt) {
-        cur[q] = nxt[q];
-        c += nxt[q];
-      }
-      if (c) atomicAdd(&s_total, c);
+      for (uint32_t q = tid; q < n; q += nt) cur[q] = nxt[q];
       __syncthreads();
-      size = s_total;
       n1 = d[size_t(n1) * k + x];
       n2 = d[size_t(n2) * k + x];
-      __syncthreads();  // s_total is reset by the next letter
     }
+    m = compact();
   }
   if (tid == 0) *out = EppOut{0, 1, total, wl};
 }
