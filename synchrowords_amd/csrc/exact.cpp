@@ -101,13 +101,11 @@ class Search {
     F_.reset();
     B_.reset();
     budget_pairs_ = cfg.dominance_pairs ? cfg.dominance_pairs : 100000000000ull;
-    adaptive_dominance_ = cfg.dominance_pairs == 0;
     forward_cap_ = cfg.forward_cap ? cfg.forward_cap : (1ull << 22);
     mem_cap_ = (cfg.max_memory_mb ? cfg.max_memory_mb : 65536ull) << 20;
   }
 
   ExactOutcome run(uint64_t lower, uint64_t upper) {
-    upper_ = upper;
     auto t0 = std::chrono::steady_clock::now();
     ExactOutcome out;
     sw_exact_result& r = out.r;
@@ -200,8 +198,7 @@ class Search {
   const uint32_t* delta_;
   sw_exact_config cfg_;
   uint64_t budget_pairs_, forward_cap_, mem_cap_;
-  bool adaptive_dominance_ = true;
-  uint64_t upper_ = 0;
+
   uint64_t pair_tests_ = 0;
   double ms_expand_ = 0, ms_dom_ = 0, ms_goal_ = 0;  // host time between synchronizations
   Workspace& ws_;
@@ -216,25 +213,6 @@ class Search {
   // and is dropped after its first few states, rarest in the tile first; and
   // sorted by size, a stream set is tested only against tiles with a set at
   // least as large.
-#ifndef SW_DOMINANCE_GAIN
-#define SW_DOMINANCE_GAIN 0.03
-#endif
-  static constexpr double kDominanceGain = SW_DOMINANCE_GAIN;
-  // Whether to prune the new level of S by dominance.  Within the fixed pair
-  // budget always.  Past it (unless the budget was set explicitly) when the
-  // pass, S.rows x nnew pairs, costs less than what pruning is expected to
-  // save: about a tenth of the frontier, which shrinks every later goal check
-  // (nnew x the other frontier now, growing about 1.9x per step) over the
-  // steps left below the upper bound.
-  bool dominance_pays(const Side& S, const Side& other, uint64_t nnew, uint64_t t) const {
-    const double pairs = double(S.rows) * double(nnew);
-    if (pairs <= double(budget_pairs_)) return true;
-    if (!adaptive_dominance_) return false;
-    const uint64_t left = upper_ > t + 1 ? std::min<uint64_t>(upper_ - t - 1, 40) : 0;
-    const double saved = kDominanceGain * std::pow(1.9, double(left)) * double(nnew) * double(other.fcount);
-    return pairs <= saved;
-  }
-
   // Cardinalities of a frontier (backward: of the preimages, i.e. the rows
   // complemented): min, deciles 1/5/9, max.  Verbose log only.
   std::string size_quantiles(Side& S, bool complement) {
@@ -405,7 +383,7 @@ class Search {
     S.prev_fcount = S.fcount;
     S.front.ensure(nnew * row_bytes() + 8);
     S.front_idx.ensure(nnew * 4 + 4);
-    const bool dominance = nnew > 0 && dominance_pays(S, backward ? F_ : B_, nnew, t);
+    const bool dominance = nnew > 0 && S.rows * nnew <= budget_pairs_;
     if (dominance) {
       // Host time up to here (the new-level size was read back, so the
       // expansion and dedup are done) counts as expansion; the commit still

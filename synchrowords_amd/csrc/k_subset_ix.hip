@@ -613,9 +613,10 @@ hipError_t launch_ix_w(const VsubParams& p0, hipStream_t st) {
     if (chunk < 128) chunk = 128;
     p.s_chunk = chunk;
   }
-  if (p.early_exit && p.s_bound && p0.s_chunk == 0) {
+  if (p.early_exit && p.s_bound && p0.s_chunk == 0 && p.s_chunk > SW_IX_EE_CHUNK) {
     // Sorted lists: chunks of the stream small enough that the workgroups
-    // running at once (consecutive tiles, one chunk) share its index in L2.
+    // running at once (consecutive tiles, one chunk) share its index in L2;
+    // smaller ones when the workgroup target asks for them.
     p.s_chunk = SW_IX_EE_CHUNK;
   }
   p.s_chunk = (p.s_chunk + 1) & ~1ull;  // even: pairs never straddle two workgroups
