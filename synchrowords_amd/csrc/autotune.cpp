@@ -54,6 +54,7 @@ void apply_tuning(sw_ctx* ctx, int W, VsubParams& p) {
   p.threads = it->second.threads;
   p.target_wgs = it->second.target_wgs;
   p.early_exit = int(it->second.early_exit);
+  p.ix_early_exit = int(it->second.early_exit);  // the class's kernel is the one tuned
   p.mask_bits = int(it->second.mask_bits);
 }
 
@@ -99,7 +100,7 @@ int sw_autotune_subset(sw_ctx* ctx, uint32_t n, uint64_t s_count, uint64_t t_cou
         p.mode = VSUB_COUNT;
         p.count_out = out.as<unsigned long long>();
         p.target_wgs = wgs;
-        p.early_exit = int(ee);
+        p.ix_early_exit = int(ee);
         run_subset(ctx, W, p);  // warm; builds the index
         SW_CHECK_HIP(hipEventRecord(e0, st));
         for (uint32_t r = 0; r < (reps ? reps : 3); ++r) SW_CHECK_HIP(launch_vsub_ix(W, p, st));
@@ -195,14 +196,15 @@ int sw_set_subset_kernel(sw_ctx* ctx, int indexed, uint64_t min_pairs) {
 }
 
 // How the indexed kernel walks the states of a stream set: as each launch
-// asks (0: the exact search asks for 2, other launches take the tuned or
-// default shape), in set order without early exit (1), or rarest-in-tile
-// first with early exit (2).
+// asks (0: the exact search asks for 2, counts for sorted lists, other
+// launches take the tuned or default shape), in set order without early exit
+// (1), rarest-in-tile first with early exit over size-sorted lists (2), or
+// size-sorted lists in set order without early exit (3).
 int sw_set_subset_walk(sw_ctx* ctx, int walk) {
   using namespace sw;
   try {
     SW_REQUIRE(ctx, "NULL argument");
-    SW_REQUIRE(walk >= 0 && walk <= 2, "walk must be 0, 1 or 2");
+    SW_REQUIRE(walk >= 0 && walk <= 3, "walk must be 0, 1, 2 or 3");
     ctx->ix_walk = walk;
     return SW_OK;
   } catch (const Error& e) {

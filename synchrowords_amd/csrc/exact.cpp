@@ -240,7 +240,7 @@ class Search {
   }
 
   static void search_walk(VsubParams& p) {
-    p.early_exit = 1;
+    p.ix_early_exit = 1;
     p.order_states = 1;
     p.sort_by_size = 1;
   }

@@ -596,7 +596,7 @@ hipError_t launch_ix_kernel(const VsubParams& p, uint64_t gx, uint64_t gy, size_
 
 template <int W, int MODE>
 hipError_t launch_ix_mode(const VsubParams& p, uint64_t gx, uint64_t gy, size_t lds, hipStream_t st) {
-  return p.early_exit ? launch_ix_kernel<W, MODE, true>(p, gx, gy, lds, st)
+  return p.ix_early_exit ? launch_ix_kernel<W, MODE, true>(p, gx, gy, lds, st)
                       : launch_ix_kernel<W, MODE, false>(p, gx, gy, lds, st);
 }
 
@@ -607,13 +607,13 @@ hipError_t launch_ix_w(const VsubParams& p0, hipStream_t st) {
   const uint64_t gx = (p.t_count + kIxTile - 1) / kIxTile;
   if (p.s_chunk == 0) {
     uint64_t target = p.target_wgs ? p.target_wgs : 1024;
-    if (p.early_exit && target < SW_IX_EE_WGS) target = SW_IX_EE_WGS;
+    if (p.ix_early_exit && target < SW_IX_EE_WGS) target = SW_IX_EE_WGS;
     const uint64_t gyt = (target + gx - 1) / gx;
     uint64_t chunk = (p.s_count + gyt - 1) / gyt;
     if (chunk < 128) chunk = 128;
     p.s_chunk = chunk;
   }
-  if (p.early_exit && p.s_bound && p0.s_chunk == 0 && p.s_chunk > SW_IX_EE_CHUNK) {
+  if (p.ix_early_exit && p.s_bound && p0.s_chunk == 0 && p.s_chunk > SW_IX_EE_CHUNK) {
     // Sorted lists: chunks of the stream small enough that the workgroups
     // running at once (consecutive tiles, one chunk) share its index in L2;
     // smaller ones when the workgroup target asks for them.
@@ -741,7 +741,7 @@ hipError_t launch_state_rank(const uint32_t* freq, uint32_t n, uint32_t* rank, u
 hipError_t launch_vsub_ix(int W, const VsubParams& p, hipStream_t st) {
   if (!p.ix_start || !p.ix_list) return hipErrorInvalidValue;
   if (p.mode == VSUB_ANY && (!p.found || !p.witness)) return hipErrorInvalidValue;
-  if (p.early_exit && !p.ix_head) return hipErrorInvalidValue;
+  if (p.ix_early_exit && !p.ix_head) return hipErrorInvalidValue;
   switch (W) {
     case 1: return launch_ix_w<1>(p, st);
     case 2: return launch_ix_w<2>(p, st);

@@ -19,19 +19,29 @@ void run_subset(sw_ctx* ctx, int W, VsubParams& p) {
     return;
   }
   if (ctx->ix_walk == 1) {
-    p.early_exit = 0;
+    p.ix_early_exit = 0;
     p.order_states = 0;
     p.sort_by_size = 0;
   } else if (ctx->ix_walk == 2) {
-    p.early_exit = 1;
+    p.ix_early_exit = 1;
     p.order_states = 1;
     p.sort_by_size = 1;
+  } else if (ctx->ix_walk == 3) {
+    p.ix_early_exit = 0;
+    p.order_states = 0;
+    p.sort_by_size = 1;
   }
+#ifndef SW_SORT_COUNTS
+#define SW_SORT_COUNTS 1
+#endif
+  // Counts walk every pair's states: sorted by size, the two sets of a pair
+  // are of like size, so the shorter one reads fewer padding rows.
+  if (SW_SORT_COUNTS && ctx->ix_walk == 0 && p.mode == VSUB_COUNT) p.sort_by_size = 1;
   const uint64_t s = p.s_count, pairs = (s + 1) / 2;
   // Sorting costs two passes over each list and a few launches: worth it
   // for launches of more than about a millisecond (unless forced).
   const bool want_sort =
-      p.sort_by_size && (ctx->ix_walk == 2 || tests >= (1ull << 34)) && p.t_count < (1ull << 32);
+      p.sort_by_size && (ctx->ix_walk >= 2 || tests >= (1ull << 34)) && p.t_count < (1ull << 32);
   if (want_sort && p.diag != INT64_MIN && p.distinct_sets && vsub_ix_fits(p.n, W, true)) {
     p.diag = INT64_MIN;
     p.proper_by_size = 1;
@@ -93,7 +103,7 @@ void run_subset(sw_ctx* ctx, int W, VsubParams& p) {
     order = f + 1024;
   }
   uint32_t* head = nullptr;
-  if (p.early_exit) {
+  if (p.ix_early_exit) {
     ctx->ix_head.ensure(pairs * size_t(ix_head_groups()) * 32 + 64);
     head = ctx->ix_head.as<uint32_t>();
   }

@@ -102,7 +102,9 @@ struct sw_ctx {
   // Eppstein batches: jobs, packed automata, pair-BFS scratch, words, results.
   sw::DevBuf epp_jobs, epp_delta, epp_scratch, epp_words, epp_out;  // sort by size: histograms and cursors, both permutations
   // How the indexed kernel walks stream sets: as each launch asks (0), always
-  // in set order without early exit (1), always rarest first with early exit (2).
+  // in set order without early exit (1), always rarest first with early exit
+  // over size-sorted lists (2), always size-sorted lists walked in set order
+  // without early exit (3).
   int ix_walk = 0;
   size_t ix_tmp_bytes = 0;
   uint32_t* ix_host = nullptr;  // pinned: the index length read back

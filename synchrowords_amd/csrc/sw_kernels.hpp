@@ -21,7 +21,8 @@ struct VsubParams {
   int complement_tile = 0;           // test S_i against the complement of T_j
   int complement_stream = 0;         // use the complement of S_i
   int mode = VSUB_COUNT;
-  int early_exit = 1;                // stop a stream set once no lane can hit
+  int early_exit = 1;                // bit-walking kernel: stop a stream set once no lane can hit
+  int ix_early_exit = 0;             // indexed kernel: the same, checked after every four states
   int proper_by_size = 0;            // skip pairs of equal cardinality (S == T)
   int64_t diag = INT64_MIN;          // skip pairs with i - j == diag
   uint64_t s_chunk = 0;              // stream sets per workgroup row (0: auto)
@@ -39,9 +40,9 @@ struct VsubParams {
   const uint32_t* ix_start = nullptr;
   const uint32_t* ix_list = nullptr;
   const uint32_t* ix_size = nullptr;
-  const uint32_t* ix_head = nullptr;  // with early_exit: each pair's first groups, fixed stride
+  const uint32_t* ix_head = nullptr;  // with ix_early_exit: each pair's first groups, fixed stride
   // Indexed kernel: list each stream set's states rarest-in-tile first
-  // (fewest tile sets hold them), so that with early_exit a stream set that
+  // (fewest tile sets hold them), so that with ix_early_exit a stream set that
   // lies in no tile set is dropped after few states.
   int order_states = 0;
   // Indexed kernel, without diag: sort both lists by
