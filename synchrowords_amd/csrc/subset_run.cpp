@@ -87,10 +87,21 @@ void run_subset(sw_ctx* ctx, int W, VsubParams& p) {
   uint32_t* start = ctx->ix_start.as<uint32_t>();
   SW_CHECK_HIP(launch_scan(ctx->ix_padded.as<uint32_t>(), start, pairs, start + pairs, ctx->ix_tmp.p,
       &ctx->ix_tmp_bytes, st));
-  SW_CHECK_HIP(hipMemcpyAsync(ctx->ix_host, start + pairs, 4, hipMemcpyDeviceToHost, st));
-  SW_CHECK_HIP(hipStreamSynchronize(st));
-  const uint64_t total = ctx->ix_host[0];
-  ctx->ix_list.ensure(total * 4 + 64);
+  // The index length: a pair holds at most 8 ceil(n / 4) entries.  While that
+  // bound fits in 1 GiB the list is sized by it and nothing is read back (a
+  // round trip per launch is a good part of a small search's step); past it,
+  // the exact length is read back.
+  const uint64_t bound = pairs * 8 * ((uint64_t(p.n) + 3) / 4);
+  if (bound * 4 <= (uint64_t(1) << 30)) {
+    // Growing frees the old list: the stream is idle first.
+    if (bound * 4 + 64 > ctx->ix_list.cap) SW_CHECK_HIP(hipStreamSynchronize(st));
+    ctx->ix_list.ensure(bound * 4 + 64);
+  } else {
+    SW_CHECK_HIP(hipMemcpyAsync(ctx->ix_host, start + pairs, 4, hipMemcpyDeviceToHost, st));
+    SW_CHECK_HIP(hipStreamSynchronize(st));
+    const uint64_t total = ctx->ix_host[0];
+    ctx->ix_list.ensure(total * 4 + 64);
+  }
   const uint32_t* rank = nullptr;
   const uint32_t* order = nullptr;
   if (p.order_states) {
