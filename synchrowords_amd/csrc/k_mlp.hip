@@ -385,22 +385,6 @@ __global__ void k_neg_popcount(const uint64_t* __restrict__ sets, uint64_t count
   y[i] = -float(c);
 }
 
-// Sort keys of the reference's Beam order: word w of each set in `perm`
-// order, and n - |S| (larger sets first).
-__global__ void k_word_keys(const uint64_t* __restrict__ sets, const uint32_t* __restrict__ perm, uint64_t count,
-    uint32_t W, uint32_t w, uint64_t* __restrict__ keys) {
-  const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j < count) keys[j] = sets[uint64_t(perm[j]) * W + w];
-}
-
-__global__ void k_size_keys(const uint64_t* __restrict__ sets, const uint32_t* __restrict__ perm, uint64_t count,
-    uint32_t W, uint32_t n, uint32_t* __restrict__ keys) {
-  const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= count) return;
-  int c = 0;
-  for (uint32_t w = 0; w < W; ++w) c += __popcll(sets[uint64_t(perm[j]) * W + w]);
-  keys[j] = n - uint32_t(c);
-}
 
 __global__ void k_iota_u32(uint32_t* p, uint64_t count) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -448,41 +432,51 @@ hipError_t sort_scores(const float* keys_in, float* keys_out, const uint32_t* va
       32, st);
 }
 
+// The reference's Beam comparator over distinct sets: larger first, then
+// word 0, word 1, ... ascending.  neg[i] = n - |S_i|.
+struct BeamLess {
+  const uint64_t* sets;
+  const uint32_t* neg;
+  uint32_t W;
+  __device__ bool operator()(uint32_t a, uint32_t b) const {
+    if (neg[a] != neg[b]) return neg[a] < neg[b];
+    for (uint32_t w = 0; w < W; ++w) {
+      const uint64_t x = sets[uint64_t(a) * W + w], y = sets[uint64_t(b) * W + w];
+      if (x != y) return x < y;
+    }
+    return false;
+  }
+};
+
+__global__ void k_neg_size(const uint64_t* __restrict__ sets, uint64_t count, uint32_t W, uint32_t n,
+    uint32_t* __restrict__ neg) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint32_t c = 0;
+  for (uint32_t w = 0; w < W; ++w) c += __popcll(sets[i * W + w]);
+  neg[i] = n - c;
+}
+
 hipError_t beam_reference_order(const uint64_t* sets, uint64_t count, uint32_t W, uint32_t n, uint32_t* perm_a,
     uint32_t* perm_b, uint64_t* key_a, uint64_t* key_b, void* temp, size_t* temp_bytes, uint32_t** result,
     hipStream_t st) {
+  (void)perm_b;
+  (void)key_b;
   if (count > 0x7fffffffull) return hipErrorInvalidValue;
   const int cnt = int(count);
-  int end_bit = 1;
-  while ((1u << end_bit) <= n) ++end_bit;
+  uint32_t* neg = reinterpret_cast<uint32_t*>(key_a);
   if (!temp) {
-    size_t a = 0, b = 0;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, a, key_a, key_b, perm_a, perm_b, cnt ? cnt : 1, 0, 64, st);
-    if (e != hipSuccess) return e;
-    e = hipcub::DeviceRadixSort::SortPairs(nullptr, b, reinterpret_cast<uint32_t*>(key_a),
-        reinterpret_cast<uint32_t*>(key_b), perm_a, perm_b, cnt ? cnt : 1, 0, end_bit, st);
-    if (e != hipSuccess) return e;
-    *temp_bytes = a > b ? a : b;
-    return hipSuccess;
+    return hipcub::DeviceMergeSort::SortKeys(nullptr, *temp_bytes, static_cast<uint32_t*>(nullptr), cnt ? cnt : 1,
+        BeamLess{nullptr, nullptr, W}, st);
   }
   *result = perm_a;
   if (!count) return hipSuccess;
-  // Least significant key first, every pass stable: word W-1 ... word 0, then size.
+  // One comparison sort of the indices; the sets are distinct, so the order
+  // is total and any correct sort gives the reference's.
   hipLaunchKernelGGL(k_iota_u32, dim3(blocks(count)), dim3(256), 0, st, perm_a, count);
-  uint32_t* in = perm_a;
-  uint32_t* out = perm_b;
-  for (int w = int(W) - 1; w >= 0; --w) {
-    hipLaunchKernelGGL(k_word_keys, dim3(blocks(count)), dim3(256), 0, st, sets, in, count, W, uint32_t(w), key_a);
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, key_a, key_b, in, out, cnt, 0, 64, st);
-    if (e != hipSuccess) return e;
-    std::swap(in, out);
-  }
-  uint32_t* ka = reinterpret_cast<uint32_t*>(key_a);
-  uint32_t* kb = reinterpret_cast<uint32_t*>(key_b);
-  hipLaunchKernelGGL(k_size_keys, dim3(blocks(count)), dim3(256), 0, st, sets, in, count, W, n, ka);
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, ka, kb, in, out, cnt, 0, end_bit, st);
+  hipLaunchKernelGGL(k_neg_size, dim3(blocks(count)), dim3(256), 0, st, sets, count, W, n, neg);
+  hipError_t e = hipcub::DeviceMergeSort::SortKeys(temp, *temp_bytes, perm_a, cnt, BeamLess{sets, neg, W}, st);
   if (e != hipSuccess) return e;
-  *result = out;
   return hipGetLastError();
 }
 
