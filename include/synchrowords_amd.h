@@ -279,14 +279,16 @@ int sw_set_subset_kernel(sw_ctx* ctx, int indexed, uint64_t min_pairs);
 
 /* How the indexed kernel walks the states of each stream set.  walk = 0 (the
  * default): as each launch asks — the exact search's goal checks and
- * dominance passes sort both lists by size, list every stream set's states
- * rarest first (fewest sets of the other list hold them) and drop a stream
- * set as soon as no set of the other list can still hold it; large counts
- * sort both lists by size; other launches take the tuned shape.  1: set
- * order, unsorted, no early exit, for every launch.  2: sorted, rarest first
- * with early exit, for every launch.  3: sorted, set order, no early exit,
- * for every launch.  Results are the same in every mode.  No reference
- * counterpart. */
+ * dominance passes list every stream set's states rarest first (fewest sets
+ * of the other list hold them), drop a stream set as soon as no set of the
+ * other list can still hold it, and test it only against the sets holding
+ * its rarest state (anchor buckets) or, where the mode does not allow that,
+ * against size-sorted lists; large counts sort both lists by size; other
+ * launches take the tuned shape.  1: set order, unsorted, no early exit, for
+ * every launch.  2: sorted by size, rarest first with early exit.  3: sorted
+ * by size, set order, no early exit.  4: anchor buckets, rarest first with
+ * early exit, where the mode allows (not counts), else as 2.  Results are the
+ * same in every mode.  No reference counterpart. */
 int sw_set_subset_walk(sw_ctx* ctx, int walk);
 
 /* -------------------------------------------------------- learned upper bound

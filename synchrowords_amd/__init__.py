@@ -460,7 +460,8 @@ class Context:
     def set_subset_walk(self, walk: int = 0) -> None:
         """How the indexed kernel walks stream sets: as each launch asks (0),
         set order without early exit (1), rarest first with early exit over
-        size-sorted lists (2), size-sorted lists in set order (3)."""
+        size-sorted lists (2), size-sorted lists in set order (3), anchor
+        buckets where the mode allows (4)."""
         _check(load().sw_set_subset_walk(self.handle, walk))
 
     def calibrate_lds(self, bytes_per_lane: int = 8) -> float:

@@ -198,13 +198,14 @@ int sw_set_subset_kernel(sw_ctx* ctx, int indexed, uint64_t min_pairs) {
 // How the indexed kernel walks the states of a stream set: as each launch
 // asks (0: the exact search asks for 2, counts for sorted lists, other
 // launches take the tuned or default shape), in set order without early exit
-// (1), rarest-in-tile first with early exit over size-sorted lists (2), or
-// size-sorted lists in set order without early exit (3).
+// (1), rarest-in-tile first with early exit over size-sorted lists (2),
+// size-sorted lists in set order without early exit (3), or rarest first with
+// early exit over anchor buckets where the mode allows, else as 2 (4).
 int sw_set_subset_walk(sw_ctx* ctx, int walk) {
   using namespace sw;
   try {
     SW_REQUIRE(ctx, "NULL argument");
-    SW_REQUIRE(walk >= 0 && walk <= 3, "walk must be 0, 1, 2 or 3");
+    SW_REQUIRE(walk >= 0 && walk <= 4, "walk must be in [0, 4]");
     ctx->ix_walk = walk;
     return SW_OK;
   } catch (const Error& e) {

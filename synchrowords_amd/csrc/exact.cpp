@@ -210,9 +210,10 @@ class Search {
   uint64_t row_bytes() const { return uint64_t(W_) * 8; }
 
   // The search's lists meet rarely: a stream set usually lies in no tile set
-  // and is dropped after its first few states, rarest in the tile first; and
-  // sorted by size, a stream set is tested only against tiles with a set at
-  // least as large.
+  // and is dropped after its first few states, rarest in the tile first; it
+  // is tested only against the tile sets holding its rarest state (anchor
+  // buckets) or, where those do not apply, sorted by size, only against tiles
+  // with a set at least as large.
   // Cardinalities of a frontier (backward: of the preimages, i.e. the rows
   // complemented): min, deciles 1/5/9, max.  Verbose log only.
   std::string size_quantiles(Side& S, bool complement) {
@@ -243,6 +244,7 @@ class Search {
     p.ix_early_exit = 1;
     p.order_states = 1;
     p.sort_by_size = 1;
+    p.bucket_anchor = 1;
   }
   static double ms_since(std::chrono::steady_clock::time_point t) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();

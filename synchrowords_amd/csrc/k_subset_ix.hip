@@ -252,6 +252,106 @@ __global__ __launch_bounds__(256) void k_size_scatter(const uint64_t* __restrict
   }
 }
 
+// Anchor of each stream set: its state of lowest rank (fewest tile sets hold
+// it), or n for the empty set; hist[anchor] += 1 (hist starts at zero).
+template <int W>
+__global__ __launch_bounds__(256) void k_anchor(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n,
+    uint64_t cmask, const uint32_t* __restrict__ rank, uint32_t* __restrict__ anchor, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[W * 64 + 1];
+  for (int t = threadIdx.x; t <= W * 64; t += blockDim.x) h[t] = 0;
+  __syncthreads();
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < count) {
+    uint32_t best = 0xffffffffu, a = n;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      uint64_t b = (sets[i * W + w] ^ cmask) & word_mask(n, w);
+      while (b) {
+        const uint32_t q = uint32_t(w) * 64 + __builtin_ctzll(b);
+        b &= b - 1;
+        const uint32_t r = rank[q];
+        if (r < best) {
+          best = r;
+          a = q;
+        }
+      }
+    }
+    anchor[i] = a;
+    atomicAdd(&h[a], 1u);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t <= W * 64; t += blockDim.x)
+    if (uint32_t(t) <= n && h[t]) atomicAdd(&hist[t], h[t]);
+}
+
+// perm[cursor[key[i]]++] = i, with one global atomic per key per workgroup
+// (keys < nkeys <= 513).
+__global__ __launch_bounds__(256) void k_key_scatter(const uint32_t* __restrict__ key, uint64_t count, uint32_t nkeys,
+    uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm) {
+  __shared__ uint32_t cnt[513], base[513];
+  for (uint32_t t = threadIdx.x; t < nkeys; t += blockDim.x) cnt[t] = 0;
+  __syncthreads();
+  const uint64_t i0 = uint64_t(blockIdx.x) * blockDim.x * kScatterPer + threadIdx.x;
+  uint32_t c[kScatterPer], local[kScatterPer];
+#pragma unroll
+  for (int e = 0; e < kScatterPer; ++e) {
+    const uint64_t i = i0 + uint64_t(e) * blockDim.x;
+    c[e] = 0;
+    local[e] = 0;
+    if (i < count) {
+      c[e] = key[i];
+      local[e] = atomicAdd(&cnt[c[e]], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < nkeys; t += blockDim.x)
+    if (cnt[t]) base[t] = atomicAdd(&cursor[t], cnt[t]);
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < kScatterPer; ++e) {
+    const uint64_t i = i0 + uint64_t(e) * blockDim.x;
+    if (i < count) perm[base[c[e]] + local[e]] = uint32_t(i);
+  }
+}
+
+// Inverted lists of the tile: for every set t and state q of it,
+// lists[cursor[q]++] = t (one global atomic per state per workgroup).
+template <int W>
+__global__ __launch_bounds__(256) void k_invert(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n,
+    uint64_t cmask, uint32_t* __restrict__ cursor, uint32_t* __restrict__ lists) {
+  __shared__ uint32_t cnt[W * 64], base[W * 64];
+  for (int t = threadIdx.x; t < W * 64; t += blockDim.x) cnt[t] = 0;
+  __syncthreads();
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint64_t x[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) x[w] = i < count ? ((sets[i * W + w] ^ cmask) & word_mask(n, w)) : 0ull;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    uint64_t b = x[w];
+    while (b) {
+      atomicAdd(&cnt[w * 64 + __builtin_ctzll(b)], 1u);
+      b &= b - 1;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < W * 64; t += blockDim.x)
+    if (uint32_t(t) < n && cnt[t]) {
+      base[t] = atomicAdd(&cursor[t], cnt[t]);
+      cnt[t] = 0;
+    }
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    uint64_t b = x[w];
+    while (b) {
+      const uint32_t q = uint32_t(w) * 64 + __builtin_ctzll(b);
+      b &= b - 1;
+      lists[base[q] + atomicAdd(&cnt[q], 1u)] = uint32_t(i);
+    }
+  }
+}
+
 // One workgroup: rank[q] = #{q' : (freq[q'], q') < (freq[q], q)}.
 __global__ __launch_bounds__(512) void k_state_rank(const uint32_t* __restrict__ freq, uint32_t n,
     uint32_t* __restrict__ rank, uint32_t* __restrict__ order) {
@@ -278,10 +378,24 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
   const int wave = rfl(threadIdx.x >> 6);
   const int nwaves = blockDim.x >> 6;
   // Lists sorted by size: the tiles of the largest sets, whose workgroups
-  // walk the most stream sets, are dispatched first.
-  const uint32_t tile_ix = (SW_IX_REV && p.t_perm) ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
-  const uint64_t tile_base = uint64_t(tile_ix) * kIxTile;
-  const uint64_t tile_left = p.t_count - tile_base;  // > 0 by grid construction
+  // walk the most stream sets, are dispatched first.  With a work list
+  // (anchor buckets), the workgroup's item names its tile (up to 4096 entries
+  // of t_perm from x) and its stream sets [z, w).
+  uint64_t tile_base, tile_left, s_begin, s_end;
+  if (p.work) {
+    const uint4 wi = p.work[blockIdx.x];
+    tile_base = wi.x;
+    tile_left = wi.y;
+    s_begin = wi.z;  // even
+    s_end = wi.w;
+  } else {
+    const uint32_t tile_ix = (SW_IX_REV && p.t_perm) ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
+    tile_base = uint64_t(tile_ix) * kIxTile;
+    tile_left = p.t_count - tile_base;  // > 0 by grid construction
+    s_begin = uint64_t(blockIdx.y) * p.s_chunk;  // even
+    s_end = s_begin + p.s_chunk;
+    if (s_end > p.s_count) s_end = p.s_count;
+  }
 
   // Lists sorted by size: the tile's largest set has tmax states, the
   // smallest c with t_bound[c] past the tile's last set; no stream set past
@@ -298,7 +412,7 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
       else lo = mid + 1;
     }
     p_bound = (uint64_t(p.s_bound[lo]) + 1) / 2;
-    if (uint64_t(blockIdx.y) * p.s_chunk / 2 >= p_bound) return;  // the whole workgroup, before any barrier
+    if (s_begin / 2 >= p_bound) return;  // the whole workgroup, before any barrier
   }
 
   // ---- transpose the tile (as k_vsub): mask g of row q = ballot over tile
@@ -339,9 +453,6 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
     }
   }
 
-  const uint64_t s_begin = uint64_t(blockIdx.y) * p.s_chunk;  // even
-  uint64_t s_end = s_begin + p.s_chunk;
-  if (s_end > p.s_count) s_end = p.s_count;
   uint64_t p_end = (s_end + 1) / 2;  // pairs [s_begin / 2, p_end)
   if (p_bound < p_end) p_end = p_bound;
   const uint32_t* __restrict__ start = p.ix_start;
@@ -604,6 +715,21 @@ template <int W>
 hipError_t launch_ix_w(const VsubParams& p0, hipStream_t st) {
   VsubParams p = p0;
   if (p.t_count == 0 || p.s_count == 0) return hipSuccess;
+  if (p.work) {
+    // Anchor buckets: one workgroup per work item.
+    if (p.work_count == 0) return hipSuccess;
+    if (p.work_count > 0x7fffffffull || !p.t_perm || !p.s_perm || p.s_bound || p.diag != INT64_MIN ||
+        p.mode == VSUB_COUNT || p.mode == VSUB_SCOUNT)
+      return hipErrorInvalidValue;
+    const size_t lds = size_t(p.n + 1) * 512 + (p.proper_by_size ? size_t(kIxTile) * 2 : 0);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    switch (p.mode) {
+      case VSUB_ANY: return launch_ix_mode<W, VSUB_ANY>(p, p.work_count, 1, lds, st);
+      case VSUB_FLAGS: return launch_ix_mode<W, VSUB_FLAGS>(p, p.work_count, 1, lds, st);
+      case VSUB_SFLAGS: return launch_ix_mode<W, VSUB_SFLAGS>(p, p.work_count, 1, lds, st);
+    }
+    return hipErrorInvalidValue;
+  }
   const uint64_t gx = (p.t_count + kIxTile - 1) / kIxTile;
   if (p.s_chunk == 0) {
     uint64_t target = p.target_wgs ? p.target_wgs : 1024;
@@ -725,6 +851,44 @@ hipError_t launch_size_scatter(int W, const uint64_t* sets, uint64_t count, uint
   const unsigned blocks = unsigned((count + 256 * kScatterPer - 1) / (256 * kScatterPer));
   switch (W) {
 #define C(w) case w: hipLaunchKernelGGL(k_size_scatter<w>, dim3(blocks), dim3(256), 0, st, sets, count, n, cmask, cursor, perm); break;
+    C(1) C(2) C(3) C(4) C(5)
+#undef C
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_anchor(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement,
+    const uint32_t* rank, uint32_t* anchor, uint32_t* hist, hipStream_t st) {
+  if (!count) return hipSuccess;
+  const uint64_t cmask = complement ? ~0ull : 0ull;
+  const unsigned blocks = unsigned((count + 255) / 256);
+  switch (W) {
+#define C(w) case w: hipLaunchKernelGGL(k_anchor<w>, dim3(blocks), dim3(256), 0, st, sets, count, n, cmask, rank, anchor, hist); break;
+    C(1) C(2) C(3) C(4) C(5)
+#undef C
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_key_scatter(const uint32_t* key, uint64_t count, uint32_t nkeys, uint32_t* cursor, uint32_t* perm,
+    hipStream_t st) {
+  if (!count) return hipSuccess;
+  if (nkeys > 513 || count > 0xffffffffull) return hipErrorInvalidValue;
+  const unsigned blocks = unsigned((count + 256 * kScatterPer - 1) / (256 * kScatterPer));
+  hipLaunchKernelGGL(k_key_scatter, dim3(blocks), dim3(256), 0, st, key, count, nkeys, cursor, perm);
+  return hipGetLastError();
+}
+
+hipError_t launch_invert(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement, uint32_t* cursor,
+    uint32_t* lists, hipStream_t st) {
+  if (!count) return hipSuccess;
+  if (count > 0xffffffffull) return hipErrorInvalidValue;
+  const uint64_t cmask = complement ? ~0ull : 0ull;
+  const unsigned blocks = unsigned((count + 255) / 256);
+  switch (W) {
+#define C(w) case w: hipLaunchKernelGGL(k_invert<w>, dim3(blocks), dim3(256), 0, st, sets, count, n, cmask, cursor, lists); break;
     C(1) C(2) C(3) C(4) C(5)
 #undef C
     default: return hipErrorInvalidValue;

@@ -2,6 +2,9 @@
 // indexed one (k_subset_ix.hip).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "sw_host.hpp"
 #include "sw_kernels.hpp"
 
@@ -22,14 +25,22 @@ void run_subset(sw_ctx* ctx, int W, VsubParams& p) {
     p.ix_early_exit = 0;
     p.order_states = 0;
     p.sort_by_size = 0;
+    p.bucket_anchor = 0;
   } else if (ctx->ix_walk == 2) {
     p.ix_early_exit = 1;
     p.order_states = 1;
     p.sort_by_size = 1;
+    p.bucket_anchor = 0;
   } else if (ctx->ix_walk == 3) {
     p.ix_early_exit = 0;
     p.order_states = 0;
     p.sort_by_size = 1;
+    p.bucket_anchor = 0;
+  } else if (ctx->ix_walk == 4) {
+    p.ix_early_exit = 1;
+    p.order_states = 1;
+    p.sort_by_size = 1;
+    p.bucket_anchor = 1;
   }
 #ifndef SW_SORT_COUNTS
 #define SW_SORT_COUNTS 1
@@ -48,7 +59,75 @@ void run_subset(sw_ctx* ctx, int W, VsubParams& p) {
   }
   const bool sorted = want_sort && p.diag == INT64_MIN;
   p.s_perm = p.t_perm = p.s_bound = p.t_bound = nullptr;
-  if (sorted) {
+  p.work = nullptr;
+  p.work_count = 0;
+  bool ranked = false;  // rank / order of the states computed already
+  uint32_t* f = nullptr;
+  if (p.bucket_anchor && (ctx->ix_walk == 0 || ctx->ix_walk == 4) && p.diag == INT64_MIN &&
+      (p.mode == VSUB_ANY || p.mode == VSUB_FLAGS || p.mode == VSUB_SFLAGS) &&
+      (ctx->ix_walk == 4 || tests >= (1ull << 34)) && s < (1ull << 32) && p.t_count < (1ull << 32)) {
+    // ---- anchor buckets
+    const uint32_t n = p.n, n1 = n + 1;
+    ctx->ix_order.ensure(3 * 512 * 4);
+    f = ctx->ix_order.as<uint32_t>();  // freq, rank, order
+    ctx->ix_sort.ensure(size_t(5) * n1 * 4);
+    uint32_t* ha = ctx->ix_sort.as<uint32_t>();  // stream sets per anchor
+    uint32_t* ca = ha + n1;                      // their cursor
+    uint32_t* ct = ca + n1;                      // inverted-list cursor
+    SW_CHECK_HIP(launch_fill_u32(f, 0u, 512, st));
+    SW_CHECK_HIP(launch_fill_u32(ha, 0u, n1, st));
+    SW_CHECK_HIP(launch_state_freq(W, p.tile, p.t_count, n, p.complement_tile, f, st));
+    SW_CHECK_HIP(launch_state_rank(f, n, f + 512, f + 1024, st));
+    ranked = true;
+    ctx->ix_anchor.ensure(s * 4 + 16);
+    SW_CHECK_HIP(launch_anchor(W, p.stream, s, n, p.complement_stream, f + 512, ctx->ix_anchor.as<uint32_t>(), ha, st));
+    ctx->ix_counts.resize(2 * size_t(n1));
+    SW_CHECK_HIP(hipMemcpyAsync(ctx->ix_counts.data(), ha, n1 * 4, hipMemcpyDeviceToHost, st));
+    SW_CHECK_HIP(hipMemcpyAsync(ctx->ix_counts.data() + n1, f, n * 4, hipMemcpyDeviceToHost, st));
+    SW_CHECK_HIP(hipStreamSynchronize(st));
+    const uint32_t* hcount = ctx->ix_counts.data();  // stream sets per anchor (n: empty sets)
+    const uint32_t* freq = hcount + n1;              // tile sets per state
+    uint64_t lists = 0;
+    for (uint32_t q = 0; q < n; ++q) lists += freq[q];
+    if (hcount[n] == 0 && lists < (1ull << 32)) {
+      SW_CHECK_HIP(launch_size_offsets(ha, n, ca, nullptr, st));
+      ctx->ix_sperm.ensure(s * 4 + 16);
+      SW_CHECK_HIP(launch_key_scatter(ctx->ix_anchor.as<uint32_t>(), s, n1, ca, ctx->ix_sperm.as<uint32_t>(), st));
+      SW_CHECK_HIP(launch_size_offsets(f, n - 1, ct, nullptr, st));  // list q starts at the count below q
+      ctx->ix_tlist.ensure(lists * 4 + 16);
+      SW_CHECK_HIP(launch_invert(W, p.tile, p.t_count, n, p.complement_tile, ct, ctx->ix_tlist.as<uint32_t>(), st));
+      // Work items: each anchor's stream sets (chunks of at most 2^17, pair
+      // aligned) against its inverted list (tiles of 4096).
+      std::vector<uint4> work;
+      uint64_t s0 = 0, l0 = 0;
+      const uint64_t chunk = uint64_t(1) << 17;
+      for (uint32_t q = 0; q < n; ++q) {
+        const uint64_t s1 = s0 + hcount[q], l1 = l0 + freq[q];
+        if (hcount[q] && freq[q]) {
+          for (uint64_t a = s0 & ~1ull; a < s1; a += chunk) {
+            uint64_t b = std::min<uint64_t>(a + chunk, s1);
+            b = std::min<uint64_t>((b + 1) & ~1ull, s);
+            for (uint64_t t = l0; t < l1; t += 4096)
+              work.push_back(make_uint4(uint32_t(t), uint32_t(std::min<uint64_t>(4096, l1 - t)), uint32_t(a),
+                  uint32_t(b)));
+          }
+        }
+        s0 = s1;
+        l0 = l1;
+      }
+      ctx->ix_work.ensure(work.size() * sizeof(uint4) + 16);
+      if (!work.empty())
+        SW_CHECK_HIP(hipMemcpyAsync(ctx->ix_work.p, work.data(), work.size() * sizeof(uint4), hipMemcpyHostToDevice,
+            st));
+      SW_CHECK_HIP(hipStreamSynchronize(st));  // `work` leaves scope
+      p.s_perm = ctx->ix_sperm.as<uint32_t>();
+      p.t_perm = ctx->ix_tlist.as<uint32_t>();
+      p.work = ctx->ix_work.as<uint4>();
+      p.work_count = work.size();
+      if (work.empty()) return;  // no stream set's anchor is in any tile set: no pair
+    }
+  }
+  if (!p.work && sorted) {
     const uint32_t n1 = p.n + 1;
     ctx->ix_sort.ensure(size_t(5) * n1 * 4);
     uint32_t* hs = ctx->ix_sort.as<uint32_t>();  // stream histogram, then tile cursor
@@ -104,12 +183,14 @@ void run_subset(sw_ctx* ctx, int W, VsubParams& p) {
   }
   const uint32_t* rank = nullptr;
   const uint32_t* order = nullptr;
-  if (p.order_states) {
-    ctx->ix_order.ensure(3 * 512 * 4);
-    uint32_t* f = ctx->ix_order.as<uint32_t>();
-    SW_CHECK_HIP(launch_fill_u32(f, 0u, 512, st));
-    SW_CHECK_HIP(launch_state_freq(W, p.tile, p.t_count, p.n, p.complement_tile, f, st));
-    SW_CHECK_HIP(launch_state_rank(f, p.n, f + 512, f + 1024, st));
+  if (p.order_states || ranked) {
+    if (!ranked) {
+      ctx->ix_order.ensure(3 * 512 * 4);
+      f = ctx->ix_order.as<uint32_t>();
+      SW_CHECK_HIP(launch_fill_u32(f, 0u, 512, st));
+      SW_CHECK_HIP(launch_state_freq(W, p.tile, p.t_count, p.n, p.complement_tile, f, st));
+      SW_CHECK_HIP(launch_state_rank(f, p.n, f + 512, f + 1024, st));
+    }
     rank = f + 512;
     order = f + 1024;
   }

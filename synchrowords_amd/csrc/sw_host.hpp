@@ -97,6 +97,8 @@ struct sw_ctx {
   sw::DevBuf ix_padded, ix_size, ix_start, ix_list, ix_tmp;
   sw::DevBuf ix_order;  // state frequencies in the tile, ranks and rank order (3 x 512 words)
   sw::DevBuf ix_sort, ix_sperm, ix_tperm, ix_head;
+  sw::DevBuf ix_anchor, ix_tlist, ix_work;  // anchor buckets: anchors, inverted lists, work items
+  std::vector<uint32_t> ix_counts;         // host copy of the bucket sizes
   // Power-set BFS batches: jobs, packed automata, frontiers, lengths.
   sw::DevBuf brute_jobs, brute_delta, brute_frontier, brute_out;
   // Eppstein batches: jobs, packed automata, pair-BFS scratch, words, results.
@@ -104,7 +106,7 @@ struct sw_ctx {
   // How the indexed kernel walks stream sets: as each launch asks (0), always
   // in set order without early exit (1), always rarest first with early exit
   // over size-sorted lists (2), always size-sorted lists walked in set order
-  // without early exit (3).
+  // without early exit (3), anchor buckets where the mode allows, else 2 (4).
   int ix_walk = 0;
   size_t ix_tmp_bytes = 0;
   uint32_t* ix_host = nullptr;  // pinned: the index length read back

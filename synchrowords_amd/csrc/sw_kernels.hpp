@@ -60,6 +60,15 @@ struct VsubParams {
   const uint32_t* t_perm = nullptr;
   const uint32_t* s_bound = nullptr;
   const uint32_t* t_bound = nullptr;  // t_bound[c] = tile sets of at most c states
+  // Indexed kernel, ANY/FLAGS/SFLAGS without diag or proper_by_size: test
+  // each stream set only against the tile sets holding its anchor (its state
+  // held by the fewest tile sets), which a subset of it must hold.
+  int bucket_anchor = 0;
+  // Set by run_subset when bucketed: one workgroup per item, x = first entry
+  // of t_perm (the anchor's inverted list), y = tile sets (<= 4096), [z, w) =
+  // stream sets (z even) in s_perm order.
+  const uint4* work = nullptr;
+  uint64_t work_count = 0;
 };
 
 // Indexed subset kernel (k_subset_ix.hip): whether its tile fits LDS for n
@@ -90,6 +99,15 @@ hipError_t launch_state_freq(int W, const uint64_t* sets, uint64_t count, uint32
 // 0 <= c <= n; hist must start at zero.  W <= 5.
 hipError_t launch_size_hist(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement, uint32_t* hist,
     hipStream_t st);
+// Anchor buckets: anchor[i] = the stream set's state of lowest rank (n if
+// empty), hist[anchor] += 1; perm[cursor[key[i]]++] = i for keys < nkeys;
+// lists[cursor[q]++] = t for every state q of every tile set t.
+hipError_t launch_anchor(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement,
+    const uint32_t* rank, uint32_t* anchor, uint32_t* hist, hipStream_t st);
+hipError_t launch_key_scatter(const uint32_t* key, uint64_t count, uint32_t nkeys, uint32_t* cursor, uint32_t* perm,
+    hipStream_t st);
+hipError_t launch_invert(int W, const uint64_t* sets, uint64_t count, uint32_t n, int complement, uint32_t* cursor,
+    uint32_t* lists, hipStream_t st);
 // rank[q] = position of state q in ascending (freq, q) order; order[r] = the
 // LDS row byte offset of the state at position r.
 hipError_t launch_state_rank(const uint32_t* freq, uint32_t n, uint32_t* rank, uint32_t* order, hipStream_t st);

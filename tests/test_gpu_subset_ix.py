@@ -97,7 +97,7 @@ def test_indexed_and_bitwalk_searches_agree(ixctx, n, k):
         bw.close()
 
 
-@pytest.mark.parametrize("walk", [1, 2, 3])
+@pytest.mark.parametrize("walk", [1, 2, 3, 4])
 @pytest.mark.parametrize("n", [7, 64, 130, 256, 300])
 def test_walk_orders_give_the_same_results(ixctx, walk, n):
     """Set-order walk and rarest-first walk with early exit, on lists where
@@ -140,14 +140,14 @@ def test_walk_orders_prune_the_same_sets(ixctx):
     n, k = 300, 2
     out = {}
     try:
-        for walk in (1, 2, 3):
+        for walk in (1, 2, 3, 4):
             ixctx.set_subset_walk(walk)
             d = gen.random_dfa(n, k, 8200)
             out[walk] = ixctx.exact(n, k, d)
     finally:
         ixctx.set_subset_walk(0)
     a = out[1]
-    for w in (2, 3):
+    for w in (2, 3, 4):
         b = out[w]
         assert a.found and b.found and a.length == b.length
         assert a.stats["sets_forward"] == b.stats["sets_forward"]
@@ -155,7 +155,7 @@ def test_walk_orders_prune_the_same_sets(ixctx):
         assert replay(n, k, d, b.word) == 1
 
 
-@pytest.mark.parametrize("walk", [1, 2, 3])
+@pytest.mark.parametrize("walk", [1, 2, 3, 4])
 @pytest.mark.parametrize("proper", [False, True])
 @pytest.mark.parametrize("n", [64, 300])
 def test_leaf_flags_with_either_walk(ixctx, walk, proper, n):
