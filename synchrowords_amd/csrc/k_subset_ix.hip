@@ -19,8 +19,19 @@
 // Index layout, per pair: G groups of 8 offsets, group k = 4 offsets of the
 // first set then 4 of the second, so lane L of half h loads its group as one
 // uint4 at entry 8k + 4h.  G is the larger set's state count rounded up to 4,
-// over 4; the shorter set is padded with the all-ones padding row.  A pair's
-// groups (up to kIxPrefetch) are loaded while the previous pair is walked.
+// over 4; the shorter set is padded with the all-ones padding row.
+//
+// Two walks.  The plain one (counts) walks every state of every pair, a
+// pair's groups (up to kIxPrefetch) loaded while the previous pair is walked.
+// The early-exit one (the exact search's launches) lists states rarest first,
+// ballots after every group and drops a pair once neither half has a tile
+// set left; its pairs' first kIxHead groups sit at a fixed stride (ix_head)
+// and are loaded kIxRing pairs ahead.  Around either walk the host may sort
+// both lists by size (a workgroup then stops at its tile's largest set) or
+// bucket them by anchor (a work list of (tile slice, stream range) items);
+// results are mapped back through the permutations.  Tuning constants below
+// can be set at build time (-D) for sweeps; the defaults are the measured
+// best.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
