@@ -100,7 +100,12 @@ class Search {
         scan_tmp_(ws_.scan_tmp), flags_(ws_.flags), probes_(ws_.probes), scan_bytes_(ws_.scan_bytes) {
     F_.reset();
     B_.reset();
+    // Default window [5e10, 1e11] pairs: smaller passes prune too little to
+    // repay themselves and larger ones cost more than they save on most
+    // automata (n = 300, k = 2 measured).  An explicit budget b means every
+    // pass of at most b pairs.
     budget_pairs_ = cfg.dominance_pairs ? cfg.dominance_pairs : 100000000000ull;
+    min_pairs_ = cfg.dominance_pairs ? 0 : 50000000000ull;
     forward_cap_ = cfg.forward_cap ? cfg.forward_cap : (1ull << 22);
     mem_cap_ = (cfg.max_memory_mb ? cfg.max_memory_mb : 65536ull) << 20;
   }
@@ -197,7 +202,7 @@ class Search {
   uint32_t n_, k_, W_, slices_;
   const uint32_t* delta_;
   sw_exact_config cfg_;
-  uint64_t budget_pairs_, forward_cap_, mem_cap_;
+  uint64_t budget_pairs_, min_pairs_, forward_cap_, mem_cap_;
 
   uint64_t pair_tests_ = 0;
   double ms_expand_ = 0, ms_dom_ = 0, ms_goal_ = 0;  // host time between synchronizations
@@ -385,7 +390,8 @@ class Search {
     S.prev_fcount = S.fcount;
     S.front.ensure(nnew * row_bytes() + 8);
     S.front_idx.ensure(nnew * 4 + 4);
-    const bool dominance = nnew > 0 && S.rows * nnew <= budget_pairs_;
+    const uint64_t pairs = S.rows * nnew;
+    const bool dominance = nnew > 0 && pairs <= budget_pairs_ && pairs >= min_pairs_;
     if (dominance) {
       // Host time up to here (the new-level size was read back, so the
       // expansion and dedup are done) counts as expansion; the commit still

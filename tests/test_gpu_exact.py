@@ -90,8 +90,11 @@ def test_dominance_pruning_keeps_the_length(ctx, n, k):
         d = gen.random_dfa(n, k, 7000 + seed)
         off = ctx.exact(n, k, d, cfg=sw.ExactConfig(dominance_pairs=1))
         on = ctx.exact(n, k, d)
-        assert on.found and off.found and on.length == off.length
-        assert replay(n, k, d, on.word) == 1 and replay(n, k, d, off.word) == 1
+        every = ctx.exact(n, k, d, cfg=sw.ExactConfig(dominance_pairs=10**11))  # small passes too
+        assert on.found and off.found and every.found
+        assert on.length == off.length == every.length
+        for r in (on, off, every):
+            assert replay(n, k, d, r.word) == 1
 
 
 @pytest.mark.parametrize("n,k", [(20, 2), (48, 2), (64, 3), (100, 2), (200, 2), (300, 2), (130, 4)])

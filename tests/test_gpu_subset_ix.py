@@ -86,10 +86,11 @@ def test_indexed_and_bitwalk_searches_agree(ixctx, n, k):
     bw = sw.Context(0)
     bw.set_subset_kernel(False)
     try:
+        cfg = sw.ExactConfig(dominance_pairs=10**11)  # every pass up to 1e11 pairs
         for seed in range(2):
             d = gen.random_dfa(n, k, 8100 + seed)
-            a = ixctx.exact(n, k, d)
-            b = bw.exact(n, k, d)
+            a = ixctx.exact(n, k, d, cfg=cfg)
+            b = bw.exact(n, k, d, cfg=cfg)
             assert a.found and b.found and a.length == b.length
             assert a.stats["sets_forward"] == b.stats["sets_forward"]
             assert a.stats["sets_backward"] == b.stats["sets_backward"]
@@ -143,7 +144,7 @@ def test_walk_orders_prune_the_same_sets(ixctx):
         for walk in (1, 2, 3, 4):
             ixctx.set_subset_walk(walk)
             d = gen.random_dfa(n, k, 8200)
-            out[walk] = ixctx.exact(n, k, d)
+            out[walk] = ixctx.exact(n, k, d, cfg=sw.ExactConfig(dominance_pairs=10**11))
     finally:
         ixctx.set_subset_walk(0)
     a = out[1]
