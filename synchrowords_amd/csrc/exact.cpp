@@ -301,7 +301,7 @@ class Search {
 
   void ensure_scan(uint64_t count) {
     size_t bytes = 0;
-    SW_CHECK_HIP(launch_scan(nullptr, nullptr, count, nullptr, nullptr, &bytes, st_));
+    SW_CHECK_HIP(keep_scan_temp_bytes(count, &bytes));
     if (bytes > scan_bytes_) {
       scan_tmp_.release();
       scan_tmp_.ensure(bytes);

@@ -202,52 +202,137 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
     dst[i] = src[i];
 }
 
-// Keep flags and their exclusive scan in one workgroup, for short lists
-// (the many small steps of a search are bound by launches, not by work):
+// Keep flags and their exclusive scan (launch_keep_scan):
 //   mode 0: keep[i] given;
 //   mode 1: keep[i] = candidate i owns its hash slot (as k_hash_resolve);
 //   mode 2: keep[i] = bit i of `dead` is clear (as k_alive).
 // pos[i] = sum of keep[j], j < i; *total = the sum of all.
-constexpr uint32_t kSmallScanThreads = 1024;
-constexpr uint64_t kSmallScanMax = 1ull << 17;
+//
+// The list is cut into tiles of kScanTile entries, kScanPer per thread at a
+// stride of kScanThreads so that every load and store is coalesced and a
+// thread's kScanPer loads are independent.  A list of one tile takes one
+// launch (k_keep_tile_scan with no tile sums); longer ones take two:
+// k_keep_tile_sums writes the flags and each tile's total, then each tile of
+// k_keep_tile_scan adds up the totals of the tiles before it and scans its own
+// entries with one ballot per row of kScanThreads.
+constexpr uint32_t kScanThreads = 256;
+constexpr uint32_t kScanPer = 8;
+constexpr uint32_t kScanTile = kScanThreads * kScanPer;
+constexpr uint32_t kScanWaves = kScanThreads / 64;
+constexpr uint64_t kScanTwoPassMax = uint64_t(kScanTile) * 1024;  // at most 1024 tiles
 
-__global__ __launch_bounds__(kSmallScanThreads) void k_keep_scan_small(int mode, const uint32_t* __restrict__ slot_of,
-    const uint32_t* __restrict__ table, const uint8_t* __restrict__ dead, uint64_t count, uint32_t* __restrict__ keep,
-    uint32_t* __restrict__ pos, uint32_t* __restrict__ total) {
-  __shared__ uint32_t part[kSmallScanThreads];
+__device__ __forceinline__ void keep_flags(int mode, const uint32_t* __restrict__ slot_of,
+    const uint32_t* __restrict__ table, const uint8_t* __restrict__ dead, uint64_t base, uint64_t count,
+    uint32_t* __restrict__ keep, uint32_t (&k)[kScanPer]) {
   const uint32_t t = threadIdx.x;
-  const uint64_t per = (count + kSmallScanThreads - 1) / kSmallScanThreads;
-  const uint64_t b = uint64_t(t) * per;
-  const uint64_t e = b + per < count ? b + per : count;
-  uint32_t sum = 0;
-  for (uint64_t i = b; i < e; ++i) {
-    uint32_t k;
-    if (mode == 1) {
-      const uint32_t s = slot_of[i];
-      k = (s != kEmpty && table[s] == (kCand | uint32_t(i))) ? 1u : 0u;
-      keep[i] = k;
-    } else if (mode == 2) {
-      k = ((dead[i >> 3] >> (i & 7)) & 1) ? 0u : 1u;
-      keep[i] = k;
-    } else {
-      k = keep[i];
+  if (mode == 1) {
+    uint32_t s[kScanPer];
+#pragma unroll
+    for (uint32_t r = 0; r < kScanPer; ++r) {
+      const uint64_t i = base + r * kScanThreads + t;
+      s[r] = i < count ? slot_of[i] : kEmpty;
     }
-    sum += k;
+#pragma unroll
+    for (uint32_t r = 0; r < kScanPer; ++r) {
+      const uint64_t i = base + r * kScanThreads + t;
+      k[r] = (s[r] != kEmpty && table[s[r]] == (kCand | uint32_t(i))) ? 1u : 0u;
+    }
+  } else if (mode == 2) {
+#pragma unroll
+    for (uint32_t r = 0; r < kScanPer; ++r) {
+      const uint64_t i = base + r * kScanThreads + t;
+      k[r] = (i < count && !((dead[i >> 3] >> (i & 7)) & 1)) ? 1u : 0u;
+    }
+  } else {
+#pragma unroll
+    for (uint32_t r = 0; r < kScanPer; ++r) {
+      const uint64_t i = base + r * kScanThreads + t;
+      k[r] = i < count ? keep[i] : 0u;
+    }
+    return;
   }
-  part[t] = sum;
+#pragma unroll
+  for (uint32_t r = 0; r < kScanPer; ++r) {
+    const uint64_t i = base + r * kScanThreads + t;
+    if (i < count) keep[i] = k[r];
+  }
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_keep_tile_sums(int mode, const uint32_t* __restrict__ slot_of,
+    const uint32_t* __restrict__ table, const uint8_t* __restrict__ dead, uint64_t count,
+    uint32_t* __restrict__ keep, uint32_t* __restrict__ tile_sum) {
+  __shared__ uint32_t wsum[kScanWaves];
+  uint32_t k[kScanPer];
+  keep_flags(mode, slot_of, table, dead, uint64_t(blockIdx.x) * kScanTile, count, keep, k);
+  uint32_t s = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < kScanPer; ++r) s += k[r];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
   __syncthreads();
-  for (uint32_t off = 1; off < kSmallScanThreads; off <<= 1) {  // inclusive Hillis-Steele scan
-    const uint32_t x = t >= off ? part[t - off] : 0u;
-    __syncthreads();
-    part[t] += x;
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kScanWaves; ++w) t += wsum[w];
+    tile_sum[blockIdx.x] = t;
   }
-  uint32_t run = part[t] - sum;
-  for (uint64_t i = b; i < e; ++i) {
-    pos[i] = run;
-    run += keep[i];
+}
+
+// tile_sum == NULL: one tile (gridDim.x == 1), flags computed here.
+__global__ __launch_bounds__(kScanThreads) void k_keep_tile_scan(int mode, const uint32_t* __restrict__ slot_of,
+    const uint32_t* __restrict__ table, const uint8_t* __restrict__ dead, uint64_t count,
+    uint32_t* __restrict__ keep, const uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ pos,
+    uint32_t* __restrict__ total) {
+  __shared__ uint32_t wcnt[kScanPer][kScanWaves];
+  __shared__ uint32_t wcarry[kScanWaves];
+  const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const uint64_t base = uint64_t(blockIdx.x) * kScanTile;
+  uint32_t k[kScanPer];
+  uint32_t c = 0;
+  if (tile_sum) {
+#pragma unroll
+    for (uint32_t r = 0; r < kScanPer; ++r) {
+      const uint64_t i = base + r * kScanThreads + t;
+      k[r] = i < count ? keep[i] : 0u;
+    }
+    for (uint32_t j = t; j < blockIdx.x; j += kScanThreads) c += tile_sum[j];
+    c = wave_sum(c);
+  } else {
+    keep_flags(mode, slot_of, table, dead, base, count, keep, k);
   }
-  if (t == kSmallScanThreads - 1) *total = part[t];
+  uint64_t m[kScanPer];
+#pragma unroll
+  for (uint32_t r = 0; r < kScanPer; ++r) m[r] = __ballot(k[r]);
+  if (lane == 0) {
+    wcarry[wv] = c;
+#pragma unroll
+    for (uint32_t r = 0; r < kScanPer; ++r) wcnt[r][wv] = uint32_t(__popcll(m[r]));
+  }
+  __syncthreads();
+  uint32_t run = 0;  // entries before the current row: earlier tiles, then earlier rows
+#pragma unroll
+  for (uint32_t w = 0; w < kScanWaves; ++w) run += wcarry[w];
+#pragma unroll
+  for (uint32_t r = 0; r < kScanPer; ++r) {
+    uint32_t before = 0, row = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kScanWaves; ++w) {
+      const uint32_t x = wcnt[r][w];
+      before += w < wv ? x : 0u;
+      row += x;
+    }
+    const uint64_t i = base + r * kScanThreads + t;
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(m[r] >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m[r]), 0u));
+    if (i < count) pos[i] = run + before + below;
+    run += row;
+  }
+  if (t == 0 && blockIdx.x == gridDim.x - 1) *total = run;
 }
 
 // Resets the goal flag and witness of a meet check.
@@ -303,11 +388,28 @@ hipError_t launch_hash_resolve(const uint32_t* slot_of, const uint32_t* table, u
   return hipGetLastError();
 }
 
+hipError_t keep_scan_temp_bytes(uint64_t count, size_t* temp_bytes) {
+  if (count > kScanTwoPassMax) return launch_scan(nullptr, nullptr, count, nullptr, nullptr, temp_bytes, nullptr);
+  *temp_bytes = size_t((count + kScanTile - 1) / kScanTile) * 4 + 4;
+  return hipSuccess;
+}
+
 hipError_t launch_keep_scan(int mode, const uint32_t* slot_of, const uint32_t* table, const void* dead_bits,
     uint64_t count, uint32_t* keep, uint32_t* pos, uint32_t* total, void* temp, size_t* temp_bytes, hipStream_t st) {
-  if (count <= kSmallScanMax) {
-    hipLaunchKernelGGL(k_keep_scan_small, dim3(1), dim3(kSmallScanThreads), 0, st, mode, slot_of, table,
-        static_cast<const uint8_t*>(dead_bits), count, keep, pos, total);
+  const uint8_t* dead = static_cast<const uint8_t*>(dead_bits);
+  if (count <= kScanTile) {
+    hipLaunchKernelGGL(k_keep_tile_scan, dim3(1), dim3(kScanThreads), 0, st, mode, slot_of, table, dead, count, keep,
+        nullptr, pos, total);
+    return hipGetLastError();
+  }
+  if (count <= kScanTwoPassMax) {
+    const uint32_t tiles = uint32_t((count + kScanTile - 1) / kScanTile);
+    if (!temp || !temp_bytes || *temp_bytes < size_t(tiles) * 4) return hipErrorInvalidValue;
+    uint32_t* sums = static_cast<uint32_t*>(temp);
+    hipLaunchKernelGGL(k_keep_tile_sums, dim3(tiles), dim3(kScanThreads), 0, st, mode, slot_of, table, dead, count,
+        keep, sums);
+    hipLaunchKernelGGL(k_keep_tile_scan, dim3(tiles), dim3(kScanThreads), 0, st, mode, slot_of, table, dead, count,
+        keep, sums, pos, total);
     return hipGetLastError();
   }
   hipError_t e = hipSuccess;

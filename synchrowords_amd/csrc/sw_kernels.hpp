@@ -217,8 +217,9 @@ hipError_t launch_hash_rebuild(int W, const uint64_t* arena, uint64_t rows, uint
 hipError_t launch_alive(int W, const void* dead_bits, uint64_t count, uint32_t* keep, hipStream_t st);
 // Keep flags (mode 0 given, 1 = hash-slot owners as launch_hash_resolve,
 // 2 = clear bits of dead_bits as launch_alive) and their exclusive scan with
-// the total, in one launch for short lists and with the scan above
-// otherwise (temp sized as for launch_scan).
+// the total: one launch up to 2048 entries, two up to 2^21, the scan above
+// beyond.  temp holds keep_scan_temp_bytes(count) bytes.
+hipError_t keep_scan_temp_bytes(uint64_t count, size_t* temp_bytes);
 hipError_t launch_keep_scan(int mode, const uint32_t* slot_of, const uint32_t* table, const void* dead_bits,
     uint64_t count, uint32_t* keep, uint32_t* pos, uint32_t* total, void* temp, size_t* temp_bytes, hipStream_t st);
 // *found = 0, *witness = ~0 (before a meet check).
