@@ -11,8 +11,14 @@ namespace sw {
 // One thread per child.  The child is the OR over slices s of table row
 // [letter][s][byte s of the parent] (the reference's
 // PreprocessedTransition::apply, preprocessed_transition.hpp:31-49); a zero
-// byte selects the empty row and is skipped.
-template <int W>
+// byte selects the empty row.  Large launches skip it (a branch per byte);
+// small ones (ALL) read every row, zero bytes included, with no branch, so
+// that a thread's loads are all in flight together: a launch of a few
+// thousand children is bound by the latency of its row reads, not by their
+// number.
+constexpr uint64_t kExpandAllMax = 1ull << 17;  // children
+
+template <int W, bool ALL>
 __global__ __launch_bounds__(256) void k_expand(const uint64_t* __restrict__ parents, uint64_t m,
     const uint64_t* __restrict__ tables, uint32_t slices, uint32_t letters, const uint32_t* __restrict__ letter_list,
     uint32_t n, int drop_mode, uint64_t* __restrict__ out, uint32_t* __restrict__ valid) {
@@ -27,16 +33,34 @@ __global__ __launch_bounds__(256) void k_expand(const uint64_t* __restrict__ par
     r[w] = 0;
   }
   const uint64_t* tab = tables + uint64_t(a) * slices * 256 * W;
+  if (ALL) {
+    // One parent word at a time: its eight rows' loads in flight together.
+    // Slices past the last read the empty row of slice 0.
+#pragma unroll 1
+    for (int w = 0; w < W; ++w) {
+      const uint64_t xw = parents[j * W + w];
 #pragma unroll
-  for (int w = 0; w < W; ++w) {
-    for (int b = 0; b < 8; ++b) {
-      const uint32_t s = uint32_t(w) * 8 + b;
-      if (s >= slices) break;
-      const uint32_t byte = uint32_t(x[w] >> (b * 8)) & 255u;
-      if (!byte) continue;
-      const uint64_t* row = tab + (uint64_t(s) * 256 + byte) * W;
+      for (int b = 0; b < 8; ++b) {
+        const uint32_t s = uint32_t(w) * 8 + b;
+        const bool in = s < slices;
+        const uint32_t byte = in ? (uint32_t(xw >> (b * 8)) & 255u) : 0u;
+        const uint64_t* row = tab + (uint64_t(in ? s : 0u) * 256 + byte) * W;
 #pragma unroll
-      for (int v = 0; v < W; ++v) r[v] |= row[v];
+        for (int v = 0; v < W; ++v) r[v] |= row[v];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      for (int b = 0; b < 8; ++b) {
+        const uint32_t s = uint32_t(w) * 8 + b;
+        if (s >= slices) break;
+        const uint32_t byte = uint32_t(x[w] >> (b * 8)) & 255u;
+        if (!byte) continue;
+        const uint64_t* row = tab + (uint64_t(s) * 256 + byte) * W;
+#pragma unroll
+        for (int v = 0; v < W; ++v) r[v] |= row[v];
+      }
     }
   }
 #pragma unroll
@@ -55,8 +79,12 @@ static hipError_t expand_w(const uint64_t* parents, uint64_t m, const uint64_t* 
   const uint64_t total = m * letters;
   if (!total) return hipSuccess;
   const uint64_t blocks = (total + 255) / 256;
-  hipLaunchKernelGGL(k_expand<W>, dim3(unsigned(blocks)), dim3(256), 0, st, parents, m, tables, slices, letters,
-      letter_list, n, drop_mode, out, valid);
+  if (total <= kExpandAllMax)
+    hipLaunchKernelGGL((k_expand<W, true>), dim3(unsigned(blocks)), dim3(256), 0, st, parents, m, tables, slices,
+        letters, letter_list, n, drop_mode, out, valid);
+  else
+    hipLaunchKernelGGL((k_expand<W, false>), dim3(unsigned(blocks)), dim3(256), 0, st, parents, m, tables, slices,
+        letters, letter_list, n, drop_mode, out, valid);
   return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
+#include "sw_kernels.hpp"
 #include "sw_mlp.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -426,8 +427,77 @@ hipError_t launch_iota(uint32_t* p, uint64_t count, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Orders of up to kBeamRankMax items are counted rather than sorted (below).
+constexpr uint32_t kRankChunk = 256;
+constexpr uint64_t kBeamRankMax = 16384;
+
+// Stable ascending order of float scores, by counting (as k_beam_rank):
+// rank[i] = #{j : key j below key i, or equal with j < i}.  Keys compare as
+// the radix sort orders them: the float bits mapped to an unsigned order.
+__device__ __forceinline__ uint32_t float_order(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return u ^ ((u >> 31) ? 0xffffffffu : 0x80000000u);
+}
+
+// Grid of (64 items i) x (kRankChunk items j): the j chunk is staged in LDS,
+// each of the four waves counts a quarter of it for the same 64 items, and
+// one atomic add per item and workgroup adds the four partial counts.
+__global__ __launch_bounds__(256) void k_score_rank(const float* __restrict__ keys, uint32_t count,
+    uint32_t* __restrict__ rank) {
+  __shared__ uint32_t jk[kRankChunk];
+  __shared__ uint32_t part[4][64];
+  const uint32_t j0 = blockIdx.y * kRankChunk;
+  const uint32_t jn = count - j0 < kRankChunk ? count - j0 : kRankChunk;
+  if (threadIdx.x < jn) jk[threadIdx.x] = float_order(keys[j0 + threadIdx.x]);
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t i = blockIdx.x * 64 + lane;
+  uint32_t before = 0;
+  if (i < count) {
+    const uint32_t ki = float_order(keys[i]);
+    const uint32_t tb = wv * (kRankChunk / 4), te = min(jn, tb + kRankChunk / 4);
+    for (uint32_t t = tb; t < te; ++t) {
+      const uint32_t kj = jk[t];
+      before += (kj < ki || (kj == ki && j0 + t < i)) ? 1u : 0u;
+    }
+  }
+  part[wv][lane] = before;
+  __syncthreads();
+  if (wv == 0 && i < count) {
+    const uint32_t tot = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+    if (tot) atomicAdd(&rank[i], tot);
+  }
+}
+
+__global__ void k_score_scatter(const uint32_t* __restrict__ rank, const float* __restrict__ keys_in,
+    const uint32_t* __restrict__ vals_in, uint32_t count, float* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t r = rank[i];
+  keys_out[r] = keys_in[i];
+  vals_out[r] = vals_in[i];
+}
+
 hipError_t sort_scores(const float* keys_in, float* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
     uint64_t count, void* temp, size_t* temp_bytes, hipStream_t st) {
+  if (!temp) {
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, *temp_bytes, keys_in, keys_out, vals_in, vals_out,
+        int(count), 0, 32, st);
+    *temp_bytes = std::max(*temp_bytes, size_t(count) * 4 + 4);  // the rank array of the counted order
+    return e;
+  }
+  if (count && count <= kBeamRankMax) {
+    if (*temp_bytes < size_t(count) * 4) return hipErrorInvalidValue;
+    uint32_t* rank = static_cast<uint32_t*>(temp);
+    hipError_t e = launch_fill_u32(rank, 0u, count, st);
+    if (e != hipSuccess) return e;
+    const dim3 grid(unsigned((count + 63) / 64), unsigned((count + kRankChunk - 1) / kRankChunk));
+    hipLaunchKernelGGL(k_score_rank, grid, dim3(256), 0, st, keys_in, uint32_t(count), rank);
+    hipLaunchKernelGGL(k_score_scatter, dim3(blocks(count)), dim3(256), 0, st, rank, keys_in, vals_in,
+        uint32_t(count), keys_out, vals_out);
+    return hipGetLastError();
+  }
   return hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, keys_in, keys_out, vals_in, vals_out, int(count), 0,
       32, st);
 }
@@ -457,6 +527,79 @@ __global__ void k_neg_size(const uint64_t* __restrict__ sets, uint64_t count, ui
   neg[i] = n - c;
 }
 
+// Up to kBeamRankMax sets, the place of each set in that order is counted
+// directly: rank[i] = #{j : set j comes before set i}, over a grid of (64
+// sets i) x (kRankChunk sets j, staged in LDS with their keys), as
+// k_score_rank; then perm[rank[i]] = i.  Two launches and a
+// clear, where the merge sort takes five or more; past kBeamRankMax the
+// quadratic count is left to the sort.
+
+// Sort key of a set: (n - |S|) in the top 9 bits, then the top 55 bits of
+// word 0; sets with equal keys are ordered by all their words.
+template <int W>
+__device__ __forceinline__ uint64_t beam_key(const uint64_t (&x)[W], uint32_t n) {
+  uint32_t c = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) c += uint32_t(__popcll(x[w]));
+  return (uint64_t(n - c) << 55) | (x[0] >> 9);
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_beam_rank(const uint64_t* __restrict__ sets, uint32_t count, uint32_t n,
+    uint32_t* __restrict__ rank) {
+  __shared__ uint64_t js[kRankChunk * W];
+  __shared__ uint64_t jkey[kRankChunk];
+  __shared__ uint32_t part[4][64];
+  const uint32_t j0 = blockIdx.y * kRankChunk;
+  const uint32_t jn = count - j0 < kRankChunk ? count - j0 : kRankChunk;
+  if (threadIdx.x < jn) {
+    uint64_t x[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      x[w] = sets[uint64_t(j0 + threadIdx.x) * W + w];
+      js[threadIdx.x * W + w] = x[w];
+    }
+    jkey[threadIdx.x] = beam_key<W>(x, n);
+  }
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t i = blockIdx.x * 64 + lane;
+  uint32_t before = 0;
+  if (i < count) {
+    uint64_t xi[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) xi[w] = sets[uint64_t(i) * W + w];
+    const uint64_t ki = beam_key<W>(xi, n);
+    const uint32_t tb = wv * (kRankChunk / 4), te = min(jn, tb + kRankChunk / 4);
+    for (uint32_t t = tb; t < te; ++t) {
+      const uint64_t kj = jkey[t];
+      if (kj != ki) {
+        before += kj < ki ? 1u : 0u;  // larger first, then the high bits of word 0
+        continue;
+      }
+#pragma unroll
+      for (int w = 0; w < W; ++w) {  // then word 0, word 1, ... ascending
+        const uint64_t y = js[t * W + w];
+        if (y != xi[w]) {
+          before += y < xi[w] ? 1u : 0u;
+          break;
+        }
+      }
+    }
+  }
+  part[wv][lane] = before;
+  __syncthreads();
+  if (wv == 0 && i < count) {
+    const uint32_t tot = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+    if (tot) atomicAdd(&rank[i], tot);
+  }
+}
+
+__global__ void k_rank_scatter(const uint32_t* __restrict__ rank, uint32_t count, uint32_t* __restrict__ perm) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) perm[rank[i]] = i;
+}
+
 hipError_t beam_reference_order(const uint64_t* sets, uint64_t count, uint32_t W, uint32_t n, uint32_t* perm_a,
     uint32_t* perm_b, uint64_t* key_a, uint64_t* key_b, void* temp, size_t* temp_bytes, uint32_t** result,
     hipStream_t st) {
@@ -471,6 +614,20 @@ hipError_t beam_reference_order(const uint64_t* sets, uint64_t count, uint32_t W
   }
   *result = perm_a;
   if (!count) return hipSuccess;
+  if (count <= kBeamRankMax && W >= 1 && W <= 8) {
+    uint32_t* rank = reinterpret_cast<uint32_t*>(key_a);
+    hipError_t e = launch_fill_u32(rank, 0u, count, st);
+    if (e != hipSuccess) return e;
+    const dim3 grid(unsigned((count + 63) / 64), unsigned((count + kRankChunk - 1) / kRankChunk));
+    switch (W) {
+#define CASE(w) \
+  case w: hipLaunchKernelGGL(k_beam_rank<w>, grid, dim3(256), 0, st, sets, uint32_t(count), n, rank); break;
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    }
+    hipLaunchKernelGGL(k_rank_scatter, dim3(blocks(count)), dim3(256), 0, st, rank, uint32_t(count), perm_a);
+    return hipGetLastError();
+  }
   // One comparison sort of the indices; the sets are distinct, so the order
   // is total and any correct sort gives the reference's.
   hipLaunchKernelGGL(k_iota_u32, dim3(blocks(count)), dim3(256), 0, st, perm_a, count);

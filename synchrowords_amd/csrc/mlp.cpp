@@ -625,16 +625,16 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
       SW_CHECK_HIP(sw::launch_fill_u32(table.as<uint32_t>(), sw::kEmpty, tcap, st));
       SW_CHECK_HIP(sw::launch_hash_insert(int(W), cand.as<uint64_t>(), nullptr, cnt, table.as<uint32_t>(), tcap - 1,
           cand.as<uint64_t>(), slot.as<uint32_t>(), nullptr, st));
-      SW_CHECK_HIP(sw::launch_hash_resolve(slot.as<uint32_t>(), table.as<uint32_t>(), cnt, keep.as<uint32_t>(), st));
       size_t need = 0;
-      SW_CHECK_HIP(sw::launch_scan(nullptr, nullptr, cnt, nullptr, nullptr, &need, st));
+      SW_CHECK_HIP(sw::keep_scan_temp_bytes(cnt, &need));
       if (need > scan_bytes) {
         scan_tmp.release();
         scan_tmp.ensure(need);
         scan_bytes = need;
       }
       uint32_t* total = small.as<uint32_t>() + 4;
-      SW_CHECK_HIP(sw::launch_scan(keep.as<uint32_t>(), pos.as<uint32_t>(), cnt, total, scan_tmp.p, &scan_bytes, st));
+      SW_CHECK_HIP(sw::launch_keep_scan(1, slot.as<uint32_t>(), table.as<uint32_t>(), nullptr, cnt,
+          keep.as<uint32_t>(), pos.as<uint32_t>(), total, scan_tmp.p, &scan_bytes, st));
       uniq.ensure(cnt * W * 8 + 8);
       uidx.ensure(cnt * 4 + 4);
       SW_CHECK_HIP(sw::launch_select(int(W), cand.as<uint64_t>(), 0, keep.as<uint32_t>(), pos.as<uint32_t>(), cnt,
