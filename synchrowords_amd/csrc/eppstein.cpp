@@ -2,7 +2,8 @@
 // reference's Eppstein<N,K>::run, synchrolib/algorithm/eppstein/eppstein.hpp:
 // 25-114): non-synchronizing detection by the pair BFS, and, when every pair
 // merges, the greedy word and its length.  One workgroup per automaton
-// (k_eppstein.hip); batches are cut so that their scratch stays under 2 GiB.
+// (k_eppstein.hip), or, for one automaton alone, the pair BFS over the whole
+// device; batches are cut so that their scratch stays under 2 GiB.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,6 +14,88 @@
 
 namespace sw {
 
+#ifndef SW_EPP_WIDE_MIN_N
+#define SW_EPP_WIDE_MIN_N 64
+#endif
+
+namespace {
+
+// One automaton over the whole device: the preimage lists built here (sources
+// ascending, as the workgroup kernel builds them), the pair BFS a level at a
+// time (three launches and a scan, the level's size read back), then the
+// greedy phase in one workgroup.  A lone automaton in one workgroup leaves
+// all but one CU idle; its BFS levels are thousands of parents wide.
+void eppstein_wide(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint64_t upper, uint64_t word_cap,
+    EppOut& res, uint32_t* words) {
+  hipStream_t st = ctx->stream;
+  const EppLayout lay = epp_layout(n, k);
+  const uint32_t kn = k * n, all = n * (n - 1) / 2;
+  std::vector<uint32_t> beg(kn + 1, 0);
+  std::vector<uint16_t> edges(kn);
+  for (uint32_t q = 0; q < n; ++q)
+    for (uint32_t a = 0; a < k; ++a) beg[a * n + delta[size_t(q) * k + a] + 1]++;
+  for (uint32_t i = 0; i < kn; ++i) beg[i + 1] += beg[i];
+  {
+    std::vector<uint32_t> fill(beg.begin(), beg.end() - 1);
+    for (uint32_t q = 0; q < n; ++q)
+      for (uint32_t a = 0; a < k; ++a) edges[fill[a * n + delta[size_t(q) * k + a]]++] = uint16_t(q);
+  }
+  ctx->epp_scratch.ensure(lay.total);
+  unsigned char* base = ctx->epp_scratch.as<unsigned char>();
+  SW_CHECK_HIP(hipMemcpyAsync(base, beg.data(), beg.size() * 4, hipMemcpyHostToDevice, st));
+  SW_CHECK_HIP(hipMemcpyAsync(base + lay.edges, edges.data(), edges.size() * 2, hipMemcpyHostToDevice, st));
+  SW_CHECK_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(base + lay.info), 0u, size_t(n) * n, st));
+  SW_CHECK_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(base + lay.keys), ~0u, size_t(2) * n * n, st));
+  const size_t most = size_t(all) + kn;
+  ctx->epp_off.ensure(most * 4 + 4);
+  ctx->epp_small.ensure(16);
+  size_t need = 0;
+  SW_CHECK_HIP(launch_scan(nullptr, nullptr, most, nullptr, nullptr, &need, st));
+  if (need > ctx->epp_tmp_bytes) {
+    ctx->epp_tmp.release();
+    ctx->epp_tmp.ensure(need);
+    ctx->epp_tmp_bytes = need;
+  }
+  const uint32_t* cnt = reinterpret_cast<const uint32_t*>(base + lay.cnt);
+  uint32_t* off = ctx->epp_off.as<uint32_t>();
+  uint32_t* total = ctx->epp_small.as<uint32_t>();
+  uint32_t head = 0, tail = 0;
+  uint64_t found = 0;
+  for (uint32_t level = 0;; ++level) {
+    const uint32_t parents = level == 0 ? kn : tail - head;
+    SW_CHECK_HIP(launch_epp_pass(0, level, parents, n, k, base, head, nullptr, 0, st));
+    SW_CHECK_HIP(launch_epp_pass(1, level, parents, n, k, base, head, nullptr, 0, st));
+    SW_CHECK_HIP(launch_scan(cnt, off, parents, total, ctx->epp_tmp.p, &ctx->epp_tmp_bytes, st));
+    SW_CHECK_HIP(launch_epp_pass(2, level, parents, n, k, base, head, off, tail, st));
+    uint32_t carry = 0;
+    SW_CHECK_HIP(hipMemcpyAsync(&carry, total, 4, hipMemcpyDeviceToHost, st));
+    SW_CHECK_HIP(hipStreamSynchronize(st));
+    found += carry;
+    head = tail;
+    tail += carry;
+    if (carry == 0 || found >= all) break;
+  }
+  if (found < all) {
+    res = EppOut{1, 0, upper, 0};
+    return;
+  }
+  const EppJob job{n, k, 0, 0, 0, word_cap, upper};
+  ctx->epp_jobs.ensure(sizeof(EppJob));
+  ctx->epp_delta.ensure(size_t(kn) * 4 + 4);
+  ctx->epp_words.ensure(word_cap * 4 + 4);
+  ctx->epp_out.ensure(sizeof(EppOut));
+  SW_CHECK_HIP(hipMemcpyAsync(ctx->epp_jobs.p, &job, sizeof(EppJob), hipMemcpyHostToDevice, st));
+  SW_CHECK_HIP(hipMemcpyAsync(ctx->epp_delta.p, delta, size_t(kn) * 4, hipMemcpyHostToDevice, st));
+  SW_CHECK_HIP(launch_epp_greedy(ctx->epp_jobs.as<EppJob>(), ctx->epp_delta.as<uint32_t>(), base,
+      ctx->epp_words.as<uint32_t>(), ctx->epp_out.as<EppOut>(), st));
+  SW_CHECK_HIP(hipMemcpyAsync(&res, ctx->epp_out.p, sizeof(EppOut), hipMemcpyDeviceToHost, st));
+  if (words && word_cap)
+    SW_CHECK_HIP(hipMemcpyAsync(words, ctx->epp_words.p, word_cap * 4, hipMemcpyDeviceToHost, st));
+  SW_CHECK_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
 void eppstein_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint32_t* ks, const uint64_t* delta_offsets,
     const uint32_t* deltas, const uint64_t* uppers, uint64_t word_cap, int32_t* non_synchro, int32_t* improved,
     uint64_t* bounds, uint64_t* word_lens, uint32_t* words) {
@@ -20,6 +103,15 @@ void eppstein_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint3
   for (uint32_t i = 0; i < count; ++i) {
     SW_REQUIRE(ns[i] >= 1 && ns[i] <= kEppMaxN, "eppstein: n must be in [1, 512]");
     SW_REQUIRE(ks[i] >= 1 && ks[i] <= 255, "eppstein: k must be in [1, 255]");
+  }
+  if (count == 1 && ns[0] >= SW_EPP_WIDE_MIN_N) {
+    EppOut r{};
+    eppstein_wide(ctx, ns[0], ks[0], deltas + delta_offsets[0], uppers[0], word_cap, r, words);
+    if (non_synchro) non_synchro[0] = r.non_synchro;
+    if (improved) improved[0] = r.improved;
+    if (bounds) bounds[0] = r.bound;
+    if (word_lens) word_lens[0] = r.word_len;
+    return;
   }
   const size_t budget = size_t(2) << 30;
   uint32_t at = 0;

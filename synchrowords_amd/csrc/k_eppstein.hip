@@ -88,6 +88,71 @@ __device__ inline void candidates(uint32_t level, uint32_t p, uint32_t n, uint32
   }
 }
 
+// Greedy merging of the closest pair, over the pair lengths in info (one
+// workgroup of kEppThreads).
+__device__ void epp_greedy(const EppJob& job, uint32_t n, uint32_t k, const uint32_t* __restrict__ d,
+    const uint32_t* __restrict__ info, uint32_t* __restrict__ words, EppOut* __restrict__ out, unsigned char* cur,
+    unsigned char* nxt, uint16_t* cs, uint32_t* wsum, uint64_t* s_best) {
+  const uint32_t tid = threadIdx.x, nt = blockDim.x;
+  for (uint32_t q = tid; q < n; q += nt) cur[q] = 1;
+  __syncthreads();
+  // The current set's states, ascending, in cs[0 .. m) (n <= 512 < threads:
+  // one flag per thread).
+  auto compact = [&]() {
+    const uint32_t f = tid < n ? uint32_t(cur[tid]) : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_scan(f, wsum, &tot);
+    if (f) cs[ex] = uint16_t(tid);
+    __syncthreads();
+    return tot;
+  };
+  uint32_t m = compact();
+  uint64_t total = 0, wl = 0;
+  uint32_t* word = words + job.word_off;
+  while (m > 1) {
+    uint64_t best = ~0ull;  // (length, first, second)
+    for (uint32_t t = tid; t < m * m; t += nt) {
+      const uint32_t i = t / m, j = t % m;
+      if (i < j) {
+        const uint32_t a = cs[i], b = cs[j];
+        const uint64_t key = (uint64_t(info[a * n + b] >> 8) << 32) | (uint64_t(a) << 16) | b;
+        best = key < best ? key : best;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint64_t o = __shfl_xor(best, off, 64);
+      best = o < best ? o : best;
+    }
+    if ((tid & 63u) == 0) s_best[tid >> 6] = best;
+    __syncthreads();
+    best = s_best[0];
+    for (uint32_t w = 1; w < (nt >> 6); ++w) best = s_best[w] < best ? s_best[w] : best;
+    __syncthreads();
+    total += best >> 32;
+    if (total > job.upper) {
+      if (tid == 0) *out = EppOut{0, 0, job.upper, 0};
+      return;
+    }
+    uint32_t n1 = uint32_t(best >> 16) & 0xffffu, n2 = uint32_t(best) & 0xffffu;
+    while (n1 != n2) {
+      const uint32_t x = info[pair_ix(n1, n2, n)] & 255u;
+      if (tid == 0 && wl < job.word_cap) word[wl] = x;
+      ++wl;
+      for (uint32_t q = tid; q < n; q += nt) nxt[q] = 0;
+      __syncthreads();
+      for (uint32_t q = tid; q < n; q += nt)
+        if (cur[q]) nxt[d[size_t(q) * k + x]] = 1;
+      __syncthreads();
+      for (uint32_t q = tid; q < n; q += nt) cur[q] = nxt[q];
+      __syncthreads();
+      n1 = d[size_t(n1) * k + x];
+      n2 = d[size_t(n2) * k + x];
+    }
+    m = compact();
+  }
+  if (tid == 0) *out = EppOut{0, 1, total, wl};
+}
+
 __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __restrict__ jobs,
     const uint32_t* __restrict__ deltas, unsigned char* __restrict__ scratch, uint32_t* __restrict__ words,
     EppOut* __restrict__ outs) {
@@ -201,64 +266,64 @@ __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __res
     return;
   }
 
-  // ---- greedy merging of the closest pair
-  for (uint32_t q = tid; q < n; q += nt) cur[q] = 1;
-  __syncthreads();
-  // The current set's states, ascending, in cs[0 .. m) (n <= 512 < threads:
-  // one flag per thread).
-  auto compact = [&]() {
-    const uint32_t f = tid < n ? uint32_t(cur[tid]) : 0u;
-    uint32_t tot;
-    const uint32_t ex = block_scan(f, wsum, &tot);
-    if (f) cs[ex] = uint16_t(tid);
-    __syncthreads();
-    return tot;
-  };
-  uint32_t m = compact();
-  uint64_t total = 0, wl = 0;
-  uint32_t* word = words + job.word_off;
-  while (m > 1) {
-    uint64_t best = ~0ull;  // (length, first, second)
-    for (uint32_t t = tid; t < m * m; t += nt) {
-      const uint32_t i = t / m, j = t % m;
-      if (i < j) {
-        const uint32_t a = cs[i], b = cs[j];
-        const uint64_t key = (uint64_t(info[a * n + b] >> 8) << 32) | (uint64_t(a) << 16) | b;
-        best = key < best ? key : best;
-      }
+  epp_greedy(job, n, k, d, info, words, out, cur, nxt, cs, wsum, s_best);
+}
+
+// ---- whole-device pair BFS of one automaton: a level is three launches over
+// the level's parents (proposals, winners per parent, winners written at the
+// scanned offsets), the same passes as the workgroup kernel above.
+__global__ __launch_bounds__(256) void k_epp_propose(uint32_t level, uint32_t parents, uint32_t n, uint32_t k,
+    const uint32_t* __restrict__ beg, const uint16_t* __restrict__ edges, const uint32_t* __restrict__ lvl,
+    const uint32_t* __restrict__ info, unsigned long long* __restrict__ keys) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= parents) return;
+  candidates(level, p, n, k, beg, edges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
+    const uint32_t ix = pair_ix(x, y, n);
+    if (info[ix] == 0) atomicMin(&keys[ix], (unsigned long long)key);
+  });
+}
+
+__global__ __launch_bounds__(256) void k_epp_count(uint32_t level, uint32_t parents, uint32_t n, uint32_t k,
+    const uint32_t* __restrict__ beg, const uint16_t* __restrict__ edges, const uint32_t* __restrict__ lvl,
+    const uint32_t* __restrict__ info, const unsigned long long* __restrict__ keys, uint32_t* __restrict__ cnt) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= parents) return;
+  uint32_t c = 0;
+  candidates(level, p, n, k, beg, edges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
+    const uint32_t ix = pair_ix(x, y, n);
+    c += (info[ix] == 0 && keys[ix] == key) ? 1u : 0u;
+  });
+  cnt[p] = c;
+}
+
+// Only a pair's winner writes its info, and it reads info before writing.
+__global__ __launch_bounds__(256) void k_epp_write(uint32_t level, uint32_t parents, uint32_t n, uint32_t k,
+    const uint32_t* __restrict__ beg, const uint16_t* __restrict__ edges, const uint32_t* __restrict__ lvl,
+    uint32_t* info, const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ off, uint32_t* queue,
+    uint32_t wbase) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= parents) return;
+  uint32_t o = wbase + off[p];
+  candidates(level, p, n, k, beg, edges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t a) {
+    const uint32_t ix = pair_ix(x, y, n);
+    if (info[ix] == 0 && keys[ix] == key) {
+      info[ix] = ((level + 1) << 8) | a;
+      queue[o++] = x | (y << 16);
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      const uint64_t o = __shfl_xor(best, off, 64);
-      best = o < best ? o : best;
-    }
-    if ((tid & 63u) == 0) s_best[tid >> 6] = best;
-    __syncthreads();
-    best = s_best[0];
-    for (uint32_t w = 1; w < (nt >> 6); ++w) best = s_best[w] < best ? s_best[w] : best;
-    __syncthreads();
-    total += best >> 32;
-    if (total > job.upper) {
-      if (tid == 0) *out = EppOut{0, 0, job.upper, 0};
-      return;
-    }
-    uint32_t n1 = uint32_t(best >> 16) & 0xffffu, n2 = uint32_t(best) & 0xffffu;
-    while (n1 != n2) {
-      const uint32_t x = info[pair_ix(n1, n2, n)] & 255u;
-      if (tid == 0 && wl < job.word_cap) word[wl] = x;
-      ++wl;
-      for (uint32_t q = tid; q < n; q += nt) nxt[q] = 0;
-      __syncthreads();
-      for (uint32_t q = tid; q < n; q += nt)
-        if (cur[q]) nxt[d[size_t(q) * k + x]] = 1;
-      __syncthreads();
-      for (uint32_t q = tid; q < n; q += nt) cur[q] = nxt[q];
-      __syncthreads();
-      n1 = d[size_t(n1) * k + x];
-      n2 = d[size_t(n2) * k + x];
-    }
-    m = compact();
-  }
-  if (tid == 0) *out = EppOut{0, 1, total, wl};
+  });
+}
+
+__global__ __launch_bounds__(kEppThreads) void k_epp_greedy(const EppJob* __restrict__ jobs,
+    const uint32_t* __restrict__ deltas, const unsigned char* __restrict__ scratch, uint32_t* __restrict__ words,
+    EppOut* __restrict__ outs) {
+  __shared__ uint32_t wsum[16];
+  __shared__ uint64_t s_best[16];
+  __shared__ unsigned char cur[kEppMaxN], nxt[kEppMaxN];
+  __shared__ uint16_t cs[kEppMaxN];
+  const EppJob job = jobs[0];
+  const EppLayout lay = epp_layout(job.n, job.k);
+  const uint32_t* info = reinterpret_cast<const uint32_t*>(scratch + job.scratch_off + lay.info);
+  epp_greedy(job, job.n, job.k, deltas + job.delta_off, info, words, outs, cur, nxt, cs, wsum, s_best);
 }
 
 }  // namespace
@@ -269,6 +334,34 @@ hipError_t launch_eppstein_wg(const EppJob* jobs, uint32_t count, const uint32_t
     uint32_t* words, EppOut* outs, hipStream_t st) {
   if (!count) return hipSuccess;
   hipLaunchKernelGGL(k_eppstein_wg, dim3(count), dim3(kEppThreads), 0, st, jobs, deltas, scratch, words, outs);
+  return hipGetLastError();
+}
+
+hipError_t launch_epp_pass(int pass, uint32_t level, uint32_t parents, uint32_t n, uint32_t k, unsigned char* base,
+    uint32_t head, const uint32_t* off, uint32_t wbase, hipStream_t st) {
+  if (!parents) return hipSuccess;
+  const EppLayout lay = epp_layout(n, k);
+  const uint32_t* beg = reinterpret_cast<const uint32_t*>(base);
+  const uint16_t* edges = reinterpret_cast<const uint16_t*>(base + lay.edges);
+  uint32_t* info = reinterpret_cast<uint32_t*>(base + lay.info);
+  unsigned long long* keys = reinterpret_cast<unsigned long long*>(base + lay.keys);
+  uint32_t* queue = reinterpret_cast<uint32_t*>(base + lay.queue);
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(base + lay.cnt);
+  const uint32_t* lvl = queue + head;
+  const dim3 grid((parents + 255) / 256), block(256);
+  if (pass == 0)
+    hipLaunchKernelGGL(k_epp_propose, grid, block, 0, st, level, parents, n, k, beg, edges, lvl, info, keys);
+  else if (pass == 1)
+    hipLaunchKernelGGL(k_epp_count, grid, block, 0, st, level, parents, n, k, beg, edges, lvl, info, keys, cnt);
+  else
+    hipLaunchKernelGGL(k_epp_write, grid, block, 0, st, level, parents, n, k, beg, edges, lvl, info, keys, off, queue,
+        wbase);
+  return hipGetLastError();
+}
+
+hipError_t launch_epp_greedy(const EppJob* job, const uint32_t* deltas, const unsigned char* scratch,
+    uint32_t* words, EppOut* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_epp_greedy, dim3(1), dim3(kEppThreads), 0, st, job, deltas, scratch, words, out);
   return hipGetLastError();
 }
 

@@ -165,6 +165,15 @@ __host__ __device__ inline EppLayout epp_layout(uint32_t n, uint32_t k) {
 size_t eppstein_scratch_bytes(uint32_t n, uint32_t k);
 hipError_t launch_eppstein_wg(const EppJob* jobs, uint32_t count, const uint32_t* deltas, unsigned char* scratch,
     uint32_t* words, EppOut* outs, hipStream_t st);
+// The pair BFS of one automaton over the whole device, one pass per launch
+// (0 proposals, 1 winners per parent into the counts region, 2 winners
+// written at wbase + off[p]); base = its scratch (epp_layout), beg and edges
+// filled, level >= 1 parents at queue[head ...).  Then the greedy phase in
+// one workgroup (job: one EppJob on the device).
+hipError_t launch_epp_pass(int pass, uint32_t level, uint32_t parents, uint32_t n, uint32_t k, unsigned char* base,
+    uint32_t head, const uint32_t* off, uint32_t wbase, hipStream_t st);
+hipError_t launch_epp_greedy(const EppJob* job, const uint32_t* deltas, const unsigned char* scratch,
+    uint32_t* words, EppOut* out, hipStream_t st);
 
 // Mask width and LDS bytes of the subset kernel.
 int vsub_mask_bits(uint32_t n, int W, int requested, bool proper_by_size);
