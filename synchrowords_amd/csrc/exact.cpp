@@ -68,7 +68,8 @@ struct Side {
 // that repeated searches (the batch's overflow path) allocate nothing and
 // never call hipFree, which would synchronise the whole device.
 struct Workspace {
-  DevBuf tables[2];
+  DevBuf tables[2], masks[2];
+  std::vector<uint64_t> host_masks[2];
   Side F, B;
   DevBuf cand, valid, slot, keep, pos, small, scan_tmp, flags, probes;
   size_t scan_bytes = 0;
@@ -256,11 +257,8 @@ class Search {
   }
 
   void upload_tables() {
-    for (int dir = 0; dir < 2; ++dir) {
-      auto t = build_tables(n_, k_, delta_, dir);
-      tables_[dir].ensure(t.size() * 8);
-      SW_CHECK_HIP(hipMemcpyAsync(tables_[dir].p, t.data(), t.size() * 8, hipMemcpyHostToDevice, st_));
-    }
+    for (int dir = 0; dir < 2; ++dir)
+      build_tables_device(n_, k_, delta_, dir, ws_.host_masks[dir], ws_.masks[dir], tables_[dir], st_);
     small_.ensure(64);
     probes_.ensure(8);
     SW_CHECK_HIP(hipMemsetAsync(probes_.p, 0, 8, st_));
@@ -585,6 +583,22 @@ std::vector<uint64_t> build_tables(uint32_t n, uint32_t k, const uint32_t* delta
         }
       }
   return t;
+}
+
+void build_tables_device(uint32_t n, uint32_t k, const uint32_t* delta, int dir, std::vector<uint64_t>& host,
+    DevBuf& masks, DevBuf& out, hipStream_t st) {
+  const uint32_t W = (n + 63) / 64, S = (n + 7) / 8;
+  host.assign(size_t(k) * n * W, 0);
+  for (uint32_t q = 0; q < n; ++q)
+    for (uint32_t a = 0; a < k; ++a) {
+      const uint32_t x = delta[size_t(q) * k + a];
+      if (dir == 0) host[(size_t(a) * n + q) * W + (x >> 6)] |= 1ull << (x & 63);  // image of q
+      else host[(size_t(a) * n + x) * W + (q >> 6)] |= 1ull << (q & 63);          // q is a preimage of x
+    }
+  masks.ensure(host.size() * 8);
+  out.ensure(size_t(k) * S * 256 * W * 8);
+  SW_CHECK_HIP(hipMemcpyAsync(masks.p, host.data(), host.size() * 8, hipMemcpyHostToDevice, st));
+  SW_CHECK_HIP(launch_tables(int(W), masks.as<uint64_t>(), n, k, out.as<uint64_t>(), st));
 }
 
 void forward_levels(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t depth, uint64_t cap_rows,

@@ -88,6 +88,39 @@ static hipError_t expand_w(const uint64_t* parents, uint64_t m, const uint64_t* 
   return hipGetLastError();
 }
 
+// Slice tables from per-state masks: row (a, s, b) = OR of masks[a][8 s + j]
+// over the set bits j of b (masks: the image singleton of each state, or its
+// preimage set).  One thread per row.
+template <int W>
+__global__ __launch_bounds__(256) void k_tables(const uint64_t* __restrict__ masks, uint32_t n, uint32_t slices,
+    uint64_t rows, uint64_t* __restrict__ out) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const uint32_t b = uint32_t(i & 255u);
+  const uint64_t as = i >> 8;
+  const uint32_t s = uint32_t(as % slices), a = uint32_t(as / slices);
+  uint64_t r[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) r[w] = 0;
+  for (uint32_t j = 0; j < 8; ++j) {
+    const uint32_t q = s * 8 + j;
+    if (!((b >> j) & 1u) || q >= n) continue;
+    const uint64_t* m = masks + (uint64_t(a) * n + q) * W;
+#pragma unroll
+    for (int w = 0; w < W; ++w) r[w] |= m[w];
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w) out[i * W + w] = r[w];
+}
+
+template <int W>
+static hipError_t tables_w(const uint64_t* masks, uint32_t n, uint32_t k, uint64_t* out, hipStream_t st) {
+  const uint32_t slices = (n + 7) / 8;
+  const uint64_t rows = uint64_t(k) * slices * 256;
+  hipLaunchKernelGGL(k_tables<W>, dim3(unsigned((rows + 255) / 256)), dim3(256), 0, st, masks, n, slices, rows, out);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------- hashing
 // Linear probing.  A candidate claims an empty slot with CAS; on meeting an
 // equal set it is a duplicate, unless that set is a candidate of this step
@@ -393,6 +426,10 @@ hipError_t launch_expand(int W, const uint64_t* parents, uint64_t m, const uint6
     uint32_t letters, const uint32_t* letter_list, uint32_t n, int drop_mode, uint64_t* out, uint32_t* valid,
     hipStream_t st) {
   SW_DISPATCH_W(W, expand_w<w_>(parents, m, tables, slices, letters, letter_list, n, drop_mode, out, valid, st))
+}
+
+hipError_t launch_tables(int W, const uint64_t* masks, uint32_t n, uint32_t k, uint64_t* out, hipStream_t st) {
+  SW_DISPATCH_W(W, tables_w<w_>(masks, n, k, out, st))
 }
 
 template <int W>

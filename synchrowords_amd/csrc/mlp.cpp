@@ -159,7 +159,8 @@ void build_guide(uint32_t n, uint32_t k, const uint32_t* delta, Guide& out, hipS
 // Device buffers of the beam, kept by the context between calls.
 struct BeamWs {
   DevBuf dtf, dtb, list, cand, table, slot, keep, pos, uniq, uidx, x, scratch, keys, keys2, vals, vals2, small,
-      scan_tmp, sort_tmp, hist;
+      scan_tmp, sort_tmp, hist, mf, mb;
+  std::vector<uint64_t> host_mf, host_mb;  // per-state masks of the tables, until the next call
   size_t scan_bytes = 0, sort_bytes = 0;
   uint64_t* host = nullptr;  // pinned read-back
   BeamWs() { SW_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), 64, hipHostMallocDefault)); }
@@ -569,8 +570,6 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
     const uint32_t W = (n + 63) / 64, S = (n + 7) / 8;
     *bound = upper;
     if (word_len) *word_len = 0;
-    auto tf = sw::build_tables(n, k, delta, 0);
-    auto tb = sw::build_tables(n, k, delta, 1);
     sw::Guide guide;
     if (m) sw::build_guide(n, k, delta, guide, st);
     sw::BeamWs& ws = sw::beam_workspace(ctx);
@@ -579,10 +578,8 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
            &keys = ws.keys, &keys2 = ws.keys2, &vals = ws.vals, &vals2 = ws.vals2, &small = ws.small,
            &scan_tmp = ws.scan_tmp, &sort_tmp = ws.sort_tmp, &histd = ws.hist;
     size_t &scan_bytes = ws.scan_bytes, &sort_bytes = ws.sort_bytes;
-    dtf.ensure(tf.size() * 8);
-    dtb.ensure(tb.size() * 8);
-    SW_CHECK_HIP(hipMemcpyAsync(dtf.p, tf.data(), tf.size() * 8, hipMemcpyHostToDevice, st));
-    SW_CHECK_HIP(hipMemcpyAsync(dtb.p, tb.data(), tb.size() * 8, hipMemcpyHostToDevice, st));
+    if (m) sw::build_tables_device(n, k, delta, 0, ws.host_mf, ws.mf, dtf, st);  // images: the features only
+    sw::build_tables_device(n, k, delta, 1, ws.host_mb, ws.mb, dtb, st);
     // The reference's start (beam.hpp:104-116): sink-component states with two
     // preimages under some letter, as singletons.
     const std::vector<char> sink = sw::sink_states(n, k, delta);

@@ -145,6 +145,10 @@ void run_subset(sw_ctx* ctx, int W, VsubParams& p);
 // Reference-layout slice tables for every letter: [k][ceil(n/8)][256][W]
 // words; dir 0 = image, 1 = preimage.
 std::vector<uint64_t> build_tables(uint32_t n, uint32_t k, const uint32_t* delta, int dir);
+// The same tables built on the device into `out` from per-state masks
+// computed here (kept in `host` and uploaded to `masks` until the next call).
+void build_tables_device(uint32_t n, uint32_t k, const uint32_t* delta, int dir, std::vector<uint64_t>& host,
+    DevBuf& masks, DevBuf& out, hipStream_t st);
 
 // States of the sink component (exact.cpp).
 std::vector<char> sink_states(uint32_t n, uint32_t k, const uint32_t* delta);

@@ -222,6 +222,9 @@ hipError_t launch_commit(int W, const uint64_t* cand, const uint32_t* keep, cons
 // Insert arena rows [0, rows) (all distinct) into an empty table.
 hipError_t launch_hash_rebuild(int W, const uint64_t* arena, uint64_t rows, uint32_t* table, uint64_t mask,
     hipStream_t st);
+// Slice tables [k][ceil(n/8)][256][W] from per-state masks [k][n][W] (row
+// (a, s, b) = OR of the masks of the states 8 s + j, j a set bit of b).
+hipError_t launch_tables(int W, const uint64_t* masks, uint32_t n, uint32_t k, uint64_t* out, hipStream_t st);
 // keep[i] = !(bit i of dead) for i < count.
 hipError_t launch_alive(int W, const void* dead_bits, uint64_t count, uint32_t* keep, hipStream_t st);
 // Keep flags (mode 0 given, 1 = hash-slot owners as launch_hash_resolve,
