@@ -54,16 +54,42 @@ def test_eppstein_random_against_oracle(ctx, n, k):
             assert (got[2] or []) == want[2]
 
 
-def test_eppstein_non_synchronizing(ctx):
-    n, k = 40, 2
+def _two_sinks(n, k=2):
+    """Two sink cycles 0..n/2-1 and n/2..n-1: no pair across them merges."""
+    h2 = n // 2
     d = [0] * (n * k)
-    for q in range(n):  # two sink cycles: 0..19 and 20..39
-        h = (q // 20) * 20
-        d[q * k + 0] = h + (q + 1) % 20
-        d[q * k + 1] = h + (q * 7) % 20
+    for q in range(n):
+        h = (q // h2) * h2
+        d[q * k + 0] = h + (q + 1) % h2
+        d[q * k + 1] = h + (q * 7) % h2
+    return d
+
+
+@pytest.mark.parametrize("n", [40, 80])  # one workgroup; the whole device (one automaton of 64 states or more)
+def test_eppstein_non_synchronizing(ctx, n):
+    k = 2
+    d = _two_sinks(n, k)
     ns, bound, word = ctx.eppstein(n, k, d, 100)
     assert ns and bound == 100 and word is None
     assert orc.eppstein(n, k, d, 100)[0]
+
+
+def test_eppstein_single_and_batch_paths_agree(ctx):
+    """One automaton of 64 states or more runs its pair BFS over the whole
+    device; in a batch the same automaton runs in one workgroup.  Flags,
+    bounds and words are the same."""
+    rng = np.random.default_rng(21)
+    inst = [(int(rng.integers(64, 260)), int(rng.integers(2, 4)), None) for _ in range(12)]
+    inst = [(n, k, gen.random_dfa(n, k, 9100 + j)) for j, (n, k, _) in enumerate(inst)]
+    inst += [(64, 2, gen.cerny(64)), (80, 2, _two_sinks(80))]
+    ups = [(n - 1) ** 2 + 1 for n, _, _ in inst]
+    bns, bbounds, bwords = ctx.eppstein_batch(inst, ups, word_cap=1 << 16)
+    for j, (n, k, d) in enumerate(inst):
+        ns, bound, word = ctx.eppstein(n, k, d, ups[j])
+        assert bool(ns) == bool(bns[j]) and bound == int(bbounds[j]), j
+        assert (list(word) if word else None) == (bwords[j] or None), j  # no word: not improved
+        if word:
+            assert replay(n, k, d, word) == 1
 
 
 def test_eppstein_upper_not_improved(ctx):
