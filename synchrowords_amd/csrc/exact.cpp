@@ -50,7 +50,7 @@ namespace sw {
 namespace {
 
 struct Side {
-  DevBuf arena, parent, letter, table, front, front_idx;
+  DevBuf arena, parent, letter, table, front, front_idx, front_idx_next;
   uint64_t rows = 0;
   uint64_t tcap = 0;
   std::vector<uint64_t> level_end;  // arena rows after each level
@@ -204,6 +204,7 @@ class Search {
   const uint32_t* delta_;
   sw_exact_config cfg_;
   uint64_t budget_pairs_, min_pairs_, forward_cap_, mem_cap_;
+  bool goal_armed_ = false;  // the last commit reset the goal flag and witness
 
   uint64_t pair_tests_ = 0;
   double ms_expand_ = 0, ms_dom_ = 0, ms_goal_ = 0;  // host time between synchronizations
@@ -379,17 +380,32 @@ class Search {
     const uint64_t nnew = read_u32(total);
     ensure_rows(S, S.rows + nnew);
     const uint64_t lo = S.rows;
+    const uint64_t pairs = (lo + nnew) * nnew;
+    const bool dominance = nnew > 0 && pairs <= budget_pairs_ && pairs >= min_pairs_;
+    // Without a dominance pass the commit also writes the next frontier (the
+    // new level itself) and arms the goal check.  front_idx holds the
+    // parents' rows, which the commit reads: the new indices go to the other
+    // buffer, and front_idx is only resized after the commit.
+    S.front.ensure(nnew * row_bytes() + 8);
+    if (!dominance) S.front_idx_next.ensure(nnew * 4 + 4);
+    int* gfound = small_.as<int>() + 2;
+    unsigned long long* gwit = reinterpret_cast<unsigned long long*>(small_.as<char>() + 16);
     SW_CHECK_HIP(launch_commit(W, cand_.as<uint64_t>(), keep_.as<uint32_t>(), pos_.as<uint32_t>(),
         slot_.as<uint32_t>(), cnt, m, S.front_idx.as<uint32_t>(), 0, nullptr, lo, S.arena.as<uint64_t>(),
-        S.parent.as<uint32_t>(), S.letter.as<uint8_t>(), S.table.as<uint32_t>(), st_));
+        S.parent.as<uint32_t>(), S.letter.as<uint8_t>(), S.table.as<uint32_t>(), st_,
+        dominance ? nullptr : S.front.as<uint64_t>(), dominance ? nullptr : S.front_idx_next.as<uint32_t>(),
+        dominance ? nullptr : gfound, dominance ? nullptr : gwit));
+    if (!dominance) {
+      std::swap(S.front_idx.p, S.front_idx_next.p);
+      std::swap(S.front_idx.cap, S.front_idx_next.cap);
+    } else {
+      S.front_idx.ensure(nnew * 4 + 4);  // a reallocation waits for the commit
+    }
+    goal_armed_ = !dominance;
     S.rows = lo + nnew;
     S.level_end.push_back(S.rows);
     S.steps++;
     S.prev_fcount = S.fcount;
-    S.front.ensure(nnew * row_bytes() + 8);
-    S.front_idx.ensure(nnew * 4 + 4);
-    const uint64_t pairs = S.rows * nnew;
-    const bool dominance = nnew > 0 && pairs <= budget_pairs_ && pairs >= min_pairs_;
     if (dominance) {
       // Host time up to here (the new-level size was read back, so the
       // expansion and dedup are done) counts as expansion; the commit still
@@ -441,8 +457,6 @@ class Search {
       S.fcount = read_u32(total);
       ms_dom_ += ms_since(t1);
     } else {
-      SW_CHECK_HIP(launch_select(W, S.arena.as<uint64_t>(), lo, nullptr, nullptr, nnew, S.front.as<uint64_t>(),
-          S.front_idx.as<uint32_t>(), st_));
       S.fcount = nnew;
       ms_expand_ += ms_since(t0);
     }
@@ -459,7 +473,8 @@ class Search {
     if (!F_.fcount || !B_.fcount) return false;
     int* found = small_.as<int>() + 2;
     unsigned long long* wit = reinterpret_cast<unsigned long long*>(small_.as<char>() + 16);
-    SW_CHECK_HIP(launch_goal_reset(found, wit, st_));
+    if (!goal_armed_) SW_CHECK_HIP(launch_goal_reset(found, wit, st_));
+    goal_armed_ = false;
     VsubParams p;
     p.stream = F_.front.as<uint64_t>();
     p.s_count = F_.fcount;

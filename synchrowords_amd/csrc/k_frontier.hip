@@ -199,8 +199,13 @@ __global__ __launch_bounds__(256) void k_commit(const uint64_t* __restrict__ can
     const uint32_t* __restrict__ pos, const uint32_t* __restrict__ slot_of, uint64_t count, uint64_t m,
     const uint32_t* __restrict__ parent_idx, uint64_t parent_base, const uint32_t* __restrict__ letter_list,
     uint64_t base, uint64_t* __restrict__ arena, uint32_t* __restrict__ parent, uint8_t* __restrict__ letter,
-    uint32_t* table) {
+    uint32_t* table, uint64_t* __restrict__ front, uint32_t* __restrict__ front_idx, int* found,
+    unsigned long long* witness) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i == 0 && found) {  // arm the goal check that follows
+    *found = 0;
+    *witness = ~0ull;
+  }
   if (i >= count || !keep[i]) return;
   const uint64_t dst = base + pos[i];
 #pragma unroll
@@ -209,6 +214,11 @@ __global__ __launch_bounds__(256) void k_commit(const uint64_t* __restrict__ can
   if (parent) parent[dst] = parent_idx ? parent_idx[i - ai * m] : uint32_t(parent_base + (i - ai * m));
   if (letter) letter[dst] = uint8_t(letter_list ? letter_list[ai] : ai);
   if (table) table[slot_of[i]] = uint32_t(dst);
+  if (front) {  // the new level as the next frontier, in row order
+#pragma unroll
+    for (int w = 0; w < W; ++w) front[uint64_t(pos[i]) * W + w] = cand[i * W + w];
+    front_idx[pos[i]] = uint32_t(dst);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_alive(const uint8_t* __restrict__ dead, uint64_t count,
@@ -504,19 +514,20 @@ hipError_t launch_scan(const uint32_t* keep, uint32_t* pos, uint64_t count, uint
 template <int W>
 static hipError_t commit_w(const uint64_t* cand, const uint32_t* keep, const uint32_t* pos, const uint32_t* slot_of,
     uint64_t count, uint64_t m, const uint32_t* parent_idx, uint64_t parent_base, const uint32_t* letter_list,
-    uint64_t base, uint64_t* arena, uint32_t* parent, uint8_t* letter, uint32_t* table, hipStream_t st) {
-  if (!count) return hipSuccess;
+    uint64_t base, uint64_t* arena, uint32_t* parent, uint8_t* letter, uint32_t* table, uint64_t* front,
+    uint32_t* front_idx, int* found, unsigned long long* witness, hipStream_t st) {
+  if (!count) return found ? launch_goal_reset(found, witness, st) : hipSuccess;
   hipLaunchKernelGGL(k_commit<W>, dim3(blocks_for(count)), dim3(256), 0, st, cand, keep, pos, slot_of, count, m,
-      parent_idx, parent_base, letter_list, base, arena, parent, letter, table);
+      parent_idx, parent_base, letter_list, base, arena, parent, letter, table, front, front_idx, found, witness);
   return hipGetLastError();
 }
 
 hipError_t launch_commit(int W, const uint64_t* cand, const uint32_t* keep, const uint32_t* pos,
     const uint32_t* slot_of, uint64_t count, uint64_t m, const uint32_t* parent_idx, uint64_t parent_base,
     const uint32_t* letter_list, uint64_t base, uint64_t* arena, uint32_t* parent, uint8_t* letter, uint32_t* table,
-    hipStream_t st) {
+    hipStream_t st, uint64_t* front, uint32_t* front_idx, int* found, unsigned long long* witness) {
   SW_DISPATCH_W(W, commit_w<w_>(cand, keep, pos, slot_of, count, m, parent_idx, parent_base, letter_list, base,
-      arena, parent, letter, table, st))
+      arena, parent, letter, table, front, front_idx, found, witness, st))
 }
 
 template <int W>

@@ -215,10 +215,14 @@ hipError_t launch_scan(const uint32_t* keep, uint32_t* pos, uint64_t count, uint
 // (parent_idx[i % m]) and letter (letter_list[i / m]); point the hash slot
 // at the new row.
 // (parent_idx NULL: parent = parent_base + i % m.)
+// front != NULL: the kept candidates are also written at front[pos] (with
+// their rows at front_idx[pos]); found != NULL: *found = 0 and *witness = ~0
+// (as launch_goal_reset) in the same launch.
 hipError_t launch_commit(int W, const uint64_t* cand, const uint32_t* keep, const uint32_t* pos,
     const uint32_t* slot_of, uint64_t count, uint64_t m, const uint32_t* parent_idx, uint64_t parent_base,
     const uint32_t* letter_list, uint64_t base, uint64_t* arena, uint32_t* parent, uint8_t* letter, uint32_t* table,
-    hipStream_t st);
+    hipStream_t st, uint64_t* front = nullptr, uint32_t* front_idx = nullptr, int* found = nullptr,
+    unsigned long long* witness = nullptr);
 // Insert arena rows [0, rows) (all distinct) into an empty table.
 hipError_t launch_hash_rebuild(int W, const uint64_t* arena, uint64_t rows, uint32_t* table, uint64_t mask,
     hipStream_t st);
