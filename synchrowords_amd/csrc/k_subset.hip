@@ -79,8 +79,12 @@ __global__ __launch_bounds__(1024) void k_vsub(VsubParams p) {
   const uint64_t tile_left = p.t_count - tile_base;  // > 0 by grid construction
 
   // ---- transpose the tile: group g = tile sets [g*64, g*64+64), one per lane.
+  // Groups past the tile's last set are skipped: only lanes whose valid mask
+  // is zero read them, and the accumulator only loses bits (a 136-set tile
+  // took 75 us when all 64 groups were transposed).
   constexpr int NGROUPS = TILE / 64;
-  for (int g = wave; g < NGROUPS; g += nwaves) {
+  const int ngroups = tile_left < uint64_t(TILE) ? int((tile_left + 63) / 64) : NGROUPS;
+  for (int g = wave; g < ngroups; g += nwaves) {
     const uint64_t t = uint64_t(g) * 64 + lane;
     const bool valid = t < tile_left;
     int pc = 0;

@@ -427,8 +427,10 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
   }
 
   // ---- transpose the tile (as k_vsub): mask g of row q = ballot over tile
-  // sets [64 g, 64 g + 64) of "q is in the set".
-  for (int g = wave; g < kIxTile / 64; g += nwaves) {
+  // sets [64 g, 64 g + 64) of "q is in the set"; groups past the tile's last
+  // set are skipped (read only under a zero valid mask).
+  const int ngroups = tile_left < uint64_t(kIxTile) ? int((tile_left + 63) / 64) : kIxTile / 64;
+  for (int g = wave; g < ngroups; g += nwaves) {
     const uint64_t t = uint64_t(g) * 64 + lane;
     const bool valid = t < tile_left;
     const uint64_t* set = p.tile + (valid && p.t_perm ? uint64_t(p.t_perm[tile_base + t]) : tile_base + t) * W;
