@@ -217,15 +217,26 @@ __global__ __launch_bounds__(256) void k_size_hist(const uint64_t* __restrict__ 
     if (uint32_t(t) <= n && h[t]) atomicAdd(&hist[t], h[t]);
 }
 
-// One thread: exclusive (cursor) and inclusive (bound) prefix sums of hist[0..n].
-__global__ void k_size_offsets(const uint32_t* __restrict__ hist, uint32_t n, uint32_t* __restrict__ cursor,
-    uint32_t* __restrict__ bound) {
-  if (threadIdx.x != 0) return;
-  uint32_t acc = 0;
-  for (uint32_t c = 0; c <= n; ++c) {
-    cursor[c] = acc;
-    acc += hist[c];
-    if (bound) bound[c] = acc;
+// Exclusive (cursor) and inclusive (bound) prefix sums of hist[0..n], n < 1024:
+// one workgroup of 1024 threads, a scan per wave, then the earlier waves'
+// totals.  (One thread walking the bins took 16 us.)
+__global__ __launch_bounds__(1024) void k_size_offsets(const uint32_t* __restrict__ hist, uint32_t n,
+    uint32_t* __restrict__ cursor, uint32_t* __restrict__ bound) {
+  __shared__ uint32_t wsum[16];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+  const uint32_t v = t <= n ? hist[t] : 0u;
+  uint32_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= uint32_t(off)) x += y;
+  }
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  uint32_t before = 0;
+  for (uint32_t w = 0; w < wv; ++w) before += wsum[w];
+  if (t <= n) {
+    cursor[t] = before + x - v;
+    if (bound) bound[t] = before + x;
   }
 }
 
@@ -852,7 +863,8 @@ hipError_t launch_size_hist(int W, const uint64_t* sets, uint64_t count, uint32_
 }
 
 hipError_t launch_size_offsets(const uint32_t* hist, uint32_t n, uint32_t* cursor, uint32_t* bound, hipStream_t st) {
-  hipLaunchKernelGGL(k_size_offsets, dim3(1), dim3(64), 0, st, hist, n, cursor, bound);
+  if (n >= 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_size_offsets, dim3(1), dim3(1024), 0, st, hist, n, cursor, bound);
   return hipGetLastError();
 }
 
