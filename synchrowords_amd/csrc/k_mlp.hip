@@ -600,6 +600,76 @@ __global__ void k_rank_scatter(const uint32_t* __restrict__ rank, uint32_t count
   if (i < count) perm[rank[i]] = i;
 }
 
+// The kept sets of a depth straight from their places: out_rows[r] =
+// sets[i] and out_idx[r] = idx[i] for the set i of place r < keep (the
+// scatter and both gathers of the sorted path in one launch).
+template <int W>
+__global__ __launch_bounds__(256) void k_rank_select(const uint32_t* __restrict__ rank,
+    const uint64_t* __restrict__ sets, const uint32_t* __restrict__ idx, uint32_t count, uint32_t keep,
+    uint64_t* __restrict__ out_rows, uint32_t* __restrict__ out_idx) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t r = rank[i];
+  if (r >= keep) return;
+#pragma unroll
+  for (int w = 0; w < W; ++w) out_rows[uint64_t(r) * W + w] = sets[uint64_t(i) * W + w];
+  out_idx[r] = idx[i];
+}
+
+static hipError_t rank_select(const uint32_t* rank, const uint64_t* sets, const uint32_t* idx, uint64_t count,
+    uint32_t W, uint64_t keep, uint64_t* out_rows, uint32_t* out_idx, hipStream_t st) {
+  switch (W) {
+#define CASE(w) \
+  case w: hipLaunchKernelGGL(k_rank_select<w>, dim3(blocks(count)), dim3(256), 0, st, rank, sets, idx, \
+      uint32_t(count), uint32_t(keep), out_rows, out_idx); return hipGetLastError();
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+  }
+  return hipErrorInvalidValue;
+}
+
+bool beam_rank_fits(uint64_t count) { return count <= kBeamRankMax; }
+
+hipError_t beam_select_reference(const uint64_t* sets, const uint32_t* idx, uint64_t count, uint32_t W, uint32_t n,
+    uint64_t keep, uint32_t* rank, uint64_t* out_rows, uint32_t* out_idx, hipStream_t st) {
+  if (!count) return hipSuccess;
+  if (count > kBeamRankMax || W < 1 || W > 8) return hipErrorInvalidValue;
+  hipError_t e = launch_fill_u32(rank, 0u, count, st);
+  if (e != hipSuccess) return e;
+  const dim3 grid(unsigned((count + 63) / 64), unsigned((count + kRankChunk - 1) / kRankChunk));
+  switch (W) {
+#define CASE(w) \
+  case w: hipLaunchKernelGGL(k_beam_rank<w>, grid, dim3(256), 0, st, sets, uint32_t(count), n, rank); break;
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+  }
+  return rank_select(rank, sets, idx, count, W, keep, out_rows, out_idx, st);
+}
+
+hipError_t beam_select_scores(const float* scores, const uint64_t* sets, const uint32_t* idx, uint64_t count,
+    uint32_t W, uint64_t keep, uint32_t* rank, uint64_t* out_rows, uint32_t* out_idx, hipStream_t st) {
+  if (!count) return hipSuccess;
+  if (count > kBeamRankMax) return hipErrorInvalidValue;
+  hipError_t e = launch_fill_u32(rank, 0u, count, st);
+  if (e != hipSuccess) return e;
+  const dim3 grid(unsigned((count + 63) / 64), unsigned((count + kRankChunk - 1) / kRankChunk));
+  hipLaunchKernelGGL(k_score_rank, grid, dim3(256), 0, st, scores, uint32_t(count), rank);
+  return rank_select(rank, sets, idx, count, W, keep, out_rows, out_idx, st);
+}
+
+// A depth's hash table cleared and its full-set index reset, in one launch.
+__global__ void k_beam_prep(uint32_t* __restrict__ table, uint64_t slots, unsigned long long* __restrict__ first) {
+  const uint64_t i0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i0 == 0) *first = ~0ull;
+  for (uint64_t i = i0; i < slots; i += uint64_t(gridDim.x) * blockDim.x) table[i] = kEmpty;
+}
+
+hipError_t launch_beam_prep(uint32_t* table, uint64_t slots, unsigned long long* first, hipStream_t st) {
+  const uint64_t b = std::min<uint64_t>(blocks(slots), 1024);
+  hipLaunchKernelGGL(k_beam_prep, dim3(unsigned(b ? b : 1)), dim3(256), 0, st, table, slots, first);
+  return hipGetLastError();
+}
+
 hipError_t beam_reference_order(const uint64_t* sets, uint64_t count, uint32_t W, uint32_t n, uint32_t* perm_a,
     uint32_t* perm_b, uint64_t* key_a, uint64_t* key_b, void* temp, size_t* temp_bytes, uint32_t** result,
     hipStream_t st) {

@@ -606,20 +606,19 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
     for (uint64_t len = 1; len < upper; ++len) {
       const uint64_t cnt = m0 * k;
       cand.ensure(cnt * W * 8 + 8);
-      SW_CHECK_HIP(sw::launch_expand(int(W), list.as<uint64_t>(), m0, dtb.as<uint64_t>(), S, k, nullptr, n, 0,
-          cand.as<uint64_t>(), nullptr, st));
-      unsigned long long* first = small.as<unsigned long long>();
-      SW_CHECK_HIP(hipMemsetAsync(first, 0xff, 8, st));
-      SW_CHECK_HIP(sw::launch_first_full(cand.as<uint64_t>(), cnt, n, W, first, st));
-      // Dedup within the step (the smallest candidate index of equal sets
-      // survives), launched before the one read-back of the step.
       uint64_t tcap = 1024;
       while (tcap < 2 * cnt) tcap <<= 1;
       table.ensure(tcap * 4);
       slot.ensure(cnt * 4 + 4);
       keep.ensure(cnt * 4 + 4);
       pos.ensure(cnt * 4 + 4);
-      SW_CHECK_HIP(sw::launch_fill_u32(table.as<uint32_t>(), sw::kEmpty, tcap, st));
+      unsigned long long* first = small.as<unsigned long long>();
+      SW_CHECK_HIP(sw::launch_beam_prep(table.as<uint32_t>(), tcap, first, st));
+      SW_CHECK_HIP(sw::launch_expand(int(W), list.as<uint64_t>(), m0, dtb.as<uint64_t>(), S, k, nullptr, n, 0,
+          cand.as<uint64_t>(), nullptr, st));
+      SW_CHECK_HIP(sw::launch_first_full(cand.as<uint64_t>(), cnt, n, W, first, st));
+      // Dedup within the step (the smallest candidate index of equal sets
+      // survives), launched before the one read-back of the step.
       SW_CHECK_HIP(sw::launch_hash_insert(int(W), cand.as<uint64_t>(), nullptr, cnt, table.as<uint32_t>(), tcap - 1,
           cand.as<uint64_t>(), slot.as<uint32_t>(), nullptr, st));
       size_t need = 0;
@@ -663,49 +662,65 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
         break;
       }
       // Order the distinct candidates and keep the first `width`: by the
-      // network's score, or (m == NULL) in the reference's Beam order.
-      const uint32_t* order = nullptr;
-      vals.ensure(uint64_t(nu) * 4 + 4);
-      vals2.ensure(uint64_t(nu) * 4 + 4);
+      // network's score, or (m == NULL) in the reference's Beam order.  Up to
+      // the counted maximum the kept sets are written straight from their
+      // places; beyond it a sort gives the order and two gathers follow.
+      const uint64_t keepn = std::min<uint64_t>(width, nu);
+      list.ensure(keepn * W * 8 + 8);
+      histd.ensure((hist_len + keepn) * 4 + 4, st, hist_len * 4);
+      const bool counted = sw::beam_rank_fits(nu);
+      const float* scores = nullptr;
       if (m) {
-        keys.ensure(uint64_t(nu) * 4 + 4);
         x.ensure(uint64_t(nu) * m->p.D * 2 + 8);
         sw::mlp_features(m, guide, dtf.as<uint64_t>(), uniq.as<uint64_t>(), nu, x.as<uint16_t>(), scratch, st);
         auto wk = sw::work_of(m, nu);
         SW_CHECK_HIP(sw::mlp_forward(m->p, x.as<uint16_t>(), nu, wk, wk.y, st));
-        keys2.ensure(uint64_t(nu) * 4 + 4);
-        SW_CHECK_HIP(sw::launch_iota(vals.as<uint32_t>(), nu, st));
-        size_t sneed = 0;
-        SW_CHECK_HIP(sw::sort_scores(nullptr, nullptr, nullptr, nullptr, nu, nullptr, &sneed, st));
-        if (sneed > sort_bytes) {
-          sort_tmp.release();
-          sort_tmp.ensure(sneed);
-          sort_bytes = sneed;
-        }
-        SW_CHECK_HIP(sw::sort_scores(wk.y, keys2.as<float>(), vals.as<uint32_t>(), vals2.as<uint32_t>(), nu,
-            sort_tmp.p, &sort_bytes, st));
-        order = vals2.as<uint32_t>();
-      } else {
-        keys.ensure(uint64_t(nu) * 8 + 8);
-        keys2.ensure(uint64_t(nu) * 8 + 8);
-        size_t sneed = 0;
-        uint32_t* res = nullptr;
-        SW_CHECK_HIP(sw::beam_reference_order(uniq.as<uint64_t>(), nu, W, n, vals.as<uint32_t>(), vals2.as<uint32_t>(),
-            keys.as<uint64_t>(), keys2.as<uint64_t>(), nullptr, &sneed, &res, st));
-        if (sneed > sort_bytes) {
-          sort_tmp.release();
-          sort_tmp.ensure(sneed);
-          sort_bytes = sneed;
-        }
-        SW_CHECK_HIP(sw::beam_reference_order(uniq.as<uint64_t>(), nu, W, n, vals.as<uint32_t>(), vals2.as<uint32_t>(),
-            keys.as<uint64_t>(), keys2.as<uint64_t>(), sort_tmp.p, &sort_bytes, &res, st));
-        order = res;
+        scores = wk.y;
       }
-      const uint64_t keepn = std::min<uint64_t>(width, nu);
-      list.ensure(keepn * W * 8 + 8);
-      SW_CHECK_HIP(sw::launch_gather_rows(uniq.as<uint64_t>(), order, keepn, W, list.as<uint64_t>(), st));
-      histd.ensure((hist_len + keepn) * 4 + 4, st, hist_len * 4);
-      SW_CHECK_HIP(sw::launch_gather_u32(uidx.as<uint32_t>(), order, keepn, histd.as<uint32_t>() + hist_len, st));
+      if (counted) {
+        vals.ensure(uint64_t(nu) * 4 + 4);
+        if (m)
+          SW_CHECK_HIP(sw::beam_select_scores(scores, uniq.as<uint64_t>(), uidx.as<uint32_t>(), nu, W, keepn,
+              vals.as<uint32_t>(), list.as<uint64_t>(), histd.as<uint32_t>() + hist_len, st));
+        else
+          SW_CHECK_HIP(sw::beam_select_reference(uniq.as<uint64_t>(), uidx.as<uint32_t>(), nu, W, n, keepn,
+              vals.as<uint32_t>(), list.as<uint64_t>(), histd.as<uint32_t>() + hist_len, st));
+      } else {
+        const uint32_t* order = nullptr;
+        vals.ensure(uint64_t(nu) * 4 + 4);
+        vals2.ensure(uint64_t(nu) * 4 + 4);
+        if (m) {
+          keys2.ensure(uint64_t(nu) * 4 + 4);
+          SW_CHECK_HIP(sw::launch_iota(vals.as<uint32_t>(), nu, st));
+          size_t sneed = 0;
+          SW_CHECK_HIP(sw::sort_scores(nullptr, nullptr, nullptr, nullptr, nu, nullptr, &sneed, st));
+          if (sneed > sort_bytes) {
+            sort_tmp.release();
+            sort_tmp.ensure(sneed);
+            sort_bytes = sneed;
+          }
+          SW_CHECK_HIP(sw::sort_scores(scores, keys2.as<float>(), vals.as<uint32_t>(), vals2.as<uint32_t>(), nu,
+              sort_tmp.p, &sort_bytes, st));
+          order = vals2.as<uint32_t>();
+        } else {
+          keys.ensure(uint64_t(nu) * 8 + 8);
+          keys2.ensure(uint64_t(nu) * 8 + 8);
+          size_t sneed = 0;
+          uint32_t* res = nullptr;
+          SW_CHECK_HIP(sw::beam_reference_order(uniq.as<uint64_t>(), nu, W, n, vals.as<uint32_t>(),
+              vals2.as<uint32_t>(), keys.as<uint64_t>(), keys2.as<uint64_t>(), nullptr, &sneed, &res, st));
+          if (sneed > sort_bytes) {
+            sort_tmp.release();
+            sort_tmp.ensure(sneed);
+            sort_bytes = sneed;
+          }
+          SW_CHECK_HIP(sw::beam_reference_order(uniq.as<uint64_t>(), nu, W, n, vals.as<uint32_t>(),
+              vals2.as<uint32_t>(), keys.as<uint64_t>(), keys2.as<uint64_t>(), sort_tmp.p, &sort_bytes, &res, st));
+          order = res;
+        }
+        SW_CHECK_HIP(sw::launch_gather_rows(uniq.as<uint64_t>(), order, keepn, W, list.as<uint64_t>(), st));
+        SW_CHECK_HIP(sw::launch_gather_u32(uidx.as<uint32_t>(), order, keepn, histd.as<uint32_t>() + hist_len, st));
+      }
       hist_off.push_back(hist_len);
       hist_m.push_back(m0);
       hist_len += keepn;

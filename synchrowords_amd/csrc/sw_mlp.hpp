@@ -69,6 +69,17 @@ hipError_t launch_neg_popcount(const uint64_t* sets, uint64_t count, uint32_t W,
 hipError_t beam_reference_order(const uint64_t* sets, uint64_t count, uint32_t W, uint32_t n, uint32_t* perm_a,
     uint32_t* perm_b, uint64_t* key_a, uint64_t* key_b, void* temp, size_t* temp_bytes, uint32_t** result,
     hipStream_t st);
+// The first `keep` sets of a depth, by place, written to out_rows/out_idx
+// (sets[i], idx[i] of the set at place r < keep), for count <= the counted
+// maximum (beam_rank_fits): in the reference's Beam order, or in ascending
+// stable order of float scores.  rank: count words of scratch.
+bool beam_rank_fits(uint64_t count);
+hipError_t beam_select_reference(const uint64_t* sets, const uint32_t* idx, uint64_t count, uint32_t W, uint32_t n,
+    uint64_t keep, uint32_t* rank, uint64_t* out_rows, uint32_t* out_idx, hipStream_t st);
+hipError_t beam_select_scores(const float* scores, const uint64_t* sets, const uint32_t* idx, uint64_t count,
+    uint32_t W, uint64_t keep, uint32_t* rank, uint64_t* out_rows, uint32_t* out_idx, hipStream_t st);
+// table[0 .. slots) = empty and *first = ~0 (the start of a beam depth).
+hipError_t launch_beam_prep(uint32_t* table, uint64_t slots, unsigned long long* first, hipStream_t st);
 // Stable ascending sort of float keys with u32 values (temp == NULL: size query).
 hipError_t sort_scores(const float* keys_in, float* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
     uint64_t count, void* temp, size_t* temp_bytes, hipStream_t st);
