@@ -131,7 +131,8 @@ typedef struct sw_exact_config {
   uint64_t max_memory_mb;     /* device bytes the search may hold (0: 64 GiB) */
   uint64_t dominance_pairs;   /* dominance passes of at most this many set pairs
                                  (0: default, passes of 5e10 to 1e11 pairs with
-                                 at least 10 steps left below the upper bound) */
+                                 at least 10 steps left below the upper bound;
+                                 for n <= 192 only once a side holds 2 GiB) */
   uint64_t forward_cap;       /* forward frontier size that freezes the forward side (0: default) */
   int32_t dfs;                /* allow the backward-only phase after the forward side freezes */
   int32_t verbose;            /* per-step log on stderr */

@@ -112,6 +112,12 @@ class Search {
     // (n = 300: seed 1, bound 41, passes at lengths 34-36, 25.7 -> 22.0 ms;
     // passes with 14 or more steps left pay on the hard seeds).
     min_left_ = cfg.dominance_pairs ? 0 : 10;
+    // And, by default, not for automata of at most 192 states unless a side
+    // holds 2 GiB of sets: on the batch's classes (n <= 160, k = 2 and 3,
+    // Eppstein bounds) the passes cost more than they save (60 k = 3 searches
+    // 310 -> 295 ms without them); the size test keeps pruning where memory
+    // would run out without it.
+    small_sets_ = !cfg.dominance_pairs && W_ <= 3;
     forward_cap_ = cfg.forward_cap ? cfg.forward_cap : (1ull << 22);
     mem_cap_ = (cfg.max_memory_mb ? cfg.max_memory_mb : 65536ull) << 20;
   }
@@ -210,6 +216,8 @@ class Search {
   const uint32_t* delta_;
   sw_exact_config cfg_;
   uint64_t budget_pairs_, min_pairs_, min_left_, forward_cap_, mem_cap_;
+  bool small_sets_ = false;
+  static constexpr uint64_t kSmallSetsDominanceBytes = 2ull << 30;
   bool goal_armed_ = false;  // the last commit reset the goal flag and witness
   uint64_t upper_ = ~0ull;     // the search's upper bound (none: forward levels only)
 
@@ -388,7 +396,8 @@ class Search {
     ensure_rows(S, S.rows + nnew);
     const uint64_t lo = S.rows;
     const uint64_t pairs = (lo + nnew) * nnew;
-    const bool dominance = nnew > 0 && pairs <= budget_pairs_ && pairs >= min_pairs_ && upper_ >= t + 1 + min_left_;
+    const bool dominance = nnew > 0 && pairs <= budget_pairs_ && pairs >= min_pairs_ && upper_ >= t + 1 + min_left_ &&
+        (!small_sets_ || (lo + nnew) * row_bytes() >= kSmallSetsDominanceBytes);
     // Without a dominance pass the commit also writes the next frontier (the
     // new level itself) and arms the goal check.  front_idx holds the
     // parents' rows, which the commit reads: the new indices go to the other
