@@ -95,11 +95,7 @@ __global__ __launch_bounds__(1024) void k_vsub(VsubParams p) {
         if (p.complement_tile) x = ~x & word_mask(n, w);
       }
       pc += __popcll(x);
-      uint64_t kept = 0;  // lane j keeps the ballot of state w*64 + j
-      for (int b = 0; b < 64; ++b) {
-        const uint64_t m = __builtin_amdgcn_ballot_w64((x >> b) & 1ull);
-        kept = (lane == b) ? m : kept;
-      }
+      const uint64_t kept = lane_transpose64(x);  // lane j: which of the 64 sets hold state w*64 + j
       const uint32_t q = uint32_t(w) * 64 + lane;
       if (q < n) {
         if (MB == 64) {

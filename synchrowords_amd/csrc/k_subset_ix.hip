@@ -185,10 +185,7 @@ __global__ __launch_bounds__(256) void k_state_freq(const uint64_t* __restrict__
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       const uint64_t x = t < count ? ((sets[t * W + w] ^ cmask) & word_mask(n, w)) : 0ull;
-      for (int b = 0; b < 64; ++b) {
-        const uint32_t pc = __popcll(__builtin_amdgcn_ballot_w64((x >> b) & 1ull));
-        c[w] += (lane == b) ? pc : 0u;
-      }
+      c[w] += uint32_t(__popcll(lane_transpose64(x)));  // lane b: sets of the 64 holding state w*64 + b
     }
   }
 #pragma unroll
@@ -437,7 +434,7 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
     if (s_begin / 2 >= p_bound) return;  // the whole workgroup, before any barrier
   }
 
-  // ---- transpose the tile (as k_vsub): mask g of row q = ballot over tile
+  // ---- transpose the tile (as k_vsub): mask g of row q = bits over tile
   // sets [64 g, 64 g + 64) of "q is in the set"; groups past the tile's last
   // set are skipped (read only under a zero valid mask).
   const int ngroups = tile_left < uint64_t(kIxTile) ? int((tile_left + 63) / 64) : kIxTile / 64;
@@ -453,11 +450,7 @@ __global__ __launch_bounds__(1024) void k_vsub_ix(VsubParams p) {
         if (p.complement_tile) x = ~x & word_mask(n, w);
       }
       pc += __popcll(x);
-      uint64_t kept = 0;
-      for (int b = 0; b < 64; ++b) {
-        const uint64_t m = __builtin_amdgcn_ballot_w64((x >> b) & 1ull);
-        kept = (lane == b) ? m : kept;
-      }
+      const uint64_t kept = lane_transpose64(x);
       const uint32_t q = uint32_t(w) * 64 + lane;
       if (q < n) rows[size_t(q) * 64 + g] = kept;
     }

@@ -22,6 +22,25 @@ __host__ __device__ inline uint64_t word_mask(uint32_t n, int w) {
 }
 
 __device__ inline uint32_t rfl(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// 64 x 64 bit transpose across the lanes of a wave: lane i holds row i (bit
+// b = entry (i, b)); lane j gets column j (bit i = bit j of lane i's row).
+// Six butterfly steps swap the off-diagonal blocks of size s = 32 ... 1
+// between lanes i and i ^ s, about 50 instructions where 64 ballots take
+// about 500.
+__device__ inline uint64_t lane_transpose64(uint64_t x) {
+  const uint32_t lane = __lane_id();
+  constexpr uint64_t kLow[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+      0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int s = 32 >> k;
+    const uint64_t lo = kLow[k];
+    const uint64_t y = __shfl_xor(x, s, 64);
+    x = (lane & uint32_t(s)) ? ((x & ~lo) | ((y & ~lo) >> s)) : ((x & lo) | ((y & lo) << s));
+  }
+  return x;
+}
 __device__ inline uint64_t rfl64(uint64_t x) {
   return (uint64_t(rfl(uint32_t(x >> 32))) << 32) | rfl(uint32_t(x));
 }
