@@ -45,7 +45,10 @@ namespace sw {
 namespace {
 
 constexpr int kIxTile = 4096;
-constexpr int kIxPrefetch = 8;  // uint4 groups of offsets (32 states) kept in registers
+#ifndef SW_IX_PREFETCH
+#define SW_IX_PREFETCH 8
+#endif
+constexpr int kIxPrefetch = SW_IX_PREFETCH;  // uint4 groups of offsets (32 states) kept in registers
 // With early exit: groups per pair kept at a fixed stride (ix_head), and
 // pairs whose head groups are loaded ahead per wave.
 #ifndef SW_IX_HEAD
