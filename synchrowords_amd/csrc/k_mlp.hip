@@ -427,7 +427,8 @@ hipError_t launch_iota(uint32_t* p, uint64_t count, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Orders of up to kBeamRankMax items are counted rather than sorted (below).
+// A beam depth of up to kBeamRankMax candidates is ordered by counting
+// (beam_select_reference, beam_select_scores) rather than sorted.
 constexpr uint32_t kRankChunk = 256;
 constexpr uint64_t kBeamRankMax = 16384;
 
@@ -469,35 +470,8 @@ __global__ __launch_bounds__(256) void k_score_rank(const float* __restrict__ ke
   }
 }
 
-__global__ void k_score_scatter(const uint32_t* __restrict__ rank, const float* __restrict__ keys_in,
-    const uint32_t* __restrict__ vals_in, uint32_t count, float* __restrict__ keys_out,
-    uint32_t* __restrict__ vals_out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  const uint32_t r = rank[i];
-  keys_out[r] = keys_in[i];
-  vals_out[r] = vals_in[i];
-}
-
 hipError_t sort_scores(const float* keys_in, float* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
     uint64_t count, void* temp, size_t* temp_bytes, hipStream_t st) {
-  if (!temp) {
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, *temp_bytes, keys_in, keys_out, vals_in, vals_out,
-        int(count), 0, 32, st);
-    *temp_bytes = std::max(*temp_bytes, size_t(count) * 4 + 4);  // the rank array of the counted order
-    return e;
-  }
-  if (count && count <= kBeamRankMax) {
-    if (*temp_bytes < size_t(count) * 4) return hipErrorInvalidValue;
-    uint32_t* rank = static_cast<uint32_t*>(temp);
-    hipError_t e = launch_fill_u32(rank, 0u, count, st);
-    if (e != hipSuccess) return e;
-    const dim3 grid(unsigned((count + 63) / 64), unsigned((count + kRankChunk - 1) / kRankChunk));
-    hipLaunchKernelGGL(k_score_rank, grid, dim3(256), 0, st, keys_in, uint32_t(count), rank);
-    hipLaunchKernelGGL(k_score_scatter, dim3(blocks(count)), dim3(256), 0, st, rank, keys_in, vals_in,
-        uint32_t(count), keys_out, vals_out);
-    return hipGetLastError();
-  }
   return hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, keys_in, keys_out, vals_in, vals_out, int(count), 0,
       32, st);
 }
@@ -530,9 +504,9 @@ __global__ void k_neg_size(const uint64_t* __restrict__ sets, uint64_t count, ui
 // Up to kBeamRankMax sets, the place of each set in that order is counted
 // directly: rank[i] = #{j : set j comes before set i}, over a grid of (64
 // sets i) x (kRankChunk sets j, staged in LDS with their keys), as
-// k_score_rank; then perm[rank[i]] = i.  Two launches and a
-// clear, where the merge sort takes five or more; past kBeamRankMax the
-// quadratic count is left to the sort.
+// k_score_rank.  A clear, the count and k_rank_select, where the merge sort
+// and two gathers take seven launches or more; past kBeamRankMax the
+// quadratic count is left to the sort (beam_reference_order).
 
 // Sort key of a set: (n - |S|) in the top 9 bits, then the top 55 bits of
 // word 0; sets with equal keys are ordered by all their words.
@@ -593,11 +567,6 @@ __global__ __launch_bounds__(256) void k_beam_rank(const uint64_t* __restrict__ 
     const uint32_t tot = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
     if (tot) atomicAdd(&rank[i], tot);
   }
-}
-
-__global__ void k_rank_scatter(const uint32_t* __restrict__ rank, uint32_t count, uint32_t* __restrict__ perm) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) perm[rank[i]] = i;
 }
 
 // The kept sets of a depth straight from their places: out_rows[r] =
@@ -684,20 +653,6 @@ hipError_t beam_reference_order(const uint64_t* sets, uint64_t count, uint32_t W
   }
   *result = perm_a;
   if (!count) return hipSuccess;
-  if (count <= kBeamRankMax && W >= 1 && W <= 8) {
-    uint32_t* rank = reinterpret_cast<uint32_t*>(key_a);
-    hipError_t e = launch_fill_u32(rank, 0u, count, st);
-    if (e != hipSuccess) return e;
-    const dim3 grid(unsigned((count + 63) / 64), unsigned((count + kRankChunk - 1) / kRankChunk));
-    switch (W) {
-#define CASE(w) \
-  case w: hipLaunchKernelGGL(k_beam_rank<w>, grid, dim3(256), 0, st, sets, uint32_t(count), n, rank); break;
-      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
-#undef CASE
-    }
-    hipLaunchKernelGGL(k_rank_scatter, dim3(blocks(count)), dim3(256), 0, st, rank, uint32_t(count), perm_a);
-    return hipGetLastError();
-  }
   // One comparison sort of the indices; the sets are distinct, so the order
   // is total and any correct sort gives the reference's.
   hipLaunchKernelGGL(k_iota_u32, dim3(blocks(count)), dim3(256), 0, st, perm_a, count);
