@@ -119,6 +119,16 @@ template <int W, bool RANKED>
 __global__ void k_ix_pair_fill(const uint64_t* __restrict__ sets, const uint32_t* __restrict__ perm, uint64_t count,
     uint32_t n, uint64_t cmask, const uint32_t* __restrict__ start, const uint32_t* __restrict__ rank,
     const uint32_t* __restrict__ order, uint32_t* __restrict__ list, uint32_t* __restrict__ head) {
+  // The rank and order tables (n <= 512 words each) in LDS: a set's walk
+  // looks one up per state.
+  __shared__ uint32_t s_rank[RANKED ? 512 : 1], s_order[RANKED ? 512 : 1];
+  if (RANKED) {
+    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
+      s_rank[t] = rank[t];
+      s_order[t] = order[t];
+    }
+    __syncthreads();
+  }
   const uint64_t s = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint64_t pairs = (count + 1) / 2;
   if (s >= 2 * pairs) return;
@@ -144,7 +154,7 @@ __global__ void k_ix_pair_fill(const uint64_t* __restrict__ sets, const uint32_t
       for (int w = 0; w < W; ++w) {
         uint64_t b = x[w];
         while (b) {
-          const uint32_t r = rank[uint32_t(w) * 64 + __builtin_ctzll(b)];
+          const uint32_t r = s_rank[uint32_t(w) * 64 + __builtin_ctzll(b)];
           b &= b - 1;
 #pragma unroll
           for (int v = 0; v < W; ++v)  // constant indices only: y stays in registers
@@ -160,7 +170,7 @@ __global__ void k_ix_pair_fill(const uint64_t* __restrict__ sets, const uint32_t
       while (b) {
         const uint32_t q = uint32_t(w) * 64 + __builtin_ctzll(b);
         b &= b - 1;
-        emit(j, RANKED ? order[q] : q * 512u);
+        emit(j, RANKED ? s_order[q] : q * 512u);
         ++j;
       }
     }
@@ -279,8 +289,9 @@ __global__ __launch_bounds__(256) void k_size_scatter(const uint64_t* __restrict
 template <int W>
 __global__ __launch_bounds__(256) void k_anchor(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n,
     uint64_t cmask, const uint32_t* __restrict__ rank, uint32_t* __restrict__ anchor, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[W * 64 + 1];
+  __shared__ uint32_t h[W * 64 + 1], s_rank[W * 64];
   for (int t = threadIdx.x; t <= W * 64; t += blockDim.x) h[t] = 0;
+  for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) s_rank[t] = rank[t];
   __syncthreads();
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < count) {
@@ -291,7 +302,7 @@ __global__ __launch_bounds__(256) void k_anchor(const uint64_t* __restrict__ set
       while (b) {
         const uint32_t q = uint32_t(w) * 64 + __builtin_ctzll(b);
         b &= b - 1;
-        const uint32_t r = rank[q];
+        const uint32_t r = s_rank[q];
         if (r < best) {
           best = r;
           a = q;
