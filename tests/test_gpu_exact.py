@@ -97,6 +97,20 @@ def test_dominance_pruning_keeps_the_length(ctx, n, k):
             assert replay(n, k, d, r.word) == 1
 
 
+@pytest.mark.parametrize("n,k,width", [(100, 2, 9000), (300, 2, 12000), (100, 3, 6000)])
+def test_device_beam_wide_crosses_the_counted_order(ctx, n, k, width):
+    """Widths whose depths hold more than 16384 distinct candidates take the
+    sorted order there and the counted order below: the oracle's bound."""
+    import oracle_lib as orc
+    d = gen.random_dfa(n, k, 9300 + n)
+    cap = (n - 1) ** 2 + 1
+    want = orc.beam_width(n, k, d, width, cap)
+    b, w = ctx.beam(n, k, d, width, cap)
+    assert b == want
+    if b < cap:
+        assert len(w) == b and replay(n, k, d, w) == 1
+
+
 @pytest.mark.parametrize("n,k", [(20, 2), (48, 2), (64, 3), (100, 2), (200, 2), (300, 2), (130, 4)])
 @pytest.mark.parametrize("width", [1, 7, 64, 500])
 def test_device_beam_matches_reference_order(ctx, n, k, width):
