@@ -119,6 +119,7 @@ void eppstein_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint3
     std::vector<EppJob> jobs;
     std::vector<uint32_t> dpack;
     size_t scratch = 0;
+    uint32_t list_lds = 0;
     uint32_t end = at;
     while (end < count) {
       const size_t need = eppstein_scratch_bytes(ns[end], ks[end]);
@@ -127,6 +128,7 @@ void eppstein_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint3
           uppers[end]});
       dpack.insert(dpack.end(), deltas + delta_offsets[end], deltas + delta_offsets[end] + size_t(ns[end]) * ks[end]);
       scratch += need;
+      list_lds = std::max(list_lds, eppstein_list_lds(ns[end], ks[end]));
       ++end;
     }
     const size_t m = jobs.size();
@@ -138,7 +140,7 @@ void eppstein_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint3
     SW_CHECK_HIP(hipMemcpyAsync(ctx->epp_jobs.p, jobs.data(), m * sizeof(EppJob), hipMemcpyHostToDevice, st));
     SW_CHECK_HIP(hipMemcpyAsync(ctx->epp_delta.p, dpack.data(), dpack.size() * 4, hipMemcpyHostToDevice, st));
     SW_CHECK_HIP(launch_eppstein_wg(ctx->epp_jobs.as<EppJob>(), uint32_t(m), ctx->epp_delta.as<uint32_t>(),
-        ctx->epp_scratch.as<unsigned char>(), ctx->epp_words.as<uint32_t>(), ctx->epp_out.as<EppOut>(), st));
+        ctx->epp_scratch.as<unsigned char>(), ctx->epp_words.as<uint32_t>(), ctx->epp_out.as<EppOut>(), list_lds, st));
     std::vector<EppOut> out(m);
     SW_CHECK_HIP(hipMemcpyAsync(out.data(), ctx->epp_out.p, m * sizeof(EppOut), hipMemcpyDeviceToHost, st));
     if (words && word_cap)

@@ -153,9 +153,28 @@ __device__ void epp_greedy(const EppJob& job, uint32_t n, uint32_t k, const uint
   if (tid == 0) *out = EppOut{0, 1, total, wl};
 }
 
+// The preimage lists in LDS when they fit the launch's dynamic LDS (every
+// candidate walk reads them; n = 300, k = 2: 3.6 KB): copies beg[k n + 1]
+// and edges[k n] there and points *b, *e at the copies.  Every thread of the
+// workgroup calls it.
+constexpr uint32_t kEppListLds = 48 * 1024;
+__device__ inline size_t epp_list_bytes(uint32_t kn) { return ((size_t(kn) + 1) * 4 + size_t(kn) * 2 + 7) & ~size_t(7); }
+__device__ inline void epp_stage_lists(uint32_t kn, uint32_t lds_cap, unsigned char* lds, const uint32_t*& b,
+    const uint16_t*& e) {
+  if (epp_list_bytes(kn) > lds_cap) return;
+  uint32_t* sb = reinterpret_cast<uint32_t*>(lds);
+  uint16_t* se = reinterpret_cast<uint16_t*>(lds + (size_t(kn) + 1) * 4);
+  for (uint32_t t = threadIdx.x; t <= kn; t += blockDim.x) sb[t] = b[t];
+  for (uint32_t t = threadIdx.x; t < kn; t += blockDim.x) se[t] = e[t];
+  __syncthreads();
+  b = sb;
+  e = se;
+}
+
 __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __restrict__ jobs,
     const uint32_t* __restrict__ deltas, unsigned char* __restrict__ scratch, uint32_t* __restrict__ words,
-    EppOut* __restrict__ outs) {
+    EppOut* __restrict__ outs, uint32_t lds_cap) {
+  extern __shared__ __align__(16) unsigned char epp_lds[];
   __shared__ uint32_t wsum[16];
   __shared__ uint64_t s_best[16];
   __shared__ unsigned char cur[kEppMaxN], nxt[kEppMaxN];
@@ -213,6 +232,9 @@ __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __res
       if (d[size_t(q) * k + a] == t) edges[o++] = uint16_t(q);
   }
   __syncthreads();
+  const uint32_t* lbeg = beg;
+  const uint16_t* ledges = edges;
+  epp_stage_lists(k * n, lds_cap, epp_lds, lbeg, ledges);
 
   // ---- pair BFS
   uint32_t head = 0, tail = 0;  // this level's parents: queue[head, tail) (level 0: k n list pairs)
@@ -222,7 +244,7 @@ __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __res
     const uint32_t* lvl = queue + head;
     // 1. proposals
     for (uint32_t p = tid; p < parents; p += nt)
-      candidates(level, p, n, k, beg, edges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
+      candidates(level, p, n, k, lbeg, ledges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
         const uint32_t ix = pair_ix(x, y, n);
         if (info[ix] == 0) atomicMin(&keys[ix], (unsigned long long)key);
       });
@@ -234,7 +256,7 @@ __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __res
       const uint32_t p = b0 + tid;
       uint32_t c = 0;
       if (p < parents)
-        candidates(level, p, n, k, beg, edges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
+        candidates(level, p, n, k, lbeg, ledges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
           const uint32_t ix = pair_ix(x, y, n);
           c += (info[ix] == 0 && keys[ix] == key) ? 1u : 0u;
         });
@@ -247,7 +269,7 @@ __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __res
     // 3. winners written in order; their length and first letter recorded
     for (uint32_t p = tid; p < parents; p += nt) {
       uint32_t o = wbase + cnt[p];
-      candidates(level, p, n, k, beg, edges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t a) {
+      candidates(level, p, n, k, lbeg, ledges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t a) {
         const uint32_t ix = pair_ix(x, y, n);
         if (info[ix] == 0 && keys[ix] == key) {
           info[ix] = ((level + 1) << 8) | a;
@@ -274,10 +296,14 @@ __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __res
 // scanned offsets), the same passes as the workgroup kernel above.
 __global__ __launch_bounds__(256) void k_epp_propose(uint32_t level, uint32_t parents, uint32_t n, uint32_t k,
     const uint32_t* __restrict__ beg, const uint16_t* __restrict__ edges, const uint32_t* __restrict__ lvl,
-    const uint32_t* __restrict__ info, unsigned long long* __restrict__ keys) {
+    const uint32_t* __restrict__ info, unsigned long long* __restrict__ keys, uint32_t lds_cap) {
+  extern __shared__ __align__(16) unsigned char epp_lds[];
+  const uint32_t* lb = beg;
+  const uint16_t* le = edges;
+  epp_stage_lists(k * n, lds_cap, epp_lds, lb, le);
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= parents) return;
-  candidates(level, p, n, k, beg, edges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
+  candidates(level, p, n, k, lb, le, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
     const uint32_t ix = pair_ix(x, y, n);
     if (info[ix] == 0) atomicMin(&keys[ix], (unsigned long long)key);
   });
@@ -285,11 +311,15 @@ __global__ __launch_bounds__(256) void k_epp_propose(uint32_t level, uint32_t pa
 
 __global__ __launch_bounds__(256) void k_epp_count(uint32_t level, uint32_t parents, uint32_t n, uint32_t k,
     const uint32_t* __restrict__ beg, const uint16_t* __restrict__ edges, const uint32_t* __restrict__ lvl,
-    const uint32_t* __restrict__ info, const unsigned long long* __restrict__ keys, uint32_t* __restrict__ cnt) {
+    const uint32_t* __restrict__ info, const unsigned long long* __restrict__ keys, uint32_t* __restrict__ cnt, uint32_t lds_cap) {
+  extern __shared__ __align__(16) unsigned char epp_lds[];
+  const uint32_t* lb = beg;
+  const uint16_t* le = edges;
+  epp_stage_lists(k * n, lds_cap, epp_lds, lb, le);
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= parents) return;
   uint32_t c = 0;
-  candidates(level, p, n, k, beg, edges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
+  candidates(level, p, n, k, lb, le, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
     const uint32_t ix = pair_ix(x, y, n);
     c += (info[ix] == 0 && keys[ix] == key) ? 1u : 0u;
   });
@@ -300,11 +330,15 @@ __global__ __launch_bounds__(256) void k_epp_count(uint32_t level, uint32_t pare
 __global__ __launch_bounds__(256) void k_epp_write(uint32_t level, uint32_t parents, uint32_t n, uint32_t k,
     const uint32_t* __restrict__ beg, const uint16_t* __restrict__ edges, const uint32_t* __restrict__ lvl,
     uint32_t* info, const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ off, uint32_t* queue,
-    uint32_t wbase) {
+    uint32_t wbase, uint32_t lds_cap) {
+  extern __shared__ __align__(16) unsigned char epp_lds[];
+  const uint32_t* lb = beg;
+  const uint16_t* le = edges;
+  epp_stage_lists(k * n, lds_cap, epp_lds, lb, le);
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= parents) return;
   uint32_t o = wbase + off[p];
-  candidates(level, p, n, k, beg, edges, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t a) {
+  candidates(level, p, n, k, lb, le, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t a) {
     const uint32_t ix = pair_ix(x, y, n);
     if (info[ix] == 0 && keys[ix] == key) {
       info[ix] = ((level + 1) << 8) | a;
@@ -330,10 +364,17 @@ __global__ __launch_bounds__(kEppThreads) void k_epp_greedy(const EppJob* __rest
 
 size_t eppstein_scratch_bytes(uint32_t n, uint32_t k) { return epp_layout(n, k).total; }
 
+uint32_t eppstein_list_lds(uint32_t n, uint32_t k) {
+  const size_t need = ((size_t(k) * n + 1) * 4 + size_t(k) * n * 2 + 7) & ~size_t(7);
+  return need <= kEppListLds ? uint32_t(need) : 0u;
+}
+
 hipError_t launch_eppstein_wg(const EppJob* jobs, uint32_t count, const uint32_t* deltas, unsigned char* scratch,
-    uint32_t* words, EppOut* outs, hipStream_t st) {
+    uint32_t* words, EppOut* outs, uint32_t list_lds, hipStream_t st) {
   if (!count) return hipSuccess;
-  hipLaunchKernelGGL(k_eppstein_wg, dim3(count), dim3(kEppThreads), 0, st, jobs, deltas, scratch, words, outs);
+  if (list_lds > kEppListLds) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_eppstein_wg, dim3(count), dim3(kEppThreads), list_lds, st, jobs, deltas, scratch, words, outs,
+      uint32_t(list_lds));
   return hipGetLastError();
 }
 
@@ -349,13 +390,15 @@ hipError_t launch_epp_pass(int pass, uint32_t level, uint32_t parents, uint32_t 
   uint32_t* cnt = reinterpret_cast<uint32_t*>(base + lay.cnt);
   const uint32_t* lvl = queue + head;
   const dim3 grid((parents + 255) / 256), block(256);
+  const uint32_t lds = eppstein_list_lds(n, k);
   if (pass == 0)
-    hipLaunchKernelGGL(k_epp_propose, grid, block, 0, st, level, parents, n, k, beg, edges, lvl, info, keys);
+    hipLaunchKernelGGL(k_epp_propose, grid, block, lds, st, level, parents, n, k, beg, edges, lvl, info, keys, lds);
   else if (pass == 1)
-    hipLaunchKernelGGL(k_epp_count, grid, block, 0, st, level, parents, n, k, beg, edges, lvl, info, keys, cnt);
+    hipLaunchKernelGGL(k_epp_count, grid, block, lds, st, level, parents, n, k, beg, edges, lvl, info, keys, cnt,
+        lds);
   else
-    hipLaunchKernelGGL(k_epp_write, grid, block, 0, st, level, parents, n, k, beg, edges, lvl, info, keys, off, queue,
-        wbase);
+    hipLaunchKernelGGL(k_epp_write, grid, block, lds, st, level, parents, n, k, beg, edges, lvl, info, keys, off, queue,
+        wbase, lds);
   return hipGetLastError();
 }
 

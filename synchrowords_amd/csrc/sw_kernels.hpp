@@ -163,8 +163,13 @@ __host__ __device__ inline EppLayout epp_layout(uint32_t n, uint32_t k) {
   return l;
 }
 size_t eppstein_scratch_bytes(uint32_t n, uint32_t k);
+// LDS bytes of an automaton's preimage lists, 0 when they are too large to
+// be staged there (the kernels then read them from the scratch).
+uint32_t eppstein_list_lds(uint32_t n, uint32_t k);
+// list_lds: the dynamic LDS of the launch, the largest eppstein_list_lds of
+// its jobs (a job whose lists need more reads them from the scratch).
 hipError_t launch_eppstein_wg(const EppJob* jobs, uint32_t count, const uint32_t* deltas, unsigned char* scratch,
-    uint32_t* words, EppOut* outs, hipStream_t st);
+    uint32_t* words, EppOut* outs, uint32_t list_lds, hipStream_t st);
 // The pair BFS of one automaton over the whole device, one pass per launch
 // (0 proposals, 1 winners per parent into the counts region, 2 winners
 // written at wbase + off[p]); base = its scratch (epp_layout), beg and edges
