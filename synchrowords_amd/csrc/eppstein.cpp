@@ -22,9 +22,10 @@ namespace {
 
 // One automaton over the whole device: the preimage lists built here (sources
 // ascending, as the workgroup kernel builds them), the pair BFS a level at a
-// time (three launches and a scan, the level's size read back), then the
-// greedy phase in one workgroup.  A lone automaton in one workgroup leaves
-// all but one CU idle; its BFS levels are thousands of parents wide.
+// time (six launches that read the level from a device state, enqueued
+// several levels ahead), then the greedy phase in one workgroup.  A lone
+// automaton in one workgroup leaves all but one CU idle; its BFS levels are
+// thousands of parents wide.
 void eppstein_wide(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint64_t upper, uint64_t word_cap,
     EppOut& res, uint32_t* words) {
   hipStream_t st = ctx->stream;
@@ -48,33 +49,25 @@ void eppstein_wide(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, u
   SW_CHECK_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(base + lay.keys), ~0u, size_t(2) * n * n, st));
   const size_t most = size_t(all) + kn;
   ctx->epp_off.ensure(most * 4 + 4);
-  ctx->epp_small.ensure(16);
-  size_t need = 0;
-  SW_CHECK_HIP(launch_scan(nullptr, nullptr, most, nullptr, nullptr, &need, st));
-  if (need > ctx->epp_tmp_bytes) {
-    ctx->epp_tmp.release();
-    ctx->epp_tmp.ensure(need);
-    ctx->epp_tmp_bytes = need;
-  }
-  const uint32_t* cnt = reinterpret_cast<const uint32_t*>(base + lay.cnt);
-  uint32_t* off = ctx->epp_off.as<uint32_t>();
-  uint32_t* total = ctx->epp_small.as<uint32_t>();
-  uint32_t head = 0, tail = 0;
-  uint64_t found = 0;
-  for (uint32_t level = 0;; ++level) {
-    const uint32_t parents = level == 0 ? kn : tail - head;
-    SW_CHECK_HIP(launch_epp_pass(0, level, parents, n, k, base, head, nullptr, 0, st));
-    SW_CHECK_HIP(launch_epp_pass(1, level, parents, n, k, base, head, nullptr, 0, st));
-    SW_CHECK_HIP(launch_scan(cnt, off, parents, total, ctx->epp_tmp.p, &ctx->epp_tmp_bytes, st));
-    SW_CHECK_HIP(launch_epp_pass(2, level, parents, n, k, base, head, off, tail, st));
-    uint32_t carry = 0;
-    SW_CHECK_HIP(hipMemcpyAsync(&carry, total, 4, hipMemcpyDeviceToHost, st));
+  ctx->epp_tmp.ensure(size_t(epp_scan_tiles(n, k)) * 4 + 4);
+  ctx->epp_small.ensure(sizeof(EppLevel));
+  EppLevel* state = ctx->epp_small.as<EppLevel>();
+  const EppLevel init{0, 0, 0, 0, 0};
+  SW_CHECK_HIP(hipMemcpyAsync(state, &init, sizeof(EppLevel), hipMemcpyHostToDevice, st));
+  // Levels are enqueued kChunk at a time; the state is read after each chunk.
+  // Every level finds at least one new pair, so there are at most `all`.
+  constexpr uint32_t kChunk = 6;
+  EppLevel host{};
+  for (uint32_t level = 0;;) {
+    for (uint32_t c = 0; c < kChunk; ++c, ++level)
+      SW_CHECK_HIP(launch_epp_level(level, n, k, base, ctx->epp_off.as<uint32_t>(), ctx->epp_tmp.as<uint32_t>(),
+          state, st));
+    SW_CHECK_HIP(hipMemcpyAsync(&host, state, sizeof(EppLevel), hipMemcpyDeviceToHost, st));
     SW_CHECK_HIP(hipStreamSynchronize(st));
-    found += carry;
-    head = tail;
-    tail += carry;
-    if (carry == 0 || found >= all) break;
+    if (host.done) break;
+    SW_REQUIRE(level <= all + 1, "eppstein: pair BFS did not end");
   }
+  const uint64_t found = host.found;
   if (found < all) {
     res = EppOut{1, 0, upper, 0};
     return;

@@ -22,6 +22,8 @@
 // The bound is the sum of those lengths; past `upper` the bound is unchanged.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "sw_kernels.hpp"
 
 namespace sw {
@@ -291,60 +293,156 @@ __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __res
   epp_greedy(job, n, k, d, info, words, out, cur, nxt, cs, wsum, s_best);
 }
 
-// ---- whole-device pair BFS of one automaton: a level is three launches over
-// the level's parents (proposals, winners per parent, winners written at the
-// scanned offsets), the same passes as the workgroup kernel above.
-__global__ __launch_bounds__(256) void k_epp_propose(uint32_t level, uint32_t parents, uint32_t n, uint32_t k,
-    const uint32_t* __restrict__ beg, const uint16_t* __restrict__ edges, const uint32_t* __restrict__ lvl,
-    const uint32_t* __restrict__ info, unsigned long long* __restrict__ keys, uint32_t lds_cap) {
+// ---- whole-device pair BFS of one automaton: the passes of the workgroup
+// kernel above (proposals, winners per parent, a scan of the winner counts,
+// winners written at the scanned offsets), one launch each over a grid sized
+// for the largest level, then k_epp_advance.  Every launch reads the level
+// from the device state, so the host enqueues several levels without
+// waiting; launches after the last level find `done` set and return.
+__device__ inline bool epp_level(const EppLevel* st, uint32_t level, uint32_t kn, EppLevel& s, uint32_t& parents) {
+  s = *st;
+  parents = level == 0 ? kn : s.tail - s.head;
+  return !s.done;
+}
+
+__global__ __launch_bounds__(256) void k_epp_propose(uint32_t level, uint32_t n, uint32_t k,
+    const uint32_t* __restrict__ beg, const uint16_t* __restrict__ edges, const uint32_t* __restrict__ queue,
+    const uint32_t* __restrict__ info, unsigned long long* __restrict__ keys, const EppLevel* __restrict__ st,
+    uint32_t lds_cap) {
   extern __shared__ __align__(16) unsigned char epp_lds[];
+  EppLevel s;
+  uint32_t parents;
+  if (!epp_level(st, level, k * n, s, parents)) return;
   const uint32_t* lb = beg;
   const uint16_t* le = edges;
   epp_stage_lists(k * n, lds_cap, epp_lds, lb, le);
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= parents) return;
-  candidates(level, p, n, k, lb, le, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
+  candidates(level, p, n, k, lb, le, queue + s.head, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
     const uint32_t ix = pair_ix(x, y, n);
     if (info[ix] == 0) atomicMin(&keys[ix], (unsigned long long)key);
   });
 }
 
-__global__ __launch_bounds__(256) void k_epp_count(uint32_t level, uint32_t parents, uint32_t n, uint32_t k,
-    const uint32_t* __restrict__ beg, const uint16_t* __restrict__ edges, const uint32_t* __restrict__ lvl,
-    const uint32_t* __restrict__ info, const unsigned long long* __restrict__ keys, uint32_t* __restrict__ cnt, uint32_t lds_cap) {
+__global__ __launch_bounds__(256) void k_epp_count(uint32_t level, uint32_t n, uint32_t k,
+    const uint32_t* __restrict__ beg, const uint16_t* __restrict__ edges, const uint32_t* __restrict__ queue,
+    const uint32_t* __restrict__ info, const unsigned long long* __restrict__ keys, uint32_t* __restrict__ cnt,
+    const EppLevel* __restrict__ st, uint32_t lds_cap) {
   extern __shared__ __align__(16) unsigned char epp_lds[];
+  EppLevel s;
+  uint32_t parents;
+  if (!epp_level(st, level, k * n, s, parents)) return;
   const uint32_t* lb = beg;
   const uint16_t* le = edges;
   epp_stage_lists(k * n, lds_cap, epp_lds, lb, le);
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= parents) return;
   uint32_t c = 0;
-  candidates(level, p, n, k, lb, le, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
+  candidates(level, p, n, k, lb, le, queue + s.head, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t) {
     const uint32_t ix = pair_ix(x, y, n);
     c += (info[ix] == 0 && keys[ix] == key) ? 1u : 0u;
   });
   cnt[p] = c;
 }
 
+// Exclusive scan of the level's winner counts over tiles of kEppScanTile:
+// tile sums, then each tile's entries offset by the earlier tiles' sums (one
+// wave scan per row of 256); the last tile also writes the level's total.
+constexpr uint32_t kEppScanPer = 8;
+constexpr uint32_t kEppScanTile = 256 * kEppScanPer;
+
+__device__ inline uint32_t epp_block_sum(uint32_t v, uint32_t* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const uint32_t t = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(256) void k_epp_scan_sums(uint32_t level, uint32_t kn, const uint32_t* __restrict__ cnt,
+    uint32_t* __restrict__ tile_sum, const EppLevel* __restrict__ st) {
+  __shared__ uint32_t red[4];
+  EppLevel s;
+  uint32_t parents;
+  if (!epp_level(st, level, kn, s, parents)) return;
+  const uint32_t base = blockIdx.x * kEppScanTile;
+  if (base >= parents) return;
+  uint32_t v = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < kEppScanPer; ++r) {
+    const uint32_t i = base + r * 256 + threadIdx.x;
+    v += i < parents ? cnt[i] : 0u;
+  }
+  v = epp_block_sum(v, red);
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(256) void k_epp_scan_offsets(uint32_t level, uint32_t kn,
+    const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ off,
+    EppLevel* st) {
+  __shared__ uint32_t red[4], wtot[4];
+  EppLevel s;
+  uint32_t parents;
+  if (!epp_level(st, level, kn, s, parents)) return;
+  const uint32_t base = blockIdx.x * kEppScanTile;
+  if (base >= parents) return;
+  uint32_t c = 0;
+  for (uint32_t j = threadIdx.x; j < blockIdx.x; j += 256) c += tile_sum[j];
+  uint32_t run = epp_block_sum(c, red);  // entries before the current row
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  for (uint32_t r = 0; r < kEppScanPer; ++r) {
+    const uint32_t i = base + r * 256 + threadIdx.x;
+    const uint32_t v = i < parents ? cnt[i] : 0u;
+    uint32_t x = v;  // inclusive scan within the wave
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, 64);
+      if (lane >= uint32_t(d)) x += y;
+    }
+    if (lane == 63) wtot[wv] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wv; ++w) before += wtot[w];
+    if (i < parents) off[i] = run + before + x - v;
+    run += wtot[0] + wtot[1] + wtot[2] + wtot[3];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && base + kEppScanTile >= parents) st->total = run;
+}
+
 // Only a pair's winner writes its info, and it reads info before writing.
-__global__ __launch_bounds__(256) void k_epp_write(uint32_t level, uint32_t parents, uint32_t n, uint32_t k,
-    const uint32_t* __restrict__ beg, const uint16_t* __restrict__ edges, const uint32_t* __restrict__ lvl,
-    uint32_t* info, const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ off, uint32_t* queue,
-    uint32_t wbase, uint32_t lds_cap) {
+__global__ __launch_bounds__(256) void k_epp_write(uint32_t level, uint32_t n, uint32_t k,
+    const uint32_t* __restrict__ beg, const uint16_t* __restrict__ edges, uint32_t* queue, uint32_t* info,
+    const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ off, const EppLevel* __restrict__ st,
+    uint32_t lds_cap) {
   extern __shared__ __align__(16) unsigned char epp_lds[];
+  EppLevel s;
+  uint32_t parents;
+  if (!epp_level(st, level, k * n, s, parents)) return;
   const uint32_t* lb = beg;
   const uint16_t* le = edges;
   epp_stage_lists(k * n, lds_cap, epp_lds, lb, le);
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= parents) return;
-  uint32_t o = wbase + off[p];
-  candidates(level, p, n, k, lb, le, lvl, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t a) {
+  uint32_t o = s.tail + off[p];
+  candidates(level, p, n, k, lb, le, queue + s.head, [&](uint32_t x, uint32_t y, uint64_t key, uint32_t a) {
     const uint32_t ix = pair_ix(x, y, n);
     if (info[ix] == 0 && keys[ix] == key) {
       info[ix] = ((level + 1) << 8) | a;
       queue[o++] = x | (y << 16);
     }
   });
+}
+
+// The next level: queue[tail, tail + total); done when a level found no
+// pair or every pair is found.
+__global__ void k_epp_advance(EppLevel* st, uint32_t all) {
+  if (st->done) return;
+  const uint32_t t = st->total;
+  st->found += t;
+  st->head = st->tail;
+  st->tail += t;
+  if (t == 0 || st->found >= all) st->done = 1;
 }
 
 __global__ __launch_bounds__(kEppThreads) void k_epp_greedy(const EppJob* __restrict__ jobs,
@@ -378,9 +476,8 @@ hipError_t launch_eppstein_wg(const EppJob* jobs, uint32_t count, const uint32_t
   return hipGetLastError();
 }
 
-hipError_t launch_epp_pass(int pass, uint32_t level, uint32_t parents, uint32_t n, uint32_t k, unsigned char* base,
-    uint32_t head, const uint32_t* off, uint32_t wbase, hipStream_t st) {
-  if (!parents) return hipSuccess;
+hipError_t launch_epp_level(uint32_t level, uint32_t n, uint32_t k, unsigned char* base, uint32_t* off,
+    uint32_t* tile_sum, EppLevel* st, hipStream_t st_) {
   const EppLayout lay = epp_layout(n, k);
   const uint32_t* beg = reinterpret_cast<const uint32_t*>(base);
   const uint16_t* edges = reinterpret_cast<const uint16_t*>(base + lay.edges);
@@ -388,18 +485,21 @@ hipError_t launch_epp_pass(int pass, uint32_t level, uint32_t parents, uint32_t 
   unsigned long long* keys = reinterpret_cast<unsigned long long*>(base + lay.keys);
   uint32_t* queue = reinterpret_cast<uint32_t*>(base + lay.queue);
   uint32_t* cnt = reinterpret_cast<uint32_t*>(base + lay.cnt);
-  const uint32_t* lvl = queue + head;
-  const dim3 grid((parents + 255) / 256), block(256);
+  const uint32_t kn = k * n, most = std::max(kn, n * (n - 1) / 2);
+  const dim3 grid((most + 255) / 256), block(256), tiles((most + kEppScanTile - 1) / kEppScanTile);
   const uint32_t lds = eppstein_list_lds(n, k);
-  if (pass == 0)
-    hipLaunchKernelGGL(k_epp_propose, grid, block, lds, st, level, parents, n, k, beg, edges, lvl, info, keys, lds);
-  else if (pass == 1)
-    hipLaunchKernelGGL(k_epp_count, grid, block, lds, st, level, parents, n, k, beg, edges, lvl, info, keys, cnt,
-        lds);
-  else
-    hipLaunchKernelGGL(k_epp_write, grid, block, lds, st, level, parents, n, k, beg, edges, lvl, info, keys, off, queue,
-        wbase, lds);
+  hipLaunchKernelGGL(k_epp_propose, grid, block, lds, st_, level, n, k, beg, edges, queue, info, keys, st, lds);
+  hipLaunchKernelGGL(k_epp_count, grid, block, lds, st_, level, n, k, beg, edges, queue, info, keys, cnt, st, lds);
+  hipLaunchKernelGGL(k_epp_scan_sums, tiles, block, 0, st_, level, kn, cnt, tile_sum, st);
+  hipLaunchKernelGGL(k_epp_scan_offsets, tiles, block, 0, st_, level, kn, cnt, tile_sum, off, st);
+  hipLaunchKernelGGL(k_epp_write, grid, block, lds, st_, level, n, k, beg, edges, queue, info, keys, off, st, lds);
+  hipLaunchKernelGGL(k_epp_advance, dim3(1), dim3(1), 0, st_, st, n * (n - 1) / 2);
   return hipGetLastError();
+}
+
+uint32_t epp_scan_tiles(uint32_t n, uint32_t k) {
+  const uint32_t most = std::max(k * n, n * (n - 1) / 2);
+  return (most + kEppScanTile - 1) / kEppScanTile;
 }
 
 hipError_t launch_epp_greedy(const EppJob* job, const uint32_t* deltas, const unsigned char* scratch,

@@ -102,11 +102,10 @@ struct sw_ctx {
   // Power-set BFS batches: jobs, packed automata, frontiers, lengths.
   sw::DevBuf brute_jobs, brute_delta, brute_frontier, brute_out;
   // Eppstein batches: jobs, packed automata, pair-BFS scratch, words, results;
-  // for a single automaton, the whole-device BFS's offsets, scan space and
-  // read-back.
+  // for a single automaton, the whole-device BFS's offsets, tile sums and
+  // level state.
   sw::DevBuf epp_jobs, epp_delta, epp_scratch, epp_words, epp_out;
   sw::DevBuf epp_off, epp_tmp, epp_small;
-  size_t epp_tmp_bytes = 0;
   // How the indexed kernel walks stream sets: as each launch asks (0), always
   // in set order without early exit (1), always rarest first with early exit
   // over size-sorted lists (2), always size-sorted lists walked in set order

@@ -170,13 +170,20 @@ uint32_t eppstein_list_lds(uint32_t n, uint32_t k);
 // its jobs (a job whose lists need more reads them from the scratch).
 hipError_t launch_eppstein_wg(const EppJob* jobs, uint32_t count, const uint32_t* deltas, unsigned char* scratch,
     uint32_t* words, EppOut* outs, uint32_t list_lds, hipStream_t st);
-// The pair BFS of one automaton over the whole device, one pass per launch
-// (0 proposals, 1 winners per parent into the counts region, 2 winners
-// written at wbase + off[p]); base = its scratch (epp_layout), beg and edges
-// filled, level >= 1 parents at queue[head ...).  Then the greedy phase in
-// one workgroup (job: one EppJob on the device).
-hipError_t launch_epp_pass(int pass, uint32_t level, uint32_t parents, uint32_t n, uint32_t k, unsigned char* base,
-    uint32_t head, const uint32_t* off, uint32_t wbase, hipStream_t st);
+// The pair BFS of one automaton over the whole device.  EppLevel is its
+// device state: the level's parents at queue[head, tail) (level 0: the k n
+// preimage-list pairs), pairs found so far, done, the last level's winners.
+// launch_epp_level enqueues one level (base = the automaton's scratch,
+// epp_layout, with beg and edges filled; off: n (n - 1) / 2 + k n words;
+// tile_sum: epp_scan_tiles words) without waiting; levels enqueued after the
+// last one return at once.  Then the greedy phase in one workgroup (job: one
+// EppJob on the device).
+struct EppLevel {
+  uint32_t head, tail, found, done, total;
+};
+hipError_t launch_epp_level(uint32_t level, uint32_t n, uint32_t k, unsigned char* base, uint32_t* off,
+    uint32_t* tile_sum, EppLevel* st, hipStream_t stream);
+uint32_t epp_scan_tiles(uint32_t n, uint32_t k);
 hipError_t launch_epp_greedy(const EppJob* job, const uint32_t* deltas, const unsigned char* scratch,
     uint32_t* words, EppOut* out, hipStream_t st);
 
