@@ -518,22 +518,27 @@ __device__ __forceinline__ uint64_t beam_key(const uint64_t (&x)[W], uint32_t n)
   return (uint64_t(n - c) << 55) | (x[0] >> 9);
 }
 
+#ifndef SW_BEAM_RANK_J
+#define SW_BEAM_RANK_J 256
+#endif
+constexpr uint32_t kBeamRankJ = SW_BEAM_RANK_J;  // sets j staged per workgroup
+
 template <int W>
 __global__ __launch_bounds__(256) void k_beam_rank(const uint64_t* __restrict__ sets, uint32_t count, uint32_t n,
     uint32_t* __restrict__ rank) {
-  __shared__ uint64_t js[kRankChunk * W];
-  __shared__ uint64_t jkey[kRankChunk];
+  __shared__ uint64_t js[kBeamRankJ * W];
+  __shared__ uint64_t jkey[kBeamRankJ];
   __shared__ uint32_t part[4][64];
-  const uint32_t j0 = blockIdx.y * kRankChunk;
-  const uint32_t jn = count - j0 < kRankChunk ? count - j0 : kRankChunk;
-  if (threadIdx.x < jn) {
+  const uint32_t j0 = blockIdx.y * kBeamRankJ;
+  const uint32_t jn = count - j0 < kBeamRankJ ? count - j0 : kBeamRankJ;
+  for (uint32_t t = threadIdx.x; t < jn; t += blockDim.x) {
     uint64_t x[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      x[w] = sets[uint64_t(j0 + threadIdx.x) * W + w];
-      js[threadIdx.x * W + w] = x[w];
+      x[w] = sets[uint64_t(j0 + t) * W + w];
+      js[t * W + w] = x[w];
     }
-    jkey[threadIdx.x] = beam_key<W>(x, n);
+    jkey[t] = beam_key<W>(x, n);
   }
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -544,7 +549,7 @@ __global__ __launch_bounds__(256) void k_beam_rank(const uint64_t* __restrict__ 
 #pragma unroll
     for (int w = 0; w < W; ++w) xi[w] = sets[uint64_t(i) * W + w];
     const uint64_t ki = beam_key<W>(xi, n);
-    const uint32_t tb = wv * (kRankChunk / 4), te = min(jn, tb + kRankChunk / 4);
+    const uint32_t tb = wv * (kBeamRankJ / 4), te = min(jn, tb + kBeamRankJ / 4);
     for (uint32_t t = tb; t < te; ++t) {
       const uint64_t kj = jkey[t];
       if (kj != ki) {
@@ -605,7 +610,7 @@ hipError_t beam_select_reference(const uint64_t* sets, const uint32_t* idx, uint
   if (count > kBeamRankMax || W < 1 || W > 8) return hipErrorInvalidValue;
   hipError_t e = launch_fill_u32(rank, 0u, count, st);
   if (e != hipSuccess) return e;
-  const dim3 grid(unsigned((count + 63) / 64), unsigned((count + kRankChunk - 1) / kRankChunk));
+  const dim3 grid(unsigned((count + 63) / 64), unsigned((count + kBeamRankJ - 1) / kBeamRankJ));
   switch (W) {
 #define CASE(w) \
   case w: hipLaunchKernelGGL(k_beam_rank<w>, grid, dim3(256), 0, st, sets, uint32_t(count), n, rank); break;
