@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_brute_level(uint32_t n, uint32_t k, con
   for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < fc; base += stride) {
     const uint64_t i = base + threadIdx.x;
     const uint32_t m = i < fc ? cur[i] : 0u;
-    // Count this round's new subsets in the workgroup, reserve one range.
+    // Count this pass's new subsets in the workgroup, reserve one range.
     uint32_t v[kBruteLevelMaxK];
     uint32_t newmask = 0;
 #pragma unroll
