@@ -94,18 +94,47 @@ def make_lists(count: int, n: int, p_small: float, p_large: float, seed: int):
     return small, large, planted
 
 
-def cpu_subset_baseline(small, large, rows: int) -> dict:
-    """Oracle count (single-thread port of the checker) on rows x all pairs."""
+GOLDEN_COUNT = ROOT / "tests" / "golden" / "c2_count.json"
+
+
+def host_cpu() -> dict:
+    """CPU model and the hardware threads this process may use."""
+    model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "threads_available": len(os.sched_getaffinity(0))}
+
+
+def cpu_subset_baseline(ctx, small, large, rows: int, golden: dict | None) -> dict:
+    """The oracle's all-pairs count (a port of the checker's semantics,
+    subsets_checker_kernel.cu:8-33) over every host thread, on the first
+    `rows` forward sets x all backward sets; the GPU counts the same rows and
+    both are compared, and with the committed golden count when it covers
+    the sample."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as orc
-    W = small.shape[1]
+    cpu = host_cpu()
+    threads = cpu["threads_available"]
     t0 = time.perf_counter()
-    total, _, _ = orc.subset_count(small[:rows], large, W)
+    total = orc.subset_count_mt(small[:rows], large, threads=threads)
     dt = time.perf_counter() - t0
     pairs = rows * large.shape[0]
-    return {"value": pairs / dt, "unit": "subset checks/s", "cores": 1, "kind": "port",
-            "sample": f"first {rows} forward sets x all {large.shape[0]} backward sets ({pairs} pairs, "
-                      f"{dt:.1f} s, count {total})"}
+    gpu_count = ctx.subset_check(small.shape[1] * 64, small[:rows], large, count=True)[1]
+    out = {"value": pairs / dt, "unit": "subset checks/s", "cores": threads, "kind": "port",
+           "cpu_model": cpu["model"],
+           "sample": f"first {rows} forward sets x all {large.shape[0]} backward sets ({pairs} pairs, "
+                     f"{dt:.1f} s, oracle count {total}, GPU count of the same rows {gpu_count})",
+           "sample_counts_equal": total == gpu_count}
+    if golden and large.shape[0] == golden["count"] and rows % golden["block_rows"] == 0:
+        want = sum(golden["block_counts"][:rows // golden["block_rows"]])
+        out["sample_count_equals_golden"] = total == want
+    assert total == gpu_count, (total, gpu_count)
+    return out
 
 
 def time_to_exact(ctx, n: int, seed: int) -> dict:
@@ -464,6 +493,13 @@ def main() -> int:
     ms_per_step = 1000.0 * wall / args.steps
     assert anyf, "planted subsets not found"
     assert len(set(counts)) == 1, "count differs between steps"
+    golden = json.loads(GOLDEN_COUNT.read_text()) if GOLDEN_COUNT.exists() else None
+    count_ok = None
+    if golden and count == golden["count"]:
+        # The oracle's count of all 2^40 pairs, computed once over every host
+        # thread (tests/golden/make_c2_count.py).
+        count_ok = counts[0] == golden["pair_count"]
+        assert count_ok, (counts[0], golden["pair_count"])
     pairs = count * count
     value = pairs / (ms_per_step / 1000.0)
     algo_bytes = 2 * count * 32  # both lists read once: (|X| + |Y|) * 8W
@@ -486,7 +522,8 @@ def main() -> int:
         "data": "synthetic (seeded Bernoulli bitsets, planted subsets)",
         "config": {"workload": f"subset-check microbench: {count} x {count} sets, n={n} (W=4), p=0.08/0.75, "
                                f"1 in 1e5 planted",
-                   "pair_count": counts[0], "exists": bool(anyf), "device": info["name"], "cus": info["cus"],
+                   "pair_count": counts[0], "pair_count_equals_oracle": count_ok,
+                   "exists": bool(anyf), "device": info["name"], "cus": info["cus"],
                    "launch_shape": tuned or "default", "subset_kernel": args.subset_kernel},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.traffic_profile),
@@ -498,8 +535,9 @@ def main() -> int:
         "lds_roofline": lds_roofline(ctx, small, count, kernel_ms, args.subset_kernel != "bitwalk"),
     }
     if not args.no_cpu:
-        rows = 256 if args.quick else 4096
-        line["cpu_baseline"] = cpu_subset_baseline(small, large, min(rows, count))
+        # About 2^38 pairs on the full lists: 10-30 s over the host's threads.
+        rows = 256 if args.quick else (1 << 18)
+        line["cpu_baseline"] = cpu_subset_baseline(ctx, small, large, min(rows, count), golden)
     if not args.no_exact:
         try:
             line["time_to_exact"] = time_to_exact(ctx, 300, 1)

@@ -25,6 +25,7 @@
 // reference's own trace, and word replay.
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -35,6 +36,7 @@
 #include <queue>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace oracle {
@@ -1475,6 +1477,63 @@ uint64_t oracle_subset_count(const uint64_t* small, uint64_t ns, const uint64_t*
     total += c;
   }
   return total;
+}
+
+// The same total count as oracle_subset_count (not proper, no flags), over
+// all host threads: the large list is transposed to one array per word so
+// that the inner loop over large sets is a plain vector loop, blocked so a
+// block of it stays in cache while a block of small sets passes over it.
+// Per-row counts go to `cnt` (may be NULL).  threads = 0: all hardware
+// threads.  Used for the full 2^20 x 2^20 microbench count and as the
+// all-core CPU baseline.
+__attribute__((optimize("O3"))) uint64_t oracle_subset_count_mt(const uint64_t* small, uint64_t ns, const uint64_t* large, uint64_t nl, uint32_t W,
+    uint32_t threads, uint32_t* cnt) {
+  if (!threads) threads = std::max(1u, std::thread::hardware_concurrency());
+  std::vector<uint64_t> soa(size_t(W) * nl);
+  for (uint64_t j = 0; j < nl; ++j)
+    for (uint32_t w = 0; w < W; ++w) soa[size_t(w) * nl + j] = large[j * W + w];
+  constexpr uint64_t kSmallBlock = 64, kLargeBlock = 4096;
+  std::atomic<uint64_t> next{0}, total{0};
+  auto work = [&]() {
+    std::vector<uint32_t> local(kSmallBlock);
+    uint64_t sum = 0;
+    for (;;) {
+      const uint64_t i0 = next.fetch_add(kSmallBlock);
+      if (i0 >= ns) break;
+      const uint64_t i1 = std::min(ns, i0 + kSmallBlock);
+      std::fill(local.begin(), local.end(), 0u);
+      for (uint64_t j0 = 0; j0 < nl; j0 += kLargeBlock) {
+        const uint64_t j1 = std::min(nl, j0 + kLargeBlock);
+        for (uint64_t i = i0; i < i1; ++i) {
+          const uint64_t* s = small + i * W;
+          uint32_t c = 0;
+          if (W == 4) {
+            const uint64_t s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3];
+            const uint64_t *l0 = &soa[0], *l1 = &soa[nl], *l2 = &soa[2 * nl], *l3 = &soa[3 * nl];
+            for (uint64_t j = j0; j < j1; ++j)
+              c += ((s0 & ~l0[j]) | (s1 & ~l1[j]) | (s2 & ~l2[j]) | (s3 & ~l3[j])) == 0;
+          } else {
+            for (uint64_t j = j0; j < j1; ++j) {
+              uint64_t miss = 0;
+              for (uint32_t w = 0; w < W; ++w) miss |= s[w] & ~soa[size_t(w) * nl + j];
+              c += miss == 0;
+            }
+          }
+          local[i - i0] += c;
+        }
+      }
+      for (uint64_t i = i0; i < i1; ++i) {
+        sum += local[i - i0];
+        if (cnt) cnt[i] = local[i - i0];
+      }
+    }
+    total.fetch_add(sum);
+  };
+  std::vector<std::thread> pool;
+  for (uint32_t t = 1; t < threads; ++t) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  return total.load();
 }
 
 // Per-state frequency over a list (subset.hpp:123-136).

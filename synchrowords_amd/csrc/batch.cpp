@@ -235,6 +235,7 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
           uint32_t(rest.size()));
       std::atomic<uint32_t> next{0};
       std::vector<std::string> errors(workers);
+      std::vector<int> codes(workers, SW_OK);
       std::vector<double> busy_ms(workers, 0.0);
       std::vector<std::thread> pool;
       for (uint32_t t = 0; t < workers; ++t)
@@ -265,15 +266,23 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
             }
           } catch (const Error& e) {
             errors[t] = e.msg;
+            codes[t] = e.code;
           } catch (const std::exception& e) {
             errors[t] = e.what();
+            codes[t] = SW_ERR_INVALID;
           }
           if (wctx && wctx != ctx) sw_ctx_destroy(wctx);
         });
       for (auto& th : pool) th.join();
       for (double b : busy_ms) st.ms_overflow_search_sum += b;
-      for (auto& e : errors)
-        if (!e.empty()) throw Error{SW_ERR_HIP, "batch overflow path: " + e};
+      // The first worker's failure by severity: a memory budget end
+      // (SW_ERR_NOMEM) is reported as such, not as a device error.
+      int worst = -1;
+      for (uint32_t t = 0; t < workers; ++t) {
+        if (codes[t] == SW_OK) continue;
+        if (worst < 0 || (codes[t] == SW_ERR_NOMEM && codes[worst] != SW_ERR_NOMEM)) worst = int(t);
+      }
+      if (worst >= 0) throw Error{codes[worst], "batch overflow path: " + errors[worst]};
       st.found += uint32_t(rest.size());
     }
     st.ms_overflow = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t2).count();

@@ -146,7 +146,13 @@ class Search {
       if (F_.frozen) fwd = false;
       else fwd = F_.fcount <= B_.fcount;
       ++t;
-      step(fwd ? F_ : B_, !fwd, t);
+      try {
+        step(fwd ? F_ : B_, !fwd, t);
+      } catch (const BudgetExceeded&) {
+        --t;  // lengths up to t were checked
+        budget_end_ = true;
+        break;
+      }
       const auto g0 = std::chrono::steady_clock::now();
       const bool met = goal_check(&wit);
       const double gms = ms_since(g0);
@@ -179,7 +185,7 @@ class Search {
       r.length = t;
       out.word = reconstruct(wit);
       r.word_len = out.word.size();
-    } else if (t + 1 >= upper || F_.fcount == 0 || B_.fcount == 0) {
+    } else if (t + 1 >= upper || (!budget_end_ && (F_.fcount == 0 || B_.fcount == 0))) {
       // No word shorter than the upper bound: the bound is the length.
       r.found = 1;
       r.bound_word = 1;
@@ -192,7 +198,12 @@ class Search {
     return out;
   }
 
-  void set_memory_error() { throw Error{SW_ERR_NOMEM, "exact search: device memory budget exceeded"}; }
+  // The memory budget is not an error: a step that would pass it ends the
+  // search with the lengths checked so far as a lower bound, as the
+  // reference's meet in the middle does (meet_in_the_middle.cpp:98-102,
+  // exact.cpp:75-96: "mlsw >= x").
+  struct BudgetExceeded {};
+  [[noreturn]] void set_memory_error() { throw BudgetExceeded{}; }
 
   // Forward levels only (distinct images of Q by depth, dominance-pruned
   // frontiers), copied to `out`; level_end[d] = rows after depth d.
@@ -219,6 +230,7 @@ class Search {
   bool small_sets_ = false;
   static constexpr uint64_t kSmallSetsDominanceBytes = 2ull << 30;
   bool goal_armed_ = false;  // the last commit reset the goal flag and witness
+  bool budget_end_ = false;  // the search ended at the memory budget
   uint64_t upper_ = ~0ull;     // the search's upper bound (none: forward levels only)
 
   uint64_t pair_tests_ = 0;
@@ -298,7 +310,9 @@ class Search {
     if (S.tcap >= need) return;
     uint64_t cap = 1024;
     while (cap < need) cap <<= 1;
-    SW_REQUIRE(cap <= (1ull << 32), "exact search: hash table over 2^32 slots");
+    // Slot indices must stay below kEmpty (0xffffffff), which marks a
+    // candidate that claimed no slot.
+    if (cap > (1ull << 31)) set_memory_error();
     S.table.ensure(cap * 4);
     S.tcap = cap;
     SW_CHECK_HIP(launch_fill_u32(S.table.as<uint32_t>(), kEmpty, cap, st_));
@@ -307,7 +321,7 @@ class Search {
 
   void ensure_rows(Side& S, uint64_t rows) {
     if (rows * row_bytes() > mem_cap_ / 4) set_memory_error();
-    SW_REQUIRE(rows < 0x7fffffffull, "exact search: more than 2^31 sets on one side");
+    if (rows >= 0x7fffffffull) set_memory_error();  // rows are 31-bit
     S.arena.ensure(rows * row_bytes(), st_, S.rows * row_bytes());
     S.parent.ensure(rows * 4, st_, S.rows * 4);
     S.letter.ensure(rows, st_, S.rows);
@@ -375,7 +389,7 @@ class Search {
     const auto t0 = std::chrono::steady_clock::now();
     const uint64_t m = S.fcount;
     const uint64_t cnt = m * k_;
-    SW_REQUIRE(cnt < 0x7fffffffull, "exact search: step over 2^31 candidates");
+    if (cnt >= 0x7fffffffull) set_memory_error();  // candidate indices are 31-bit
     if ((S.rows + cnt) * row_bytes() * 2 > mem_cap_) set_memory_error();
     cand_.ensure(cnt * row_bytes());
     valid_.ensure(cnt * 4);

@@ -508,14 +508,15 @@ __global__ void k_neg_size(const uint64_t* __restrict__ sets, uint64_t count, ui
 // and two gathers take seven launches or more; past kBeamRankMax the
 // quadratic count is left to the sort (beam_reference_order).
 
-// Sort key of a set: (n - |S|) in the top 9 bits, then the top 55 bits of
-// word 0; sets with equal keys are ordered by all their words.
+// Sort key of a set: (n - |S|) in the top 10 bits (n <= 512, so the empty
+// set's 512 still fits), then the top 54 bits of word 0; sets with equal keys
+// are ordered by all their words.
 template <int W>
 __device__ __forceinline__ uint64_t beam_key(const uint64_t (&x)[W], uint32_t n) {
   uint32_t c = 0;
 #pragma unroll
   for (int w = 0; w < W; ++w) c += uint32_t(__popcll(x[w]));
-  return (uint64_t(n - c) << 55) | (x[0] >> 9);
+  return (uint64_t(n - c) << 54) | (x[0] >> 10);
 }
 
 #ifndef SW_BEAM_RANK_J

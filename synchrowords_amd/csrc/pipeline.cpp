@@ -2,8 +2,12 @@
 // (jit/jitmain.cpp:11-49) over a config-JSON plan (configs/exact.json).
 //
 // Brute, Eppstein and Beam are the bound suppliers that sit before Exact in
-// the reference's plans; they run on the host here as they do in the
-// reference.  Exact runs on the device (exact.cpp).
+// the reference's plans.  All four run on the device: Brute as a power-set
+// BFS (brute.cpp), Eppstein's pair tree and greedy word (eppstein.cpp), the
+// Beam in the reference's order (mlp.cpp, sw_beam_device) and Exact
+// (exact.cpp).  This file parses the plan, evaluates its C++ expressions,
+// keeps [lower, upper] and the word, and does the Beam's presort and dynamic
+// sizing on the host.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -435,7 +439,11 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
       if (open) {
         sw_exact_config ec{};
         ec.dfs = eval_bool(cfg->get("dfs"), true, n, k) ? 1 : 0;
+        // The device budget: gpu_max_memory_mb when the plan gives one, else
+        // the reference's max_memory_mb (exact/config.hpp:24-46; 12 GiB in
+        // configs/exact.json).
         const Json* mm = cfg->get("gpu_max_memory_mb");
+        if (!mm) mm = cfg->get("max_memory_mb");
         if (mm) ec.max_memory_mb = uint64_t(eval_num(mm, "0", n, k));
         const Json* vb = cfg->get("verbose");
         ec.verbose = vb ? eval_bool(vb, false, n, k) : 0;

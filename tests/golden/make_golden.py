@@ -144,15 +144,22 @@ def make_readme():
     i = 0
     idx = 0
     expected = {0: [0, 0], 1: None, 2: [3, 3], 3: [18, 18]}  # docs/install.md:101-105
+    # The word printed for automaton 3 (docs/install.md:104) is Eppstein's
+    # greedy word (find_word: true in readme_config.json); Exact returns none.
+    doc_line = (REFERENCE / "docs" / "install.md").read_text().splitlines()[103]
+    doc_word = [int(x) for x in doc_line[doc_line.index("{") + 1:doc_line.index("}")].split()]
     while i < len(vals):
         k, n = vals[i], vals[i + 1]
         d = vals[i + 2:i + 2 + n * k]
         i += 2 + n * k
         build(n, k)
         c = case(f"readme{idx}", n, k, d, docs=expected[idx])
+        if idx == 3:
+            c["docs_eppstein_word"] = doc_word
         cases.append(c)
         idx += 1
-    write("readme.json", cases, "data/readme_input.txt; docs = docs/install.md:101-105 (None: NON SYNCHRO)")
+    write("readme.json", cases, "data/readme_input.txt; docs = docs/install.md:101-105 (None: NON SYNCHRO); "
+          "docs_eppstein_word = the word of docs/install.md:104")
 
 
 def make_misc():

@@ -119,3 +119,16 @@ def count_bits(sets, n):
     acc = np.zeros(n, dtype=np.uint64)
     lib().oracle_count_bits(_p(s), C.c_uint64(s.shape[0]), n, _p(acc))
     return acc
+
+
+def subset_count_mt(small, large, threads=0, per_row=False):
+    """All-pairs count over host threads (same semantics as subset_count, not
+    proper): the total, and per-row counts when asked."""
+    s = np.ascontiguousarray(small, dtype=np.uint64)
+    lg = np.ascontiguousarray(large, dtype=np.uint64)
+    W = s.shape[1]
+    cnt = np.zeros(s.shape[0], dtype=np.uint32) if per_row else None
+    f = lib().oracle_subset_count_mt
+    f.restype = C.c_uint64
+    total = f(_p(s), C.c_uint64(s.shape[0]), _p(lg), C.c_uint64(lg.shape[0]), W, threads, _p(cnt))
+    return (int(total), cnt) if per_row else int(total)

@@ -116,3 +116,14 @@ def test_eppstein_batch(ctx):
         assert bool(ns[j]) == w[0] and int(bounds[j]) == w[1], j
         if not ns[j] and w[2]:
             assert words[j] == w[2], j
+
+
+def test_eppstein_word_matches_reference_docs(ctx):
+    """README automaton 3: the reference prints Eppstein's 25-letter word
+    (docs/install.md:104); the device pair tree and greedy phase give the
+    same word letter for letter."""
+    case = [c for c in golden("readme.json") if c["name"] == "readme3"][0]
+    n, k, d = case["n"], case["k"], case["delta"]
+    ns, bound, word = ctx.eppstein(n, k, d, (n - 1) ** 2 + 1)
+    assert not ns and bound == 25
+    assert word == case["docs_eppstein_word"]

@@ -35,7 +35,8 @@ def test_exact_trivial_bound(ctx, case):
         _check_word(case, r.length, r.word)
 
 
-@pytest.mark.parametrize("case", _cases("c1_random48.json", "readme.json"))
+@pytest.mark.parametrize("case", _cases("c1_random48.json", "readme.json", "c3_random200.json",
+                                        "c5_random300.json"))
 def test_exact_with_beam_bound(ctx, case):
     """Exact given the reference's Beam bound: proves it optimal (or finds shorter)."""
     n, k = case["n"], case["k"]
@@ -43,12 +44,14 @@ def test_exact_with_beam_bound(ctx, case):
         pytest.skip("Brute decides n <= 15")
     r = ctx.exact(n, k, case["delta"], upper=case["beam"])
     assert r.found and r.length == case["upper"]
+    if not r.bound_word:
+        _check_word(case, r.length, r.word)
 
 
 def _pipeline_cases():
     out = []
     for nm in ("c1_random48.json", "readme.json", "cerny.json", "hard.json", "slowly_sink.json",
-               "random_misc.json", "c3_random200.json"):
+               "random_misc.json", "c3_random200.json", "c5_random300.json"):
         try:
             out += [pytest.param(c, id=c["name"]) for c in golden(nm)]
         except FileNotFoundError:
@@ -111,7 +114,7 @@ def test_device_beam_wide_crosses_the_counted_order(ctx, n, k, width):
         assert len(w) == b and replay(n, k, d, w) == 1
 
 
-@pytest.mark.parametrize("n,k", [(20, 2), (48, 2), (64, 3), (100, 2), (200, 2), (300, 2), (130, 4)])
+@pytest.mark.parametrize("n,k", [(20, 2), (48, 2), (64, 3), (100, 2), (200, 2), (300, 2), (130, 4), (512, 2)])
 @pytest.mark.parametrize("width", [1, 7, 64, 500])
 def test_device_beam_matches_reference_order(ctx, n, k, width):
     """The device beam in the reference's order gives the oracle's Beam bound
@@ -125,3 +128,24 @@ def test_device_beam_matches_reference_order(ctx, n, k, width):
         assert b == want, (seed, b, want)
         if b < cap:
             assert len(w) == b and replay(n, k, d, w) == 1
+
+
+@pytest.mark.parametrize("case", _cases("c5_random300.json"))
+def test_learned_bound_exact_search(ctx, case):
+    """The learned-bound search of the n=300 workload: the committed network
+    orders a device beam (width 64), its bound and word cap the exact
+    search, which ends at the reference's length with a word that replays."""
+    import synchrowords_amd as sw
+    from pathlib import Path
+    from synchrowords_amd.train_scorer import load_scorer
+    n, k, d = case["n"], case["k"], case["delta"]
+    m = load_scorer(ctx, Path(sw.__file__).parent / "models" / "scorer_n300_k2.npz")
+    try:
+        bound, bword = m.beam(d, 64, (n - 1) ** 2 + 1)
+    finally:
+        m.close()
+    assert len(bword) == bound and replay(n, k, d, bword) == 1
+    assert bound >= case["upper"]
+    r = ctx.exact(n, k, d, upper=bound, upper_word=bword)
+    assert r.found and r.length == case["upper"]
+    _check_word(case, r.length, r.word)

@@ -142,3 +142,19 @@ def test_autotuned_shape_keeps_results():
         assert got == total and anyf
     finally:
         c.close()
+
+
+def test_microbench_count_matches_oracle_golden(ctx):
+    """The full subset-check microbench (2^20 x 2^20 sets, n = 256): the
+    device count of all 2^40 pairs equals the oracle's all-pairs count
+    committed in tests/golden/c2_count.json, and the planted subsets make the
+    existence flag true."""
+    import json
+    import bench
+    from conftest import GOLDEN
+    g = json.loads((GOLDEN / "c2_count.json").read_text())
+    small, large, _ = bench.make_lists(g["count"], g["n"], g["p_small"], g["p_large"], seed=g["seed"])
+    _, cnt, anyf = ctx.subset_check(g["n"], small, large, count=True)
+    assert anyf and cnt == g["pair_count"]
+    _, cnt, _ = ctx.subset_check(g["n"], small[:g["block_rows"]], large, count=True)
+    assert cnt == g["block_counts"][0]

@@ -132,3 +132,48 @@ def test_apply_against_definition():
             pr = {p for p in range(n) if int(d[p, a]) in set(members)}
             got = {q for q in range(n) if (int(pre[a * 40 + i, q // 64]) >> (q % 64)) & 1}
             assert got == pr
+
+
+def test_count_bits_against_unpackbits():
+    """oracle_count_bits (subset.hpp:123-136) against numpy's bit unpacking,
+    across word boundaries and with the tail bits above n clear."""
+    rng = np.random.default_rng(11)
+    for n in (1, 5, 63, 64, 65, 130, 256, 300, 512):
+        sets = random_sets(rng, 257, n, 0.37)
+        got = orc.count_bits(sets, n)
+        bits = np.unpackbits(sets.view(np.uint8), axis=1, bitorder="little")
+        want = bits.sum(axis=0, dtype=np.uint64)[:n]
+        np.testing.assert_array_equal(got, want)
+        assert bits[:, n:].sum() == 0
+
+
+def test_eppstein_word_matches_reference_docs():
+    """The reference prints Eppstein's greedy word for README automaton 3
+    (docs/install.md:104, 25 letters): the oracle's pair tree and greedy
+    choice give the same word, letter for letter."""
+    case = [c for c in golden("readme.json") if c["name"] == "readme3"][0]
+    n, k, d = case["n"], case["k"], case["delta"]
+    ns, bound, w = orc.eppstein(n, k, d, (n - 1) ** 2 + 1)
+    assert not ns
+    assert w == case["docs_eppstein_word"]
+    assert bound == len(w) == 25 == case["eppstein"]
+
+
+def test_subset_count_mt_against_numpy():
+    """The threaded all-pairs count (the microbench's full count) against a
+    numpy all-pairs test and the scalar count, per row."""
+    rng = np.random.default_rng(12)
+    for n in (7, 64, 200, 256, 300):
+        W = words(n)
+        a = random_sets(rng, 150, n, 0.08)
+        b = random_sets(rng, 3000, n, 0.75)
+        a[7] = b[11]
+        a[8] = 0
+        m = np.ones((150, 3000), dtype=bool)
+        for w in range(W):
+            m &= (a[:, w:w + 1] & ~b[None, :, w]) == 0
+        for threads in (1, 3, 0):
+            total, per_row = orc.subset_count_mt(a, b, threads=threads, per_row=True)
+            assert total == m.sum()
+            np.testing.assert_array_equal(per_row, m.sum(axis=1))
+        assert per_row[8] == 3000  # the empty set lies in every set
