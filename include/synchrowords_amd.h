@@ -117,6 +117,24 @@ void sw_leaf_destroy(sw_leaf* leaf);
 int sw_subset_check(sw_ctx* ctx, uint32_t n, const uint64_t* small, uint64_t small_cnt, const uint64_t* large,
     uint64_t large_cnt, uint32_t thresh, int where, uint8_t* results, uint64_t* pair_count, int32_t* any);
 
+/* The same checker for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  small and large are device
+ * pointers; the outputs are device pointers too and each may be NULL:
+ * results[i] = (cnt_i >= thresh) as one byte per small set, *pair_count = the
+ * total count, *any = whether any pair exists (with results and pair_count
+ * both NULL the kernel stops at the first pair).  Asynchronous on the
+ * context's stream (read the outputs after sw_sync); it calls no hipMalloc,
+ * hipFree, hipMemset or synchronisation.  Its scratch is a static device
+ * array of 16 MiB + 256 B per GPU, held until the process ends, so
+ * small_cnt is limited to 4194304 (2^22) when results are asked for
+ * (SW_ERR_INVALID beyond).  Every call on one GPU shares that scratch: calls
+ * that overlap on one GPU race.  Uses the bit-walking kernel (no stream
+ * index). */
+int sw_subset_check_noalloc(sw_ctx* ctx, uint32_t n, const uint64_t* small, uint64_t small_cnt,
+    const uint64_t* large, uint64_t large_cnt, uint32_t thresh, uint8_t* results, uint64_t* pair_count,
+    int32_t* any);
+
 /* Per-state frequency over a list: acc[q] = #{i : q in sets[i]} (acc host,
  * n entries).  Replaces count_bits_kernel<N>(subsets, acc, cnt)
  * (synchrolib/data_structures/cuda/count_bits_kernel.hpp:9-10), with 64-bit

@@ -99,6 +99,8 @@ _SIGS = {
     "sw_leaf_destroy": (None, [_P]),
     "sw_subset_check": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, _P, C.c_uint64, C.c_uint32, C.c_int, _P, _U64P,
                                   C.POINTER(C.c_int32)]),
+    "sw_subset_check_noalloc": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, _P, C.c_uint64, C.c_uint32, _P, _P,
+                                          _P]),
     "sw_count_bits": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, _P, C.c_int]),
     "sw_exact_solve": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint64, C.c_uint64, _P, C.c_uint64,
                                  C.POINTER(_ExactConfig), C.POINTER(_ExactResult), _P, C.c_uint64]),
@@ -434,6 +436,16 @@ class Context:
         _check(load().sw_subset_check(self.handle, n, small.ptr, small_cnt, large.ptr, large_cnt, 1, SW_DEVICE,
                                       None, C.byref(cnt) if count else None, C.byref(anyf)))
         return (cnt.value if count else None), bool(anyf.value)
+
+    def subset_check_noalloc(self, n: int, small: DeviceArray, small_cnt: int, large: DeviceArray, large_cnt: int,
+                             thresh: int = 1, results: DeviceArray | None = None,
+                             pair_count: DeviceArray | None = None, any_out: DeviceArray | None = None) -> None:
+        """The checker without allocation (sw_subset_check_noalloc): every
+        output is device memory, written asynchronously on the stream."""
+        _check(load().sw_subset_check_noalloc(self.handle, n, small.ptr, small_cnt, large.ptr, large_cnt, thresh,
+                                              results.ptr if results else None,
+                                              pair_count.ptr if pair_count else None,
+                                              any_out.ptr if any_out else None))
 
     def beam(self, n: int, k: int, delta, width: int, upper: int, mlp: "Mlp | None" = None):
         """(bound, word) of a device beam: cardinality order, or scored by `mlp`."""

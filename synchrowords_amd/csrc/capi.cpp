@@ -87,6 +87,7 @@ int sw_ctx_create(int device, sw_ctx** out) {
   ctx->name = prop.gcnArchName;
   ctx->cus = prop.multiProcessorCount;
   ctx->mem = prop.totalGlobalMem;
+  sw::noalloc_init(device);  // the no-allocation entry point's scratch address
   *out = ctx.release();
   SW_API_END
 }

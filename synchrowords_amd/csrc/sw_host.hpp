@@ -122,6 +122,10 @@ namespace sw {
 // Make the context's device current.
 void use(sw_ctx* ctx);
 
+// Look up the device address of the no-allocation entry point's static
+// scratch on `device` (current); once per device (noalloc.hip).
+void noalloc_init(int device);
+
 struct VsubParams;
 // Fill the launch shape of a subset-kernel call from the context's tuning
 // table (the nearest tuned size class of the same W), if any.
