@@ -141,6 +141,34 @@ int sw_subset_check_noalloc(sw_ctx* ctx, uint32_t n, const uint64_t* small, uint
  * counts. */
 int sw_count_bits(sw_ctx* ctx, uint32_t n, const uint64_t* sets, uint64_t cnt, uint64_t* acc, int where);
 
+/* Subset trie over a list of sets, resident on the device: the reference's
+ * SubsetsTrie<N,Threads> (synchrolib/data_structures/subsets_trie.hpp:18-468),
+ * which its DFS builds over the forward frontier (algorithm/exact/dfs.cpp:155)
+ * and queries with backward sets.  Nodes split on the state held by the
+ * most of their sets that not all hold (lowest state on ties,
+ * subsets_trie.hpp:299-350); leaves hold at most 8 sets (any number at depth
+ * 48); each node keeps its smallest and largest cardinality.
+ * sw_trie_build == SubsetsTrie::build(vec) (subsets_trie.hpp:57-70), on the
+ * device; the sets are copied (host) or read (device), not kept. */
+typedef struct sw_trie sw_trie;
+int sw_trie_build(sw_ctx* ctx, uint32_t n, const uint64_t* sets, uint64_t count, int where, sw_trie** out);
+/* For each query, whether some trie set lies inside it (superset == 0: the
+ * reference's contains_subset_of, subsets_trie.hpp:76-89,353-468, per query)
+ * or contains it (superset == 1).  results (host, one byte per query) may be
+ * NULL: then the walk stops at the first hit, *any says whether there was one
+ * and *witness = (query index << 32) | row of the trie set in the built list
+ * (~0 when none).  With results, *any = OR of the results. */
+int sw_trie_query(sw_trie* t, int superset, const uint64_t* queries, uint64_t count, int where, uint8_t* results,
+    int32_t* any, uint64_t* witness);
+/* Sets, node ids used and level-synchronous build passes of the last build. */
+int sw_trie_info(sw_trie* t, uint64_t* sets, uint64_t* nodes, uint32_t* top_levels);
+/* Copies of the trie (host): node records as 4 x u32 each (internal: zero
+ * child, one child, bit | minpop << 16, maxpop; leaf: first set, count,
+ * 0xffff | minpop << 16, maxpop), the sets in trie order and their rows in
+ * the built list.  Any output may be NULL. */
+int sw_trie_nodes(sw_trie* t, uint32_t* nodes, uint64_t cap_nodes, uint64_t* sets, uint32_t* rows);
+void sw_trie_destroy(sw_trie* t);
+
 /* ------------------------------------------------ algorithm-level entry points */
 
 /* Exact search settings (the reference's ExactConfig keys,
@@ -152,8 +180,20 @@ typedef struct sw_exact_config {
                                  at least 10 steps left below the upper bound;
                                  for n <= 192 only once a side holds 2 GiB) */
   uint64_t forward_cap;       /* forward frontier size that freezes the forward side (0: default) */
-  int32_t dfs;                /* allow the backward-only phase after the forward side freezes */
+  int32_t dfs;                /* when a step would pass the memory budget, deepen the backward side
+                                 in parts that fit (the reference's DFS, dfs.cpp:63-347) instead of
+                                 ending with a lower bound */
   int32_t verbose;            /* per-step log on stderr */
+  uint64_t dfs_min_list_size; /* smallest backward part of that phase (0: 8192, the reference's
+                                 default, config.hpp:24-46) */
+  int32_t strict_memory_limit;/* 1: when even parts of dfs_min_list_size sets do not fit the budget,
+                                 end with a lower bound (the reference's STRICT_MEMORY_LIMIT);
+                                 0: use that size anyway */
+  int32_t goal_check;         /* meet checks: 0 automatic (the subset kernel; the trie over the
+                                 forward frontier once it is reused: frozen forward side, runs of
+                                 backward steps, the deepening phase), 1 always the vertical subset
+                                 kernel, 2 always the forward trie, 3 tries over both frontiers
+                                 (the one that did not step) */
 } sw_exact_config;
 
 typedef struct sw_exact_result {
@@ -169,6 +209,13 @@ typedef struct sw_exact_result {
   double ms_expand;     /* expansion, dedup and commit of new levels */
   double ms_dominance;  /* dominance pruning of new levels */
   double ms_goal;       /* meet checks between the frontiers */
+  int32_t dfs_entered;  /* 1: the search deepened the backward side in parts (memory budget) */
+  int32_t out_of_memory;/* 1: it ended at the budget; `length` is a lower bound */
+  uint64_t dfs_calls;   /* parts expanded by that phase */
+  uint64_t dfs_sets;    /* sets it generated */
+  uint64_t trie_builds; /* device subset tries built for meet checks */
+  uint64_t trie_queries;/* sets queried against them */
+  double ms_dfs;        /* wall time of that phase */
 } sw_exact_result;
 
 /* Shortest reset word of a synchronizing automaton, given bounds

@@ -42,7 +42,8 @@ def words(n: int) -> int:
 
 class _ExactConfig(C.Structure):
     _fields_ = [("max_memory_mb", C.c_uint64), ("dominance_pairs", C.c_uint64), ("forward_cap", C.c_uint64),
-                ("dfs", C.c_int32), ("verbose", C.c_int32)]
+                ("dfs", C.c_int32), ("verbose", C.c_int32), ("dfs_min_list_size", C.c_uint64),
+                ("strict_memory_limit", C.c_int32), ("goal_check", C.c_int32)]
 
 
 class _ExactResult(C.Structure):
@@ -50,7 +51,9 @@ class _ExactResult(C.Structure):
                 ("steps_forward", C.c_uint64), ("steps_backward", C.c_uint64), ("sets_forward", C.c_uint64),
                 ("sets_backward", C.c_uint64), ("pair_tests", C.c_uint64), ("hash_probes", C.c_uint64),
                 ("ms_total", C.c_double), ("ms_expand", C.c_double), ("ms_dominance", C.c_double),
-                ("ms_goal", C.c_double)]
+                ("ms_goal", C.c_double), ("dfs_entered", C.c_int32), ("out_of_memory", C.c_int32),
+                ("dfs_calls", C.c_uint64), ("dfs_sets", C.c_uint64), ("trie_builds", C.c_uint64),
+                ("trie_queries", C.c_uint64), ("ms_dfs", C.c_double)]
 
 
 class _Result(C.Structure):
@@ -102,6 +105,11 @@ _SIGS = {
     "sw_subset_check_noalloc": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, _P, C.c_uint64, C.c_uint32, _P, _P,
                                           _P]),
     "sw_count_bits": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, _P, C.c_int]),
+    "sw_trie_build": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, C.c_int, C.POINTER(_P)]),
+    "sw_trie_query": (C.c_int, [_P, C.c_int, _P, C.c_uint64, C.c_int, _P, C.POINTER(C.c_int32), _U64P]),
+    "sw_trie_info": (C.c_int, [_P, _U64P, _U64P, _U32P]),
+    "sw_trie_nodes": (C.c_int, [_P, _P, C.c_uint64, _P, _P]),
+    "sw_trie_destroy": (None, [_P]),
     "sw_exact_solve": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint64, C.c_uint64, _P, C.c_uint64,
                                  C.POINTER(_ExactConfig), C.POINTER(_ExactResult), _P, C.c_uint64]),
     "sw_pipeline_solve": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_char_p, C.POINTER(_Result), _P,
@@ -187,10 +195,14 @@ class ExactConfig:
     forward_cap: int = 0
     dfs: bool = True
     verbose: bool = False
+    dfs_min_list_size: int = 0
+    strict_memory_limit: bool = False
+    goal_check: int = 0  # 0 automatic, 1 vertical subset kernel, 2 device subset trie
 
     def _c(self) -> _ExactConfig:
         return _ExactConfig(self.max_memory_mb, self.dominance_pairs, self.forward_cap, int(self.dfs),
-                            int(self.verbose))
+                            int(self.verbose), self.dfs_min_list_size, int(self.strict_memory_limit),
+                            self.goal_check)
 
 
 @dataclass
@@ -271,6 +283,52 @@ class Leaf:
         if self.handle and self.ctx.handle:
             load().sw_leaf_destroy(self.handle)
         self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Trie:
+    """Device subset trie over a list of sets (sw_trie_build)."""
+
+    def __init__(self, ctx: "Context", n: int, sets):
+        self.ctx, self.n = ctx, n
+        s = _sets(sets, n)
+        h = _P()
+        _check(load().sw_trie_build(ctx.handle, n, _ptr(s), s.shape[0], SW_HOST, C.byref(h)))
+        self.handle = h
+
+    def query(self, queries, superset: bool = False, per_query: bool = True):
+        """(per-query flags or None, any, witness) for host queries."""
+        q = _sets(queries, self.n)
+        res = np.zeros(q.shape[0], dtype=np.uint8) if per_query else None
+        anyf = C.c_int32()
+        wit = C.c_uint64()
+        _check(load().sw_trie_query(self.handle, int(superset), _ptr(q), q.shape[0], SW_HOST,
+                                    _ptr(res) if per_query and q.shape[0] else None, C.byref(anyf), C.byref(wit)))
+        return (res.astype(bool) if per_query else None), bool(anyf.value), int(wit.value)
+
+    def info(self) -> dict:
+        s, nd, lv = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        _check(load().sw_trie_info(self.handle, C.byref(s), C.byref(nd), C.byref(lv)))
+        return {"sets": s.value, "nodes": nd.value, "top_levels": lv.value}
+
+    def dump(self):
+        """(nodes [count, 4] u32, sets in trie order, their rows in the built list)."""
+        inf = self.info()
+        nodes = np.zeros((inf["nodes"], 4), dtype=np.uint32)
+        sets = np.zeros((inf["sets"], words(self.n)), dtype=np.uint64)
+        rows = np.zeros(inf["sets"], dtype=np.uint32)
+        _check(load().sw_trie_nodes(self.handle, _ptr(nodes), inf["nodes"], _ptr(sets), _ptr(rows)))
+        return nodes, sets, rows
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().sw_trie_destroy(self.handle)
+            self.handle = None
 
     def __del__(self):
         try:

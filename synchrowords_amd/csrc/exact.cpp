@@ -28,11 +28,24 @@
 // reference's own rule (dfs.cpp:285-287): on a shortest word every backward
 // set strictly contains the forward image it meets, which has >= 2 states.
 //
-// The reference's cost model (meet_in_the_middle.cpp:346-551) and its DFS
-// phase (dfs.cpp) exist to stay inside host memory; with hundreds of GB of HBM
-// the device keeps whole levels, expands the smaller frontier, and freezes the
-// forward side past a size cap (the backward-only continuation, the role of
-// the reference's DFS).  Choices of side never change the answer.
+// Meet checks of large frontiers walk a device subset trie (k_trie.hip)
+// built over the frontier that did not just change: a trie over the forward
+// frontier answers "is some forward set inside this backward set", one over
+// the backward frontier "does some backward set contain this forward set";
+// the tries are rebuilt only when their side steps.
+//
+// The reference's cost model (meet_in_the_middle.cpp:346-551) picks sides
+// and the switch to its DFS (dfs.cpp) to stay inside host memory.  Here the
+// device keeps whole levels while they fit the budget (max_memory_mb),
+// expands the smaller frontier, and freezes the forward side past a size
+// cap (backward-only from then on).  When a step would pass the budget, the
+// search deepens the backward frontier in parts instead, as the reference's
+// DFS does (dfs.cpp:63-347): each part's preimages are deduplicated (against
+// the backward side's rows and among themselves), checked against the trie
+// over the forward frontier, and deepened in parts again while below the
+// best length found; a find lowers that bound (dfs.cpp:179-189).  If even
+// that does not fit, the result is a lower bound (exact.cpp:75-96 of the
+// reference: "mlsw >= x").  Choices of side never change the answer.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,8 +55,11 @@
 #include <cmath>
 #include <string>
 
+#include <memory>
+
 #include "sw_host.hpp"
 #include "sw_kernels.hpp"
+#include "sw_trie.hpp"
 
 namespace sw {
 
@@ -73,6 +89,17 @@ struct Workspace {
   Side F, B;
   DevBuf cand, valid, slot, keep, pos, small, scan_tmp, flags, probes;
   size_t scan_bytes = 0;
+  // Subset tries over the forward and the backward frontier (meet checks,
+  // and the forward one for the backward-deepening phase).
+  DeviceTrie ftrie, btrie;
+  // Backward-deepening phase: one list per depth below the frontier (sets
+  // complemented, parent index in the list above, letter) and the hash
+  // table that deduplicates a part's preimages.
+  struct Level {
+    DevBuf sets, parent, letter;
+  };
+  std::vector<std::unique_ptr<Level>> dfs;
+  DevBuf dfs_table;
   // Pinned landing zone for the few scalars read back per step (a copy to
   // pageable memory would go through a staging buffer).
   uint64_t* host = nullptr;
@@ -120,6 +147,11 @@ class Search {
     small_sets_ = !cfg.dominance_pairs && W_ <= 3;
     forward_cap_ = cfg.forward_cap ? cfg.forward_cap : (1ull << 22);
     mem_cap_ = (cfg.max_memory_mb ? cfg.max_memory_mb : 65536ull) << 20;
+    dfs_min_list_ = cfg.dfs_min_list_size ? cfg.dfs_min_list_size : 8192;
+    // With the deepening on, the levels kept whole leave a quarter of the
+    // budget to it, so that its parts are not starved.
+    mitm_cap_ = cfg.dfs ? mem_cap_ - mem_cap_ / 4 : mem_cap_;
+    goal_mode_ = cfg.goal_check;
   }
 
   ExactOutcome run(uint64_t lower, uint64_t upper) {
@@ -146,15 +178,26 @@ class Search {
       if (F_.frozen) fwd = false;
       else fwd = F_.fcount <= B_.fcount;
       ++t;
-      try {
-        step(fwd ? F_ : B_, !fwd, t);
-      } catch (const BudgetExceeded&) {
+      // A step of the other side when the chosen one would pass the budget:
+      // every level that still fits shortens the deepening that follows.
+      bool stepped = false;
+      for (int attempt = 0; attempt < 2 && !stepped; ++attempt) {
+        const bool f = attempt == 0 ? fwd : !fwd;
+        if (f && F_.frozen) continue;
+        try {
+          step(f ? F_ : B_, !f, t);
+          fwd = f;
+          stepped = true;
+        } catch (const BudgetExceeded&) {
+        }
+      }
+      if (!stepped) {
         --t;  // lengths up to t were checked
         budget_end_ = true;
         break;
       }
       const auto g0 = std::chrono::steady_clock::now();
-      const bool met = goal_check(&wit);
+      const bool met = goal_check(&wit, fwd);
       const double gms = ms_since(g0);
       ms_goal_ += gms;
       if (cfg_.verbose) {
@@ -169,7 +212,31 @@ class Search {
         found = true;
         break;
       }
-      if (!cfg_.dfs && F_.frozen) break;
+    }
+    bool oom = false;
+    uint64_t length = 0;
+    std::vector<uint32_t> word;
+    if (found) {
+      length = t;
+      word = reconstruct(wit);
+    } else if (budget_end_ && t + 1 < upper && F_.fcount && B_.fcount) {
+      if (cfg_.dfs) {
+        // Deepen the backward frontier in parts (dfs.cpp:63-91): found,
+        // the bound's word optimal, or the budget exhausted again.
+        const auto d0 = std::chrono::steady_clock::now();
+        oom = !deepen(t, upper);
+        ms_dfs_ = ms_since(d0);
+        r.dfs_entered = 1;
+        if (dfs_best_) {
+          word = dfs_word_;
+          length = dfs_best_;
+          found = !oom;
+        } else if (!oom) {
+          length = upper;
+        }
+      } else {
+        oom = true;
+      }
     }
     r.steps_forward = F_.steps;
     r.steps_backward = B_.steps;
@@ -180,19 +247,29 @@ class Search {
     r.ms_expand = ms_expand_;
     r.ms_dominance = ms_dom_;
     r.ms_goal = ms_goal_;
+    r.dfs_calls = dfs_calls_;
+    r.dfs_sets = dfs_sets_;
+    r.trie_builds = trie_builds_;
+    r.trie_queries = trie_queries_;
+    r.ms_dfs = ms_dfs_;
     if (found) {
       r.found = 1;
-      r.length = t;
-      out.word = reconstruct(wit);
+      r.length = length;
+      out.word = word;
       r.word_len = out.word.size();
-    } else if (t + 1 >= upper || (!budget_end_ && (F_.fcount == 0 || B_.fcount == 0))) {
+    } else if (oom) {
+      // Lengths up to t are excluded; a word the deepening found before the
+      // budget ran out is returned as a tighter upper bound.
+      r.found = 0;
+      r.out_of_memory = 1;
+      r.length = std::max<uint64_t>(lower, t + 1);
+      out.word = word;
+      r.word_len = out.word.size();
+    } else {
       // No word shorter than the upper bound: the bound is the length.
       r.found = 1;
       r.bound_word = 1;
       r.length = upper;
-    } else {
-      r.found = 0;
-      r.length = std::max<uint64_t>(lower, t + 1);
     }
     r.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return out;
@@ -212,7 +289,11 @@ class Search {
     init_forward();
     for (uint32_t d = 1; d <= depth; ++d) {
       if (!F_.fcount || F_.rows + F_.fcount * k_ > cap_rows) break;
-      step(F_, false, d);
+      try {
+        step(F_, false, d);
+      } catch (const BudgetExceeded&) {
+        break;
+      }
     }
     out.ensure(F_.rows * row_bytes() + 8);
     SW_CHECK_HIP(hipMemcpyAsync(out.p, F_.arena.p, F_.rows * row_bytes(), hipMemcpyDeviceToDevice, st_));
@@ -231,6 +312,18 @@ class Search {
   static constexpr uint64_t kSmallSetsDominanceBytes = 2ull << 30;
   bool goal_armed_ = false;  // the last commit reset the goal flag and witness
   bool budget_end_ = false;  // the search ended at the memory budget
+  uint64_t dfs_min_list_ = 8192;
+  uint64_t mitm_cap_ = 0;  // budget of the levels kept whole
+  int goal_mode_ = 0;
+  // Which frontier each trie holds: the side's step count when built (~0: none).
+  uint64_t ftrie_steps_ = ~0ull, btrie_steps_ = ~0ull;
+  uint64_t trie_builds_ = 0, trie_queries_ = 0;
+  uint64_t backward_run_ = 0;  // backward steps since the last forward one
+  // Backward deepening: parts of at most part_ sets, words of at most
+  // max_depth_ letters still wanted, the best word found.
+  uint64_t part_ = 0, max_depth_ = 0, dfs_best_ = 0, dfs_calls_ = 0, dfs_sets_ = 0;
+  std::vector<uint32_t> dfs_word_;
+  double ms_dfs_ = 0;
   uint64_t upper_ = ~0ull;     // the search's upper bound (none: forward levels only)
 
   uint64_t pair_tests_ = 0;
@@ -389,8 +482,12 @@ class Search {
     const auto t0 = std::chrono::steady_clock::now();
     const uint64_t m = S.fcount;
     const uint64_t cnt = m * k_;
-    if (cnt >= 0x7fffffffull) set_memory_error();  // candidate indices are 31-bit
-    if ((S.rows + cnt) * row_bytes() * 2 > mem_cap_) set_memory_error();
+    // Every budget test comes before anything changes, so that a step that
+    // would pass the budget leaves both sides as they were (the backward
+    // deepening starts from them).  Indices of rows and candidates are 31-bit.
+    if (cnt >= 0x7fffffffull || S.rows + cnt >= 0x7fffffffull) set_memory_error();
+    if ((S.rows + cnt) * row_bytes() > mitm_cap_ / 3) set_memory_error();
+    if (used_bytes() + cnt * (3 * row_bytes() + 37) > mitm_cap_) set_memory_error();
     cand_.ensure(cnt * row_bytes());
     valid_.ensure(cnt * 4);
     slot_.ensure(cnt * 4);
@@ -498,33 +595,223 @@ class Search {
           dominance ? "" : " (no dominance)");
   }
 
-  // Some forward frontier set disjoint from some backward complement.
-  bool goal_check(uint64_t* witness) {
+  // Meet check: some forward frontier set inside some backward set.
+  // Large checks walk a subset trie over the frontier that did not just
+  // step (the frozen forward side keeps its trie); small ones take the
+  // vertical subset kernel (stream: forward sets; tile: the backward sets,
+  // complemented back on load).
+  static constexpr uint64_t kTriePairs = 1ull << 26;
+  // Query lists up to this size are walked in list order (the cardinality
+  // order costs four launches).
+  static constexpr uint64_t kSortQueries = 1ull << 14;
+  uint32_t trie_grid() const { return uint32_t(std::max(1, ctx_->cus)) * 3; }
+
+  void ensure_ftrie() {
+    if (ftrie_steps_ == F_.steps) return;
+    ws_.ftrie.build(st_, n_, F_.front.as<uint64_t>(), F_.fcount, 0, nullptr);
+    ftrie_steps_ = F_.steps;
+    ++trie_builds_;
+  }
+  void ensure_btrie() {
+    if (btrie_steps_ == B_.steps) return;
+    ws_.btrie.build(st_, n_, B_.front.as<uint64_t>(), B_.fcount, 1, nullptr);
+    btrie_steps_ = B_.steps;
+    ++trie_builds_;
+  }
+
+  bool goal_check(uint64_t* witness, bool stepped_forward) {
     if (!F_.fcount || !B_.fcount) return false;
     int* found = small_.as<int>() + 2;
     unsigned long long* wit = reinterpret_cast<unsigned long long*>(small_.as<char>() + 16);
     if (!goal_armed_) SW_CHECK_HIP(launch_goal_reset(found, wit, st_));
     goal_armed_ = false;
-    VsubParams p;
-    p.stream = F_.front.as<uint64_t>();
-    p.s_count = F_.fcount;
-    p.tile = B_.front.as<uint64_t>();
-    p.t_count = B_.fcount;
-    p.n = n_;
-    p.complement_tile = 1;
-    p.mode = VSUB_ANY;
-    p.found = found;
-    p.witness = wit;
-    apply_tuning(ctx_, int(W_), p);
-    search_walk(p);
-    run_subset(ctx_, int(W_), p);
-    pair_tests_ += F_.fcount * B_.fcount;
+    const uint64_t pairs = F_.fcount * B_.fcount;
+    // The forward trie answers a backward set's query by walking the few
+    // branches its states allow (backward sets are small), but building it
+    // costs more than one kernel check; so by default it is used once it is
+    // built for the current forward frontier anyway: after backward steps
+    // (it is built at the first of them), when the forward side is frozen,
+    // and in the deepening phase.  A trie over the backward frontier asked
+    // for supersets of forward sets visits most of its nodes (forward sets
+    // are the larger ones), so only goal_check 3 uses it.
+    backward_run_ = stepped_forward ? 0 : backward_run_ + 1;
+    const bool reuse = F_.frozen || ftrie_steps_ == F_.steps || backward_run_ >= 2;
+    const bool trie = goal_mode_ == 2 || goal_mode_ == 3 || (goal_mode_ == 0 && pairs >= kTriePairs && reuse);
+    if (trie && (goal_mode_ != 3 || F_.frozen || !stepped_forward)) {
+      // Forward trie, backward queries: witness (forward tag << 32) | query.
+      ensure_ftrie();
+      ws_.ftrie.query(st_, trie_grid(), 0, B_.front.as<uint64_t>(), B_.fcount, 1, B_.fcount > kSortQueries, 0,
+          found, wit, nullptr);
+      trie_queries_ += B_.fcount;
+    } else if (trie) {
+      // Backward trie (the complements complemented back), forward queries:
+      // witness (query << 32) | backward tag.
+      ensure_btrie();
+      ws_.btrie.query(st_, trie_grid(), 1, F_.front.as<uint64_t>(), F_.fcount, 0, F_.fcount > kSortQueries, 1,
+          found, wit, nullptr);
+      trie_queries_ += F_.fcount;
+    } else {
+      VsubParams p;
+      p.stream = F_.front.as<uint64_t>();
+      p.s_count = F_.fcount;
+      p.tile = B_.front.as<uint64_t>();
+      p.t_count = B_.fcount;
+      p.n = n_;
+      p.complement_tile = 1;
+      p.mode = VSUB_ANY;
+      p.found = found;
+      p.witness = wit;
+      apply_tuning(ctx_, int(W_), p);
+      search_walk(p);
+      run_subset(ctx_, int(W_), p);
+      pair_tests_ += pairs;
+    }
     // found (4 bytes at +8) and witness (8 bytes at +16) in one copy
     SW_CHECK_HIP(hipMemcpyAsync(ws_.host, small_.as<char>() + 8, 16, hipMemcpyDeviceToHost, st_));
     SW_CHECK_HIP(hipStreamSynchronize(st_));
     const int f = int(uint32_t(ws_.host[0]));
     *witness = ws_.host[1];
     return f != 0;
+  }
+
+  // Device bytes this search holds, from its counts (buffers the context
+  // kept from earlier, larger searches are not charged to it).
+  uint64_t used_bytes() const {
+    const uint64_t rb = row_bytes();
+    uint64_t b = 2ull * k_ * slices_ * 256 * rb;  // slice tables of both directions
+    for (const Side* sd : {&F_, &B_}) b += sd->rows * (rb + 5) + sd->tcap * 4 + sd->fcount * (rb + 8);
+    if (ftrie_steps_ != ~0ull) b += ws_.ftrie.m * 3 * (rb + 4) + uint64_t(ws_.ftrie.node_count) * 16;
+    if (btrie_steps_ != ~0ull) b += ws_.btrie.m * 3 * (rb + 4) + uint64_t(ws_.btrie.node_count) * 16;
+    return b;
+  }
+
+  // Backward deepening below the frontier, lengths t + 1 .. max_depth_
+  // (the reference's Dfs::run, dfs.cpp:63-91, 161-218): the frontier is cut
+  // into parts; each part's preimages form the next list, which is checked
+  // against the forward trie and deepened in parts in turn.  Parts hold at
+  // most part_ sets, sized so that one list per remaining depth fits the
+  // budget (dfs.cpp:349-373).  False when the budget does not allow it.
+  bool deepen(uint64_t t, uint64_t upper) {
+    max_depth_ = upper - 1;
+    try {
+      ensure_ftrie();
+      const uint64_t rb = row_bytes();
+      const uint64_t used = used_bytes();
+      const uint64_t avail = mem_cap_ > used ? mem_cap_ - used : 0;
+      const uint64_t levels = max_depth_ - t;
+      const uint64_t per = uint64_t(k_) * (rb + 5) * levels + uint64_t(k_) * (rb + 32);
+      part_ = avail / per;
+      if (part_ < dfs_min_list_) {
+        if (cfg_.strict_memory_limit) return false;
+        part_ = dfs_min_list_;
+      }
+      part_ = std::min<uint64_t>(part_, (1ull << 30) / k_);
+      if (cfg_.verbose)
+        std::fprintf(stderr, "[sw exact] deepening: lengths %llu..%llu, forward trie %llu sets, %llu backward sets, "
+            "parts of %llu\n", (unsigned long long)(t + 1), (unsigned long long)max_depth_,
+            (unsigned long long)F_.fcount, (unsigned long long)B_.fcount, (unsigned long long)part_);
+      for (uint64_t b = 0; b < B_.fcount && t < max_depth_; b += part_)
+        dfs_part(0, b, std::min(B_.fcount, b + part_), t);
+    } catch (const BudgetExceeded&) {
+      return false;
+    } catch (const Error& e) {
+      if (e.code != SW_ERR_NOMEM) throw;
+      return false;  // the device ran out of memory
+    }
+    return true;
+  }
+
+  // Preimages of rows [b, e) of list d (d = 0: the backward frontier), words
+  // of length lsw + 1, as list d + 1 (dfs.cpp:228-347): sets of at most one
+  // state dropped, sets the backward side already holds dropped (produced
+  // at no greater depth), duplicates among them dropped; then the meet check
+  // and, below max_depth_, the parts of the new list.
+  void dfs_part(uint32_t d, uint64_t b, uint64_t e, uint64_t lsw) {
+    const int W = int(W_);
+    const uint64_t* list = d == 0 ? B_.front.as<uint64_t>() : ws_.dfs[d - 1]->sets.as<uint64_t>();
+    const uint64_t p = e - b, cnt = p * k_;
+    cand_.ensure(cnt * row_bytes());
+    valid_.ensure(cnt * 4);
+    slot_.ensure(cnt * 4);
+    keep_.ensure(cnt * 4 + 4);
+    pos_.ensure(cnt * 4 + 4);
+    ensure_scan(cnt);
+    uint64_t tcap = 1024;
+    while (tcap < 2 * cnt + 1024) tcap <<= 1;
+    ws_.dfs_table.ensure(tcap * 4);
+    uint32_t* table = ws_.dfs_table.as<uint32_t>();
+    SW_CHECK_HIP(launch_expand(W, list + b * W_, p, tables_[1].as<uint64_t>(), slices_, k_, nullptr, n_, 1,
+        cand_.as<uint64_t>(), valid_.as<uint32_t>(), st_));
+    SW_CHECK_HIP(launch_hash_lookup(W, cand_.as<uint64_t>(), valid_.as<uint32_t>(), cnt, B_.table.as<uint32_t>(),
+        B_.tcap - 1, B_.arena.as<uint64_t>(), st_));
+    SW_CHECK_HIP(launch_fill_u32(table, kEmpty, tcap, st_));
+    SW_CHECK_HIP(launch_hash_insert(W, cand_.as<uint64_t>(), valid_.as<uint32_t>(), cnt, table, tcap - 1,
+        B_.arena.as<uint64_t>(), slot_.as<uint32_t>(), probes_.as<unsigned long long>(), st_));
+    uint32_t* total = small_.as<uint32_t>();
+    SW_CHECK_HIP(launch_keep_scan(1, slot_.as<uint32_t>(), table, nullptr, cnt, keep_.as<uint32_t>(),
+        pos_.as<uint32_t>(), total, scan_tmp_.p, &scan_bytes_, st_));
+    const uint64_t c = read_u32(total);
+    ++dfs_calls_;
+    if (!c) return;
+    while (ws_.dfs.size() <= d) ws_.dfs.emplace_back(new Workspace::Level());
+    Workspace::Level& lv = *ws_.dfs[d];
+    lv.sets.ensure(c * row_bytes());
+    lv.parent.ensure(c * 4);
+    lv.letter.ensure(c);
+    SW_CHECK_HIP(launch_commit(W, cand_.as<uint64_t>(), keep_.as<uint32_t>(), pos_.as<uint32_t>(),
+        slot_.as<uint32_t>(), cnt, p, nullptr, b, nullptr, 0, lv.sets.as<uint64_t>(), lv.parent.as<uint32_t>(),
+        lv.letter.as<uint8_t>(), table, st_));
+    dfs_sets_ += c;
+    int* found = small_.as<int>() + 2;
+    unsigned long long* wit = reinterpret_cast<unsigned long long*>(small_.as<char>() + 16);
+    SW_CHECK_HIP(launch_goal_reset(found, wit, st_));
+    ws_.ftrie.query(st_, trie_grid(), 0, lv.sets.as<uint64_t>(), c, 1, c > kSortQueries, 0, found, wit, nullptr);
+    trie_queries_ += c;
+    if (cfg_.verbose && (dfs_calls_ & 1023) == 0)
+      std::fprintf(stderr, "[sw exact] deepening: %llu parts, %llu sets, list %u, length %llu\n",
+          (unsigned long long)dfs_calls_, (unsigned long long)dfs_sets_, d + 1, (unsigned long long)(lsw + 1));
+    SW_CHECK_HIP(hipMemcpyAsync(ws_.host, small_.as<char>() + 8, 16, hipMemcpyDeviceToHost, st_));
+    SW_CHECK_HIP(hipStreamSynchronize(st_));
+    if (uint32_t(ws_.host[0])) {
+      // A word of length lsw + 1: only shorter ones are wanted now.
+      dfs_word_ = dfs_reconstruct(d + 1, ws_.host[1]);
+      dfs_best_ = lsw + 1;
+      max_depth_ = lsw;
+      if (cfg_.verbose)
+        std::fprintf(stderr, "[sw exact] deepening: word of length %llu (list %u)\n", (unsigned long long)(lsw + 1),
+            d + 1);
+      return;
+    }
+    if (lsw + 1 >= max_depth_) return;
+    for (uint64_t q = 0; q < c; q += part_) {
+      dfs_part(d + 1, q, std::min(c, q + part_), lsw + 1);
+      if (lsw + 1 >= max_depth_) return;
+    }
+  }
+
+  // The word of a meet found in list `level`: forward letters to the forward
+  // set, then each list's letter up to the frontier, then the frontier set's
+  // backward letters to its singleton.
+  std::vector<uint32_t> dfs_reconstruct(uint32_t level, uint64_t witness) {
+    const uint32_t f = uint32_t(witness >> 32);
+    uint32_t idx = uint32_t(witness & 0xffffffffu);
+    std::vector<uint32_t> v;
+    for (uint32_t l = level; l >= 1; --l) {
+      Workspace::Level& lv = *ws_.dfs[l - 1];
+      uint32_t par = 0;
+      uint8_t let = 0;
+      SW_CHECK_HIP(hipMemcpyAsync(&par, lv.parent.as<uint32_t>() + idx, 4, hipMemcpyDeviceToHost, st_));
+      SW_CHECK_HIP(hipMemcpyAsync(&let, lv.letter.as<uint8_t>() + idx, 1, hipMemcpyDeviceToHost, st_));
+      SW_CHECK_HIP(hipStreamSynchronize(st_));
+      v.push_back(let);
+      idx = par;
+    }
+    std::vector<uint32_t> tail = path(B_, idx);
+    v.insert(v.end(), tail.begin(), tail.end());
+    std::vector<uint32_t> u = path(F_, f);
+    std::reverse(u.begin(), u.end());
+    u.insert(u.end(), v.begin(), v.end());
+    return u;
   }
 
   std::vector<uint32_t> path(Side& S, uint32_t frontier_pos) {

@@ -186,6 +186,30 @@ __global__ __launch_bounds__(256) void k_hash_insert(const uint64_t* __restrict_
   }
 }
 
+// valid[i] = 0 for a candidate equal to a row the table holds (the table
+// holds rows of `arena`; marks of candidates of an unfinished step are
+// skipped).  Read only: the backward-deepening phase drops sets the side
+// already produced at a smaller or equal depth.
+template <int W>
+__global__ __launch_bounds__(256) void k_hash_lookup(const uint64_t* __restrict__ cand, uint32_t* __restrict__ valid,
+    uint64_t count, const uint32_t* __restrict__ table, uint64_t mask, const uint64_t* __restrict__ arena) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count || !valid[i]) return;
+  uint64_t s[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) s[w] = cand[i * W + w];
+  uint64_t pos = set_hash<W>(s) & mask;
+  while (true) {
+    const uint32_t v = table[pos];
+    if (v == kEmpty) return;
+    if (!(v & kCand) && set_eq<W>(s, arena + uint64_t(v) * W)) {
+      valid[i] = 0;
+      return;
+    }
+    pos = (pos + 1) & mask;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_hash_resolve(const uint32_t* __restrict__ slot_of,
     const uint32_t* __restrict__ table, uint64_t count, uint32_t* __restrict__ keep) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -595,6 +619,19 @@ hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, static_cast<const uint4*>(src), static_cast<uint4*>(dst),
       n16);
+  return hipGetLastError();
+}
+
+hipError_t launch_hash_lookup(int W, const uint64_t* cand, uint32_t* valid, uint64_t count, const uint32_t* table,
+    uint64_t mask, const uint64_t* arena, hipStream_t st) {
+  if (!count) return hipSuccess;
+  const unsigned blocks = unsigned((count + 255) / 256);
+  switch (W) {
+#define C(w) case w: hipLaunchKernelGGL(k_hash_lookup<w>, dim3(blocks), dim3(256), 0, st, cand, valid, count, table, mask, arena); break;
+    C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8)
+#undef C
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

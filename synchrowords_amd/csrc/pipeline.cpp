@@ -447,6 +447,13 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
         if (mm) ec.max_memory_mb = uint64_t(eval_num(mm, "0", n, k));
         const Json* vb = cfg->get("verbose");
         ec.verbose = vb ? eval_bool(vb, false, n, k) : 0;
+        // The backward deepening's smallest part and whether the budget is
+        // strict (exact/config.hpp:24-46: dfs_min_list_size 8192,
+        // strict_memory_limit false).
+        ec.dfs_min_list_size = uint64_t(eval_num(cfg->get("dfs_min_list_size"), "8192", n, k));
+        ec.strict_memory_limit = eval_bool(cfg->get("strict_memory_limit"), false, n, k) ? 1 : 0;
+        const Json* gc = cfg->get("gpu_goal_check");
+        if (gc) ec.goal_check = int32_t(eval_num(gc, "0", n, k));
         auto r = exact_search(ctx, n, k, d, lower, upper, ec);
         out->exact = r.r;
         if (r.r.found) {
@@ -454,7 +461,13 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
           if (!r.r.bound_word) word = r.word;
           else if (word.size() != upper) word.clear();
         } else {
+          // The budget ran out ("mlsw >= x"): a lower bound, and any word the
+          // search found on the way as a tighter upper bound.
           lower = std::max(lower, r.r.length);
+          if (!r.word.empty() && r.word.size() < upper) {
+            upper = r.word.size();
+            word = r.word;
+          }
         }
       }
     } else if (name == "Reduce") {

@@ -112,6 +112,85 @@ hipError_t launch_invert(int W, const uint64_t* sets, uint64_t count, uint32_t n
 // LDS row byte offset of the state at position r.
 hipError_t launch_state_rank(const uint32_t* freq, uint32_t n, uint32_t* rank, uint32_t* order, hipStream_t st);
 
+// Device subset trie (k_trie.hip; built by trie.cpp).  A node is a uint4:
+// internal {zero child, one child, bit | minpop << 16, maxpop}, leaf {first
+// set, set count, kTrieLeafBit | minpop << 16, maxpop}; node 0 is the root,
+// and no child is node 0.
+constexpr uint32_t kTrieLeafBit = 0xffffu;
+constexpr uint32_t kTrieMaxDepth = 48;  // nodes at this depth are leaves
+constexpr uint32_t kTrieLeafSets = 8;   // leaves hold at most this many sets above the depth cap
+constexpr uint32_t kTrieChunk = 2048;   // sets per workgroup of the level-synchronous passes
+// Largest node built whole in one workgroup's LDS (two set and two tag buffers).
+__host__ __device__ constexpr uint32_t trie_local_cap(int W) {
+  return W == 1 ? 4096u : (W <= 3 ? 2048u : (W <= 7 ? 1024u : 512u));
+}
+__host__ __device__ constexpr size_t trie_local_lds(int W) {
+  return 2 * size_t(trie_local_cap(W)) * (8 * size_t(W) + 4);
+}
+struct TrieSeg {
+  uint32_t node, begin, end, depth;
+};
+struct TrieChunk {
+  uint32_t seg, begin, end, pad;
+};
+// A node built by one workgroup (k_trie_local; k_trie_bigleaf for a leaf
+// too large for LDS): rows [begin, end) of buffer buf (0 input, 1, 2).
+struct TrieLocalJob {
+  uint32_t root, begin, end, depth, buf, desc_base, pad0, pad1;
+};
+struct TrieBuildArgs {
+  const uint64_t* in;        // input sets (complemented on load when in_cmask is all ones)
+  uint64_t in_cmask;
+  const uint32_t* in_tags;   // tags of the input sets (NULL: row index)
+  const uint64_t* buf1;
+  const uint64_t* buf2;
+  const uint32_t* tags1;
+  const uint32_t* tags2;
+  uint64_t* out;             // the trie's sets, in trie order
+  uint32_t* out_tags;
+  uint4* nodes;
+  uint32_t n;
+};
+struct TrieQueryArgs {
+  const uint4* nodes;
+  uint32_t node_count;
+  const uint64_t* tsets;
+  const uint32_t* ttags;
+  const uint64_t* queries;
+  uint64_t q_cmask;          // all ones: the queries are stored complemented
+  uint32_t qcount, n;
+  const uint32_t* perm;      // query order (NULL: list order)
+  int reverse;               // walk perm from its end
+  int sorted;                // perm is a cardinality order in the walk's favour
+  int query_high;            // ANY witness: (query << 32) | tag, else (tag << 32) | query
+  uint32_t* next;            // work counter, zero at launch
+  int* found;
+  unsigned long long* witness;
+  uint8_t* flags;            // FLAGS: flags[query] = 1 on a hit
+};
+hipError_t launch_trie_hist(int W, const uint64_t* src, int complement, uint32_t n, const TrieChunk* chunks,
+    uint32_t nchunks, uint32_t* hist, uint32_t* seg_min, uint32_t* seg_max, hipStream_t st);
+hipError_t launch_trie_choose(const uint32_t* hist, const TrieSeg* segs, uint32_t nsegs, uint32_t n, uint32_t* bits,
+    uint32_t* ones, hipStream_t st);
+hipError_t launch_trie_chunk_zeros(int W, const uint64_t* src, int complement, const TrieChunk* chunks,
+    uint32_t nchunks, const uint32_t* bits, uint32_t* czero, hipStream_t st);
+hipError_t launch_trie_scatter(int W, const uint64_t* src, int complement, uint32_t n, const uint32_t* src_tags,
+    uint64_t* dst, uint32_t* dst_tags, const TrieChunk* chunks, uint32_t nchunks, const uint32_t* bits,
+    const uint2* coff, hipStream_t st);
+hipError_t launch_trie_bigleaf(int W, const TrieBuildArgs& a, const TrieLocalJob* jobs, uint32_t njobs,
+    hipStream_t st);
+hipError_t launch_trie_local(int W, const TrieBuildArgs& a, const TrieLocalJob* jobs, uint32_t njobs,
+    hipStream_t st);
+// superset: trie sets containing the query (else inside it); any: stop at
+// the first hit with a witness (else per-query flags).
+hipError_t launch_trie_query(int W, int superset, int any, const TrieQueryArgs& a, uint32_t grid, hipStream_t st);
+// Cardinality order (any W): hist[c] += sets of c states; then with
+// launch_size_offsets' cursor, perm[cursor[|S_i|]++] = i.
+hipError_t launch_card_hist(int W, const uint64_t* sets, uint64_t count, int complement, uint32_t n, uint32_t* hist,
+    hipStream_t st);
+hipError_t launch_card_scatter(int W, const uint64_t* sets, uint64_t count, int complement, uint32_t n,
+    uint32_t* cursor, uint32_t* perm, hipStream_t st);
+
 // Power-set BFS (k_brute.hip).  One automaton per workgroup with its bitmap
 // in LDS (brute_lds_bytes(n, k) <= kBruteMaxLds): frontier_off
 // indexes two buffers of 2^n subsets each.  Or one level over the device
@@ -236,6 +315,9 @@ hipError_t launch_commit(int W, const uint64_t* cand, const uint32_t* keep, cons
     hipStream_t st, uint64_t* front = nullptr, uint32_t* front_idx = nullptr, int* found = nullptr,
     unsigned long long* witness = nullptr);
 // Insert arena rows [0, rows) (all distinct) into an empty table.
+// valid[i] = 0 for each valid candidate equal to an arena row in the table.
+hipError_t launch_hash_lookup(int W, const uint64_t* cand, uint32_t* valid, uint64_t count, const uint32_t* table,
+    uint64_t mask, const uint64_t* arena, hipStream_t st);
 hipError_t launch_hash_rebuild(int W, const uint64_t* arena, uint64_t rows, uint32_t* table, uint64_t mask,
     hipStream_t st);
 // Slice tables [k][ceil(n/8)][256][W] from per-state masks [k][n][W] (row

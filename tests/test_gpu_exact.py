@@ -149,3 +149,80 @@ def test_learned_bound_exact_search(ctx, case):
     r = ctx.exact(n, k, d, upper=bound, upper_word=bword)
     assert r.found and r.length == case["upper"]
     _check_word(case, r.length, r.word)
+
+
+def _trie_cases():
+    out = []
+    for nm in ("c1_random48.json", "c3_random200.json", "c5_random300.json"):
+        cs = [c for c in golden(nm) if not c["non_synchro"]]
+        out += [pytest.param(c, id=c["name"]) for c in (cs[:12] if nm == "c1_random48.json" else cs)]
+    return out
+
+
+@pytest.mark.parametrize("case", _trie_cases())
+@pytest.mark.parametrize("mode", [1, 2, 3])
+def test_meet_checks_by_kernel_or_trie(ctx, case, mode):
+    """Meet checks by the vertical subset kernel (1), by the device subset
+    trie over the forward frontier (2), or by tries over the frontier that
+    did not step (3: superset queries after forward steps), from the trivial
+    bound so that the search itself finds the meet: the reference's length,
+    and the word built from the witness replays."""
+    import synchrowords_amd as sw
+    n, k, d = case["n"], case["k"], case["delta"]
+    r = ctx.exact(n, k, d, cfg=sw.ExactConfig(goal_check=mode))
+    assert r.found and not r.bound_word and r.length == case["upper"]
+    _check_word(case, r.length, r.word)
+    if mode >= 2:
+        assert r.stats["trie_builds"] > 0 and r.stats["trie_queries"] > 0
+    else:
+        assert r.stats["trie_builds"] == 0
+
+
+def _budget_cases():
+    out = [pytest.param(c, id=c["name"]) for c in golden("c3_random200.json")[:5]]
+    out += [pytest.param(c, id=c["name"]) for c in golden("c5_random300.json")[:2]]
+    return out
+
+
+def _search_mb(r, n):
+    """Device MB an unconstrained search held (sets of both sides, row plus
+    parent and letter, their hash tables and frontiers), rounded up."""
+    W = (n + 63) // 64
+    sets = r.stats["sets_forward"] + r.stats["sets_backward"]
+    return max(1, (sets * (8 * W + 5 + 16 + 8 * W)) >> 20)
+
+
+@pytest.mark.parametrize("case", _budget_cases())
+@pytest.mark.parametrize("fraction", [2, 4])
+def test_memory_budget_deepens_the_backward_side(ctx, case, fraction):
+    """With a device budget of a half or a quarter of what the search's
+    levels take, a step passes the budget and the search deepens the
+    backward frontier in parts against the forward trie (the reference's
+    DFS, dfs.cpp:63-347): it still ends at the reference's length, with a
+    word that replays unless the Beam bound's word is the answer."""
+    import synchrowords_amd as sw
+    n, k, d = case["n"], case["k"], case["delta"]
+    full = ctx.exact(n, k, d, upper=case["beam"])
+    mb = max(1, _search_mb(full, n) // fraction)
+    r = ctx.exact(n, k, d, upper=case["beam"], cfg=sw.ExactConfig(max_memory_mb=mb, dfs_min_list_size=1024))
+    assert r.stats["dfs_entered"] == 1, (mb, r.stats)
+    assert r.found and not r.stats["out_of_memory"] and r.length == case["upper"]
+    if not r.bound_word:
+        _check_word(case, r.length, r.word)
+
+
+@pytest.mark.parametrize("case", _budget_cases()[:3])
+def test_memory_budget_without_deepening_gives_a_lower_bound(ctx, case):
+    """With the deepening off (dfs: false) or a strict budget too small for
+    its smallest parts, the search ends at the budget with a lower bound on
+    the length ("mlsw >= x", exact.cpp:75-96 of the reference), never an
+    error."""
+    import synchrowords_amd as sw
+    n, k, d = case["n"], case["k"], case["delta"]
+    full = ctx.exact(n, k, d, upper=case["beam"])
+    mb = max(1, _search_mb(full, n) // 8)
+    for cfg in (sw.ExactConfig(max_memory_mb=mb, dfs=False),
+                sw.ExactConfig(max_memory_mb=1, strict_memory_limit=True, dfs_min_list_size=1 << 26)):
+        r = ctx.exact(n, k, d, upper=case["beam"], cfg=cfg)
+        assert not r.found and r.stats["out_of_memory"] == 1
+        assert 1 <= r.length <= case["upper"]

@@ -177,3 +177,15 @@ def test_subset_count_mt_against_numpy():
             assert total == m.sum()
             np.testing.assert_array_equal(per_row, m.sum(axis=1))
         assert per_row[8] == 3000  # the empty set lies in every set
+
+
+@pytest.mark.parametrize("name", ["random200_s0", "random200_s2"])
+@pytest.mark.parametrize("mb", [1, 5, 10])
+def test_oracle_memory_budget_keeps_the_length(name, mb):
+    """The oracle's meet in the middle under a small memory budget switches
+    to its DFS (dfs.cpp:63-347 restated) and still gives the reference's
+    length."""
+    c = [c for c in golden("c3_random200.json") if c["name"] == name][0]
+    o = orc.solve(c["n"], c["k"], c["delta"], max_memory_mb=mb)
+    assert o["entered_dfs"]
+    assert (o["lower"], o["upper"]) == (c["upper"], c["upper"])
