@@ -177,15 +177,89 @@ def time_to_exact(ctx, n: int, seed: int) -> dict:
     return out
 
 
-def bench_batch(ctx, count: int) -> dict:
+def run_reference(n: int, k: int, threads: int, automata) -> dict | None:
+    """The reference binary (oracle/_ref/ref_N_K_T, the reference's own
+    sources built for N, K and T threads) on a file of automata: per
+    automaton its [lower, upper] and the plan's time (JIT compile excluded,
+    as in the reference's output), or None when it was not built."""
+    from synchrowords_amd import gen
+    exe = ROOT / "oracle" / "_ref" / f"ref_{n}_{k}_{threads}"
+    if not exe.exists():
+        return None
+    with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as f:
+        for d in automata:
+            f.write(gen.to_text(n, k, d))
+        path = f.name
+    try:
+        t0 = time.perf_counter()
+        res = subprocess.run([str(exe), path], capture_output=True, text=True, timeout=1800)
+        wall = time.perf_counter() - t0
+    finally:
+        os.unlink(path)
+    rows = []
+    for line in res.stdout.splitlines():
+        m = {kv.split("=")[0]: kv.split("=")[1] for kv in line.split("|")[-1].split() if "=" in kv}
+        if "total_us" in m:
+            rows.append((int(m["lo"]), int(m["hi"]), float(m["total_us"]) / 1e6))
+    return {"rows": rows, "s_total": round(sum(r[2] for r in rows), 4), "wall_s": round(wall, 3)}
+
+
+def bench_times(ctx) -> dict:
+    """Time to the exact length through the configs/exact.json plan, per
+    workload: random binary DFAs with n = 48 (seeds 0..19), n = 200 (seeds
+    0..9), n = 300 (seeds 1000..1004), and the Cerny automaton C_64 (length
+    63^2 = 3969).  GPU: one plan call per automaton after a warm call, wall
+    clock.  Reference: its own binary on the host CPU at one thread and at
+    16 threads (min(64, host threads) on the box), its plan time per
+    automaton.  Every length is compared."""
+    from synchrowords_amd import gen
+    cpu = host_cpu()
+    work = [("random n=48 k=2, seeds 0..19", 48, [gen.random_dfa(48, 2, s) for s in range(20)]),
+            ("random n=200 k=2, seeds 0..9", 200, [gen.random_dfa(200, 2, s) for s in range(10)]),
+            ("Cerny C_64", 64, [gen.cerny(64)]),
+            ("random n=300 k=2, seeds 1000..1004", 300, [gen.random_dfa(300, 2, s) for s in range(1000, 1005)])]
+    out = {"workload": "time to exact reset length, configs/exact.json plan", "cpu_model": cpu["model"],
+           "host_threads": cpu["threads_available"], "rows": []}
+    for name, n, autos in work:
+        ctx.pipeline(n, 2, autos[0])  # warm
+        gpu, lengths, stages = [], [], {}
+        for d in autos:
+            t0 = time.perf_counter()
+            r = ctx.pipeline(n, 2, d)
+            gpu.append(time.perf_counter() - t0)
+            lengths.append((r.lower, r.upper))
+            for st, ms in r.times_ms:
+                stages[st] = stages.get(st, 0.0) + ms
+            assert r.word is not None and len(r.word) == r.upper and replay_ok(n, d, r.word)
+        row = {"workload": name, "automata": len(autos), "gpu_s_total": round(sum(gpu), 4),
+               "gpu_s_max": round(max(gpu), 4), "gpu_stage_ms": {k: round(v, 2) for k, v in stages.items()}}
+        for t in (1, 16):
+            ref = run_reference(n, 2, t, autos)
+            if ref is None:
+                continue
+            same = [tuple(x[:2]) for x in ref["rows"]] == [tuple(x) for x in lengths]
+            row[f"reference_{t}t_s_total"] = ref["s_total"]
+            row[f"speedup_vs_{t}t"] = round(ref["s_total"] / sum(gpu), 2)
+            row[f"lengths_equal_{t}t"] = same
+        print(f"[bench] times {row}", file=sys.stderr, flush=True)
+        out["rows"].append(row)
+    return out
+
+
+def replay_ok(n: int, d, word) -> bool:
+    import synchrowords_amd as sw
+    return sw.replay(n, 2, d, word) == 1
+
+
+def bench_batch(ctx, count: int, cap_rows: int = 0, max_pairs: int = 0) -> dict:
     """The 4096-instance batch (n in [16,160], k in {2,3}) in one persistent launch."""
     from synchrowords_amd import gen
     import synchrowords_amd as sw
     specs = gen.batch_specs(4096, 2025)[:count]
     inst = [(n, k, sw.random_dfa(n, k, seed)) for n, k, seed in specs]
-    ctx.batch(inst[:64])  # warm
+    ctx.batch(inst[:64], cap_rows=cap_rows, max_pairs=max_pairs)  # warm
     t0 = time.perf_counter()
-    res, st = ctx.batch(inst)
+    res, st = ctx.batch(inst, cap_rows=cap_rows, max_pairs=max_pairs)
     wall = time.perf_counter() - t0
     out = {"workload": f"batch of {count} random DFAs, n in [16,160], k in {{2,3}} (batch_specs(4096, 2025))",
            "wall_s": round(wall, 4), "instances_per_s": count / wall, "stats": st}
@@ -382,7 +456,7 @@ def bench_learned(ctx, scorer: str, seeds, width: int) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["subset", "batch", "learned", "exact", "brute", "eppstein"],
+    ap.add_argument("--workload", choices=["subset", "batch", "learned", "exact", "brute", "eppstein", "times"],
                     default="subset",
                     help="subset: the headline line; batch / learned: the batched and learned-bound reports; "
                          "exact: one verbose exact search (--exact-n, --exact-seed); brute: power-set BFS; "
@@ -390,6 +464,8 @@ def main() -> int:
     ap.add_argument("--exact-n", type=int, default=300)
     ap.add_argument("--exact-seed", type=int, default=1)
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--batch-cap-rows", type=int, default=0, help="sets per side inside a workgroup (0: default)")
+    ap.add_argument("--batch-max-pairs", type=int, default=0, help="largest goal check inside a workgroup (0: default)")
     ap.add_argument("--scorer", type=str, default=str(ROOT / "synchrowords_amd" / "models" / "scorer_n300_k2.npz"))
     ap.add_argument("--learned-width", type=int, default=64)
     ap.add_argument("--learned-seeds", type=int, default=20)
@@ -433,11 +509,13 @@ def main() -> int:
     if args.workload != "subset":
         ctx = sw.Context(0)
         if args.workload == "batch":
-            line = bench_batch(ctx, args.batch_count)
+            line = bench_batch(ctx, args.batch_count, args.batch_cap_rows, args.batch_max_pairs)
         elif args.workload == "brute":
             line = bench_brute(ctx, args.batch_count, args.no_cpu)
         elif args.workload == "eppstein":
             line = bench_eppstein(ctx, args.batch_count, args.no_cpu)
+        elif args.workload == "times":
+            line = bench_times(ctx)
         elif args.workload == "exact":
             d = sw.random_dfa(args.exact_n, 2, args.exact_seed)
             ub = ctx.pipeline(args.exact_n, 2, d).beam_bound  # warm; the plan's Beam bound caps the search
@@ -535,8 +613,9 @@ def main() -> int:
         "lds_roofline": lds_roofline(ctx, small, count, kernel_ms, args.subset_kernel != "bitwalk"),
     }
     if not args.no_cpu:
-        # About 2^38 pairs on the full lists: 10-30 s over the host's threads.
-        rows = 256 if args.quick else (1 << 18)
+        # The whole 2^40-pair count over every host thread: about 16 s on
+        # the box's 16 threads.
+        rows = 256 if args.quick else count
         line["cpu_baseline"] = cpu_subset_baseline(ctx, small, large, min(rows, count), golden)
     if not args.no_exact:
         try:

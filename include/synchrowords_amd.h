@@ -190,8 +190,8 @@ typedef struct sw_exact_config {
                                  end with a lower bound (the reference's STRICT_MEMORY_LIMIT);
                                  0: use that size anyway */
   int32_t goal_check;         /* meet checks: 0 automatic (the subset kernel; the trie over the
-                                 forward frontier once it is reused: frozen forward side, runs of
-                                 backward steps, the deepening phase), 1 always the vertical subset
+                                 forward frontier where one build serves many checks: frozen
+                                 forward side, the deepening phase), 1 always the vertical subset
                                  kernel, 2 always the forward trie, 3 tries over both frontiers
                                  (the one that did not step) */
 } sw_exact_config;
@@ -286,11 +286,11 @@ int sw_pipeline_solve(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta
 /* ----------------------------------------------------------- batched search */
 
 typedef struct sw_batch_config {
-  uint32_t grid;       /* workgroups of the persistent launch (0: 4 per CU) */
+  uint32_t grid;       /* workgroups of the persistent launch (0: 4 per CU, 3 when n > 128) */
   uint32_t cap_rows;   /* sets per side a workgroup may hold (0: 65536) */
   uint32_t cap_cand;   /* candidates of one step (0: 98304) */
   uint32_t overflow_workers; /* host threads finishing overflowed instances (0: 8) */
-  uint64_t max_pairs;  /* largest goal check done inside a workgroup (0: 2^26) */
+  uint64_t max_pairs;  /* largest goal check done inside a workgroup (0: 2^32) */
 } sw_batch_config;
 
 typedef struct sw_batch_stats {

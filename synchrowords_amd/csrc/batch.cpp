@@ -18,8 +18,84 @@
 
 #include "sw_batch.hpp"
 #include "sw_host.hpp"
+#include "sw_kernels.hpp"
 
 namespace sw {
+
+// One automaton of at most 64 states (one word per set) through the batch
+// kernel's search in a single workgroup: the whole meet in the middle runs
+// inside one launch with its tables in LDS, where the device-wide search
+// pays about ten launches and two read-backs per step (the Cerny automaton
+// C_64 takes 3968 steps with frontiers of one or two sets).  False when the
+// instance outgrows the workgroup (then the device-wide search runs).
+bool exact_search_workgroup(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, uint64_t upper,
+    ExactOutcome& out) {
+  if (n < 2 || n > 64 || k > kBatchMaxK || upper < 2 || upper > 4096) return false;
+  const auto t0 = std::chrono::steady_clock::now();
+  BatchInstance b{};
+  b.n = n;
+  b.k = k;
+  b.delta_off = 0;
+  b.upper = uint32_t(upper);
+  const std::vector<char> sink = sink_states(n, k, d);
+  std::vector<uint32_t> indeg(size_t(n) * k, 0);
+  for (uint32_t q = 0; q < n; ++q)
+    for (uint32_t a = 0; a < k; ++a) indeg[size_t(d[size_t(q) * k + a]) * k + a]++;
+  for (uint32_t q = 0; q < n; ++q) {
+    if (!sink[q]) continue;
+    bool ok = false;
+    for (uint32_t a = 0; a < k; ++a) ok |= indeg[size_t(q) * k + a] > 1;
+    if (ok) b.b0[0] |= 1ull << q;
+  }
+  std::vector<uint8_t> bytes(size_t(n) * k);
+  for (size_t j = 0; j < bytes.size(); ++j) bytes[j] = uint8_t(d[j]);
+  BatchLimits L;
+  L.cap_rows = 1u << 16;
+  L.cap_cand = 3u * (1u << 15);
+  uint32_t hs = 1024;
+  while (hs < 2 * (L.cap_rows + L.cap_cand)) hs <<= 1;
+  L.hash_slots = hs;
+  L.max_pairs = 1ull << 32;
+  const uint32_t wcap = uint32_t(std::max<uint64_t>(upper, 64));
+  const size_t stride = batch_scratch_bytes(1, L);
+  hipStream_t s = ctx->stream;
+  ctx->wg_in.ensure(sizeof(BatchInstance) + 64 + bytes.size() + 64);
+  ctx->wg_scratch.ensure(stride);
+  ctx->wg_out.ensure(sizeof(BatchResult) + 64 + wcap);
+  uint8_t* din = ctx->wg_in.as<uint8_t>();
+  BatchInstance* dinst = reinterpret_cast<BatchInstance*>(din);
+  uint32_t* dorder = reinterpret_cast<uint32_t*>(din + sizeof(BatchInstance));      // [0] = 0: the one instance
+  uint32_t* dhead = dorder + 4;                                                    // queue head
+  uint8_t* dbytes = din + sizeof(BatchInstance) + 64;
+  BatchResult* dres = ctx->wg_out.as<BatchResult>();
+  uint8_t* dwords = ctx->wg_out.as<uint8_t>() + sizeof(BatchResult) + 64;
+  std::vector<uint8_t> head(sizeof(BatchInstance) + 64, 0);
+  std::memcpy(head.data(), &b, sizeof(b));
+  SW_CHECK_HIP(hipMemcpyAsync(din, head.data(), head.size(), hipMemcpyHostToDevice, s));
+  SW_CHECK_HIP(hipMemcpyAsync(dbytes, bytes.data(), bytes.size(), hipMemcpyHostToDevice, s));
+  SW_CHECK_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(dres), 0u, sizeof(BatchResult) / 4, s));
+  SW_CHECK_HIP(launch_batch(1, dinst, dbytes, dorder, 1, dhead, dres, dwords, wcap, ctx->wg_scratch.as<uint8_t>(),
+      stride, L, 1, s));
+  BatchResult r{};
+  SW_CHECK_HIP(hipMemcpyAsync(&r, dres, sizeof(r), hipMemcpyDeviceToHost, s));
+  SW_CHECK_HIP(hipStreamSynchronize(s));
+  if (r.status != kBatchFound && r.status != kBatchBound) return false;
+  std::memset(&out.r, 0, sizeof(out.r));
+  out.word.clear();
+  out.r.found = 1;
+  out.r.length = r.length;
+  out.r.steps_forward = r.steps;
+  if (r.status == kBatchFound) {
+    std::vector<uint8_t> w(r.word_len);
+    if (r.word_len) SW_CHECK_HIP(hipMemcpy(w.data(), dwords, r.word_len, hipMemcpyDeviceToHost));
+    out.word.assign(w.begin(), w.end());
+    out.r.word_len = out.word.size();
+  } else {
+    out.r.bound_word = 1;
+  }
+  out.r.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return true;
+}
 
 }  // namespace sw
 
@@ -141,21 +217,28 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
     auto t1 = std::chrono::steady_clock::now();
     st.ms_host_prep = std::chrono::duration<double, std::milli>(t1 - t0).count();
 
-    // ---- the persistent launch
+    // ---- the persistent launch: 256-thread workgroups, several per CU.
+    // Instances that outgrow a workgroup leave for the whole-device search
+    // below.  A second launch of 1024-thread workgroups with 2^20 rows per
+    // side for them was measured slower (5.2 s for the 1099 such instances
+    // of the 4096-instance batch, against 1.5 s on the whole device).
     std::vector<BatchResult> res(inst.size());
     const uint32_t wc = std::max<uint32_t>(word_cap ? word_cap : 0, 2);
     const uint32_t kernel_wcap = std::min<uint32_t>(std::max<uint32_t>(wc, 64), 4096);
     std::vector<uint8_t> wbuf(size_t(inst.size()) * kernel_wcap);
     if (!inst.empty()) {
-      BatchLimits L;
+      BatchLimits L{};
       L.cap_rows = cfg.cap_rows ? cfg.cap_rows : (1u << 16);
       L.cap_cand = cfg.cap_cand ? cfg.cap_cand : 3u * (1u << 15);
+      L.max_pairs = cfg.max_pairs ? cfg.max_pairs : (1ull << 32);
       uint32_t hs = 1024;
       while (hs < 2 * (L.cap_rows + L.cap_cand)) hs <<= 1;
       L.hash_slots = hs;
-      L.max_pairs = cfg.max_pairs ? cfg.max_pairs : (1ull << 26);
       const size_t stride = batch_scratch_bytes(int(wmax), L);
-      uint32_t grid = cfg.grid ? cfg.grid : uint32_t(ctx->cus) * 4;
+      // Workgroups resident at once: four per CU, three when the widest
+      // instances' LDS (tables and goal tile, about 45 KB at n > 128) allows
+      // no more.
+      uint32_t grid = cfg.grid ? cfg.grid : uint32_t(ctx->cus) * (wmax >= 3 ? 3 : 4);
       grid = std::min<uint32_t>(grid, uint32_t(inst.size()));
       DevBuf d_inst, d_bytes, d_order, d_head, d_res, d_words, d_scratch;
       d_inst.ensure(inst.size() * sizeof(BatchInstance));
@@ -170,10 +253,10 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
       SW_CHECK_HIP(hipMemcpyAsync(d_bytes.p, bytes.data(), bytes.size(), hipMemcpyHostToDevice, s));
       SW_CHECK_HIP(hipMemcpyAsync(d_order.p, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
       SW_CHECK_HIP(hipMemsetAsync(d_head.p, 0, 8, s));
-      SW_CHECK_HIP(hipMemsetAsync(d_res.p, 0, res.size() * sizeof(BatchResult), s));
+      SW_CHECK_HIP(hipMemcpyAsync(d_res.p, res.data(), res.size() * sizeof(BatchResult), hipMemcpyHostToDevice, s));
       auto k0 = std::chrono::steady_clock::now();
       SW_CHECK_HIP(launch_batch(int(wmax), d_inst.as<BatchInstance>(), d_bytes.as<uint8_t>(), d_order.as<uint32_t>(),
-          uint32_t(inst.size()), d_head.as<uint32_t>(), d_res.as<BatchResult>(), d_words.as<uint8_t>(), kernel_wcap,
+          uint32_t(order.size()), d_head.as<uint32_t>(), d_res.as<BatchResult>(), d_words.as<uint8_t>(), kernel_wcap,
           d_scratch.as<uint8_t>(), stride, L, grid, s));
       SW_CHECK_HIP(hipStreamSynchronize(s));
       st.ms_kernel = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - k0).count();
@@ -254,7 +337,7 @@ extern "C" int sw_batch_solve(sw_ctx* ctx, uint32_t count, const uint32_t* ns, c
               sw_exact_config ec{};
               ec.dfs = 1;
               const auto s0 = std::chrono::steady_clock::now();
-              auto r = exact_search(wctx, ns[i], ks[i], d, 0, upper[i], ec);
+              auto r = exact_search(wctx, ns[i], ks[i], d, 0, upper[i], ec, false);
               busy_ms[t] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - s0).count();
               if (!r.r.found) throw Error{SW_ERR_NOMEM, "exact search ended without a length (memory budget)"};
               lengths[i] = r.r.length;

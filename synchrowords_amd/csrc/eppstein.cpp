@@ -14,8 +14,11 @@
 
 namespace sw {
 
+// Below 128 states one workgroup takes a lone automaton: the Cerny automaton
+// C_64's pair BFS has thousands of levels a few pairs wide, where the
+// whole-device path pays its launches per level (57 ms for C_64).
 #ifndef SW_EPP_WIDE_MIN_N
-#define SW_EPP_WIDE_MIN_N 64
+#define SW_EPP_WIDE_MIN_N 128
 #endif
 
 namespace {

@@ -318,7 +318,6 @@ class Search {
   // Which frontier each trie holds: the side's step count when built (~0: none).
   uint64_t ftrie_steps_ = ~0ull, btrie_steps_ = ~0ull;
   uint64_t trie_builds_ = 0, trie_queries_ = 0;
-  uint64_t backward_run_ = 0;  // backward steps since the last forward one
   // Backward deepening: parts of at most part_ sets, words of at most
   // max_depth_ letters still wanted, the best word found.
   uint64_t part_ = 0, max_depth_ = 0, dfs_best_ = 0, dfs_calls_ = 0, dfs_sets_ = 0;
@@ -628,14 +627,13 @@ class Search {
     const uint64_t pairs = F_.fcount * B_.fcount;
     // The forward trie answers a backward set's query by walking the few
     // branches its states allow (backward sets are small), but building it
-    // costs more than one kernel check; so by default it is used once it is
-    // built for the current forward frontier anyway: after backward steps
-    // (it is built at the first of them), when the forward side is frozen,
-    // and in the deepening phase.  A trie over the backward frontier asked
-    // for supersets of forward sets visits most of its nodes (forward sets
-    // are the larger ones), so only goal_check 3 uses it.
-    backward_run_ = stepped_forward ? 0 : backward_run_ + 1;
-    const bool reuse = F_.frozen || ftrie_steps_ == F_.steps || backward_run_ >= 2;
+    // costs more than a kernel check of the same frontiers (n = 300 seed 1:
+    // 12.1 -> 17.3 ms when it was rebuilt for runs of backward steps); so by
+    // default it serves only where one build meets many queries: the frozen
+    // forward side and the deepening phase.  A trie over the backward
+    // frontier asked for supersets of forward sets visits most of its nodes
+    // (forward sets are the larger ones), so only goal_check 3 uses it.
+    const bool reuse = F_.frozen;
     const bool trie = goal_mode_ == 2 || goal_mode_ == 3 || (goal_mode_ == 0 && pairs >= kTriePairs && reuse);
     if (trie && (goal_mode_ != 3 || F_.frozen || !stepped_forward)) {
       // Forward trie, backward queries: witness (forward tag << 32) | query.
@@ -943,8 +941,19 @@ void forward_levels(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, 
 }
 
 ExactOutcome exact_search(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint64_t lower, uint64_t upper,
-    const sw_exact_config& cfg) {
+    const sw_exact_config& cfg, bool allow_workgroup) {
   use(ctx);
+  // Small automata: the whole search in one workgroup when it fits (no
+  // per-step launches; a few MB of scratch, inside any budget a plan sets);
+  // the other settings below only concern the device-wide search.
+  if (allow_workgroup && n <= 64 && !cfg.verbose && !cfg.goal_check && !cfg.dominance_pairs && !cfg.forward_cap &&
+      (!cfg.max_memory_mb || cfg.max_memory_mb >= 64)) {
+    ExactOutcome o;
+    if (exact_search_workgroup(ctx, n, k, delta, upper, o)) {
+      o.r.length = std::max(o.r.length, lower);
+      return o;
+    }
+  }
   Search s(ctx, n, k, delta, cfg);
   return s.run(lower, upper);
 }

@@ -20,7 +20,8 @@ namespace sw {
 
 namespace {
 
-constexpr int kThreads = 256;
+constexpr uint32_t kGoalWords = 16;               // 64-set words per tile row
+constexpr uint32_t kGoalTile = kGoalWords * 64;   // backward sets per tile
 
 template <int W>
 struct SetW {
@@ -70,35 +71,43 @@ size_t batch_scratch_bytes(int W, const BatchLimits& L) {
   return (b + 4095) & ~size_t(4095);
 }
 
-template <int W>
-__device__ inline void block_scan(uint32_t v, uint32_t* sh, uint32_t& excl, uint32_t& total) {
-  // Inclusive Hillis-Steele scan over kThreads values in LDS.
-  const int t = threadIdx.x;
-  sh[t] = v;
+// Exclusive scan of 0/1 flags over the workgroup of T threads: a ballot per
+// wave, the lanes below in it, and the totals of the waves before (two
+// barriers).
+template <int T>
+__device__ inline void block_scan(bool v, uint32_t* sh, uint32_t& excl, uint32_t& total) {
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t b = __ballot(v);
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  if (lane == 0) sh[wv] = uint32_t(__popcll(b));
   __syncthreads();
-  for (int off = 1; off < kThreads; off <<= 1) {
-    uint32_t x = t >= off ? sh[t - off] : 0u;
-    __syncthreads();
-    sh[t] += x;
-    __syncthreads();
+  uint32_t before = 0, tot = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < T / 64; ++w) {
+    before += w < wv ? sh[w] : 0u;
+    tot += sh[w];
   }
-  excl = sh[t] - v;
-  total = sh[kThreads - 1];
+  excl = before + uint32_t(__popcll(b & below));
+  total = tot;
   __syncthreads();
 }
 
-template <int W>
-__global__ __launch_bounds__(kThreads) void k_batch(const BatchInstance* __restrict__ inst,
+// T threads per workgroup (256: several instances per CU at a time).
+template <int W, int T>
+__global__ __launch_bounds__(T) void k_batch(const BatchInstance* __restrict__ inst,
     const uint8_t* __restrict__ deltas, const uint32_t* __restrict__ order, uint32_t count, uint32_t* head,
     BatchResult* __restrict__ out, uint8_t* __restrict__ words, uint32_t word_cap, uint8_t* scratch_base,
     size_t scratch_stride, BatchLimits L) {
   constexpr int MAXN = W * 64;
   __shared__ uint8_t sd[MAXN * kBatchMaxK];             // delta
   __shared__ uint64_t pre[kBatchMaxK][MAXN][W];         // preimage of each state per letter
-  __shared__ uint32_t sh[kThreads];
+  __shared__ uint32_t sh[T];
   __shared__ uint32_t s_inst, s_found, s_stop;
   __shared__ unsigned long long s_wit;
   __shared__ uint32_t s_tcap[2];
+  // Goal-check tile: kGoalTile backward sets transposed, row q = which of
+  // them hold state q (64 sets per word, kGoalWords words per state).
+  __shared__ uint64_t gtile[MAXN * kGoalWords];
 
   Scratch<W> S = carve<W>(scratch_base + size_t(blockIdx.x) * scratch_stride, L);
   const int tid = threadIdx.x;
@@ -117,10 +126,10 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchInstance* __restr
       continue;
     }
     // ---- per-instance tables in LDS
-    for (uint32_t i = tid; i < n * K; i += kThreads) sd[i] = deltas[I.delta_off + i];
-    for (uint32_t i = tid; i < K * MAXN * W; i += kThreads) (&pre[0][0][0])[i] = 0;
+    for (uint32_t i = tid; i < n * K; i += T) sd[i] = deltas[I.delta_off + i];
+    for (uint32_t i = tid; i < K * MAXN * W; i += T) (&pre[0][0][0])[i] = 0;
     __syncthreads();
-    for (uint32_t i = tid; i < n * K; i += kThreads) {
+    for (uint32_t i = tid; i < n * K; i += T) {
       const uint32_t q = i / K, a = i - q * K, t = sd[i];
       atomicOr(reinterpret_cast<unsigned long long*>(&pre[a][t][q >> 6]), 1ull << (q & 63));
     }
@@ -133,7 +142,7 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchInstance* __restr
     // hash tables start small and grow with the side
     uint32_t tcap[2] = {1024, 1024};
     for (int side = 0; side < 2; ++side)
-      for (uint32_t i = tid; i < tcap[side]; i += kThreads) S.table[side][i] = kEmpty;
+      for (uint32_t i = tid; i < tcap[side]; i += T) S.table[side][i] = kEmpty;
     __syncthreads();
     if (tid == 0) {
 #pragma unroll
@@ -192,9 +201,9 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchInstance* __restr
           break;
         }
         tcap[side] = nc;
-        for (uint32_t i = tid; i < nc; i += kThreads) S.table[side][i] = kEmpty;
+        for (uint32_t i = tid; i < nc; i += T) S.table[side][i] = kEmpty;
         __syncthreads();
-        for (uint32_t r = tid; r < rows[side]; r += kThreads) {
+        for (uint32_t r = tid; r < rows[side]; r += T) {
           uint64_t pos = set_hash<W>(S.arena[side] + size_t(r) * W) & (nc - 1);
           while (atomicCAS(&S.table[side][pos], kEmpty, r) != kEmpty) pos = (pos + 1) & (nc - 1);
         }
@@ -202,7 +211,7 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchInstance* __restr
       }
       const uint32_t tmask = tcap[side] - 1;
       // ---- expand and insert
-      for (uint32_t c = tid; c < ncand; c += kThreads) {
+      for (uint32_t c = tid; c < ncand; c += T) {
         const uint32_t a = c / m, j = lo[side] + (c - a * m);
         const uint64_t* x = S.arena[side] + size_t(j) * W;
         uint64_t r[W];
@@ -296,7 +305,7 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchInstance* __restr
       __syncthreads();
       // ---- resolve, scan, commit (candidate order, so rows are deterministic)
       uint32_t base = rows[side];
-      for (uint32_t c0 = 0; c0 < ncand; c0 += kThreads) {
+      for (uint32_t c0 = 0; c0 < ncand; c0 += T) {
         const uint32_t c = c0 + tid;
         uint32_t keep = 0, slot = kEmpty;
         if (c < ncand) {
@@ -304,7 +313,7 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchInstance* __restr
           keep = (slot != kEmpty && S.table[side][slot] == (kCand | c)) ? 1u : 0u;
         }
         uint32_t excl, total;
-        block_scan<W>(keep, sh, excl, total);
+        block_scan<T>(keep != 0, sh, excl, total);
         if (keep) {
           const uint32_t dst = base + excl;
           const uint64_t* src = S.cand + size_t(c) * W;
@@ -335,34 +344,52 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchInstance* __restr
       }
       __syncthreads();
       if (nf && nbk) {
-        // The smaller list is spread over the threads; groups of threads split
-        // the other list.
-        const bool f_small = nf <= nbk;
-        const uint32_t ns = f_small ? nf : nbk, nl = f_small ? nbk : nf;
-        const uint64_t* sbase = f_small ? S.arena[0] + size_t(lo[0]) * W : S.arena[1] + size_t(lo[1]) * W;
-        const uint64_t* lbase = f_small ? S.arena[1] + size_t(lo[1]) * W : S.arena[0] + size_t(lo[0]) * W;
-        for (uint32_t s0 = 0; s0 < ns; s0 += kThreads) {
-          const uint32_t span = ns - s0 < kThreads ? ns - s0 : kThreads;
-          const uint32_t groups = kThreads / span;
-          const uint32_t e = s0 + tid % span, g = tid / span;
-          if (g < groups) {
-            uint64_t x[W];
+        // Backward sets B = Q \ C by tiles of kGoalTile, transposed into LDS
+        // with 64 x 64 lane transposes; forward sets streamed by groups of
+        // kGoalWords lanes (lane l holds tile sets 64 l .. 64 l + 63): the
+        // AND of the rows of a forward set's states keeps the tile sets
+        // that contain it, and the group stops once none can.
+        const uint32_t lane = tid & 63, wv = tid >> 6;
+        const uint32_t gi = tid / kGoalWords, li = tid % kGoalWords;
+        const uint32_t gshift = (tid & 63) & ~(kGoalWords - 1);
+        const uint64_t gmask = ((1ull << kGoalWords) - 1) << gshift;
+        const uint64_t* fbase = S.arena[0] + size_t(lo[0]) * W;
+        const uint64_t* bbase = S.arena[1] + size_t(lo[1]) * W;
+        for (uint32_t t0 = 0; t0 < nbk; t0 += kGoalTile) {
+          const uint32_t tn = nbk - t0 < kGoalTile ? nbk - t0 : kGoalTile;
+          const uint32_t ng = (tn + 63) / 64;
+          for (uint32_t g = wv; g < ng; g += T / 64) {
+            const uint32_t j = g * 64 + lane;
+            const bool ok = j < tn;
 #pragma unroll
-            for (int w = 0; w < W; ++w) x[w] = sbase[size_t(e) * W + w];
-            uint32_t it = 0;
-            for (uint32_t j = g; j < nl; j += groups, ++it) {
-              if ((it & 31) == 0 && __hip_atomic_load(&s_found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
-                break;
-              const uint64_t* y = lbase + size_t(j) * W;
-              uint64_t acc = 0;
+            for (int w = 0; w < W; ++w) {
+              const uint64_t v = ok ? (~bbase[size_t(t0 + j) * W + w] & mask[w]) : 0ull;
+              const uint64_t tr = lane_transpose64(v);
+              if (uint32_t(w) * 64 + lane < n) gtile[(w * 64 + lane) * kGoalWords + g] = tr;
+            }
+          }
+          __syncthreads();
+          const uint64_t valid = li < ng ? (li + 1 < ng || (tn & 63) == 0 ? ~0ull : ((1ull << (tn & 63)) - 1)) : 0ull;
+          const uint32_t groups = T / kGoalWords;
+          for (uint32_t s0 = 0; s0 < nf; s0 += groups) {
+            if (__hip_atomic_load(&s_found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+            const uint32_t si = s0 + gi;
+            uint64_t acc = si < nf ? valid : 0ull;
+            if (si < nf) {
 #pragma unroll
-              for (int w = 0; w < W; ++w) acc |= x[w] & y[w];
-              if (!acc) {
-                const uint32_t fi = f_small ? e : j, bi = f_small ? j : e;
-                atomicMin(&s_wit, (unsigned long long)(uint64_t(fi) << 32 | bi));
-                s_found = 1;
-                break;
+              for (int w = 0; w < W; ++w) {
+                uint64_t bits = fbase[size_t(si) * W + w];
+                while (bits && (__ballot(acc != 0) & gmask)) {
+                  const uint32_t q = uint32_t(w) * 64 + __builtin_ctzll(bits);
+                  bits &= bits - 1;
+                  acc &= gtile[q * kGoalWords + li];
+                }
               }
+            }
+            if (acc) {
+              const uint32_t bi = t0 + li * 64 + uint32_t(__builtin_ctzll(acc));
+              atomicMin(&s_wit, (unsigned long long)(uint64_t(si) << 32 | bi));
+              s_found = 1;
             }
           }
           __syncthreads();
@@ -409,13 +436,15 @@ __global__ __launch_bounds__(kThreads) void k_batch(const BatchInstance* __restr
 hipError_t launch_batch(int W, const BatchInstance* inst, const uint8_t* deltas, const uint32_t* order,
     uint32_t count, uint32_t* head, BatchResult* out, uint8_t* words, uint32_t word_cap, uint8_t* scratch,
     size_t scratch_stride, const BatchLimits& L, uint32_t grid, hipStream_t st) {
+#define CASE(w)                                                                                              \
+  case w:                                                                                                    \
+    hipLaunchKernelGGL((k_batch<w, kBatchThreads>), dim3(grid), dim3(kBatchThreads), 0, st, inst, deltas, order, \
+        count, head, out, words, word_cap, scratch, scratch_stride, L);                                      \
+    return hipGetLastError();
   switch (W) {
-#define CASE(w) \
-  case w: hipLaunchKernelGGL(k_batch<w>, dim3(grid), dim3(kThreads), 0, st, inst, deltas, order, count, head, out, \
-      words, word_cap, scratch, scratch_stride, L); return hipGetLastError();
     CASE(1) CASE(2) CASE(3)
-#undef CASE
   }
+#undef CASE
   return hipErrorInvalidValue;
 }
 

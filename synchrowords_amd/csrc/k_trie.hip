@@ -88,29 +88,40 @@ __global__ __launch_bounds__(256) void k_trie_hist(const uint64_t* __restrict__ 
   __shared__ uint32_t h[kMaxW * 64];
   __shared__ uint32_t smin, smax;
   const TrieChunk c = chunks[blockIdx.x];
-  for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) h[t] = 0;
+  for (uint32_t t = threadIdx.x; t < W * 64; t += blockDim.x) h[t] = 0;
   if (threadIdx.x == 0) {
     smin = 0xffffffffu;
     smax = 0;
   }
   __syncthreads();
+  // 64-set groups per wave, counted by 64 x 64 transposes (lane j of the
+  // transposed word w holds the group's column of state 64 w + j).
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   uint32_t mn = 0xffffffffu, mx = 0;
-  for (uint32_t i = c.begin + threadIdx.x; i < c.end; i += blockDim.x) {
+  uint32_t part[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) part[w] = 0;
+  for (uint32_t g = c.begin + wv * 64; g < c.end; g += blockDim.x) {
+    const uint32_t i = g + lane;
+    const bool ok = i < c.end;
     uint64_t x[W];
-    trie_load<W>(src, i, cmask, n, x);
+#pragma unroll
+    for (int w = 0; w < W; ++w) x[w] = 0;
+    if (ok) trie_load<W>(src, i, cmask, n, x);
     uint32_t p = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      uint64_t v = x[w];
-      p += uint32_t(__popcll(v));
-      while (v) {
-        atomicAdd(&h[w * 64 + __builtin_ctzll(v)], 1u);
-        v &= v - 1;
-      }
+      p += uint32_t(__popcll(x[w]));
+      part[w] += uint32_t(__popcll(lane_transpose64(x[w])));
     }
-    mn = p < mn ? p : mn;
-    mx = p > mx ? p : mx;
+    if (ok) {
+      mn = p < mn ? p : mn;
+      mx = p > mx ? p : mx;
+    }
   }
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    if (part[w]) atomicAdd(&h[w * 64 + lane], part[w]);
   atomicMin(&smin, mn);
   atomicMax(&smax, mx);
   __syncthreads();
@@ -238,12 +249,104 @@ __global__ __launch_bounds__(256) void k_trie_bigleaf(TrieBuildArgs a, const Tri
 }
 
 // Whole subtrees of at most trie_local_cap(W) sets, one per workgroup (256
-// threads), in LDS: two set buffers and two tag buffers (dynamic LDS), the
-// state histogram and a stack of pending nodes.  Node ids: the job's root,
-// then job.desc_base + 2 per split (at most 2 (size - 1)).
+// threads), in LDS: two set buffers and two tag buffers (dynamic LDS).  Nodes
+// of more than 64 sets are split by the whole workgroup, one at a time
+// (state histogram, division bit, stable partition); a node of at most 64
+// sets goes to a queue, and then the four waves build those subtrees
+// side by side, one lane per set, with no workgroup barrier.  Node ids: the
+// job's root, then job.desc_base onwards in the order the workgroup meets
+// nodes: 2 per split of the workgroup, and 2 (s - 1) reserved for a queued
+// subtree of s sets (at most s - 1 splits below its root).  In all at most
+// 2 (size - 1): the workgroup's splits number one less than the queued
+// subtrees and its own leaves, and each of those leaves holds a set.
 struct LocalFrame {
   uint32_t node, b, e, depth_buf;  // depth << 1 | buffer holding [b, e)
 };
+struct SmallFrame {
+  uint32_t node, b, e, depth_buf, id_base;
+};
+constexpr uint32_t kTrieSmallQueue = 256;
+
+// One wave builds the subtree of at most 64 sets of frame f: node records
+// and the leaves' sets and tags written out; stack = this wave's frames.
+template <int W>
+__device__ void wave_subtree(const TrieBuildArgs& a, uint32_t job_begin, SmallFrame sf, uint64_t* S0, uint64_t* S1,
+    uint32_t* T0, uint32_t* T1, LocalFrame* stack, uint32_t lane) {
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  uint32_t sp = 1, next = sf.id_base;
+  stack[0] = LocalFrame{sf.node, sf.b, sf.e, sf.depth_buf};
+  while (sp) {
+    // Every lane writes and reads the same frames: each sees its own writes.
+    const LocalFrame f = stack[--sp];
+    const uint32_t depth = f.depth_buf >> 1, which = f.depth_buf & 1;
+    const uint64_t* A = which ? S1 : S0;
+    uint64_t* B = which ? S0 : S1;
+    const uint32_t* TA = which ? T1 : T0;
+    uint32_t* TB = which ? T0 : T1;
+    const uint32_t cnt = f.e - f.b;
+    const bool ok = lane < cnt;
+    uint64_t x[W];
+    uint32_t p = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      x[w] = ok ? A[size_t(f.b + lane) * W + w] : 0ull;
+      p += uint32_t(__popcll(x[w]));
+    }
+    const uint32_t tag = ok ? TA[f.b + lane] : 0u;
+    uint32_t mn = ok ? p : 0xffffffffu, mx = ok ? p : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      mn = min(mn, uint32_t(__shfl_xor(mn, off, 64)));
+      mx = max(mx, uint32_t(__shfl_xor(mx, off, 64)));
+    }
+    uint64_t best = 0;
+    if (cnt > kTrieLeafSets && depth < kTrieMaxDepth) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const uint32_t c = uint32_t(__popcll(lane_transpose64(x[w])));  // sets holding state 64 w + lane
+        const uint32_t q = uint32_t(w) * 64 + lane;
+        if (c > 0 && c < cnt && q < a.n) {
+          const uint64_t key = (uint64_t(c) << 32) | uint64_t(0xffffffffu - q);
+          best = key > best ? key : best;
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const uint64_t o = __shfl_xor(best, off, 64);
+        best = o > best ? o : best;
+      }
+    }
+    if (!best) {
+      if (lane == 0) a.nodes[f.node] = make_uint4(job_begin + f.b, cnt, kTrieLeafBit | (mn << 16), mx);
+      if (ok) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) a.out[uint64_t(job_begin + f.b + lane) * W + w] = x[w];
+        a.out_tags[job_begin + f.b + lane] = tag;
+      }
+      continue;
+    }
+    const uint32_t bit = 0xffffffffu - uint32_t(best);
+    const bool one = ok && ((x[bit >> 6] >> (bit & 63)) & 1ull);
+    const uint64_t bz = __ballot(ok && !one), bo = __ballot(one);
+    const uint32_t zc = uint32_t(__popcll(bz));
+    if (ok) {
+      const uint32_t d = one ? f.b + zc + uint32_t(__popcll(bo & below)) : f.b + uint32_t(__popcll(bz & below));
+#pragma unroll
+      for (int w = 0; w < W; ++w) B[size_t(d) * W + w] = x[w];
+      TB[d] = tag;
+    }
+    // The children's reads of B follow these writes in the wave's LDS order.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t z = next, o = next + 1;
+    next += 2;
+    if (lane == 0) a.nodes[f.node] = make_uint4(z, o, bit | (mn << 16), mx);
+    const uint32_t db = ((depth + 1) << 1) | (which ^ 1u);
+    stack[sp++] = LocalFrame{o, f.b + zc, f.e, db};
+    stack[sp++] = LocalFrame{z, f.b, f.b + zc, db};
+  }
+}
 
 template <int W>
 __global__ __launch_bounds__(256) void k_trie_local(TrieBuildArgs a, const TrieLocalJob* __restrict__ jobs) {
@@ -255,8 +358,10 @@ __global__ __launch_bounds__(256) void k_trie_local(TrieBuildArgs a, const TrieL
   uint32_t* T1 = T0 + CAP;
   __shared__ uint32_t h[kMaxW * 64];
   __shared__ LocalFrame stack[kTrieMaxDepth + 2];
+  __shared__ LocalFrame wstack[4][kTrieMaxDepth + 2];
+  __shared__ SmallFrame queue[kTrieSmallQueue];
   __shared__ LocalFrame cur;
-  __shared__ uint32_t sp, next_id, smin, smax, sbit, sones;
+  __shared__ uint32_t sp, qn, qhead, next_id, smin, smax, sbit, sones;
   __shared__ uint32_t wz[4], wo[4];
   const TrieLocalJob job = jobs[blockIdx.x];
   const uint32_t size = job.end - job.begin;
@@ -273,10 +378,19 @@ __global__ __launch_bounds__(256) void k_trie_local(TrieBuildArgs a, const TrieL
     T0[i] = stags ? stags[job.begin + i] : job.begin + i;
   }
   if (tid == 0) {
-    sp = 1;
-    stack[0] = LocalFrame{job.root, 0, size, job.depth << 1};
     next_id = job.desc_base;
+    qn = 0;
+    qhead = 0;
+    if (size <= 64) {  // the whole job is one small subtree
+      queue[0] = SmallFrame{job.root, 0, size, job.depth << 1, job.desc_base};
+      qn = 1;
+      sp = 0;
+    } else {
+      sp = 1;
+      stack[0] = LocalFrame{job.root, 0, size, job.depth << 1};
+    }
   }
+  // Block phase: nodes of more than 64 sets (or small ones the queue had no room for).
   while (true) {
     __syncthreads();
     if (tid == 0) {
@@ -296,24 +410,36 @@ __global__ __launch_bounds__(256) void k_trie_local(TrieBuildArgs a, const TrieL
     const uint32_t cnt = f.e - f.b;
     const bool split = cnt > kTrieLeafSets && depth < kTrieMaxDepth;
     if (split)
-      for (uint32_t t = tid; t < a.n; t += blockDim.x) h[t] = 0;
+      for (uint32_t t = tid; t < W * 64; t += blockDim.x) h[t] = 0;
     __syncthreads();
+    // State counts by 64-set groups, one per wave at a time: lane l takes
+    // set l of the group; a 64 x 64 transpose of each word puts the
+    // group's column of state 64 w + j in lane j, whose popcount is that
+    // state's count.  Each wave adds its counts to LDS once (no two lanes
+    // on one address), instead of one contended atomic per set bit.
     uint32_t mn = 0xffffffffu, mx = 0;
-    for (uint32_t i = f.b + tid; i < f.e; i += blockDim.x) {
+    uint32_t part[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) part[w] = 0;
+    for (uint32_t g = f.b + wv * 64; g < f.e; g += 256) {
+      const uint32_t i = g + lane;
+      const bool ok = i < f.e;
       uint32_t p = 0;
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        uint64_t v = A[size_t(i) * W + w];
+        const uint64_t v = ok ? A[size_t(i) * W + w] : 0ull;
         p += uint32_t(__popcll(v));
-        if (split)
-          while (v) {
-            atomicAdd(&h[w * 64 + __builtin_ctzll(v)], 1u);
-            v &= v - 1;
-          }
+        if (split) part[w] += uint32_t(__popcll(lane_transpose64(v)));
       }
-      mn = p < mn ? p : mn;
-      mx = p > mx ? p : mx;
+      if (ok) {
+        mn = p < mn ? p : mn;
+        mx = p > mx ? p : mx;
+      }
     }
+    if (split)
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        if (part[w]) atomicAdd(&h[w * 64 + lane], part[w]);
     atomicMin(&smin, mn);
     atomicMax(&smax, mx);
     __syncthreads();
@@ -374,9 +500,25 @@ __global__ __launch_bounds__(256) void k_trie_local(TrieBuildArgs a, const TrieL
       next_id += 2;
       a.nodes[f.node] = make_uint4(z, o, bit | (smin << 16), smax);
       const uint32_t db = ((depth + 1) << 1) | (which ^ 1u);
-      stack[sp++] = LocalFrame{o, f.b + zc, f.e, db};
-      stack[sp++] = LocalFrame{z, f.b, f.b + zc, db};
+      const LocalFrame kids[2] = {LocalFrame{o, f.b + zc, f.e, db}, LocalFrame{z, f.b, f.b + zc, db}};
+      for (const LocalFrame& k : kids) {
+        const uint32_t ks = k.e - k.b;
+        if (ks <= 64 && qn < kTrieSmallQueue) {
+          queue[qn++] = SmallFrame{k.node, k.b, k.e, k.depth_buf, next_id};
+          next_id += 2 * (ks - 1);
+        } else {
+          stack[sp++] = k;
+        }
+      }
     }
+  }
+  // Wave phase: the queued subtrees, one wave each at a time.
+  while (true) {
+    uint32_t qi = 0;
+    if (lane == 0) qi = atomicAdd(&qhead, 1u);
+    qi = __shfl(qi, 0, 64);
+    if (qi >= qn) break;
+    wave_subtree<W>(a, job.begin, queue[qi], S0, S1, T0, T1, wstack[wv], lane);
   }
 }
 
@@ -496,14 +638,39 @@ __global__ __launch_bounds__(256) void k_card_hist(const uint64_t* __restrict__ 
     if (h[t]) atomicAdd(&hist[t], h[t]);
 }
 
+// A workgroup takes 8 sets per thread, counts them per cardinality in LDS
+// and reserves each cardinality's range with one global atomic.
+constexpr int kCardPer = 8;
+
 template <int W>
 __global__ __launch_bounds__(256) void k_card_scatter(const uint64_t* __restrict__ sets, uint64_t count,
     uint64_t cmask, uint32_t n, uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  uint64_t x[W];
-  trie_load<W>(sets, i, cmask, n, x);
-  perm[atomicAdd(&cursor[popc_set<W>(x)], 1u)] = uint32_t(i);
+  __shared__ uint32_t cnt[kMaxW * 64 + 1], base[kMaxW * 64 + 1];
+  for (uint32_t t = threadIdx.x; t <= n; t += blockDim.x) cnt[t] = 0;
+  __syncthreads();
+  const uint64_t i0 = uint64_t(blockIdx.x) * blockDim.x * kCardPer + threadIdx.x;
+  uint32_t c[kCardPer], local[kCardPer];
+#pragma unroll
+  for (int e = 0; e < kCardPer; ++e) {
+    const uint64_t i = i0 + uint64_t(e) * blockDim.x;
+    c[e] = 0;
+    local[e] = 0;
+    if (i < count) {
+      uint64_t x[W];
+      trie_load<W>(sets, i, cmask, n, x);
+      c[e] = popc_set<W>(x);
+      local[e] = atomicAdd(&cnt[c[e]], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t <= n; t += blockDim.x)
+    if (cnt[t]) base[t] = atomicAdd(&cursor[t], cnt[t]);
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < kCardPer; ++e) {
+    const uint64_t i = i0 + uint64_t(e) * blockDim.x;
+    if (i < count) perm[base[c[e]] + local[e]] = uint32_t(i);
+  }
 }
 
 // ------------------------------------------------------------- launchers
@@ -631,7 +798,7 @@ hipError_t launch_card_scatter(int W, const uint64_t* sets, uint64_t count, int 
   if (!count) return hipSuccess;
   if (count > 0xffffffffull) return hipErrorInvalidValue;
   const uint64_t cm = complement ? ~0ull : 0ull;
-  const unsigned blocks = unsigned((count + 255) / 256);
+  const unsigned blocks = unsigned((count + 256 * kCardPer - 1) / (256 * kCardPer));
   switch (W) {
 #define C(w) case w: hipLaunchKernelGGL(k_card_scatter<w>, dim3(blocks), dim3(256), 0, st, sets, count, cm, n, cursor, perm); break;
     SW_TRIE_W(C)

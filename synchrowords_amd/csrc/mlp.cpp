@@ -160,6 +160,7 @@ void build_guide(uint32_t n, uint32_t k, const uint32_t* delta, Guide& out, hipS
 struct BeamWs {
   DevBuf dtf, dtb, list, cand, table, slot, keep, pos, uniq, uidx, x, scratch, keys, keys2, vals, vals2, small,
       scan_tmp, sort_tmp, hist, mf, mb;
+  DevBuf sb_in, sb_meta, sb_word;  // one-workgroup beam: seed and transitions, depth table, word
   std::vector<uint64_t> host_mf, host_mb;  // per-state masks of the tables, until the next call
   size_t scan_bytes = 0, sort_bytes = 0;
   uint64_t* host = nullptr;  // pinned read-back
@@ -579,7 +580,11 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
            &scan_tmp = ws.scan_tmp, &sort_tmp = ws.sort_tmp, &histd = ws.hist;
     size_t &scan_bytes = ws.scan_bytes, &sort_bytes = ws.sort_bytes;
     if (m) sw::build_tables_device(n, k, delta, 0, ws.host_mf, ws.mf, dtf, st);  // images: the features only
-    sw::build_tables_device(n, k, delta, 1, ws.host_mb, ws.mb, dtb, st);
+    bool tables_built = false;
+    auto build_preimage_tables = [&]() {
+      if (!tables_built) sw::build_tables_device(n, k, delta, 1, ws.host_mb, ws.mb, dtb, st);
+      tables_built = true;
+    };
     // The reference's start (beam.hpp:104-116): sink-component states with two
     // preimages under some letter, as singletons.
     const std::vector<char> sink = sw::sink_states(n, k, delta);
@@ -603,7 +608,68 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
     // kept set (its parent is index % parents, its letter index / parents).
     std::vector<uint64_t> hist_off, hist_m;
     uint64_t hist_len = 0;
-    for (uint64_t len = 1; len < upper; ++len) {
+    uint64_t len0 = 1;
+    // Automata of at most 64 states without a network: depths in one
+    // workgroup, one launch (k_beam_small.hip), while they fit its LDS; a
+    // depth that does not is continued below from the list and history the
+    // kernel hands back.
+    constexpr uint64_t kSmallDepths = 1u << 20;
+    constexpr uint32_t kSmallHist = 1u << 22;
+    if (!m && n <= 64 && k <= sw::kBeamSmallMaxK && upper <= kSmallDepths && m0 <= sw::kBeamSmallCand) {
+      DevBuf &sin = ws.sb_in, &smeta = ws.sb_meta, &sword = ws.sb_word;
+      const size_t din_bytes = size_t(n) * k * 4 + 64 * 8;
+      sin.ensure(din_bytes);
+      smeta.ensure(size_t(upper) * 8 + 8);
+      sword.ensure(size_t(upper) * 4 + 16 + 4);
+      histd.ensure(size_t(kSmallHist) * 4, st, hist_len * 4);
+      list.ensure(size_t(sw::kBeamSmallCand) * 8 + 8);
+      std::vector<uint8_t> hin(din_bytes, 0);
+      std::memcpy(hin.data(), seed.data(), seed.size() * 8);
+      std::memcpy(hin.data() + 64 * 8, delta, size_t(n) * k * 4);
+      SW_CHECK_HIP(hipMemcpyAsync(sin.p, hin.data(), din_bytes, hipMemcpyHostToDevice, st));
+      sw::BeamSmallArgs a{};
+      a.n = n;
+      a.k = k;
+      a.width = width;
+      a.upper = uint32_t(upper);
+      a.m0 = uint32_t(m0);
+      a.seed = sin.as<uint64_t>();
+      a.delta = reinterpret_cast<const uint32_t*>(sin.as<uint8_t>() + 64 * 8);
+      a.hist = histd.as<uint32_t>();
+      a.hist_cap = kSmallHist;
+      a.meta = smeta.as<uint32_t>();
+      a.meta_cap = uint32_t(upper);
+      // The four result words, then the word: one read-back for both.
+      a.out = sword.as<uint32_t>();
+      a.word = sword.as<uint32_t>() + 4;
+      a.list_out = list.as<uint64_t>();
+      SW_CHECK_HIP(sw::launch_beam_small(a, st));
+      std::vector<uint32_t> back(4 + size_t(upper));
+      SW_CHECK_HIP(hipMemcpyAsync(back.data(), sword.p, back.size() * 4, hipMemcpyDeviceToHost, st));
+      SW_CHECK_HIP(hipStreamSynchronize(st));
+      const uint32_t* o = back.data();
+      if (o[0] == sw::kBeamSmallFound) {
+        std::vector<uint32_t> w(back.begin() + 4, back.begin() + 4 + o[1]);
+        *bound = o[1];
+        if (word_len) *word_len = w.size();
+        if (word && w.size() <= word_cap) std::copy(w.begin(), w.end(), word);
+        return SW_OK;
+      }
+      if (o[0] != sw::kBeamSmallHandover) return SW_OK;  // no depth below upper produced Q
+      // Continue at depth o[1] with the kernel's list and history.
+      std::vector<uint32_t> meta(size_t(o[1] - 1) * 2);
+      if (!meta.empty())
+        SW_CHECK_HIP(hipMemcpy(meta.data(), smeta.p, meta.size() * 4, hipMemcpyDeviceToHost));
+      for (size_t d = 0; d + 1 < size_t(o[1]); ++d) {
+        hist_off.push_back(meta[2 * d]);
+        hist_m.push_back(meta[2 * d + 1]);
+      }
+      hist_len = o[3];
+      m0 = o[2];
+      len0 = o[1];
+    }
+    build_preimage_tables();
+    for (uint64_t len = len0; len < upper; ++len) {
       const uint64_t cnt = m0 * k;
       cand.ensure(cnt * W * 8 + 8);
       uint64_t tcap = 1024;

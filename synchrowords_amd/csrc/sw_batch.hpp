@@ -8,6 +8,7 @@ namespace sw {
 
 constexpr uint32_t kBatchMaxK = 4;  // letters handled inside the persistent kernel
 constexpr uint32_t kBatchMaxW = 3;  // words per set (n <= 192)
+constexpr int kBatchThreads = 256;  // threads per workgroup of the persistent launch
 
 enum BatchStatus : uint32_t {
   kBatchPending = 0,

@@ -99,6 +99,8 @@ struct sw_ctx {
   sw::DevBuf ix_sort, ix_sperm, ix_tperm, ix_head;
   sw::DevBuf ix_anchor, ix_tlist, ix_work;  // anchor buckets: anchors, inverted lists, work items
   std::vector<uint32_t> ix_counts;         // host copy of the bucket sizes
+  // One-workgroup exact search of a small automaton: instance, scratch, result and word.
+  sw::DevBuf wg_in, wg_scratch, wg_out;
   // Power-set BFS batches: jobs, packed automata, frontiers, lengths.
   sw::DevBuf brute_jobs, brute_delta, brute_frontier, brute_out;
   // Eppstein batches: jobs, packed automata, pair-BFS scratch, words, results;
@@ -164,7 +166,11 @@ struct ExactOutcome {
 // Forward levels of the exact search (images of Q up to `depth`).
 void forward_levels(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t depth, uint64_t cap_rows,
     DevBuf& out, std::vector<uint64_t>& level_end);
+// allow_workgroup: automata of at most 64 states first try the one-workgroup
+// search (batch.cpp, exact_search_workgroup).
 ExactOutcome exact_search(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint64_t lower, uint64_t upper,
-    const sw_exact_config& cfg);
+    const sw_exact_config& cfg, bool allow_workgroup = true);
+bool exact_search_workgroup(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint64_t upper,
+    ExactOutcome& out);
 
 }  // namespace sw

@@ -191,6 +191,34 @@ hipError_t launch_card_hist(int W, const uint64_t* sets, uint64_t count, int com
 hipError_t launch_card_scatter(int W, const uint64_t* sets, uint64_t count, int complement, uint32_t n,
     uint32_t* cursor, uint32_t* perm, hipStream_t st);
 
+// Beam of an automaton of at most 64 states in one workgroup
+// (k_beam_small.hip): every depth in one launch while a depth's candidates
+// fit kBeamSmallCand; out[0] = status, out[1] = the depth it stopped at (the
+// bound when found), out[2] = the list size there (list_out), out[3] = the
+// history entries written; meta[2 d], meta[2 d + 1] = history offset and list
+// size of depth d + 1; word[0 .. bound) on a find.
+constexpr uint32_t kBeamSmallCand = 2048;
+constexpr uint32_t kBeamSmallMaxK = 16;
+enum BeamSmallStatus : uint32_t {
+  kBeamSmallFound = 1,      // a depth produced Q
+  kBeamSmallExhausted = 2,  // every depth below upper done, no Q
+  kBeamSmallEmpty = 3,      // the list became empty
+  kBeamSmallHandover = 4,   // a depth too large for LDS: continue on the device
+};
+struct BeamSmallArgs {
+  uint32_t n, k, width, upper, m0;
+  const uint64_t* seed;
+  const uint32_t* delta;
+  uint32_t* hist;
+  uint32_t hist_cap;
+  uint32_t* meta;
+  uint32_t meta_cap;
+  uint32_t* word;
+  uint64_t* list_out;
+  uint32_t* out;
+};
+hipError_t launch_beam_small(const BeamSmallArgs& a, hipStream_t st);
+
 // Power-set BFS (k_brute.hip).  One automaton per workgroup with its bitmap
 // in LDS (brute_lds_bytes(n, k) <= kBruteMaxLds): frontier_off
 // indexes two buffers of 2^n subsets each.  Or one level over the device

@@ -75,13 +75,13 @@ def test_eppstein_non_synchronizing(ctx, n):
 
 
 def test_eppstein_single_and_batch_paths_agree(ctx):
-    """One automaton of 64 states or more runs its pair BFS over the whole
+    """One automaton of 128 states or more runs its pair BFS over the whole
     device; in a batch the same automaton runs in one workgroup.  Flags,
     bounds and words are the same."""
     rng = np.random.default_rng(21)
-    inst = [(int(rng.integers(64, 260)), int(rng.integers(2, 4)), None) for _ in range(12)]
+    inst = [(int(rng.integers(128, 300)), int(rng.integers(2, 4)), None) for _ in range(12)]
     inst = [(n, k, gen.random_dfa(n, k, 9100 + j)) for j, (n, k, _) in enumerate(inst)]
-    inst += [(64, 2, gen.cerny(64)), (80, 2, _two_sinks(80))]
+    inst += [(128, 2, gen.cerny(128)), (64, 2, gen.cerny(64)), (160, 2, _two_sinks(160))]
     ups = [(n - 1) ** 2 + 1 for n, _, _ in inst]
     bns, bbounds, bwords = ctx.eppstein_batch(inst, ups, word_cap=1 << 16)
     for j, (n, k, d) in enumerate(inst):

@@ -114,6 +114,23 @@ def test_device_beam_wide_crosses_the_counted_order(ctx, n, k, width):
         assert len(w) == b and replay(n, k, d, w) == 1
 
 
+@pytest.mark.parametrize("n,k,width", [(64, 3, 3000), (50, 4, 1500), (64, 2, 1100), (33, 16, 400)])
+def test_device_beam_small_automata_hand_over(ctx, n, k, width):
+    """Automata of at most 64 states run the Beam's depths in one workgroup
+    until a depth's candidates outgrow its LDS (2048), then the device-wide
+    beam continues from the list and history it hands back: the oracle's
+    bound at the same width, and a word that replays."""
+    import oracle_lib as orc
+    for seed in range(2):
+        d = gen.random_dfa(n, k, 9500 + seed + n)
+        cap = (n - 1) ** 2 + 1
+        want = orc.beam_width(n, k, d, width, cap)
+        b, w = ctx.beam(n, k, d, width, cap)
+        assert b == want, (seed, b, want)
+        if b < cap:
+            assert len(w) == b and replay(n, k, d, w) == 1
+
+
 @pytest.mark.parametrize("n,k", [(20, 2), (48, 2), (64, 3), (100, 2), (200, 2), (300, 2), (130, 4), (512, 2)])
 @pytest.mark.parametrize("width", [1, 7, 64, 500])
 def test_device_beam_matches_reference_order(ctx, n, k, width):
@@ -152,16 +169,19 @@ def test_learned_bound_exact_search(ctx, case):
 
 
 def _trie_cases():
+    """(mode, case): every mode on C1 and C3; modes 1 and 2 on C5 (superset
+    walks over a backward trie visit most of its nodes, long at n = 300)."""
     out = []
     for nm in ("c1_random48.json", "c3_random200.json", "c5_random300.json"):
         cs = [c for c in golden(nm) if not c["non_synchro"]]
-        out += [pytest.param(c, id=c["name"]) for c in (cs[:12] if nm == "c1_random48.json" else cs)]
+        cs = cs[:12] if nm == "c1_random48.json" else cs
+        for mode in ((1, 2) if nm == "c5_random300.json" else (1, 2, 3)):
+            out += [pytest.param(mode, c, id=f"{mode}-{c['name']}") for c in cs]
     return out
 
 
-@pytest.mark.parametrize("case", _trie_cases())
-@pytest.mark.parametrize("mode", [1, 2, 3])
-def test_meet_checks_by_kernel_or_trie(ctx, case, mode):
+@pytest.mark.parametrize("mode,case", _trie_cases())
+def test_meet_checks_by_kernel_or_trie(ctx, mode, case):
     """Meet checks by the vertical subset kernel (1), by the device subset
     trie over the forward frontier (2), or by tries over the frontier that
     did not step (3: superset queries after forward steps), from the trivial
