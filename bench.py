@@ -251,6 +251,74 @@ def replay_ok(n: int, d, word) -> bool:
     return sw.replay(n, 2, d, word) == 1
 
 
+def bench_frontier(ctx, steps: int, warmup: int, log2_frontier: int = 23) -> dict:
+    """One search step at scale (sw_frontier_step): a random binary DFA with
+    n = 300 (W = 5, the size of the n=300 exact searches), a frontier of
+    2^23 random sets (one in eight repeated), a visited list of the images
+    of the frontier's first half, so that about half of the candidates are
+    visited and an eighth are duplicates.  Per kernel: HIP-event time on the
+    library's stream, algorithmic bytes and the fraction of 8 TB/s; for the
+    hash insert, probes per second."""
+    import synchrowords_amd as sw
+    n, k = 300, 2
+    W = (n + 63) // 64
+    m = 1 << log2_frontier
+    d = np.array(sw.random_dfa(n, k, 7), dtype=np.uint32)
+    rng = np.random.default_rng(20251204)
+    f = rng.integers(0, 2**64 - 1, size=(m, W), dtype=np.uint64, endpoint=True)
+    f &= rng.integers(0, 2**64 - 1, size=(m, W), dtype=np.uint64, endpoint=True)  # p = 1/4 per state
+    f[:, W - 1] &= np.uint64((1 << (n - 64 * (W - 1))) - 1)
+    rep = m // 8
+    f[m - rep:] = f[rng.integers(0, m - rep, size=rep)]
+    df = sw.DeviceArray.from_host(ctx, f)
+    del f
+    half = m // 2
+    dv = sw.DeviceArray(ctx, half * k * W * 8)
+    nv, _ = ctx.frontier_step_device(n, k, d, df, half, None, 0, dv)
+    dout = sw.DeviceArray(ctx, m * k * W * 8)
+    dpar = sw.DeviceArray(ctx, m * k * 4)
+    dlet = sw.DeviceArray(ctx, m * k)
+    for _ in range(warmup):
+        ctx.frontier_step_device(n, k, d, df, m, dv, nv, dout, dpar, dlet)
+    runs = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        nnew, st = ctx.frontier_step_device(n, k, d, df, m, dv, nv, dout, dpar, dlet)
+        runs.append(st)
+    wall = (time.perf_counter() - t0) / max(1, steps)
+    for a in (df, dv, dout, dpar, dlet):
+        a.free()
+    med = lambda key: float(np.median([r[key] for r in runs]))
+    cand = k * m
+    probes = runs[-1]["probes"]
+    rb = 8 * W
+    kernels = {
+        # parents read once per letter's pass are L2 hits; children written
+        "k_expand": (med("ms_expand"), m * rb + cand * rb, "parent 8W read + child 8W written"),
+        # candidate read, slot index written, one 4-byte slot read per probe
+        "k_hash_insert": (med("ms_insert"), cand * (rb + 4) + probes * 4,
+                          "candidate 8W read + slot_of 4 B written + 4 B per probe"),
+        # slot_of read, owner slot read, keep and position written
+        "keep_scan": (med("ms_scan"), cand * 16, "slot_of 4 B + slot 4 B read, keep and pos 4 B each written"),
+        # keep flags read; per new set: position read, set read and written, parent and letter written
+        "k_commit": (med("ms_commit"), cand * 4 + nnew * (4 + 2 * rb + 4 + 1),
+                     "keep 4 B per candidate; per new set pos 4 B, set 8W read + 8W written, parent 4 B, letter 1 B"),
+    }
+    out = {"workload": f"one search step, random binary DFA n={n} (W={W}), frontier 2^{log2_frontier} sets "
+                       f"(1 in 8 repeated), visited list {nv} sets (images of the first half)",
+           "candidates": cand, "new_sets": nnew, "visited": nv, "probes": probes,
+           "mean_probes_per_candidate": round(probes / cand, 4), "table_slots": runs[-1]["table_slots"],
+           "steps": steps, "wall_ms_per_step": round(wall * 1e3, 3), "kernels": {}}
+    for name, (ms, nbytes, unit) in kernels.items():
+        gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        out["kernels"][name] = {"ms": round(ms, 4), "algorithmic_bytes": int(nbytes), "unit_bytes": unit,
+                                "GB_s": round(gbs, 1), "frac_of_8TBs": round(gbs / 8000.0, 4)}
+    ins = kernels["k_hash_insert"][0]
+    out["probes_per_s"] = probes / (ins * 1e-3) if ins > 0 else 0.0
+    out["candidates_per_s_step"] = cand / (sum(v[0] for v in kernels.values()) * 1e-3)
+    return out
+
+
 def bench_batch(ctx, count: int, cap_rows: int = 0, max_pairs: int = 0) -> dict:
     """The 4096-instance batch (n in [16,160], k in {2,3}) in one persistent launch."""
     from synchrowords_amd import gen
@@ -456,11 +524,12 @@ def bench_learned(ctx, scorer: str, seeds, width: int) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["subset", "batch", "learned", "exact", "brute", "eppstein", "times"],
+    ap.add_argument("--workload", choices=["subset", "batch", "learned", "exact", "brute", "eppstein", "times",
+                                           "frontier"],
                     default="subset",
                     help="subset: the headline line; batch / learned: the batched and learned-bound reports; "
                          "exact: one verbose exact search (--exact-n, --exact-seed); brute: power-set BFS; "
-                         "eppstein: Eppstein's bound")
+                         "eppstein: Eppstein's bound; frontier: one search step at 2^23 frontier sets")
     ap.add_argument("--exact-n", type=int, default=300)
     ap.add_argument("--exact-seed", type=int, default=1)
     ap.add_argument("--batch-count", type=int, default=4096)
@@ -516,6 +585,8 @@ def main() -> int:
             line = bench_eppstein(ctx, args.batch_count, args.no_cpu)
         elif args.workload == "times":
             line = bench_times(ctx)
+        elif args.workload == "frontier":
+            line = bench_frontier(ctx, args.steps, args.warmup)
         elif args.workload == "exact":
             d = sw.random_dfa(args.exact_n, 2, args.exact_seed)
             ub = ctx.pipeline(args.exact_n, 2, d).beam_bound  # warm; the plan's Beam bound caps the search

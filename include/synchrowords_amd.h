@@ -93,6 +93,27 @@ int sw_transition_apply(sw_ctx* ctx, uint32_t n, const uint64_t* table, const ui
 int sw_expand(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, int dir, const uint64_t* in,
     uint64_t cnt, uint64_t* out, int where);
 
+/* One breadth-first step of the meet-in-the-middle search with a visited
+ * list: the images (dir 0) or preimages (dir 1) of every frontier set under
+ * every letter, each distinct set once, none that `visited` holds.  Replaces
+ * the apply, sort_keep_unique and visited-merge part of
+ * MeetInTheMiddle::bfs_step with bfs_novisited false
+ * (synchrolib/algorithm/exact/meet_in_the_middle.cpp:186-229), without its
+ * dominance reduction; dir 1 applies preimages, as invbfs_step does
+ * (:245-263), with the same deduplication.  Order: candidate order
+ * (candidate a*cnt + i is frontier[i] under letter a), each set at its first
+ * candidate; the reference returns them sorted.  `out` holds k*cnt sets;
+ * out_parent[j] (frontier index) and out_letter[j] may be NULL.  visited may
+ * hold duplicates.  where: SW_HOST or SW_DEVICE for every list.  stats may be
+ * NULL; its kernel times are HIP events on the context's stream. */
+typedef struct sw_frontier_stats {
+  uint64_t candidates, new_sets, probes, table_slots;
+  double ms_expand, ms_insert, ms_scan, ms_commit;
+} sw_frontier_stats;
+int sw_frontier_step(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, int dir, const uint64_t* visited,
+    uint64_t visited_cnt, const uint64_t* frontier, uint64_t cnt, uint64_t* out, uint32_t* out_parent,
+    uint8_t* out_letter, uint64_t* out_count, int where, sw_frontier_stats* stats);
+
 /* Implicit-trie leaf test, the reference's SubsetsImplicitTrieKernel<N,Proper>
  * (synchrolib/data_structures/cuda/subsets_implicit_trie_kernel.hpp:10-34):
  * create == allocate(small, small_cnt, large_cnt) (copies `small`, host);

@@ -68,6 +68,12 @@ class _BatchConfig(C.Structure):
                 ("overflow_workers", C.c_uint32), ("max_pairs", C.c_uint64)]
 
 
+class _FrontierStats(C.Structure):
+    _fields_ = [("candidates", C.c_uint64), ("new_sets", C.c_uint64), ("probes", C.c_uint64),
+                ("table_slots", C.c_uint64), ("ms_expand", C.c_double), ("ms_insert", C.c_double),
+                ("ms_scan", C.c_double), ("ms_commit", C.c_double)]
+
+
 class _BatchStats(C.Structure):
     _fields_ = [("ms_host_prep", C.c_double), ("ms_kernel", C.c_double), ("ms_overflow", C.c_double),
                 ("ms_total", C.c_double), ("non_synchro", C.c_uint32), ("found", C.c_uint32),
@@ -97,6 +103,8 @@ _SIGS = {
     "sw_dev_copy": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "sw_transition_apply": (C.c_int, [_P, C.c_uint32, _P, _P, _P, C.c_uint64, C.c_int]),
     "sw_expand": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_int, _P, C.c_uint64, _P, C.c_int]),
+    "sw_frontier_step": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_int, _P, C.c_uint64, _P, C.c_uint64, _P, _P, _P,
+                                   _U64P, C.c_int, _P]),
     "sw_leaf_create": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "sw_leaf_run": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_uint32, _P, C.c_uint32, _P]),
     "sw_leaf_destroy": (None, [_P]),
@@ -475,6 +483,25 @@ class Context:
         _check(load().sw_expand(self.handle, n, k, _ptr(d), int(backward), _ptr(s), s.shape[0], _ptr(out), SW_HOST))
         return out
 
+    def frontier_step(self, n: int, k: int, delta, frontier, visited=None, backward: bool = False):
+        """One search step (sw_frontier_step): (new sets, parent index, letter,
+        stats dict), new sets in candidate order, each at its first candidate."""
+        f = _sets(frontier, n)
+        v = _sets(visited if visited is not None else np.zeros((0, words(n)), dtype=np.uint64), n)
+        d = _delta(n, k, delta)
+        cap = k * f.shape[0]
+        out = np.zeros((max(cap, 1), words(n)), dtype=np.uint64)
+        par = np.zeros(max(cap, 1), dtype=np.uint32)
+        let = np.zeros(max(cap, 1), dtype=np.uint8)
+        cnt = C.c_uint64(0)
+        st = _FrontierStats()
+        _check(load().sw_frontier_step(self.handle, n, k, _ptr(d), int(backward), _ptr(v), v.shape[0], _ptr(f),
+                                       f.shape[0], _ptr(out), _ptr(par), _ptr(let), C.byref(cnt), SW_HOST,
+                                       C.byref(st)))
+        m = cnt.value
+        stats = {name: getattr(st, name) for name, _ in _FrontierStats._fields_}
+        return out[:m], par[:m], let[:m], stats
+
     def subset_check(self, n: int, small, large, thresh: int = 1, per_row: bool = False, count: bool = True):
         """(results or None, pair_count or None, any) of the list-vs-list checker."""
         s = _sets(small, n)
@@ -494,6 +521,20 @@ class Context:
         _check(load().sw_subset_check(self.handle, n, small.ptr, small_cnt, large.ptr, large_cnt, 1, SW_DEVICE,
                                       None, C.byref(cnt) if count else None, C.byref(anyf)))
         return (cnt.value if count else None), bool(anyf.value)
+
+    def frontier_step_device(self, n: int, k: int, delta, frontier: DeviceArray, frontier_cnt: int,
+                             visited: DeviceArray | None, visited_cnt: int, out: DeviceArray,
+                             parent: DeviceArray | None = None, letter: DeviceArray | None = None,
+                             backward: bool = False):
+        """sw_frontier_step on device-resident lists: (new-set count, stats)."""
+        d = _delta(n, k, delta)
+        cnt = C.c_uint64(0)
+        st = _FrontierStats()
+        _check(load().sw_frontier_step(self.handle, n, k, _ptr(d), int(backward), visited.ptr if visited else None,
+                                       visited_cnt, frontier.ptr, frontier_cnt, out.ptr,
+                                       parent.ptr if parent else None, letter.ptr if letter else None,
+                                       C.byref(cnt), SW_DEVICE, C.byref(st)))
+        return cnt.value, {name: getattr(st, name) for name, _ in _FrontierStats._fields_}
 
     def subset_check_noalloc(self, n: int, small: DeviceArray, small_cnt: int, large: DeviceArray, large_cnt: int,
                              thresh: int = 1, results: DeviceArray | None = None,

@@ -105,6 +105,8 @@ void sw_ctx_destroy(sw_ctx* ctx) {
   ctx->exact_ws = nullptr;
   if (ctx->beam_ws && ctx->beam_ws_free) ctx->beam_ws_free(ctx->beam_ws);
   ctx->beam_ws = nullptr;
+  if (ctx->frontier_ws && ctx->frontier_ws_free) ctx->frontier_ws_free(ctx->frontier_ws);
+  ctx->frontier_ws = nullptr;
   if (ctx->ix_host) (void)hipHostFree(ctx->ix_host);
   ctx->ix_host = nullptr;
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);

@@ -84,8 +84,9 @@ struct Side {
 // that repeated searches (the batch's overflow path) allocate nothing and
 // never call hipFree, which would synchronise the whole device.
 struct Workspace {
-  DevBuf tables[2], masks[2];
+  DevBuf tables[2], masks[2], targets;
   std::vector<uint64_t> host_masks[2];
+  std::vector<uint16_t> host_targets;
   Side F, B;
   DevBuf cand, valid, slot, keep, pos, small, scan_tmp, flags, probes;
   size_t scan_bytes = 0;
@@ -379,6 +380,7 @@ class Search {
   void upload_tables() {
     for (int dir = 0; dir < 2; ++dir)
       build_tables_device(n_, k_, delta_, dir, ws_.host_masks[dir], ws_.masks[dir], tables_[dir], st_);
+    upload_targets(n_, k_, delta_, ws_.host_targets, ws_.targets, st_);
     small_.ensure(64);
     probes_.ensure(8);
     SW_CHECK_HIP(hipMemsetAsync(probes_.p, 0, 8, st_));
@@ -494,8 +496,9 @@ class Search {
     pos_.ensure(cnt * 4 + 4);
     ensure_scan(cnt);
     const int W = int(W_);
-    SW_CHECK_HIP(launch_expand(W, S.front.as<uint64_t>(), m, tables_[backward ? 1 : 0].as<uint64_t>(), slices_, k_,
-        nullptr, n_, backward ? 1 : 0, cand_.as<uint64_t>(), valid_.as<uint32_t>(), st_));
+    SW_CHECK_HIP(launch_expand_sets(W, S.front.as<uint64_t>(), m, tables_[backward ? 1 : 0].as<uint64_t>(), slices_,
+        ws_.targets.as<uint16_t>(), backward ? 1 : 0, k_, nullptr, n_, backward ? 1 : 0, cand_.as<uint64_t>(),
+        valid_.as<uint32_t>(), st_));
     ensure_table(S, cnt);
     SW_CHECK_HIP(launch_hash_insert(W, cand_.as<uint64_t>(), valid_.as<uint32_t>(), cnt, S.table.as<uint32_t>(),
         S.tcap - 1, S.arena.as<uint64_t>(), slot_.as<uint32_t>(), probes_.as<unsigned long long>(), st_));
@@ -738,8 +741,8 @@ class Search {
     while (tcap < 2 * cnt + 1024) tcap <<= 1;
     ws_.dfs_table.ensure(tcap * 4);
     uint32_t* table = ws_.dfs_table.as<uint32_t>();
-    SW_CHECK_HIP(launch_expand(W, list + b * W_, p, tables_[1].as<uint64_t>(), slices_, k_, nullptr, n_, 1,
-        cand_.as<uint64_t>(), valid_.as<uint32_t>(), st_));
+    SW_CHECK_HIP(launch_expand_sets(W, list + b * W_, p, tables_[1].as<uint64_t>(), slices_,
+        ws_.targets.as<uint16_t>(), 1, k_, nullptr, n_, 1, cand_.as<uint64_t>(), valid_.as<uint32_t>(), st_));
     SW_CHECK_HIP(launch_hash_lookup(W, cand_.as<uint64_t>(), valid_.as<uint32_t>(), cnt, B_.table.as<uint32_t>(),
         B_.tcap - 1, B_.arena.as<uint64_t>(), st_));
     SW_CHECK_HIP(launch_fill_u32(table, kEmpty, tcap, st_));
@@ -929,6 +932,15 @@ void build_tables_device(uint32_t n, uint32_t k, const uint32_t* delta, int dir,
   out.ensure(size_t(k) * S * 256 * W * 8);
   SW_CHECK_HIP(hipMemcpyAsync(masks.p, host.data(), host.size() * 8, hipMemcpyHostToDevice, st));
   SW_CHECK_HIP(launch_tables(int(W), masks.as<uint64_t>(), n, k, out.as<uint64_t>(), st));
+}
+
+void upload_targets(uint32_t n, uint32_t k, const uint32_t* delta, std::vector<uint16_t>& host, DevBuf& out,
+    hipStream_t st) {
+  host.resize(size_t(k) * n);
+  for (uint32_t q = 0; q < n; ++q)
+    for (uint32_t a = 0; a < k; ++a) host[size_t(a) * n + q] = uint16_t(delta[size_t(q) * k + a]);
+  out.ensure(host.size() * 2 + 8);
+  SW_CHECK_HIP(hipMemcpyAsync(out.p, host.data(), host.size() * 2, hipMemcpyHostToDevice, st));
 }
 
 void forward_levels(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t depth, uint64_t cap_rows,

@@ -3,8 +3,6 @@
 #include "sw_device.hpp"
 #include "sw_kernels.hpp"
 
-#include <hipcub/hipcub.hpp>
-
 namespace sw {
 
 // ---------------------------------------------------------------- expansion
@@ -88,6 +86,106 @@ static hipError_t expand_w(const uint64_t* parents, uint64_t m, const uint64_t* 
   return hipGetLastError();
 }
 
+// Expansion of large launches, bit-sliced: a wave takes 64 parents, loads
+// them coalesced through LDS and transposes each word across the lanes
+// (lane_transpose64), so that lane l holds, per word w, the 64-bit mask of
+// the parents that contain state q = 64 w + l.  Then per letter a:
+//   image:    the child mask of state t is the OR of the masks of the states
+//             q with delta(q, a) = t, one 64-bit LDS atomic OR per state;
+//   preimage: the child mask of state q is the parent mask of delta(q, a),
+//             one 64-bit LDS read per state
+// (the preimage of a set commutes with the complement, so the complemented
+// rows of the backward side expand the same way), and the masks are
+// transposed back and written coalesced through LDS.  About W transposes per
+// letter and per 64 children, against ceil(n/8) slice-table gathers per
+// child in k_expand (1.77 ms for 2^24 children at n = 300, VALU-bound by set
+// bits at 1.68 ms).  targets[a * n + q] = delta(q, a).
+constexpr uint32_t kSlicedWaves = 4;
+
+template <int W, bool BACK>
+__global__ __launch_bounds__(64 * kSlicedWaves) void k_expand_sliced(const uint64_t* __restrict__ parents, uint64_t m,
+    const uint16_t* __restrict__ targets, uint32_t letters, const uint32_t* __restrict__ letter_list, uint32_t n,
+    int drop_mode, uint64_t* __restrict__ out, uint32_t* __restrict__ valid) {
+  __shared__ uint64_t stage[kSlicedWaves][64 * W];
+  __shared__ uint64_t pmask[BACK ? kSlicedWaves : 1][BACK ? 64 * W : 1];
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t j0 = (uint64_t(blockIdx.x) * kSlicedWaves + wv) * 64;
+  const uint64_t words_here = j0 < m ? (m - j0 < 64 ? (m - j0) * W : uint64_t(64) * W) : 0;
+  uint64_t* my = stage[wv];
+  // Parents in, coalesced: word e of the wave's block is my[e].
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const uint32_t e = uint32_t(w) * 64 + lane;
+    my[e] = e < words_here ? parents[j0 * W + e] : 0ull;
+  }
+  __syncthreads();
+  uint64_t pres[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) pres[w] = lane_transpose64(my[lane * W + w]);
+  if (BACK) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) pmask[wv][w * 64 + lane] = pres[w];
+  }
+  __syncthreads();
+  for (uint32_t ai = 0; ai < letters; ++ai) {
+    const uint32_t a = letter_list ? letter_list[ai] : ai;
+    const uint16_t* ta = targets + uint64_t(a) * n;
+    uint64_t y[W];
+    if (BACK) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const uint32_t q = uint32_t(w) * 64 + lane;
+        y[w] = q < n ? pmask[wv][ta[q]] : 0ull;
+      }
+    } else {
+#pragma unroll
+      for (int w = 0; w < W; ++w) my[w * 64 + lane] = 0ull;
+      __syncthreads();
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const uint32_t q = uint32_t(w) * 64 + lane;
+        if (q < n && pres[w]) atomicOr(reinterpret_cast<unsigned long long*>(&my[ta[q]]), pres[w]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int w = 0; w < W; ++w) y[w] = my[w * 64 + lane];
+      __syncthreads();
+    }
+    // Back to one child per lane, then out through LDS.
+    int pc = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      y[w] = lane_transpose64(y[w]);
+      pc += __popcll(y[w]);
+      my[lane * W + w] = y[w];
+    }
+    const uint64_t base = uint64_t(ai) * m + j0;
+    if (valid && j0 + lane < m) valid[base + lane] = drop_mode == 1 ? (pc + 1 < int(n) ? 1u : 0u) : 1u;
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const uint32_t e = uint32_t(w) * 64 + lane;
+      if (e < words_here) out[base * W + e] = my[e];
+    }
+    __syncthreads();
+  }
+}
+
+template <int W>
+static hipError_t expand_sliced_w(const uint64_t* parents, uint64_t m, const uint16_t* targets, int dir,
+    uint32_t letters, const uint32_t* letter_list, uint32_t n, int drop_mode, uint64_t* out, uint32_t* valid,
+    hipStream_t st) {
+  if (!m || !letters) return hipSuccess;
+  const uint64_t blocks = (m + 64 * kSlicedWaves - 1) / (64 * kSlicedWaves);
+  if (dir)
+    hipLaunchKernelGGL((k_expand_sliced<W, true>), dim3(unsigned(blocks)), dim3(64 * kSlicedWaves), 0, st, parents, m,
+        targets, letters, letter_list, n, drop_mode, out, valid);
+  else
+    hipLaunchKernelGGL((k_expand_sliced<W, false>), dim3(unsigned(blocks)), dim3(64 * kSlicedWaves), 0, st, parents,
+        m, targets, letters, letter_list, n, drop_mode, out, valid);
+  return hipGetLastError();
+}
+
 // Slice tables from per-state masks: row (a, s, b) = OR of masks[a][8 s + j]
 // over the set bits j of b (masks: the image singleton of each state, or its
 // preimage set).  One thread per row.
@@ -143,7 +241,13 @@ __global__ __launch_bounds__(256) void k_hash_insert(const uint64_t* __restrict_
       uint64_t pos = set_hash<W>(s) & mask;
       while (true) {
         ++nprobe;
-        uint32_t v = __hip_atomic_load(&table[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // Claim first: at most half the slots are taken, so the first CAS
+        // usually succeeds; a failed one returns the slot's holder.
+        uint32_t v = atomicCAS(&table[pos], kEmpty, mine);
+        if (v == kEmpty) {
+          res = uint32_t(pos);
+          goto done;
+        }
         bool advance = false;
         while (!advance) {
           if (v == kEmpty) {
@@ -380,10 +484,12 @@ __global__ __launch_bounds__(kScanThreads) void k_keep_tile_sums(int mode, const
 }
 
 // tile_sum == NULL: one tile (gridDim.x == 1), flags computed here.
+// tile_off != NULL: tile_off[t] already holds the entries before tile t (the
+// scanned tile sums of a list of more than 1024 tiles).
 __global__ __launch_bounds__(kScanThreads) void k_keep_tile_scan(int mode, const uint32_t* __restrict__ slot_of,
     const uint32_t* __restrict__ table, const uint8_t* __restrict__ dead, uint64_t count,
     uint32_t* __restrict__ keep, const uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ pos,
-    uint32_t* __restrict__ total) {
+    uint32_t* __restrict__ total, const uint32_t* __restrict__ tile_off) {
   __shared__ uint32_t wcnt[kScanPer][kScanWaves];
   __shared__ uint32_t wcarry[kScanWaves];
   const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -396,7 +502,11 @@ __global__ __launch_bounds__(kScanThreads) void k_keep_tile_scan(int mode, const
       const uint64_t i = base + r * kScanThreads + t;
       k[r] = i < count ? keep[i] : 0u;
     }
-    for (uint32_t j = t; j < blockIdx.x; j += kScanThreads) c += tile_sum[j];
+    if (tile_off) {
+      c = t == 0 ? tile_off[blockIdx.x] : 0u;
+    } else {
+      for (uint32_t j = t; j < blockIdx.x; j += kScanThreads) c += tile_sum[j];
+    }
     c = wave_sum(c);
   } else {
     keep_flags(mode, slot_of, table, dead, base, count, keep, k);
@@ -436,8 +546,79 @@ __global__ void k_goal_reset(int* found, unsigned long long* witness) {
   *witness = ~0ull;
 }
 
-__global__ void k_scan_total(const uint32_t* keep, const uint32_t* pos, uint64_t count, uint32_t* total) {
-  *total = count ? pos[count - 1] + keep[count - 1] : 0u;
+
+// Exclusive scan of arbitrary 32-bit values (sums wrap mod 2^32): tiles of
+// kScanTile entries, eight consecutive entries per thread (two 16-byte loads),
+// a block scan of the 256 thread totals (wave shuffles, then the waves'
+// totals in LDS).  Longer lists scan their tile sums first, with the same
+// kernels, and add them back in k_scan_apply.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t lane) {
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= uint32_t(off)) x += y;
+  }
+  return x - v;
+}
+
+__device__ __forceinline__ void load8(const uint32_t* __restrict__ in, uint64_t i0, uint64_t count, uint32_t (&v)[8]) {
+  if (i0 + 8 <= count && ((reinterpret_cast<uintptr_t>(in + i0) & 15) == 0)) {
+    const uint4 a = *reinterpret_cast<const uint4*>(in + i0);
+    const uint4 b = *reinterpret_cast<const uint4*>(in + i0 + 4);
+    v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = i0 + r < count ? in[i0 + r] : 0u;
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile_sums(const uint32_t* __restrict__ in, uint64_t count,
+    uint32_t* __restrict__ tile_sum) {
+  __shared__ uint32_t wsum[kScanWaves];
+  uint32_t v[8];
+  load8(in, uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * 8, count, v);
+  uint32_t s = 0;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) s += v[r];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kScanWaves; ++w) t += wsum[w];
+    tile_sum[blockIdx.x] = t;
+  }
+}
+
+// out may alias in.  tile_off == NULL: one tile.  The last tile writes *total.
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* in, uint64_t count,
+    const uint32_t* __restrict__ tile_off, uint32_t* out, uint32_t* __restrict__ total) {
+  __shared__ uint32_t wsum[kScanWaves];
+  const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const uint64_t i0 = uint64_t(blockIdx.x) * kScanTile + uint64_t(t) * 8;
+  uint32_t v[8];
+  load8(in, i0, count, v);
+  uint32_t s = 0;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) s += v[r];
+  const uint32_t ex = wave_excl_scan(s, lane);
+  if (lane == 63) wsum[wv] = ex + s;
+  __syncthreads();
+  uint32_t run = tile_off ? tile_off[blockIdx.x] : 0u, all = run;
+#pragma unroll
+  for (uint32_t w = 0; w < kScanWaves; ++w) {
+    run += w < wv ? wsum[w] : 0u;
+    all += wsum[w];
+  }
+  run += ex;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    if (i0 + r < count) out[i0 + r] = run;
+    run += v[r];
+  }
+  if (t == 0 && blockIdx.x == gridDim.x - 1 && total) *total = all;
 }
 
 // ------------------------------------------------------------------ launchers
@@ -460,6 +641,14 @@ hipError_t launch_expand(int W, const uint64_t* parents, uint64_t m, const uint6
     uint32_t letters, const uint32_t* letter_list, uint32_t n, int drop_mode, uint64_t* out, uint32_t* valid,
     hipStream_t st) {
   SW_DISPATCH_W(W, expand_w<w_>(parents, m, tables, slices, letters, letter_list, n, drop_mode, out, valid, st))
+}
+
+hipError_t launch_expand_sets(int W, const uint64_t* parents, uint64_t m, const uint64_t* tables, uint32_t slices,
+    const uint16_t* targets, int dir, uint32_t letters, const uint32_t* letter_list, uint32_t n, int drop_mode,
+    uint64_t* out, uint32_t* valid, hipStream_t st) {
+  if (m * letters <= kExpandAllMax || !targets)
+    return launch_expand(W, parents, m, tables, slices, letters, letter_list, n, drop_mode, out, valid, st);
+  SW_DISPATCH_W(W, expand_sliced_w<w_>(parents, m, targets, dir, letters, letter_list, n, drop_mode, out, valid, st))
 }
 
 hipError_t launch_tables(int W, const uint64_t* masks, uint32_t n, uint32_t k, uint64_t* out, hipStream_t st) {
@@ -487,35 +676,61 @@ hipError_t launch_hash_resolve(const uint32_t* slot_of, const uint32_t* table, u
   return hipGetLastError();
 }
 
+// Words of scratch the scan of `count` values needs: the tile sums of every
+// level above the first.
+static uint64_t scan_temp_words(uint64_t count) {
+  const uint64_t tiles = (count + kScanTile - 1) / kScanTile;
+  return tiles <= 1 ? 0 : tiles + scan_temp_words(tiles);
+}
+
+static hipError_t scan_levels(const uint32_t* in, uint32_t* out, uint64_t count, uint32_t* total, uint32_t* temp,
+    hipStream_t st) {
+  const uint64_t tiles = (count + kScanTile - 1) / kScanTile;
+  if (tiles <= 1) {
+    hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(kScanThreads), 0, st, in, count, nullptr, out, total);
+    return hipGetLastError();
+  }
+  uint32_t* sums = temp;
+  hipLaunchKernelGGL(k_scan_tile_sums, dim3(unsigned(tiles)), dim3(kScanThreads), 0, st, in, count, sums);
+  const hipError_t e = scan_levels(sums, sums, tiles, nullptr, temp + tiles, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scan_apply, dim3(unsigned(tiles)), dim3(kScanThreads), 0, st, in, count, sums, out, total);
+  return hipGetLastError();
+}
+
 hipError_t keep_scan_temp_bytes(uint64_t count, size_t* temp_bytes) {
-  if (count > kScanTwoPassMax) return launch_scan(nullptr, nullptr, count, nullptr, nullptr, temp_bytes, nullptr);
-  *temp_bytes = size_t((count + kScanTile - 1) / kScanTile) * 4 + 4;
+  const uint64_t tiles = (count + kScanTile - 1) / kScanTile;
+  if (count > kScanTwoPassMax) *temp_bytes = size_t(tiles + scan_temp_words(tiles)) * 4 + 4;
+  else *temp_bytes = size_t(tiles) * 4 + 4;
   return hipSuccess;
 }
 
 hipError_t launch_keep_scan(int mode, const uint32_t* slot_of, const uint32_t* table, const void* dead_bits,
     uint64_t count, uint32_t* keep, uint32_t* pos, uint32_t* total, void* temp, size_t* temp_bytes, hipStream_t st) {
   const uint8_t* dead = static_cast<const uint8_t*>(dead_bits);
+  if (count > 0x7fffffffull) return hipErrorInvalidValue;
   if (count <= kScanTile) {
     hipLaunchKernelGGL(k_keep_tile_scan, dim3(1), dim3(kScanThreads), 0, st, mode, slot_of, table, dead, count, keep,
-        nullptr, pos, total);
+        nullptr, pos, total, nullptr);
     return hipGetLastError();
   }
-  if (count <= kScanTwoPassMax) {
-    const uint32_t tiles = uint32_t((count + kScanTile - 1) / kScanTile);
-    if (!temp || !temp_bytes || *temp_bytes < size_t(tiles) * 4) return hipErrorInvalidValue;
-    uint32_t* sums = static_cast<uint32_t*>(temp);
-    hipLaunchKernelGGL(k_keep_tile_sums, dim3(tiles), dim3(kScanThreads), 0, st, mode, slot_of, table, dead, count,
-        keep, sums);
-    hipLaunchKernelGGL(k_keep_tile_scan, dim3(tiles), dim3(kScanThreads), 0, st, mode, slot_of, table, dead, count,
-        keep, sums, pos, total);
-    return hipGetLastError();
+  const uint32_t tiles = uint32_t((count + kScanTile - 1) / kScanTile);
+  size_t need = 0;
+  keep_scan_temp_bytes(count, &need);
+  if (!temp || !temp_bytes || *temp_bytes < need) return hipErrorInvalidValue;
+  uint32_t* sums = static_cast<uint32_t*>(temp);
+  hipLaunchKernelGGL(k_keep_tile_sums, dim3(tiles), dim3(kScanThreads), 0, st, mode, slot_of, table, dead, count,
+      keep, sums);
+  const uint32_t* off = nullptr;
+  if (count > kScanTwoPassMax) {
+    // More tiles than one workgroup adds up: scan the tile sums first.
+    const hipError_t e = scan_levels(sums, sums, tiles, nullptr, sums + tiles, st);
+    if (e != hipSuccess) return e;
+    off = sums;
   }
-  hipError_t e = hipSuccess;
-  if (mode == 1) e = launch_hash_resolve(slot_of, table, count, keep, st);
-  else if (mode == 2) e = launch_alive(1, dead_bits, count, keep, st);
-  if (e != hipSuccess) return e;
-  return launch_scan(keep, pos, count, total, temp, temp_bytes, st);
+  hipLaunchKernelGGL(k_keep_tile_scan, dim3(tiles), dim3(kScanThreads), 0, st, mode, slot_of, table, dead, count,
+      keep, sums, pos, total, off);
+  return hipGetLastError();
 }
 
 hipError_t launch_goal_reset(int* found, unsigned long long* witness, hipStream_t st) {
@@ -526,13 +741,13 @@ hipError_t launch_goal_reset(int* found, unsigned long long* witness, hipStream_
 hipError_t launch_scan(const uint32_t* keep, uint32_t* pos, uint64_t count, uint32_t* total, void* temp,
     size_t* temp_bytes, hipStream_t st) {
   if (count > 0x7fffffffull) return hipErrorInvalidValue;
-  if (!temp) return hipcub::DeviceScan::ExclusiveSum(nullptr, *temp_bytes, keep, pos, int(count ? count : 1), st);
-  if (count) {
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, *temp_bytes, keep, pos, int(count), st);
-    if (e != hipSuccess) return e;
+  const size_t need = size_t(scan_temp_words(count)) * 4 + 4;
+  if (!temp) {
+    *temp_bytes = need;
+    return hipSuccess;
   }
-  hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, st, keep, pos, count, total);
-  return hipGetLastError();
+  if (!temp_bytes || *temp_bytes < need) return hipErrorInvalidValue;
+  return scan_levels(keep, pos, count, total, static_cast<uint32_t*>(temp), st);
 }
 
 template <int W>

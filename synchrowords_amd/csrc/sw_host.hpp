@@ -92,6 +92,9 @@ struct sw_ctx {
   // Buffers of the device beam, reused between calls.
   void* beam_ws = nullptr;
   void (*beam_ws_free)(void*) = nullptr;
+  // Buffers and events of sw_frontier_step, reused between calls.
+  void* frontier_ws = nullptr;
+  void (*frontier_ws_free)(void*) = nullptr;
   // Stream index of the indexed subset kernel, reused between launches, and
   // when it is used: never (0), or for launches of at least ix_min_pairs pairs.
   sw::DevBuf ix_padded, ix_size, ix_start, ix_list, ix_tmp;
@@ -154,6 +157,10 @@ std::vector<uint64_t> build_tables(uint32_t n, uint32_t k, const uint32_t* delta
 // computed here (kept in `host` and uploaded to `masks` until the next call).
 void build_tables_device(uint32_t n, uint32_t k, const uint32_t* delta, int dir, std::vector<uint64_t>& host,
     DevBuf& masks, DevBuf& out, hipStream_t st);
+// targets[a * n + q] = delta(q, a) as 16-bit words (the bit-sliced
+// expansion's table), staged in `host` and uploaded to `out`.
+void upload_targets(uint32_t n, uint32_t k, const uint32_t* delta, std::vector<uint16_t>& host, DevBuf& out,
+    hipStream_t st);
 
 // States of the sink component (exact.cpp).
 std::vector<char> sink_states(uint32_t n, uint32_t k, const uint32_t* delta);

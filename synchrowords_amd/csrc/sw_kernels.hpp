@@ -316,6 +316,12 @@ hipError_t launch_random_sets(int W, uint64_t* out, uint64_t count, uint32_t n, 
 hipError_t launch_expand(int W, const uint64_t* parents, uint64_t m, const uint64_t* tables, uint32_t slices,
     uint32_t letters, const uint32_t* letter_list, uint32_t n, int drop_mode, uint64_t* out, uint32_t* valid,
     hipStream_t st);
+// The same, for launches past 2^17 children bit-sliced (64 parents per wave,
+// transposed across the lanes; targets[a * n + q] = delta(q, a), u16);
+// smaller launches, or targets NULL, take the slice tables.
+hipError_t launch_expand_sets(int W, const uint64_t* parents, uint64_t m, const uint64_t* tables, uint32_t slices,
+    const uint16_t* targets, int dir, uint32_t letters, const uint32_t* letter_list, uint32_t n, int drop_mode,
+    uint64_t* out, uint32_t* valid, hipStream_t st);
 
 // Open-addressing hash set over rows of an arena.  Slots hold an arena index,
 // or kCand | candidate index while a step is being merged.
