@@ -211,10 +211,10 @@ typedef struct sw_exact_config {
                                  end with a lower bound (the reference's STRICT_MEMORY_LIMIT);
                                  0: use that size anyway */
   int32_t goal_check;         /* meet checks: 0 automatic (the subset kernel; the trie over the
-                                 forward frontier where one build serves many checks: frozen
-                                 forward side, the deepening phase), 1 always the vertical subset
-                                 kernel, 2 always the forward trie, 3 tries over both frontiers
-                                 (the one that did not step) */
+                                 frozen forward frontier, where one build serves many checks),
+                                 1 always the vertical subset kernel, 2 always the forward trie
+                                 (the deepening phase's checks too), 3 tries over both frontiers
+                                 (the one that did not step; the forward trie when deepening) */
 } sw_exact_config;
 
 typedef struct sw_exact_result {

@@ -41,9 +41,12 @@
 // cap (backward-only from then on).  When a step would pass the budget, the
 // search deepens the backward frontier in parts instead, as the reference's
 // DFS does (dfs.cpp:63-347): each part's preimages are deduplicated (against
-// the backward side's rows and among themselves), checked against the trie
-// over the forward frontier, and deepened in parts again while below the
-// best length found; a find lowers that bound (dfs.cpp:179-189).  If even
+// the backward side's rows and among themselves), checked against the
+// forward frontier (the vertical subset kernel; the forward trie when the
+// goal-check mode asks for it: on the budget cases the kernel took a fifth
+// to a third of the trie's time, whose subset queries are the dense
+// backward sets), and deepened in parts again while below the best length
+// found; a find lowers that bound (dfs.cpp:179-189).  If even
 // that does not fit, the result is a lower bound (exact.cpp:75-96 of the
 // reference: "mlsw >= x").  Choices of side never change the answer.
 #include <hip/hip_runtime.h>
@@ -313,6 +316,7 @@ class Search {
   static constexpr uint64_t kSmallSetsDominanceBytes = 2ull << 30;
   bool goal_armed_ = false;  // the last commit reset the goal flag and witness
   bool budget_end_ = false;  // the search ended at the memory budget
+  bool dfs_trie_ = false;    // the deepening's meet checks walk the forward trie
   uint64_t dfs_min_list_ = 8192;
   uint64_t mitm_cap_ = 0;  // budget of the levels kept whole
   int goal_mode_ = 0;
@@ -689,13 +693,14 @@ class Search {
   // Backward deepening below the frontier, lengths t + 1 .. max_depth_
   // (the reference's Dfs::run, dfs.cpp:63-91, 161-218): the frontier is cut
   // into parts; each part's preimages form the next list, which is checked
-  // against the forward trie and deepened in parts in turn.  Parts hold at
+  // against the forward frontier and deepened in parts in turn.  Parts hold at
   // most part_ sets, sized so that one list per remaining depth fits the
   // budget (dfs.cpp:349-373).  False when the budget does not allow it.
   bool deepen(uint64_t t, uint64_t upper) {
     max_depth_ = upper - 1;
+    dfs_trie_ = goal_mode_ == 2 || goal_mode_ == 3;
     try {
-      ensure_ftrie();
+      if (dfs_trie_) ensure_ftrie();
       const uint64_t rb = row_bytes();
       const uint64_t used = used_bytes();
       const uint64_t avail = mem_cap_ > used ? mem_cap_ - used : 0;
@@ -766,8 +771,27 @@ class Search {
     int* found = small_.as<int>() + 2;
     unsigned long long* wit = reinterpret_cast<unsigned long long*>(small_.as<char>() + 16);
     SW_CHECK_HIP(launch_goal_reset(found, wit, st_));
-    ws_.ftrie.query(st_, trie_grid(), 0, lv.sets.as<uint64_t>(), c, 1, c > kSortQueries, 0, found, wit, nullptr);
-    trie_queries_ += c;
+    if (dfs_trie_) {
+      ws_.ftrie.query(st_, trie_grid(), 0, lv.sets.as<uint64_t>(), c, 1, c > kSortQueries, 0, found, wit, nullptr);
+      trie_queries_ += c;
+    } else {
+      // Forward frontier (stream) against the new list (tile, complemented
+      // back on load): witness (forward row << 32) | list row, as the trie's.
+      VsubParams vp;
+      vp.stream = F_.front.as<uint64_t>();
+      vp.s_count = F_.fcount;
+      vp.tile = lv.sets.as<uint64_t>();
+      vp.t_count = c;
+      vp.n = n_;
+      vp.complement_tile = 1;
+      vp.mode = VSUB_ANY;
+      vp.found = found;
+      vp.witness = wit;
+      apply_tuning(ctx_, W, vp);
+      search_walk(vp);
+      run_subset(ctx_, W, vp);
+      pair_tests_ += F_.fcount * c;
+    }
     if (cfg_.verbose && (dfs_calls_ & 1023) == 0)
       std::fprintf(stderr, "[sw exact] deepening: %llu parts, %llu sets, list %u, length %llu\n",
           (unsigned long long)dfs_calls_, (unsigned long long)dfs_sets_, d + 1, (unsigned long long)(lsw + 1));

@@ -213,19 +213,22 @@ def _search_mb(r, n):
 
 
 @pytest.mark.parametrize("case", _budget_cases())
-@pytest.mark.parametrize("fraction", [2, 4])
-def test_memory_budget_deepens_the_backward_side(ctx, case, fraction):
+@pytest.mark.parametrize("fraction,goal", [(2, 0), (4, 0), (4, 2)])
+def test_memory_budget_deepens_the_backward_side(ctx, case, fraction, goal):
     """With a device budget of a half or a quarter of what the search's
     levels take, a step passes the budget and the search deepens the
-    backward frontier in parts against the forward trie (the reference's
-    DFS, dfs.cpp:63-347): it still ends at the reference's length, with a
-    word that replays unless the Beam bound's word is the answer."""
+    backward frontier in parts (the reference's DFS, dfs.cpp:63-347), each
+    list checked against the forward frontier by the subset kernel (goal 0)
+    or the forward trie (goal 2): it still ends at the reference's length,
+    with a word that replays unless the Beam bound's word is the answer."""
     import synchrowords_amd as sw
     n, k, d = case["n"], case["k"], case["delta"]
     full = ctx.exact(n, k, d, upper=case["beam"])
     mb = max(1, _search_mb(full, n) // fraction)
-    r = ctx.exact(n, k, d, upper=case["beam"], cfg=sw.ExactConfig(max_memory_mb=mb, dfs_min_list_size=1024))
+    r = ctx.exact(n, k, d, upper=case["beam"],
+                  cfg=sw.ExactConfig(max_memory_mb=mb, dfs_min_list_size=1024, goal_check=goal))
     assert r.stats["dfs_entered"] == 1, (mb, r.stats)
+    assert (r.stats["trie_queries"] > 0) == (goal == 2)
     assert r.found and not r.stats["out_of_memory"] and r.length == case["upper"]
     if not r.bound_word:
         _check_word(case, r.length, r.word)
