@@ -52,6 +52,14 @@ void sw_gen_random_dfa(uint32_t n, uint32_t k, uint64_t seed, uint32_t* out);
 /* Create a context on HIP device `device` (its own stream and scratch). */
 int sw_ctx_create(int device, sw_ctx** out);
 void sw_ctx_destroy(sw_ctx* ctx);
+/* Free every device buffer the context holds (scratch of the entry points,
+ * search, beam and batch workspaces), keeping the context and its stream;
+ * later calls allocate again.  Replaces reset_cuda_device_impl()
+ * (synchrolib/data_structures/cuda/subsets_checker_kernel.hpp:8, a
+ * cudaDeviceReset at subsets_checker_kernel.cu:64-66), which frees every
+ * allocation on the device; this frees only the library's own, so other
+ * users of the GPU in the process keep theirs. */
+int sw_ctx_reset(sw_ctx* ctx);
 /* Device name, compute-unit count and global memory bytes. */
 int sw_device_info(sw_ctx* ctx, char* name, size_t name_cap, int* cus, uint64_t* mem_bytes);
 /* Device memory for callers without their own allocator. */

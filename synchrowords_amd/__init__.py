@@ -99,6 +99,7 @@ _SIGS = {
     "sw_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "sw_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "sw_sync": (C.c_int, [_P]),
+    "sw_ctx_reset": (C.c_int, [_P]),
     "sw_stream_timer": (C.c_int, [_P, C.c_int, C.POINTER(C.c_float)]),
     "sw_dev_copy": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "sw_transition_apply": (C.c_int, [_P, C.c_uint32, _P, _P, _P, C.c_uint64, C.c_int]),
@@ -457,6 +458,10 @@ class Context:
 
     def sync(self) -> None:
         _check(load().sw_sync(self.handle))
+
+    def reset(self) -> None:
+        """Free the context's device buffers (sw_ctx_reset); later calls allocate again."""
+        _check(load().sw_ctx_reset(self.handle))
 
     def timer_start(self) -> None:
         """Record a HIP event on the context's stream."""

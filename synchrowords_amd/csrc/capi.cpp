@@ -202,6 +202,28 @@ int sw_calibrate_lds(sw_ctx* ctx, uint32_t bytes_per_lane, double* tb_per_s) {
   SW_API_END
 }
 
+int sw_ctx_reset(sw_ctx* ctx) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx, "ctx is NULL");
+  sw::use(ctx);
+  SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  for (DevBuf* b : {&ctx->a, &ctx->b, &ctx->c, &ctx->d, &ctx->e, &ctx->ix_padded, &ctx->ix_size, &ctx->ix_start,
+           &ctx->ix_list, &ctx->ix_tmp, &ctx->ix_order, &ctx->ix_sort, &ctx->ix_sperm, &ctx->ix_tperm,
+           &ctx->ix_head, &ctx->ix_anchor, &ctx->ix_tlist, &ctx->ix_work, &ctx->wg_in, &ctx->wg_scratch,
+           &ctx->wg_out, &ctx->brute_jobs, &ctx->brute_delta, &ctx->brute_frontier, &ctx->brute_out,
+           &ctx->epp_jobs, &ctx->epp_delta, &ctx->epp_scratch, &ctx->epp_words, &ctx->epp_out, &ctx->epp_off,
+           &ctx->epp_tmp, &ctx->epp_small})
+    b->release();
+  ctx->ix_tmp_bytes = 0;
+  if (ctx->exact_ws && ctx->exact_ws_free) ctx->exact_ws_free(ctx->exact_ws);
+  ctx->exact_ws = nullptr;
+  if (ctx->beam_ws && ctx->beam_ws_free) ctx->beam_ws_free(ctx->beam_ws);
+  ctx->beam_ws = nullptr;
+  if (ctx->frontier_ws && ctx->frontier_ws_free) ctx->frontier_ws_free(ctx->frontier_ws);
+  ctx->frontier_ws = nullptr;
+  SW_API_END
+}
+
 int sw_stream_timer(sw_ctx* ctx, int op, float* ms) {
   SW_API_BEGIN
   SW_REQUIRE(ctx, "ctx is NULL");
