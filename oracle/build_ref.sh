@@ -16,15 +16,25 @@
 #   N*N*log2 N, beam_exact_ratio 0.01; Exact dfs, dfs_shortcut, 12 GiB,
 #   dfs_min_list_size 8192, bfs_small_list_size N*K.
 #
-# usage: oracle/build_ref.sh N K [THREADS]
+# With PLAN "reduce" the binary is ref_reduce_<N>_<K>_<T>, built with the
+# settings of configs/exact_reduce.json: the same stages plus Reduce (min_n
+# 80, list_size_threshold N*K*8) before Exact, whose bfs_small_list_size is
+# N*K*K*8.
+#
+# usage: oracle/build_ref.sh N K [THREADS] [PLAN]
 set -euo pipefail
 N=${1:?N}
 K=${2:?K}
 T=${3:-1}
+PLAN=${4:-exact}
 REF=${SYNCHRO_REFERENCE:-/root/reference}
 HERE="$(cd "$(dirname "$0")" && pwd)"
 OUT="$HERE/_ref"
-BIN="$OUT/ref_${N}_${K}_${T}"
+if [ "$PLAN" = reduce ]; then
+  BIN="$OUT/ref_reduce_${N}_${K}_${T}"
+else
+  BIN="$OUT/ref_${N}_${K}_${T}"
+fi
 if [ ! -d "$REF/synchrolib" ]; then
   echo "reference tree not found at $REF" >&2
   exit 3
@@ -50,8 +60,13 @@ DEFS=(
   '-DBEAM_EXACT_RATIO=(0.01)' '-DMAX_ITER=(-1)' -DPRESORT_INDEG
   '-DDFS_SHORTCUT=(true)' '-DDFS=(true)' '-DSTRICT_MEMORY_LIMIT=(false)'
   '-DMAX_MEMORY=(12 * 1024)' '-DDFS_MIN_LIST_SIZE=(8192)'
-  '-DBFS_SMALL_LIST_SIZE=(AUT_N * AUT_K)'
 )
+if [ "$PLAN" = reduce ]; then
+  DEFS+=(-DCOMPILE_REDUCE '-DMIN_N=(80)' '-DLIST_SIZE_THRESHOLD=(AUT_N * AUT_K * 8)'
+    '-DBFS_SMALL_LIST_SIZE=(AUT_N * AUT_K * AUT_K * 8)')
+else
+  DEFS+=('-DBFS_SMALL_LIST_SIZE=(AUT_N * AUT_K)')
+fi
 # x86-64-v3 (AVX2) instead of the reference's -march=native so the binary also
 # runs on the host CPU of another machine.
 FLAGS=(-std=c++17 -O3 -march=x86-64-v3 -DNDEBUG -pthread -w -I "$REF" -I "$REF/jit")
@@ -65,7 +80,7 @@ SRCS=(
   "$REF/synchrolib/utils/connectivity.cpp"
   "$HERE/ref_driver.cpp"
 )
-OBJ="$OUT/obj_${N}_${K}_${T}"
+OBJ="$OUT/obj_$(basename "$BIN")"
 mkdir -p "$OBJ"
 pids=()
 objs=()

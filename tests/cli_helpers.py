@@ -5,7 +5,8 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parents[1]
 CLI = ROOT / "synchrowords_amd" / "synchro_amd"
 
-# The reference's configs/exact.json and configs/readme_config.json plans, as data.
+# The reference's configs/exact.json, configs/exact_reduce.json and
+# configs/readme_config.json plans, as data.
 EXACT_PLAN = {
     "upper_bound": "1ULL * (AUT_N - 1) * (AUT_N - 1) + 1", "threads": 1, "gpu": True,
     "algorithms": [
@@ -14,6 +15,16 @@ EXACT_PLAN = {
         {"name": "Beam", "config": {"presort": "indeg", "dynamic": True, "min_beam_size": "AUT_N * std::log2(AUT_N)",
                                     "max_beam_size": "AUT_N * AUT_N * std::log2(AUT_N)", "beam_exact_ratio": 0.01}},
         {"name": "Exact", "config": {"dfs_min_list_size": "8192", "bfs_small_list_size": "AUT_N * AUT_K",
+                                     "dfs": True, "dfs_shortcut": True, "max_memory_mb": "12 * 1024"}}]}
+EXACT_REDUCE_PLAN = {
+    "upper_bound": "1ULL * (AUT_N - 1) * (AUT_N - 1) + 1", "threads": 1, "gpu": False,
+    "algorithms": [
+        {"name": "Brute", "config": {"max_n": "15"}},
+        {"name": "Eppstein", "config": {"transition_tables": False, "find_word": False}},
+        {"name": "Beam", "config": {"presort": "indeg", "dynamic": True, "min_beam_size": "AUT_N * std::log2(AUT_N)",
+                                    "max_beam_size": "AUT_N * AUT_N * std::log2(AUT_N)", "beam_exact_ratio": 0.01}},
+        {"name": "Reduce", "config": {"min_n": "80", "list_size_threshold": "AUT_N * AUT_K * 8"}},
+        {"name": "Exact", "config": {"dfs_min_list_size": "8192", "bfs_small_list_size": "AUT_N * AUT_K * AUT_K * 8",
                                      "dfs": True, "dfs_shortcut": True, "max_memory_mb": "12 * 1024"}}]}
 README_PLAN = {
     "upper_bound": "1ULL * AUT_N * AUT_N * AUT_N / 6", "threads": 1, "gpu": False,

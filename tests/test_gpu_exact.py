@@ -249,3 +249,20 @@ def test_memory_budget_without_deepening_gives_a_lower_bound(ctx, case):
         r = ctx.exact(n, k, d, upper=case["beam"], cfg=cfg)
         assert not r.found and r.stats["out_of_memory"] == 1
         assert 1 <= r.length <= case["upper"]
+
+
+@pytest.mark.parametrize("case", [pytest.param(c, id=c["name"]) for c in golden("reduce_plan.json")])
+def test_pipeline_with_the_reduce_plan(ctx, case):
+    """configs/exact_reduce.json (Reduce before Exact): the bounds the
+    reference printed for this plan, after its reduction and the second
+    build for the reduced N (tests/golden/reduce_plan.json), and a word that
+    replays.  The device pipeline records Reduce and searches the whole
+    automaton."""
+    import json
+    from cli_helpers import EXACT_REDUCE_PLAN
+    n, k = case["n"], case["k"]
+    r = ctx.pipeline(n, k, case["delta"], plan=json.dumps(EXACT_REDUCE_PLAN))
+    assert r.non_synchro == bool(case["non_synchro"])
+    assert (r.lower, r.upper) == (case["lower"], case["upper"])
+    assert "Reduce" in [name for name, _ in r.times_ms]
+    _check_word(case, r.upper, r.word)

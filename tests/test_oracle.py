@@ -189,3 +189,23 @@ def test_oracle_memory_budget_keeps_the_length(name, mb):
     o = orc.solve(c["n"], c["k"], c["delta"], max_memory_mb=mb)
     assert o["entered_dfs"]
     assert (o["lower"], o["upper"]) == (c["upper"], c["upper"])
+
+
+def test_reference_reduce_plan_keeps_the_exact_lengths():
+    """The reference's configs/exact_reduce.json plan (its Reduce shrank
+    every one of these automata, reduce.hpp:31-103, and a second build for
+    the reduced N finished the search) ends at the same bounds as its
+    configs/exact.json plan: the reduction never changes a length, which is
+    why the device pipeline may run Reduce's automaton unreduced."""
+    red = {c["name"]: c for c in golden("reduce_plan.json")}
+    assert len(red) >= 20 and all(c["reduced_n"] and c["reduced_n"] < c["n"] for c in red.values())
+    seen = 0
+    for name in ("c3_random200.json", "c5_random300.json", "random_misc.json", "readme.json"):
+        for c in golden(name):
+            if c["name"] in red:
+                r = red[c["name"]]
+                assert r["delta"] == c["delta"]
+                assert (r["non_synchro"], r["lower"], r["upper"]) == (int(bool(c["non_synchro"])), c["lower"],
+                                                                      c["upper"]), name
+                seen += 1
+    assert seen == len(red)
