@@ -7,9 +7,11 @@
 // kBeamSmallCand candidates are handed back to the device-wide beam
 // (mlp.cpp) with the list and the word history so far, which it continues.
 //
-// Same results as the device-wide beam in the reference's order: candidates
+// Same bounds as the device-wide beam in the reference's order: candidates
 // are numbered letter-major (c = a m + i), among equal sets the smallest
-// number survives, and the history keeps each kept set's candidate number.
+// number survives, each depth keeps the same `width` distinct sets (chosen
+// by selection, kept in candidate order), and the history keeps each kept
+// set's candidate number.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -59,7 +61,8 @@ __global__ __launch_bounds__(kThreadsBS) void k_beam_small(BeamSmallArgs a) {
   __shared__ uint32_t uidx[kBeamSmallCand];
   __shared__ uint32_t tab[kTab];
   __shared__ uint32_t wsum[kThreadsBS / 64];
-  __shared__ uint32_t s_first;
+  __shared__ uint32_t pch[65];  // sets per size, for the selection
+  __shared__ uint32_t s_first, s_cls, s_cstar, s_need;
   const uint32_t tid = threadIdx.x;
   const uint32_t n = a.n, k = a.k;
   const uint64_t full = n == 64 ? ~0ull : ((1ull << n) - 1);
@@ -147,22 +150,61 @@ __global__ __launch_bounds__(kThreadsBS) void k_beam_small(BeamSmallArgs a) {
       nu += tot;
     }
     __syncthreads();
-    // Place of each distinct set: larger first, then word ascending; the
-    // first `width` are kept, with their candidate numbers as history.
+    // The `width` first distinct sets in the reference's order (larger
+    // first, then word ascending), kept in candidate order: that order is
+    // total on distinct sets, so which sets a depth keeps does not depend on
+    // the order the list is stored in, and the bound does not either (the
+    // word is any word of the bound's length).  All are kept when they fit;
+    // otherwise the sets larger than the boundary size are kept, and of the
+    // boundary size those with fewer than `need` smaller words in it.
     const uint32_t keepn = nu < a.width ? nu : a.width;
     uint64_t* next = lists[cur ^ 1];
-    for (uint32_t u = tid; u < nu; u += kThreadsBS) {
-      const uint64_t x = uval[u];
-      const uint32_t px = uint32_t(__popcll(x));
-      uint32_t before = 0;
-      for (uint32_t v = 0; v < nu; ++v) {
-        const uint64_t y = uval[v];
-        const uint32_t py = uint32_t(__popcll(y));
-        before += (py > px || (py == px && y < x)) ? 1u : 0u;
+    if (nu <= a.width) {
+      for (uint32_t u = tid; u < nu; u += kThreadsBS) {
+        next[u] = uval[u];
+        a.hist[hist_len + u] = uidx[u];
       }
-      if (before < keepn) {
-        next[before] = x;
-        a.hist[hist_len + before] = uidx[u];
+    } else {
+      for (uint32_t i = tid; i < 65; i += kThreadsBS) pch[i] = 0;
+      if (tid == 0) s_cls = 0;
+      __syncthreads();
+      for (uint32_t u = tid; u < nu; u += kThreadsBS) atomicAdd(&pch[__popcll(uval[u])], 1u);
+      __syncthreads();
+      if (tid == 0) {
+        uint32_t above = 0, c = 64;
+        while (above + pch[c] < a.width) above += pch[c--];
+        s_cstar = c;
+        s_need = a.width - above;
+      }
+      __syncthreads();
+      const uint32_t cs = s_cstar;
+      // The boundary class, gathered (its order only feeds counts).
+      for (uint32_t u = tid; u < nu; u += kThreadsBS)
+        if (uint32_t(__popcll(uval[u])) == cs) cand[atomicAdd(&s_cls, 1u)] = uval[u];
+      __syncthreads();
+      const uint32_t ncls = s_cls, need = s_need;
+      uint32_t kept = 0;
+      for (uint32_t u0 = 0; u0 < nu; u0 += kThreadsBS) {
+        const uint32_t u = u0 + tid;
+        bool keep = false;
+        if (u < nu) {
+          const uint64_t x = uval[u];
+          const uint32_t px = uint32_t(__popcll(x));
+          if (px > cs) {
+            keep = true;
+          } else if (px == cs) {
+            uint32_t below = 0;
+            for (uint32_t j = 0; j < ncls; ++j) below += cand[j] < x ? 1u : 0u;
+            keep = below < need;
+          }
+        }
+        uint32_t tot;
+        const uint32_t at = flag_scan(keep, wsum, &tot);
+        if (keep) {
+          next[kept + at] = uval[u];
+          a.hist[hist_len + kept + at] = uidx[u];
+        }
+        kept += tot;
       }
     }
     if (tid == 0) {
