@@ -3,9 +3,10 @@
 // preimages of the list under every letter, the first depth producing Q
 // bounds the length, duplicates dropped, the `width` largest sets kept, ties
 // by their word ascending).  Sets are one 64-bit word; the preimage masks, the
-// list, the candidates and the dedup table live in LDS.  Depths of more than
-// kBeamSmallCand candidates are handed back to the device-wide beam
-// (mlp.cpp) with the list and the word history so far, which it continues.
+// list, the candidates and the dedup table live in LDS (about 156 KB).
+// Depths of more than kBeamSmallCand candidates, or that keep more than
+// kBeamSmallList sets, are handed back to the device-wide beam (mlp.cpp)
+// with the list and the word history so far, which it continues.
 //
 // Same bounds as the device-wide beam in the reference's order: candidates
 // are numbered letter-major (c = a m + i), among equal sets the smallest
@@ -55,7 +56,7 @@ __device__ __forceinline__ uint32_t flag_scan(bool v, uint32_t* wsum, uint32_t* 
 
 __global__ __launch_bounds__(kThreadsBS) void k_beam_small(BeamSmallArgs a) {
   __shared__ uint64_t pre[kBeamSmallMaxK][64];
-  __shared__ uint64_t lists[2][kBeamSmallCand];
+  __shared__ uint64_t lists[2][kBeamSmallList];
   __shared__ uint64_t cand[kBeamSmallCand];
   __shared__ uint64_t uval[kBeamSmallCand];
   __shared__ uint32_t uidx[kBeamSmallCand];
@@ -158,6 +159,10 @@ __global__ __launch_bounds__(kThreadsBS) void k_beam_small(BeamSmallArgs a) {
     // otherwise the sets larger than the boundary size are kept, and of the
     // boundary size those with fewer than `need` smaller words in it.
     const uint32_t keepn = nu < a.width ? nu : a.width;
+    if (keepn > kBeamSmallList) {  // this depth continues on the device-wide beam
+      status = kBeamSmallHandover;
+      break;
+    }
     uint64_t* next = lists[cur ^ 1];
     if (nu <= a.width) {
       for (uint32_t u = tid; u < nu; u += kThreadsBS) {

@@ -615,7 +615,7 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
     // kernel hands back.
     constexpr uint64_t kSmallDepths = 1u << 20;
     constexpr uint32_t kSmallHist = 1u << 22;
-    if (!m && n <= 64 && k <= sw::kBeamSmallMaxK && upper <= kSmallDepths && m0 <= sw::kBeamSmallCand) {
+    if (!m && n <= 64 && k <= sw::kBeamSmallMaxK && upper <= kSmallDepths && m0 <= sw::kBeamSmallList) {
       DevBuf &sin = ws.sb_in, &smeta = ws.sb_meta, &sword = ws.sb_word;
       const size_t din_bytes = size_t(n) * k * 4 + 64 * 8;
       sin.ensure(din_bytes);
@@ -732,8 +732,26 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
       // the counted maximum the kept sets are written straight from their
       // places; beyond it a sort gives the order and two gathers follow.
       const uint64_t keepn = std::min<uint64_t>(width, nu);
-      list.ensure(keepn * W * 8 + 8);
       histd.ensure((hist_len + keepn) * 4 + 4, st, hist_len * 4);
+      if (nu <= width) {
+        // Every distinct set is kept.  Which sets a depth keeps never depends
+        // on their stored order (the order is total on distinct sets, and a
+        // network's only says which `width` come first), and neither does
+        // the bound, so they stay in candidate order: no ranking, and the
+        // list is the compacted buffer itself.
+        std::swap(list.p, uniq.p);
+        std::swap(list.cap, uniq.cap);
+        if (nu)
+          SW_CHECK_HIP(hipMemcpyAsync(histd.as<uint32_t>() + hist_len, uidx.p, size_t(nu) * 4,
+              hipMemcpyDeviceToDevice, st));
+        hist_off.push_back(hist_len);
+        hist_m.push_back(m0);
+        hist_len += keepn;
+        m0 = keepn;
+        if (keepn == 0) break;  // every continuation was empty
+        continue;
+      }
+      list.ensure(keepn * W * 8 + 8);
       const bool counted = sw::beam_rank_fits(nu);
       const float* scores = nullptr;
       if (m) {

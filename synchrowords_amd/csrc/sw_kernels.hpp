@@ -193,11 +193,12 @@ hipError_t launch_card_scatter(int W, const uint64_t* sets, uint64_t count, int 
 
 // Beam of an automaton of at most 64 states in one workgroup
 // (k_beam_small.hip): every depth in one launch while a depth's candidates
-// fit kBeamSmallCand; out[0] = status, out[1] = the depth it stopped at (the
+// fit kBeamSmallCand and the sets it keeps kBeamSmallList; out[0] = status, out[1] = the depth it stopped at (the
 // bound when found), out[2] = the list size there (list_out), out[3] = the
 // history entries written; meta[2 d], meta[2 d + 1] = history offset and list
 // size of depth d + 1; word[0 .. bound) on a find.
-constexpr uint32_t kBeamSmallCand = 2048;
+constexpr uint32_t kBeamSmallCand = 4096;  // candidates of one depth
+constexpr uint32_t kBeamSmallList = 2048;  // sets a depth keeps
 constexpr uint32_t kBeamSmallMaxK = 16;
 enum BeamSmallStatus : uint32_t {
   kBeamSmallFound = 1,      // a depth produced Q

@@ -114,11 +114,14 @@ def test_device_beam_wide_crosses_the_counted_order(ctx, n, k, width):
         assert len(w) == b and replay(n, k, d, w) == 1
 
 
-@pytest.mark.parametrize("n,k,width", [(64, 3, 3000), (50, 4, 1500), (64, 2, 1100), (33, 16, 400)])
+@pytest.mark.parametrize("n,k,width", [(64, 3, 3000), (50, 4, 1500), (64, 2, 3000), (33, 16, 400), (64, 2, 1800),
+                                       (48, 2, 268)])
 def test_device_beam_small_automata_hand_over(ctx, n, k, width):
     """Automata of at most 64 states run the Beam's depths in one workgroup
-    until a depth's candidates outgrow its LDS (2048), then the device-wide
-    beam continues from the list and history it hands back: the oracle's
+    until a depth's candidates outgrow its LDS (4096) or it keeps more than
+    2048 sets, then the device-wide beam continues from the list and history
+    it hands back; narrower beams (1800, and the plan's 268 at n = 48) stay
+    in the workgroup and keep each depth's sets by selection: the oracle's
     bound at the same width, and a word that replays."""
     import oracle_lib as orc
     for seed in range(2):
