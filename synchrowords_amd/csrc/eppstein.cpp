@@ -82,7 +82,7 @@ void eppstein_wide(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, u
   ctx->epp_out.ensure(sizeof(EppOut));
   SW_CHECK_HIP(hipMemcpyAsync(ctx->epp_jobs.p, &job, sizeof(EppJob), hipMemcpyHostToDevice, st));
   SW_CHECK_HIP(hipMemcpyAsync(ctx->epp_delta.p, delta, size_t(kn) * 4, hipMemcpyHostToDevice, st));
-  SW_CHECK_HIP(launch_epp_greedy(ctx->epp_jobs.as<EppJob>(), ctx->epp_delta.as<uint32_t>(), base,
+  SW_CHECK_HIP(launch_epp_greedy(ctx->epp_jobs.as<EppJob>(), n, ctx->epp_delta.as<uint32_t>(), base,
       ctx->epp_words.as<uint32_t>(), ctx->epp_out.as<EppOut>(), st));
   SW_CHECK_HIP(hipMemcpyAsync(&res, ctx->epp_out.p, sizeof(EppOut), hipMemcpyDeviceToHost, st));
   if (words && word_cap)
@@ -115,7 +115,7 @@ void eppstein_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint3
     std::vector<EppJob> jobs;
     std::vector<uint32_t> dpack;
     size_t scratch = 0;
-    uint32_t list_lds = 0;
+    uint32_t list_lds = 0, max_n = 0;
     uint32_t end = at;
     while (end < count) {
       const size_t need = eppstein_scratch_bytes(ns[end], ks[end]);
@@ -125,6 +125,7 @@ void eppstein_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint3
       dpack.insert(dpack.end(), deltas + delta_offsets[end], deltas + delta_offsets[end] + size_t(ns[end]) * ks[end]);
       scratch += need;
       list_lds = std::max(list_lds, eppstein_list_lds(ns[end], ks[end]));
+      max_n = std::max(max_n, ns[end]);
       ++end;
     }
     const size_t m = jobs.size();
@@ -136,7 +137,8 @@ void eppstein_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint3
     SW_CHECK_HIP(hipMemcpyAsync(ctx->epp_jobs.p, jobs.data(), m * sizeof(EppJob), hipMemcpyHostToDevice, st));
     SW_CHECK_HIP(hipMemcpyAsync(ctx->epp_delta.p, dpack.data(), dpack.size() * 4, hipMemcpyHostToDevice, st));
     SW_CHECK_HIP(launch_eppstein_wg(ctx->epp_jobs.as<EppJob>(), uint32_t(m), ctx->epp_delta.as<uint32_t>(),
-        ctx->epp_scratch.as<unsigned char>(), ctx->epp_words.as<uint32_t>(), ctx->epp_out.as<EppOut>(), list_lds, st));
+        ctx->epp_scratch.as<unsigned char>(), ctx->epp_words.as<uint32_t>(), ctx->epp_out.as<EppOut>(), list_lds,
+        eppstein_letter_lds(max_n, list_lds), max_n <= 64, st));
     std::vector<EppOut> out(m);
     SW_CHECK_HIP(hipMemcpyAsync(out.data(), ctx->epp_out.p, m * sizeof(EppOut), hipMemcpyDeviceToHost, st));
     if (words && word_cap)

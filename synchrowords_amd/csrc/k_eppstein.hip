@@ -24,6 +24,7 @@
 
 #include <algorithm>
 
+#include "sw_device.hpp"
 #include "sw_kernels.hpp"
 
 namespace sw {
@@ -91,27 +92,40 @@ __device__ inline void candidates(uint32_t level, uint32_t p, uint32_t n, uint32
 }
 
 // Greedy merging of the closest pair, over the pair lengths in info (one
-// workgroup of kEppThreads).
+// workgroup of kEppThreads).  The current set is W 64-bit words in LDS
+// (mask).  Per merge the block compacts the set's states (each thread its
+// own state's place, from popcounts of the words below it) and searches the
+// closest pair; then wave 0 alone applies the pair's word letter by letter:
+// lane l holds states l + 64 j of the set in registers, and the image under
+// a letter is one OR over the wave per word, with no barrier per letter.
+// letters[x n + y] (LDS, may be NULL: then info's low byte) is the first
+// letter of pair {x, y}; dl (LDS, may be NULL: then d) the transitions as
+// 16-bit words.
 __device__ void epp_greedy(const EppJob& job, uint32_t n, uint32_t k, const uint32_t* __restrict__ d,
-    const uint32_t* __restrict__ info, uint32_t* __restrict__ words, EppOut* __restrict__ out, unsigned char* cur,
-    unsigned char* nxt, uint16_t* cs, uint32_t* wsum, uint64_t* s_best) {
-  const uint32_t tid = threadIdx.x, nt = blockDim.x;
-  for (uint32_t q = tid; q < n; q += nt) cur[q] = 1;
+    const uint32_t* __restrict__ info, uint32_t* __restrict__ words, EppOut* __restrict__ out, uint16_t* cs,
+    uint64_t* mask, uint64_t* s_best, uint32_t* s_stop, const unsigned char* letters, const uint16_t* dl) {
+  const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = tid & 63u;
+  const uint32_t W = (n + 63) / 64;
+  if (tid < W) mask[tid] = word_mask(n, int(tid));
+  if (tid == 0) *s_stop = 0;
   __syncthreads();
-  // The current set's states, ascending, in cs[0 .. m) (n <= 512 < threads:
-  // one flag per thread).
-  auto compact = [&]() {
-    const uint32_t f = tid < n ? uint32_t(cur[tid]) : 0u;
-    uint32_t tot;
-    const uint32_t ex = block_scan(f, wsum, &tot);
-    if (f) cs[ex] = uint16_t(tid);
-    __syncthreads();
-    return tot;
-  };
-  uint32_t m = compact();
   uint64_t total = 0, wl = 0;
   uint32_t* word = words + job.word_off;
-  while (m > 1) {
+  while (true) {
+    // The set's states, ascending, in cs[0 .. m).
+    uint32_t m = 0;
+    for (uint32_t w = 0; w < W; ++w) m += uint32_t(__popcll(mask[w]));
+    if (tid < n) {
+      const uint32_t w = tid >> 6, b = tid & 63u;
+      const uint64_t x = mask[w];
+      if ((x >> b) & 1ull) {
+        uint32_t pos = uint32_t(__popcll(x & ((1ull << b) - 1ull)));
+        for (uint32_t v = 0; v < w; ++v) pos += uint32_t(__popcll(mask[v]));
+        cs[pos] = uint16_t(tid);
+      }
+    }
+    __syncthreads();
+    if (m <= 1) break;
     uint64_t best = ~0ull;  // (length, first, second)
     for (uint32_t t = tid; t < m * m; t += nt) {
       const uint32_t i = t / m, j = t % m;
@@ -125,32 +139,57 @@ __device__ void epp_greedy(const EppJob& job, uint32_t n, uint32_t k, const uint
       const uint64_t o = __shfl_xor(best, off, 64);
       best = o < best ? o : best;
     }
-    if ((tid & 63u) == 0) s_best[tid >> 6] = best;
+    if (lane == 0) s_best[tid >> 6] = best;
     __syncthreads();
-    best = s_best[0];
-    for (uint32_t w = 1; w < (nt >> 6); ++w) best = s_best[w] < best ? s_best[w] : best;
+    if (tid < 64) {
+      best = s_best[0];
+      for (uint32_t w = 1; w < (nt >> 6); ++w) best = s_best[w] < best ? s_best[w] : best;
+      total += best >> 32;
+      if (total > job.upper) {
+        if (tid == 0) {
+          *out = EppOut{0, 0, job.upper, 0};
+          *s_stop = 1;
+        }
+      } else {
+        uint64_t S[8];
+#pragma unroll
+        for (uint32_t w = 0; w < 8; ++w) S[w] = w < W ? mask[w] : 0ull;
+        uint32_t n1 = uint32_t(best >> 16) & 0xffffu, n2 = uint32_t(best) & 0xffffu;
+        while (n1 != n2) {
+          const uint32_t ix = pair_ix(n1, n2, n);
+          const uint32_t x = letters ? uint32_t(letters[ix]) : (info[ix] & 255u);
+          if (tid == 0 && wl < job.word_cap) word[wl] = x;
+          ++wl;
+          uint64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+          for (uint32_t j = 0; j < 8; ++j) {
+            const uint32_t q = lane + 64 * j;
+            if (j < W && ((S[j] >> lane) & 1ull)) {
+              const uint32_t t = dl ? uint32_t(dl[q * k + x]) : d[size_t(q) * k + x];
+              const uint32_t tw = t >> 6;
+              const uint64_t bit = 1ull << (t & 63u);
+#pragma unroll
+              for (uint32_t v = 0; v < 8; ++v) acc[v] |= v == tw ? bit : 0ull;
+            }
+          }
+#pragma unroll
+          for (uint32_t v = 0; v < 8; ++v) {
+            uint64_t y = acc[v];
+            if (v < W)
+              for (int off = 32; off > 0; off >>= 1) y |= __shfl_xor(y, off, 64);
+            S[v] = y;
+          }
+          n1 = dl ? uint32_t(dl[n1 * k + x]) : d[size_t(n1) * k + x];
+          n2 = dl ? uint32_t(dl[n2 * k + x]) : d[size_t(n2) * k + x];
+        }
+        uint64_t mine = 0;
+#pragma unroll
+        for (uint32_t v = 0; v < 8; ++v) mine = v == lane ? S[v] : mine;
+        if (lane < W) mask[lane] = mine;
+      }
+    }
     __syncthreads();
-    total += best >> 32;
-    if (total > job.upper) {
-      if (tid == 0) *out = EppOut{0, 0, job.upper, 0};
-      return;
-    }
-    uint32_t n1 = uint32_t(best >> 16) & 0xffffu, n2 = uint32_t(best) & 0xffffu;
-    while (n1 != n2) {
-      const uint32_t x = info[pair_ix(n1, n2, n)] & 255u;
-      if (tid == 0 && wl < job.word_cap) word[wl] = x;
-      ++wl;
-      for (uint32_t q = tid; q < n; q += nt) nxt[q] = 0;
-      __syncthreads();
-      for (uint32_t q = tid; q < n; q += nt)
-        if (cur[q]) nxt[d[size_t(q) * k + x]] = 1;
-      __syncthreads();
-      for (uint32_t q = tid; q < n; q += nt) cur[q] = nxt[q];
-      __syncthreads();
-      n1 = d[size_t(n1) * k + x];
-      n2 = d[size_t(n2) * k + x];
-    }
-    m = compact();
+    if (*s_stop) return;
   }
   if (tid == 0) *out = EppOut{0, 1, total, wl};
 }
@@ -160,6 +199,15 @@ __device__ void epp_greedy(const EppJob& job, uint32_t n, uint32_t k, const uint
 // and edges[k n] there and points *b, *e at the copies.  Every thread of the
 // workgroup calls it.
 constexpr uint32_t kEppListLds = 48 * 1024;
+// Pair letters for the greedy walk (n * n bytes) after the lists in the
+// dynamic LDS, 64 KB of dynamic LDS in all; transitions of up to
+// kEppDeltaLds entries in static LDS.
+constexpr uint32_t kEppDynLds = 64 * 1024;
+constexpr uint32_t kEppDeltaLds = 4096;
+// Pair tables of automata up to 64 states in static LDS (about 56 KB).
+constexpr uint32_t kEppSmallPairs = 64 * 64;
+constexpr uint32_t kEppSmallCnt = 2048;
+__host__ __device__ inline uint32_t epp_letters_off(uint32_t list_lds) { return (list_lds + 15u) & ~15u; }
 __device__ inline size_t epp_list_bytes(uint32_t kn) { return ((size_t(kn) + 1) * 4 + size_t(kn) * 2 + 7) & ~size_t(7); }
 __device__ inline void epp_stage_lists(uint32_t kn, uint32_t lds_cap, unsigned char* lds, const uint32_t*& b,
     const uint16_t*& e) {
@@ -173,14 +221,20 @@ __device__ inline void epp_stage_lists(uint32_t kn, uint32_t lds_cap, unsigned c
   e = se;
 }
 
+template <bool SMALL>
 __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __restrict__ jobs,
     const uint32_t* __restrict__ deltas, unsigned char* __restrict__ scratch, uint32_t* __restrict__ words,
-    EppOut* __restrict__ outs, uint32_t lds_cap) {
+    EppOut* __restrict__ outs, uint32_t lds_cap, uint32_t letter_lds) {
   extern __shared__ __align__(16) unsigned char epp_lds[];
   __shared__ uint32_t wsum[16];
   __shared__ uint64_t s_best[16];
-  __shared__ unsigned char cur[kEppMaxN], nxt[kEppMaxN];
+  __shared__ uint64_t mask[8];
+  __shared__ uint32_t s_stop;
   __shared__ uint16_t cs[kEppMaxN];
+  __shared__ uint16_t dls[kEppDeltaLds];
+  __shared__ uint32_t s_info[SMALL ? kEppSmallPairs : 1];
+  __shared__ unsigned long long s_keys[SMALL ? kEppSmallPairs : 1];
+  __shared__ uint32_t s_queue[SMALL ? kEppSmallCnt : 1], s_cnt[SMALL ? kEppSmallCnt : 1];
   const EppJob job = jobs[blockIdx.x];
   const uint32_t n = job.n, k = job.k;
   const uint32_t* d = deltas + job.delta_off;
@@ -201,6 +255,16 @@ __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __res
   const uint32_t all = n * (n - 1) / 2;
   uint32_t* queue = reinterpret_cast<uint32_t*>(base + lay.queue);
   uint32_t* cnt = reinterpret_cast<uint32_t*>(base + lay.cnt);
+  // Automata of at most 64 states keep the pair tables in LDS: every
+  // candidate of the pair BFS reads its pair's entry and may race for its
+  // key, a chain of dependent accesses per parent that device memory makes
+  // an L2 round trip each.
+  if (SMALL && n * n <= kEppSmallPairs && k * n <= kEppSmallCnt && all <= kEppSmallCnt) {
+    info = s_info;
+    keys = s_keys;
+    queue = s_queue;
+    cnt = s_cnt;
+  }
 
   // Preimage lists, sources ascending: list (a, t) at beg[a n + t].
   for (uint32_t at = tid; at < k * n; at += nt) {
@@ -290,7 +354,20 @@ __global__ __launch_bounds__(kEppThreads) void k_eppstein_wg(const EppJob* __res
     return;
   }
 
-  epp_greedy(job, n, k, d, info, words, out, cur, nxt, cs, wsum, s_best);
+  // The greedy walk's tables in LDS where they fit: the first letter of
+  // every pair (after the list copy in the dynamic LDS) and the transitions.
+  unsigned char* letters = nullptr;
+  if (letter_lds && uint64_t(n) * n <= letter_lds) {
+    letters = epp_lds + epp_letters_off(lds_cap);
+    for (uint32_t i = tid; i < n * n; i += nt) letters[i] = (unsigned char)(info[i] & 255u);
+  }
+  const uint16_t* dl = nullptr;
+  if (n * k <= kEppDeltaLds) {
+    for (uint32_t i = tid; i < n * k; i += nt) dls[i] = uint16_t(d[i]);
+    dl = dls;
+  }
+  __syncthreads();
+  epp_greedy(job, n, k, d, info, words, out, cs, mask, s_best, &s_stop, letters, dl);
 }
 
 // ---- whole-device pair BFS of one automaton: the passes of the workgroup
@@ -447,15 +524,30 @@ __global__ void k_epp_advance(EppLevel* st, uint32_t all) {
 
 __global__ __launch_bounds__(kEppThreads) void k_epp_greedy(const EppJob* __restrict__ jobs,
     const uint32_t* __restrict__ deltas, const unsigned char* __restrict__ scratch, uint32_t* __restrict__ words,
-    EppOut* __restrict__ outs) {
-  __shared__ uint32_t wsum[16];
+    EppOut* __restrict__ outs, uint32_t letter_lds) {
+  extern __shared__ __align__(16) unsigned char epp_lds[];
   __shared__ uint64_t s_best[16];
-  __shared__ unsigned char cur[kEppMaxN], nxt[kEppMaxN];
+  __shared__ uint64_t mask[8];
+  __shared__ uint32_t s_stop;
   __shared__ uint16_t cs[kEppMaxN];
+  __shared__ uint16_t dls[kEppDeltaLds];
   const EppJob job = jobs[0];
-  const EppLayout lay = epp_layout(job.n, job.k);
+  const uint32_t n = job.n, k = job.k;
+  const EppLayout lay = epp_layout(n, k);
   const uint32_t* info = reinterpret_cast<const uint32_t*>(scratch + job.scratch_off + lay.info);
-  epp_greedy(job, job.n, job.k, deltas + job.delta_off, info, words, outs, cur, nxt, cs, wsum, s_best);
+  const uint32_t* d = deltas + job.delta_off;
+  unsigned char* letters = nullptr;
+  if (letter_lds && uint64_t(n) * n <= letter_lds) {
+    letters = epp_lds;
+    for (uint32_t i = threadIdx.x; i < n * n; i += blockDim.x) letters[i] = (unsigned char)(info[i] & 255u);
+  }
+  const uint16_t* dl = nullptr;
+  if (n * k <= kEppDeltaLds) {
+    for (uint32_t i = threadIdx.x; i < n * k; i += blockDim.x) dls[i] = uint16_t(d[i]);
+    dl = dls;
+  }
+  __syncthreads();
+  epp_greedy(job, n, k, d, info, words, outs, cs, mask, s_best, &s_stop, letters, dl);
 }
 
 }  // namespace
@@ -467,12 +559,23 @@ uint32_t eppstein_list_lds(uint32_t n, uint32_t k) {
   return need <= kEppListLds ? uint32_t(need) : 0u;
 }
 
+uint32_t eppstein_letter_lds(uint32_t max_n, uint32_t list_lds) {
+  const uint64_t need = uint64_t(max_n) * max_n;
+  return epp_letters_off(list_lds) + need <= kEppDynLds ? uint32_t(need) : 0u;
+}
+
 hipError_t launch_eppstein_wg(const EppJob* jobs, uint32_t count, const uint32_t* deltas, unsigned char* scratch,
-    uint32_t* words, EppOut* outs, uint32_t list_lds, hipStream_t st) {
+    uint32_t* words, EppOut* outs, uint32_t list_lds, uint32_t letter_lds, bool small, hipStream_t st) {
   if (!count) return hipSuccess;
-  if (list_lds > kEppListLds) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_eppstein_wg, dim3(count), dim3(kEppThreads), list_lds, st, jobs, deltas, scratch, words, outs,
-      uint32_t(list_lds));
+  if (list_lds > kEppListLds || (letter_lds && epp_letters_off(list_lds) + letter_lds > kEppDynLds))
+    return hipErrorInvalidValue;
+  const uint32_t dyn = letter_lds ? epp_letters_off(list_lds) + letter_lds : list_lds;
+  if (small)
+    hipLaunchKernelGGL(k_eppstein_wg<true>, dim3(count), dim3(kEppThreads), dyn, st, jobs, deltas, scratch, words,
+        outs, uint32_t(list_lds), letter_lds);
+  else
+    hipLaunchKernelGGL(k_eppstein_wg<false>, dim3(count), dim3(kEppThreads), dyn, st, jobs, deltas, scratch, words,
+        outs, uint32_t(list_lds), letter_lds);
   return hipGetLastError();
 }
 
@@ -502,9 +605,11 @@ uint32_t epp_scan_tiles(uint32_t n, uint32_t k) {
   return (most + kEppScanTile - 1) / kEppScanTile;
 }
 
-hipError_t launch_epp_greedy(const EppJob* job, const uint32_t* deltas, const unsigned char* scratch,
+hipError_t launch_epp_greedy(const EppJob* job, uint32_t n, const uint32_t* deltas, const unsigned char* scratch,
     uint32_t* words, EppOut* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_epp_greedy, dim3(1), dim3(kEppThreads), 0, st, job, deltas, scratch, words, out);
+  const uint32_t letters = eppstein_letter_lds(n, 0);
+  hipLaunchKernelGGL(k_epp_greedy, dim3(1), dim3(kEppThreads), letters, st, job, deltas, scratch, words, out,
+      letters);
   return hipGetLastError();
 }
 

@@ -274,10 +274,17 @@ size_t eppstein_scratch_bytes(uint32_t n, uint32_t k);
 // LDS bytes of an automaton's preimage lists, 0 when they are too large to
 // be staged there (the kernels then read them from the scratch).
 uint32_t eppstein_list_lds(uint32_t n, uint32_t k);
+// Bytes of the greedy walk's pair letters (max_n * max_n) when they fit the
+// launch's dynamic LDS after list_lds bytes of lists, else 0 (the walk then
+// reads them from the scratch).
+uint32_t eppstein_letter_lds(uint32_t max_n, uint32_t list_lds);
 // list_lds: the dynamic LDS of the launch, the largest eppstein_list_lds of
-// its jobs (a job whose lists need more reads them from the scratch).
+// its jobs (a job whose lists need more reads them from the scratch);
+// letter_lds: eppstein_letter_lds of its largest n; small: every job has at
+// most 64 states (the pair tables then live in LDS, about 56 KB per
+// workgroup, which larger launches do without to keep two per CU).
 hipError_t launch_eppstein_wg(const EppJob* jobs, uint32_t count, const uint32_t* deltas, unsigned char* scratch,
-    uint32_t* words, EppOut* outs, uint32_t list_lds, hipStream_t st);
+    uint32_t* words, EppOut* outs, uint32_t list_lds, uint32_t letter_lds, bool small, hipStream_t st);
 // The pair BFS of one automaton over the whole device.  EppLevel is its
 // device state: the level's parents at queue[head, tail) (level 0: the k n
 // preimage-list pairs), pairs found so far, done, the last level's winners.
@@ -292,7 +299,7 @@ struct EppLevel {
 hipError_t launch_epp_level(uint32_t level, uint32_t n, uint32_t k, unsigned char* base, uint32_t* off,
     uint32_t* tile_sum, EppLevel* st, hipStream_t stream);
 uint32_t epp_scan_tiles(uint32_t n, uint32_t k);
-hipError_t launch_epp_greedy(const EppJob* job, const uint32_t* deltas, const unsigned char* scratch,
+hipError_t launch_epp_greedy(const EppJob* job, uint32_t n, const uint32_t* deltas, const unsigned char* scratch,
     uint32_t* words, EppOut* out, hipStream_t st);
 
 // Mask width and LDS bytes of the subset kernel.
