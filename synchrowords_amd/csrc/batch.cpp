@@ -75,7 +75,7 @@ bool exact_search_workgroup(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t*
   SW_CHECK_HIP(hipMemcpyAsync(dbytes, bytes.data(), bytes.size(), hipMemcpyHostToDevice, s));
   SW_CHECK_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(dres), 0u, sizeof(BatchResult) / 4, s));
   SW_CHECK_HIP(launch_batch(1, dinst, dbytes, dorder, 1, dhead, dres, dwords, wcap, ctx->wg_scratch.as<uint8_t>(),
-      stride, L, 1, s));
+      stride, L, 1, s, 1024));
   BatchResult r{};
   SW_CHECK_HIP(hipMemcpyAsync(&r, dres, sizeof(r), hipMemcpyDeviceToHost, s));
   SW_CHECK_HIP(hipStreamSynchronize(s));

@@ -92,7 +92,9 @@ __device__ inline void block_scan(bool v, uint32_t* sh, uint32_t& excl, uint32_t
   __syncthreads();
 }
 
-// T threads per workgroup (256: several instances per CU at a time).
+// T threads per workgroup: 256 in the batch's persistent launch (several
+// instances per CU at a time), 1024 for a lone automaton (one workgroup has
+// the whole device to itself).
 template <int W, int T>
 __global__ __launch_bounds__(T) void k_batch(const BatchInstance* __restrict__ inst,
     const uint8_t* __restrict__ deltas, const uint32_t* __restrict__ order, uint32_t count, uint32_t* head,
@@ -435,11 +437,16 @@ __global__ __launch_bounds__(T) void k_batch(const BatchInstance* __restrict__ i
 
 hipError_t launch_batch(int W, const BatchInstance* inst, const uint8_t* deltas, const uint32_t* order,
     uint32_t count, uint32_t* head, BatchResult* out, uint8_t* words, uint32_t word_cap, uint8_t* scratch,
-    size_t scratch_stride, const BatchLimits& L, uint32_t grid, hipStream_t st) {
+    size_t scratch_stride, const BatchLimits& L, uint32_t grid, hipStream_t st, uint32_t threads) {
+  if (threads != 256 && threads != 1024) return hipErrorInvalidValue;
 #define CASE(w)                                                                                              \
   case w:                                                                                                    \
-    hipLaunchKernelGGL((k_batch<w, kBatchThreads>), dim3(grid), dim3(kBatchThreads), 0, st, inst, deltas, order, \
-        count, head, out, words, word_cap, scratch, scratch_stride, L);                                      \
+    if (threads == 256)                                                                                      \
+      hipLaunchKernelGGL((k_batch<w, 256>), dim3(grid), dim3(256), 0, st, inst, deltas, order, count, head, out, \
+          words, word_cap, scratch, scratch_stride, L);                                                      \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_batch<w, 1024>), dim3(grid), dim3(1024), 0, st, inst, deltas, order, count, head,    \
+          out, words, word_cap, scratch, scratch_stride, L);                                                 \
     return hipGetLastError();
   switch (W) {
     CASE(1) CASE(2) CASE(3)

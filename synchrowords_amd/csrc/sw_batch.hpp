@@ -8,7 +8,7 @@ namespace sw {
 
 constexpr uint32_t kBatchMaxK = 4;  // letters handled inside the persistent kernel
 constexpr uint32_t kBatchMaxW = 3;  // words per set (n <= 192)
-constexpr int kBatchThreads = 256;  // threads per workgroup of the persistent launch
+constexpr uint32_t kBatchThreads = 256;  // threads per workgroup of the persistent launch
 
 enum BatchStatus : uint32_t {
   kBatchPending = 0,
@@ -50,6 +50,6 @@ struct BatchLimits {
 size_t batch_scratch_bytes(int W, const BatchLimits& L);
 hipError_t launch_batch(int W, const BatchInstance* inst, const uint8_t* deltas, const uint32_t* order,
     uint32_t count, uint32_t* head, BatchResult* out, uint8_t* words, uint32_t word_cap, uint8_t* scratch,
-    size_t scratch_stride, const BatchLimits& L, uint32_t grid, hipStream_t st);
+    size_t scratch_stride, const BatchLimits& L, uint32_t grid, hipStream_t st, uint32_t threads = kBatchThreads);
 
 }  // namespace sw
