@@ -303,6 +303,8 @@ typedef struct sw_result {
   double ms[8];                 /* per algorithm, plan order */
   char names[8][16];
   sw_exact_result exact;
+  uint32_t reduced_n;           /* states of the automaton Exact searched after a Reduce (0: not reduced) */
+  uint32_t reduce_steps;        /* forward steps the Reduce took before handing over (0: not reduced) */
 } sw_result;
 
 /* Run a plan (the reference's config JSON, e.g. configs/exact.json) on one

@@ -1302,6 +1302,83 @@ static Aut make_aut(uint32_t n, uint32_t k, const uint32_t* delta) {
   return A;
 }
 
+// ------------------------------------------------------------------ Reduce
+// Reduce::run (reduce/reduce.hpp:31-103): a forward search from {Q} whose
+// level is the images of the last one (reduce.hpp:190-214), less those that
+// contain a set of an earlier level and those that properly contain another
+// image (both through the implicit trie, which sorts and deduplicates its
+// lists first); it stops at a singleton, at the upper bound, or once a
+// level holds more than `threshold` sets (reduce.hpp:108-122).  Then the
+// states reachable from the level (reduce.hpp:165-188) give the reduced N.
+struct ReduceOut {
+  u64 steps = 0;      // mlsw after the search
+  int decided = 0;    // 1: bounds set to `length`
+  u64 length = 0;
+  u32 reduced_n = 0;  // 0: no reduction
+};
+
+template <int W>
+static ReduceOut reduce_w(const Aut& A, u64 upper, u64 threshold) {
+  const u32 N = A.n, K = A.k;
+  ReduceOut o;
+  std::vector<Table<W>> T;
+  for (u32 a = 0; a < K; ++a) T.emplace_back(A, a);
+  List<W> list{Set<W>::full(N)}, visited, next;
+  const u64 max_mlsw = upper - 1;
+  u64 mlsw = 0;
+  bool found = false;
+  while (mlsw < max_mlsw) {
+    if (list.size() > threshold) break;
+    mlsw++;
+    next.resize(list.size() * K);
+    for (size_t i = 0; i < list.size(); ++i)
+      for (u32 a = 0; a < K; ++a) next[i * K + a] = T[a].apply(list[i]);
+    std::swap(list, next);
+    // Each call sorts and deduplicates its lists again (the first leaves
+    // `list` reordered).
+    reduce_by<W, false>(visited, list, false);
+    reduce_self(list, false);
+    visited.insert(visited.end(), list.begin(), list.end());
+    bool single = false;
+    for (const auto& x : list) single |= x.size() == 1;
+    if (single) {
+      found = true;
+      break;
+    }
+  }
+  if (!found && mlsw == max_mlsw) {
+    mlsw++;
+    found = true;
+  }
+  o.steps = mlsw;
+  if (found) {
+    o.decided = 1;
+    o.length = mlsw;
+    return o;
+  }
+  std::vector<char> reach(N, 0);
+  std::vector<u32> stack;
+  for (const auto& x : list)
+    for (u32 q = 0; q < N; ++q)
+      if (x.has(q) && !reach[q]) {
+        reach[q] = 1;
+        stack.push_back(q);
+      }
+  while (!stack.empty()) {
+    const u32 q = stack.back();
+    stack.pop_back();
+    for (u32 a = 0; a < K; ++a)
+      if (!reach[A.at(q, a)]) {
+        reach[A.at(q, a)] = 1;
+        stack.push_back(A.at(q, a));
+      }
+  }
+  u32 rn = 0;
+  for (u32 q = 0; q < N; ++q) rn += reach[q];
+  if (rn != N && rn != 0) o.reduced_n = rn;
+  return o;
+}
+
 }  // namespace oracle
 
 // ================================================================== C API
@@ -1359,6 +1436,30 @@ int oracle_solve(uint32_t n, uint32_t k, const uint32_t* delta, int run_exact, u
     if (tr_b) tr_b[i] = tr.bsize[i];
   }
   if (trace_len) *trace_len = tr.phase.size();
+  return 0;
+}
+
+// The Reduce stage on an automaton with upper bound `upper` and the plan's
+// list_size_threshold: steps taken, whether the bounds were settled (and to
+// what), and the reduced number of states (0: no reduction).  Returns 0, or
+// -1 on bad input.
+int oracle_reduce(uint32_t n, uint32_t k, const uint32_t* delta, uint64_t upper, uint64_t threshold, uint64_t* steps,
+    int32_t* decided, uint64_t* length, uint32_t* reduced_n) {
+  if (!n || !k || !delta || !upper) return -1;
+  for (size_t i = 0; i < size_t(n) * k; ++i)
+    if (delta[i] >= n) return -1;
+  Aut A = make_aut(n, k, delta);
+  ReduceOut o;
+  switch ((n + 63) / 64) {
+#define CASE(w) case w: o = reduce_w<w>(A, upper, threshold); break;
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    default: return -1;
+  }
+  *steps = o.steps;
+  *decided = o.decided;
+  *length = o.length;
+  *reduced_n = o.reduced_n;
   return 0;
 }
 

@@ -60,7 +60,7 @@ class _Result(C.Structure):
     _fields_ = [("non_synchro", C.c_int32), ("n_algorithms", C.c_int32), ("lower", C.c_uint64),
                 ("upper", C.c_uint64), ("word_len", C.c_uint64), ("eppstein_bound", C.c_uint64),
                 ("beam_bound", C.c_uint64), ("ms", C.c_double * 8), ("names", (C.c_char * 16) * 8),
-                ("exact", _ExactResult)]
+                ("exact", _ExactResult), ("reduced_n", C.c_uint32), ("reduce_steps", C.c_uint32)]
 
 
 class _BatchConfig(C.Structure):
@@ -233,6 +233,8 @@ class PipelineResult:
     beam_bound: int
     times_ms: list[tuple[str, float]]
     exact: dict
+    reduced_n: int = 0      # states Exact searched after a Reduce stage shrank the automaton (0: none)
+    reduce_steps: int = 0   # forward steps that Reduce took first
 
 
 class DeviceArray:
@@ -730,7 +732,7 @@ class Context:
         w = [int(x) for x in word[: r.word_len]] if r.word_len else None
         ex = {f: getattr(r.exact, f) for f, _ in _ExactResult._fields_}
         return PipelineResult(bool(r.non_synchro), int(r.lower), int(r.upper), w, int(r.eppstein_bound),
-                              int(r.beam_bound), times, ex)
+                              int(r.beam_bound), times, ex, int(r.reduced_n), int(r.reduce_steps))
 
 
 def random_dfa(n: int, k: int, seed: int) -> np.ndarray:

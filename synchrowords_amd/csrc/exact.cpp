@@ -125,8 +125,9 @@ Workspace& workspace(sw_ctx* ctx) {
 
 class Search {
  public:
-  Search(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, const sw_exact_config& cfg)
-      : ctx_(ctx), st_(ctx->stream), n_(n), k_(k), W_((n + 63) / 64), slices_((n + 7) / 8), delta_(delta),
+  Search(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, const sw_exact_config& cfg,
+      const ExactSeed* seed = nullptr)
+      : seed_(seed), ctx_(ctx), st_(ctx->stream), n_(n), k_(k), W_((n + 63) / 64), slices_((n + 7) / 8), delta_(delta),
         cfg_(cfg), ws_(workspace(ctx)), tables_(ws_.tables), F_(ws_.F), B_(ws_.B), cand_(ws_.cand),
         valid_(ws_.valid), slot_(ws_.slot), keep_(ws_.keep), pos_(ws_.pos), small_(ws_.small),
         scan_tmp_(ws_.scan_tmp), flags_(ws_.flags), probes_(ws_.probes), scan_bytes_(ws_.scan_bytes) {
@@ -220,9 +221,11 @@ class Search {
     bool oom = false;
     uint64_t length = 0;
     std::vector<uint32_t> word;
+    int64_t root = -1;
     if (found) {
       length = t;
       word = reconstruct(wit);
+      root = word_root_;
     } else if (budget_end_ && t + 1 < upper && F_.fcount && B_.fcount) {
       if (cfg_.dfs) {
         // Deepen the backward frontier in parts (dfs.cpp:63-91): found,
@@ -233,6 +236,7 @@ class Search {
         r.dfs_entered = 1;
         if (dfs_best_) {
           word = dfs_word_;
+          root = dfs_root_;
           length = dfs_best_;
           found = !oom;
         } else if (!oom) {
@@ -260,6 +264,7 @@ class Search {
       r.found = 1;
       r.length = length;
       out.word = word;
+      out.root = root;
       r.word_len = out.word.size();
     } else if (oom) {
       // Lengths up to t are excluded; a word the deepening found before the
@@ -268,6 +273,7 @@ class Search {
       r.out_of_memory = 1;
       r.length = std::max<uint64_t>(lower, t + 1);
       out.word = word;
+      out.root = root;
       r.word_len = out.word.size();
     } else {
       // No word shorter than the upper bound: the bound is the length.
@@ -306,6 +312,8 @@ class Search {
   }
 
  private:
+  const ExactSeed* seed_;  // forward level 0 other than {Q} (a Reduce stage's list)
+  int64_t word_root_ = -1, dfs_root_ = -1, last_root_ = -1;  // forward level-0 row of a word
   sw_ctx* ctx_;
   hipStream_t st_;
   uint32_t n_, k_, W_, slices_;
@@ -454,6 +462,24 @@ class Search {
   }
 
   void init_forward() {
+    if (seed_) {
+      // Distinct device rows as level 0 (the caller deduplicated them).
+      const uint64_t cnt = seed_->count;
+      ensure_rows(F_, cnt);
+      SW_CHECK_HIP(hipMemcpyAsync(F_.arena.p, seed_->rows, cnt * row_bytes(), hipMemcpyDeviceToDevice, st_));
+      SW_CHECK_HIP(hipMemsetAsync(F_.parent.p, 0xff, cnt * 4, st_));
+      SW_CHECK_HIP(hipMemsetAsync(F_.letter.p, 0xff, cnt, st_));
+      F_.rows = cnt;
+      F_.level_end.push_back(cnt);
+      ensure_table(F_, 0);
+      F_.front.ensure(cnt * row_bytes());
+      F_.front_idx.ensure(cnt * 4);
+      SW_CHECK_HIP(launch_select(int(W_), F_.arena.as<uint64_t>(), 0, nullptr, nullptr, cnt, F_.front.as<uint64_t>(),
+          F_.front_idx.as<uint32_t>(), st_));
+      F_.fcount = cnt;
+      F_.prev_fcount = cnt;
+      return;
+    }
     std::vector<uint64_t> q(W_, 0);
     for (uint32_t s = 0; s < n_; ++s) q[s >> 6] |= 1ull << (s & 63);
     seed(F_, q);
@@ -800,6 +826,7 @@ class Search {
     if (uint32_t(ws_.host[0])) {
       // A word of length lsw + 1: only shorter ones are wanted now.
       dfs_word_ = dfs_reconstruct(d + 1, ws_.host[1]);
+      dfs_root_ = word_root_;
       dfs_best_ = lsw + 1;
       max_depth_ = lsw;
       if (cfg_.verbose)
@@ -834,6 +861,7 @@ class Search {
     std::vector<uint32_t> tail = path(B_, idx);
     v.insert(v.end(), tail.begin(), tail.end());
     std::vector<uint32_t> u = path(F_, f);
+    word_root_ = last_root_;
     std::reverse(u.begin(), u.end());
     u.insert(u.end(), v.begin(), v.end());
     return u;
@@ -853,13 +881,15 @@ class Search {
       w.push_back(letter[row]);
       row = parent[row];
     }
+    last_root_ = int64_t(row);
     return w;  // letters from the set back to level 0
   }
 
   std::vector<uint32_t> reconstruct(uint64_t witness) {
     const uint32_t s = uint32_t(witness >> 32), t = uint32_t(witness & 0xffffffffu);
     std::vector<uint32_t> u = path(F_, s);
-    std::reverse(u.begin(), u.end());  // forward letters from Q
+    word_root_ = last_root_;
+    std::reverse(u.begin(), u.end());  // forward letters from level 0
     std::vector<uint32_t> v = path(B_, t);  // backward letters towards the singleton
     u.insert(u.end(), v.begin(), v.end());
     return u;
@@ -977,8 +1007,12 @@ void forward_levels(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, 
 }
 
 ExactOutcome exact_search(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint64_t lower, uint64_t upper,
-    const sw_exact_config& cfg, bool allow_workgroup) {
+    const sw_exact_config& cfg, bool allow_workgroup, const ExactSeed* seed) {
   use(ctx);
+  if (seed) {
+    SW_REQUIRE(seed->rows && seed->count && seed->count < 0x7fffffffull, "empty or oversized forward seed");
+    allow_workgroup = false;  // the one-workgroup search starts from Q
+  }
   // Small automata: the whole search in one workgroup when it fits (no
   // per-step launches; a few MB of scratch, inside any budget a plan sets);
   // the other settings below only concern the device-wide search.
@@ -990,7 +1024,7 @@ ExactOutcome exact_search(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d
       return o;
     }
   }
-  Search s(ctx, n, k, delta, cfg);
+  Search s(ctx, n, k, delta, cfg, seed);
   return s.run(lower, upper);
 }
 

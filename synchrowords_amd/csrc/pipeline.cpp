@@ -2,12 +2,13 @@
 // (jit/jitmain.cpp:11-49) over a config-JSON plan (configs/exact.json).
 //
 // Brute, Eppstein and Beam are the bound suppliers that sit before Exact in
-// the reference's plans.  All four run on the device: Brute as a power-set
-// BFS (brute.cpp), Eppstein's pair tree and greedy word (eppstein.cpp), the
-// Beam in the reference's order (mlp.cpp, sw_beam_device) and Exact
-// (exact.cpp).  This file parses the plan, evaluates its C++ expressions,
-// keeps [lower, upper] and the word, and does the Beam's presort and dynamic
-// sizing on the host.
+// the reference's plans, Reduce may shrink the automaton Exact searches.
+// All five run on the device: Brute as a power-set BFS (brute.cpp),
+// Eppstein's pair tree and greedy word (eppstein.cpp), the Beam in the
+// reference's order (mlp.cpp, sw_beam_device), Reduce's short forward search
+// (reduce.cpp) and Exact (exact.cpp).  This file parses the plan, evaluates
+// its C++ expressions, keeps [lower, upper] and the word, and does the
+// Beam's presort and dynamic sizing on the host.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -20,6 +21,7 @@
 #include <vector>
 
 #include "sw_host.hpp"
+#include "sw_reduce.hpp"
 
 namespace sw {
 namespace {
@@ -372,6 +374,11 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
   bool non_synchro = false;
   word.clear();
   int idx = 0;
+  // After a Reduce that shrank the automaton, only Exact acts, on the
+  // reduced automaton from the Reduce's list (the reference's other
+  // algorithms return at once then: brute.hpp:39, eppstein.hpp:36,
+  // beam.hpp:48, reduce.hpp:35).
+  ReduceState red;
   for (const Json& a : algos->arr) {
     const Json* nm = a.get("name");
     SW_REQUIRE(nm && nm->kind == Json::STR, "algorithm without a name");
@@ -380,7 +387,9 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
     const std::string name = nm->str;
     auto t0 = std::chrono::steady_clock::now();
     const bool open = !non_synchro && lower != upper;
-    if (name == "Brute") {
+    if (red.reduced && name != "Exact") {
+      // Not run on the reduced automaton.
+    } else if (name == "Brute") {
       const long long max_n = (long long)eval_num(cfg->get("max_n"), "20", n, k);
       SW_REQUIRE(max_n <= 32, "max_n can not be greater than 32");
       if (open && n <= uint64_t(max_n)) {
@@ -454,7 +463,26 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
         ec.strict_memory_limit = eval_bool(cfg->get("strict_memory_limit"), false, n, k) ? 1 : 0;
         const Json* gc = cfg->get("gpu_goal_check");
         if (gc) ec.goal_check = int32_t(eval_num(gc, "0", n, k));
-        auto r = exact_search(ctx, n, k, d, lower, upper, ec);
+        ExactOutcome r;
+        if (red.reduced) {
+          // The reduced automaton from the Reduce's list; lengths and words
+          // get the Reduce's steps back (exact.cpp:83-87 of the reference).
+          const uint64_t s = red.steps;
+          ExactSeed seed;
+          seed.rows = red.list.as<uint64_t>();
+          seed.count = red.list_count;
+          r = exact_search(ctx, red.n_red, k, red.delta_red.data(), lower > s ? lower - s : 0, upper - s, ec, true,
+              &seed);
+          r.r.length += s;
+          if (!r.word.empty() && r.root >= 0) {
+            std::vector<uint32_t> full = red.prefix(uint32_t(r.root));
+            full.insert(full.end(), r.word.begin(), r.word.end());
+            r.word = std::move(full);
+            r.r.word_len = r.word.size();
+          }
+        } else {
+          r = exact_search(ctx, n, k, d, lower, upper, ec);
+        }
         out->exact = r.r;
         if (r.r.found) {
           lower = upper = r.r.length;
@@ -471,15 +499,21 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
         }
       }
     } else if (name == "Reduce") {
-      // The reference's Reduce (algorithm/reduce/reduce.hpp:31-103) runs a
-      // short forward search and, when the states reachable from its list
-      // are fewer than N, restarts the plan on that smaller automaton (a new
-      // JIT build) so that Exact fits the host's memory.  It never changes a
-      // bound Exact computes.  The device search runs on the whole automaton,
-      // so the stage is accepted, checked and recorded, and leaves the
-      // bounds to the stages after it.
-      (void)eval_num(cfg->get("min_n"), "80", n, k);
-      (void)eval_num(cfg->get("list_size_threshold"), "AUT_N * AUT_K * 8", n, k);
+      // The reference's Reduce (algorithm/reduce/reduce.hpp:31-103): a short
+      // forward search that may settle the length, or hand Exact a smaller
+      // automaton and a list to start from (reduce.cpp).
+      const long double min_n = eval_num(cfg->get("min_n"), "80", n, k);
+      const long double thr = eval_num(cfg->get("list_size_threshold"), "AUT_N * AUT_K * 8", n, k);
+      if (open && (long double)n >= min_n && upper != UINT64_MAX && !red.ran) {
+        reduce_stage(ctx, n, k, d, upper, thr <= 0 ? 0 : uint64_t(thr), red);
+        if (red.decided) {
+          lower = upper = red.length;
+          if (!red.word.empty()) word = red.word;
+        } else if (red.reduced) {
+          out->reduced_n = red.n_red;
+          out->reduce_steps = uint32_t(red.steps);
+        }
+      }
     } else {
       throw Error{SW_ERR_UNSUPPORTED, "algorithm not supported by this library: " + name};
     }

@@ -68,6 +68,23 @@ struct DevBuf {
   }
 };
 
+// Buffers of the Reduce stage (reduce.cpp), reused between calls.
+struct ReduceBufs {
+  DevBuf masks, tables, list, next, cand, uniq, uidx, visited, flags, keep, pos, idx, scan, small, map, table, slot;
+  size_t scan_bytes = 0;
+  uint64_t* pinned = nullptr;  // read-back landing zone (64 bytes)
+  uint64_t* host() {
+    if (!pinned) SW_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned), 64, hipHostMallocDefault));
+    return pinned;
+  }
+  ReduceBufs() = default;
+  ReduceBufs(const ReduceBufs&) = delete;
+  ReduceBufs& operator=(const ReduceBufs&) = delete;
+  ~ReduceBufs() {
+    if (pinned) (void)hipHostFree(pinned);
+  }
+};
+
 }  // namespace sw
 
 struct sw_ctx {
@@ -120,6 +137,8 @@ struct sw_ctx {
   uint32_t* ix_host = nullptr;  // pinned: the index length read back
   int ix_enabled = 1;
   uint64_t ix_min_pairs = 1ull << 30;
+  // The plan's Reduce stage.
+  sw::ReduceBufs reduce;
 };
 
 namespace sw {
@@ -169,14 +188,22 @@ std::vector<char> sink_states(uint32_t n, uint32_t k, const uint32_t* delta);
 struct ExactOutcome {
   sw_exact_result r{};
   std::vector<uint32_t> word;
+  int64_t root = -1;  // the forward level-0 row the word starts from (-1: no word)
+};
+// A forward level 0 other than {Q}: `count` distinct device rows (the list a
+// Reduce stage hands to Exact, exact.cpp:131-140 of the reference).
+struct ExactSeed {
+  const uint64_t* rows = nullptr;
+  uint64_t count = 0;
 };
 // Forward levels of the exact search (images of Q up to `depth`).
 void forward_levels(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t depth, uint64_t cap_rows,
     DevBuf& out, std::vector<uint64_t>& level_end);
 // allow_workgroup: automata of at most 64 states first try the one-workgroup
 // search (batch.cpp, exact_search_workgroup).
+// seed: start the forward side from these rows instead of {Q}.
 ExactOutcome exact_search(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint64_t lower, uint64_t upper,
-    const sw_exact_config& cfg, bool allow_workgroup = true);
+    const sw_exact_config& cfg, bool allow_workgroup = true, const ExactSeed* seed = nullptr);
 bool exact_search_workgroup(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, uint64_t upper,
     ExactOutcome& out);
 

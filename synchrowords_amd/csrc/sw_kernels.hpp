@@ -390,4 +390,15 @@ hipError_t launch_lds_read(int bytes_per_lane, uint64_t iters, uint32_t blocks, 
 // Fill 32-bit words.
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t count, hipStream_t st);
 
+// Reduce stage (k_reduce.hip).  out[w] |= the OR of word w over `count` rows
+// of W words (out zeroed by the caller).
+hipError_t launch_or_rows(const uint64_t* rows, uint64_t count, uint32_t W, unsigned long long* out, hipStream_t st);
+// *first = min(*first, the lowest index of a row with exactly one state).
+hipError_t launch_first_singleton(const uint64_t* rows, uint64_t count, uint32_t W, unsigned long long* first,
+    hipStream_t st);
+// out row i (w_out words) = { map[q] : q in in row i (w_in words) }; map has
+// n_in entries, defined for every state the rows hold.
+hipError_t launch_remap_sets(const uint64_t* in, uint64_t count, uint32_t w_in, const uint16_t* map, uint32_t n_in,
+    uint64_t* out, uint32_t w_out, hipStream_t st);
+
 }  // namespace sw

@@ -61,6 +61,17 @@ def solve(n, k, delta, run_exact=True, trace=False, max_memory_mb=0):
     return out
 
 
+def reduce(n, k, delta, upper, threshold):
+    """The Reduce stage: (steps, decided, length, reduced_n); reduced_n 0 when
+    the automaton was not reduced."""
+    d = np.ascontiguousarray(delta, dtype=np.uint32)
+    steps, dec, length, rn = C.c_uint64(), C.c_int32(), C.c_uint64(), C.c_uint32()
+    rc = lib().oracle_reduce(n, k, _p(d), C.c_uint64(upper), C.c_uint64(threshold), C.byref(steps), C.byref(dec),
+                             C.byref(length), C.byref(rn))
+    assert rc == 0, rc
+    return int(steps.value), bool(dec.value), int(length.value), int(rn.value)
+
+
 def brute(n, k, delta) -> int:
     d = np.ascontiguousarray(delta, dtype=np.uint32)
     return int(lib().oracle_brute(n, k, _p(d)))

@@ -256,16 +256,71 @@ def test_memory_budget_without_deepening_gives_a_lower_bound(ctx, case):
 
 @pytest.mark.parametrize("case", [pytest.param(c, id=c["name"]) for c in golden("reduce_plan.json")])
 def test_pipeline_with_the_reduce_plan(ctx, case):
-    """configs/exact_reduce.json (Reduce before Exact): the bounds the
-    reference printed for this plan, after its reduction and the second
-    build for the reduced N (tests/golden/reduce_plan.json), and a word that
-    replays.  The device pipeline records Reduce and searches the whole
-    automaton."""
+    """configs/exact_reduce.json (Reduce before Exact): the device Reduce
+    takes the reference's number of forward steps and reduces to its number
+    of states, and the Exact search on the reduced automaton from the
+    Reduce's list ends at the bounds the reference printed after its second
+    build for the reduced N (tests/golden/reduce_plan.json), with a word
+    (the Reduce's prefix, then the reduced search's letters) that replays on
+    the original automaton."""
     import json
     from cli_helpers import EXACT_REDUCE_PLAN
     n, k = case["n"], case["k"]
     r = ctx.pipeline(n, k, case["delta"], plan=json.dumps(EXACT_REDUCE_PLAN))
     assert r.non_synchro == bool(case["non_synchro"])
+    assert (r.reduced_n, r.reduce_steps) == (case["reduced_n"], case["reduce_bfs_steps"])
     assert (r.lower, r.upper) == (case["lower"], case["upper"])
     assert "Reduce" in [name for name, _ in r.times_ms]
     _check_word(case, r.upper, r.word)
+
+
+def _reduce_plan(threshold: str) -> str:
+    import json
+    from cli_helpers import EXACT_REDUCE_PLAN
+    plan = json.loads(json.dumps(EXACT_REDUCE_PLAN))
+    for a in plan["algorithms"]:
+        if a["name"] == "Reduce":
+            a["config"]["list_size_threshold"] = threshold
+    return json.dumps(plan)
+
+
+@pytest.mark.parametrize("n,k,seed", [(n, k, s) for n, k in ((80, 2), (96, 3), (130, 2), (160, 3), (220, 2))
+                                      for s in range(3)])
+def test_reduce_matches_oracle(ctx, n, k, seed):
+    """Random automata through the Reduce plan: the device Reduce's steps and
+    reduced number of states equal the oracle's (reduce.hpp restated, itself
+    checked against the reference's golden runs) from the same Beam bound,
+    and the final length equals the one of the plan without Reduce."""
+    import oracle_lib as orc
+    d = gen.random_dfa(n, k, 7000 + seed)
+    r = ctx.pipeline(n, k, d, plan=_reduce_plan("AUT_N * AUT_K * 8"))
+    plain = ctx.pipeline(n, k, d)
+    assert r.beam_bound == plain.beam_bound
+    steps, decided, length, rn = orc.reduce(n, k, d, plain.beam_bound, n * k * 8)
+    if decided:
+        assert (r.lower, r.upper) == (length, length) and r.reduced_n == 0
+    else:
+        assert (r.reduced_n, r.reduce_steps) == (rn, steps if rn else 0)
+    assert (r.lower, r.upper) == (plain.lower, plain.upper)
+    assert r.word is not None and len(r.word) == r.upper and replay(n, k, d, r.word) == 1
+
+
+@pytest.mark.parametrize("case", [c for c in _cases("random_misc.json") if c.values[0]["n"] >= 80])
+def test_reduce_settles_the_length(ctx, case):
+    """A plan without Beam (so the upper bound is Eppstein's, above the
+    length) and a list threshold no level reaches: the Reduce's own forward
+    search runs until a level holds a singleton, which settles the length
+    (reduce.hpp:55-67).  The reference's length, the oracle's Reduce agrees,
+    and the word is the Reduce's own (a path to that singleton)."""
+    import json
+    import oracle_lib as orc
+    n, k = case["n"], case["k"]
+    plan = json.loads(_reduce_plan("AUT_N * 100000"))
+    plan["algorithms"] = [a for a in plan["algorithms"] if a["name"] != "Beam"]
+    r = ctx.pipeline(n, k, case["delta"], plan=json.dumps(plan))
+    assert r.eppstein_bound > case["upper"]
+    assert (r.lower, r.upper) == (case["upper"], case["upper"])
+    assert r.reduced_n == 0 and r.exact["found"] == 0
+    _check_word(case, r.upper, r.word)
+    _, decided, length, _ = orc.reduce(n, k, case["delta"], r.eppstein_bound, n * 100000)
+    assert decided and length == case["upper"]

@@ -195,8 +195,7 @@ def test_reference_reduce_plan_keeps_the_exact_lengths():
     """The reference's configs/exact_reduce.json plan (its Reduce shrank
     every one of these automata, reduce.hpp:31-103, and a second build for
     the reduced N finished the search) ends at the same bounds as its
-    configs/exact.json plan: the reduction never changes a length, which is
-    why the device pipeline may run Reduce's automaton unreduced."""
+    configs/exact.json plan: the reduction never changes a length."""
     red = {c["name"]: c for c in golden("reduce_plan.json")}
     assert len(red) >= 20 and all(c["reduced_n"] and c["reduced_n"] < c["n"] for c in red.values())
     seen = 0
@@ -209,3 +208,18 @@ def test_reference_reduce_plan_keeps_the_exact_lengths():
                                                                       c["upper"]), name
                 seen += 1
     assert seen == len(red)
+
+
+@pytest.mark.parametrize("case", [pytest.param(c, id=c["name"]) for c in golden("reduce_plan.json")])
+def test_oracle_reduce_matches_reference(case):
+    """The oracle's Reduce (reduce.hpp:31-103 restated) takes as many
+    forward steps as the reference's and reduces to the same number of
+    states, from the plan's Beam bound with the plan's list_size_threshold
+    (N * K * 8).  A restatement that skipped the re-sort before the
+    self-reduction (the implicit trie needs its lists sorted) stopped one
+    step early on ten of these automata."""
+    n, k = case["n"], case["k"]
+    up = orc.solve(n, k, case["delta"], run_exact=False)["upper"]
+    steps, decided, _, rn = orc.reduce(n, k, case["delta"], up, n * k * 8)
+    assert not decided
+    assert (steps, rn) == (case["reduce_bfs_steps"], case["reduced_n"])
