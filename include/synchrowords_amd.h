@@ -223,6 +223,9 @@ typedef struct sw_exact_config {
                                  1 always the vertical subset kernel, 2 always the forward trie
                                  (the deepening phase's checks too), 3 tries over both frontiers
                                  (the one that did not step; the forward trie when deepening) */
+  int32_t side_rule;          /* which side steps: 0 default (1), 1 the smaller frontier, 2 planned
+                                 by the upper bound: the cheapest order of the steps left below it
+                                 under a cost model of expansion and meet checks, first move taken */
 } sw_exact_config;
 
 typedef struct sw_exact_result {

@@ -43,7 +43,7 @@ def words(n: int) -> int:
 class _ExactConfig(C.Structure):
     _fields_ = [("max_memory_mb", C.c_uint64), ("dominance_pairs", C.c_uint64), ("forward_cap", C.c_uint64),
                 ("dfs", C.c_int32), ("verbose", C.c_int32), ("dfs_min_list_size", C.c_uint64),
-                ("strict_memory_limit", C.c_int32), ("goal_check", C.c_int32)]
+                ("strict_memory_limit", C.c_int32), ("goal_check", C.c_int32), ("side_rule", C.c_int32)]
 
 
 class _ExactResult(C.Structure):
@@ -207,11 +207,12 @@ class ExactConfig:
     dfs_min_list_size: int = 0
     strict_memory_limit: bool = False
     goal_check: int = 0  # 0 automatic, 1 vertical subset kernel, 2 device subset trie
+    side_rule: int = 0   # 0 default, 1 the smaller frontier steps, 2 steps planned by the upper bound
 
     def _c(self) -> _ExactConfig:
         return _ExactConfig(self.max_memory_mb, self.dominance_pairs, self.forward_cap, int(self.dfs),
                             int(self.verbose), self.dfs_min_list_size, int(self.strict_memory_limit),
-                            self.goal_check)
+                            self.goal_check, self.side_rule)
 
 
 @dataclass

@@ -84,6 +84,20 @@ def test_cerny64_word(ctx):
     assert replay(64, 2, d, r.word) == 1
 
 
+@pytest.mark.parametrize("case", _cases("c5_random300.json") + _cases("c3_random200.json")[:3])
+def test_bound_planned_side_order_keeps_the_length(ctx, case):
+    """The side order planned by the upper bound (side_rule 2: the remaining
+    steps below the Beam bound ordered by a cost model) reaches the
+    reference's length, like the balanced order: the order of the steps
+    never changes the answer."""
+    import synchrowords_amd as sw
+    n, k = case["n"], case["k"]
+    r = ctx.exact(n, k, case["delta"], upper=case["beam"], cfg=sw.ExactConfig(side_rule=2))
+    assert r.found and r.length == case["upper"]
+    if not r.bound_word:
+        _check_word(case, r.length, r.word)
+
+
 @pytest.mark.parametrize("n,k", [(40, 3), (90, 2), (90, 3), (150, 2), (260, 2), (300, 2)])
 def test_dominance_pruning_keeps_the_length(ctx, n, k):
     """Dropping dominated sets (both sides) never changes the shortest length:
