@@ -17,7 +17,6 @@
 #include "sw_kernels.hpp"
 #include "sw_mlp.hpp"
 
-#include <hipcub/hipcub.hpp>
 
 #include <utility>
 
@@ -364,14 +363,6 @@ __global__ void k_first_full(const uint64_t* __restrict__ sets, uint64_t count, 
   if (uint32_t(c) == n) atomicMin(first, (unsigned long long)i);
 }
 
-__global__ void k_gather_rows(const uint64_t* __restrict__ rows, const uint32_t* __restrict__ idx, uint64_t count,
-    uint32_t W, uint64_t* __restrict__ out) {
-  const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= count) return;
-  const uint64_t r = idx[j];
-  for (uint32_t w = 0; w < W; ++w) out[j * W + w] = rows[r * W + w];
-}
-
 __global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t count,
     uint32_t* __restrict__ out) {
   const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -386,25 +377,12 @@ __global__ void k_neg_popcount(const uint64_t* __restrict__ sets, uint64_t count
   y[i] = -float(c);
 }
 
-
-__global__ void k_iota_u32(uint32_t* p, uint64_t count) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < count) p[i] = uint32_t(i);
-}
-
 }  // namespace
 
 hipError_t launch_first_full(const uint64_t* sets, uint64_t count, uint32_t n, uint32_t W, unsigned long long* first,
     hipStream_t st) {
   if (!count) return hipSuccess;
   hipLaunchKernelGGL(k_first_full, dim3(blocks(count)), dim3(256), 0, st, sets, count, n, W, first);
-  return hipGetLastError();
-}
-
-hipError_t launch_gather_rows(const uint64_t* rows, const uint32_t* idx, uint64_t count, uint32_t W, uint64_t* out,
-    hipStream_t st) {
-  if (!count) return hipSuccess;
-  hipLaunchKernelGGL(k_gather_rows, dim3(blocks(count)), dim3(256), 0, st, rows, idx, count, W, out);
   return hipGetLastError();
 }
 
@@ -418,12 +396,6 @@ hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint64_t 
 hipError_t launch_neg_popcount(const uint64_t* sets, uint64_t count, uint32_t W, float* y, hipStream_t st) {
   if (!count) return hipSuccess;
   hipLaunchKernelGGL(k_neg_popcount, dim3(blocks(count)), dim3(256), 0, st, sets, count, W, y);
-  return hipGetLastError();
-}
-
-hipError_t launch_iota(uint32_t* p, uint64_t count, hipStream_t st) {
-  if (!count) return hipSuccess;
-  hipLaunchKernelGGL(k_iota_u32, dim3(blocks(count)), dim3(256), 0, st, p, count);
   return hipGetLastError();
 }
 
@@ -470,36 +442,6 @@ __global__ __launch_bounds__(256) void k_score_rank(const float* __restrict__ ke
   }
 }
 
-hipError_t sort_scores(const float* keys_in, float* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
-    uint64_t count, void* temp, size_t* temp_bytes, hipStream_t st) {
-  return hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, keys_in, keys_out, vals_in, vals_out, int(count), 0,
-      32, st);
-}
-
-// The reference's Beam comparator over distinct sets: larger first, then
-// word 0, word 1, ... ascending.  neg[i] = n - |S_i|.
-struct BeamLess {
-  const uint64_t* sets;
-  const uint32_t* neg;
-  uint32_t W;
-  __device__ bool operator()(uint32_t a, uint32_t b) const {
-    if (neg[a] != neg[b]) return neg[a] < neg[b];
-    for (uint32_t w = 0; w < W; ++w) {
-      const uint64_t x = sets[uint64_t(a) * W + w], y = sets[uint64_t(b) * W + w];
-      if (x != y) return x < y;
-    }
-    return false;
-  }
-};
-
-__global__ void k_neg_size(const uint64_t* __restrict__ sets, uint64_t count, uint32_t W, uint32_t n,
-    uint32_t* __restrict__ neg) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  uint32_t c = 0;
-  for (uint32_t w = 0; w < W; ++w) c += __popcll(sets[i * W + w]);
-  neg[i] = n - c;
-}
 
 // Up to kBeamRankMax sets, the place of each set in that order is counted
 // directly: rank[i] = #{j : set j comes before set i}, over a grid of (64
@@ -645,26 +587,98 @@ hipError_t launch_beam_prep(uint32_t* table, uint64_t slots, unsigned long long*
   return hipGetLastError();
 }
 
-hipError_t beam_reference_order(const uint64_t* sets, uint64_t count, uint32_t W, uint32_t n, uint32_t* perm_a,
-    uint32_t* perm_b, uint64_t* key_a, uint64_t* key_b, void* temp, size_t* temp_bytes, uint32_t** result,
-    hipStream_t st) {
-  (void)perm_b;
-  (void)key_b;
-  if (count > 0x7fffffffull) return hipErrorInvalidValue;
-  const int cnt = int(count);
-  uint32_t* neg = reinterpret_cast<uint32_t*>(key_a);
-  if (!temp) {
-    return hipcub::DeviceMergeSort::SortKeys(nullptr, *temp_bytes, static_cast<uint32_t*>(nullptr), cnt ? cnt : 1,
-        BeamLess{nullptr, nullptr, W}, st);
+// Past the counted maximum a depth keeps its first `keep` sets by selection
+// instead of a sort: which sets a depth keeps does not depend on their
+// stored order, so one workgroup finds them by a most-significant-digit
+// radix selection over the order's key, 8 bits a pass, and the kept sets
+// are then compacted in candidate order.  The key of a set in the
+// reference's order is the bit string (n - |S|) in 10 bits, word 0, word 1,
+// ... (64 bits each), read as 64-bit chunks: chunk 0 = (n - |S|) << 54 |
+// word 0 >> 10, chunk c = word c-1 << 54 | word c >> 10, chunk W = word W-1
+// << 54.  Distinct sets have distinct keys.  Scores (the learned order) have
+// the one chunk float_order(score) << 32 | index (ascending, stable).
+// Each pass histograms the digit of the items still undecided, finds the
+// digit d where the `need` first end, keeps those below d, drops those above
+// and continues with d's bucket, or stops when all of d's bucket is needed.
+// state[i]: 0 undecided, 1 kept, 2 dropped; keep[i] = state != 2 at the end.
+template <int W, bool SCORES>
+__device__ __forceinline__ uint64_t select_chunk(const uint64_t* __restrict__ sets, const float* __restrict__ scores,
+    uint32_t i, uint32_t n, uint32_t c) {
+  if (SCORES) return (uint64_t(float_order(scores[i])) << 32) | i;
+  const uint64_t* x = sets + uint64_t(i) * W;
+  if (c == 0) {
+    uint32_t pc = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) pc += uint32_t(__popcll(x[w]));
+    return (uint64_t(n - pc) << 54) | (x[0] >> 10);
   }
-  *result = perm_a;
+  return (x[c - 1] << 54) | (c < uint32_t(W) ? (x[c] >> 10) : 0ull);
+}
+
+constexpr uint32_t kSelectThreads = 1024;
+
+template <int W, bool SCORES>
+__global__ __launch_bounds__(kSelectThreads) void k_select_first(const uint64_t* __restrict__ sets,
+    const float* __restrict__ scores, uint32_t count, uint32_t n, uint32_t keep, uint8_t* __restrict__ state,
+    uint32_t* __restrict__ keep_flag) {
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t s_need, s_digit, s_done;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < count; i += kSelectThreads) state[i] = 0;
+  if (tid == 0) {
+    s_need = keep;
+    s_done = keep >= count ? 1u : 0u;
+  }
+  __syncthreads();
+  const uint32_t chunks = SCORES ? 1u : uint32_t(W) + 1u;
+  for (uint32_t c = 0; c < chunks && !s_done; ++c) {
+    for (int dp = 7; dp >= 0 && !s_done; --dp) {
+      const uint32_t shift = uint32_t(dp) * 8u;
+      for (uint32_t b = tid; b < 256; b += kSelectThreads) hist[b] = 0;
+      __syncthreads();
+      for (uint32_t i = tid; i < count; i += kSelectThreads)
+        if (state[i] == 0) atomicAdd(&hist[(select_chunk<W, SCORES>(sets, scores, i, n, c) >> shift) & 255u], 1u);
+      __syncthreads();
+      if (tid == 0) {
+        // The undecided items number at least s_need, so some digit ends the need.
+        uint32_t below = 0, d = 0;
+        for (; d < 255; ++d) {
+          if (below + hist[d] >= s_need) break;
+          below += hist[d];
+        }
+        s_digit = d;
+        s_need -= below;
+        s_done = hist[d] == s_need ? 1u : 0u;
+      }
+      __syncthreads();
+      const uint32_t ds = s_digit;
+      for (uint32_t i = tid; i < count; i += kSelectThreads) {
+        if (state[i] != 0) continue;
+        const uint32_t dg = uint32_t(select_chunk<W, SCORES>(sets, scores, i, n, c) >> shift) & 255u;
+        if (dg != ds) state[i] = dg < ds ? 1 : 2;
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = tid; i < count; i += kSelectThreads) keep_flag[i] = state[i] != 2 ? 1u : 0u;
+}
+
+hipError_t beam_select_first(const uint64_t* sets, const float* scores, uint64_t count, uint32_t W, uint32_t n,
+    uint64_t keep, uint8_t* state, uint32_t* keep_flag, hipStream_t st) {
   if (!count) return hipSuccess;
-  // One comparison sort of the indices; the sets are distinct, so the order
-  // is total and any correct sort gives the reference's.
-  hipLaunchKernelGGL(k_iota_u32, dim3(blocks(count)), dim3(256), 0, st, perm_a, count);
-  hipLaunchKernelGGL(k_neg_size, dim3(blocks(count)), dim3(256), 0, st, sets, count, W, n, neg);
-  hipError_t e = hipcub::DeviceMergeSort::SortKeys(temp, *temp_bytes, perm_a, cnt, BeamLess{sets, neg, W}, st);
-  if (e != hipSuccess) return e;
+  if (count > 0xffffffffull || W < 1 || W > 8) return hipErrorInvalidValue;
+  const dim3 g(1), b(kSelectThreads);
+  const uint32_t c32 = uint32_t(count), k32 = uint32_t(keep < count ? keep : count);
+  if (scores) {
+    hipLaunchKernelGGL((k_select_first<1, true>), g, b, 0, st, sets, scores, c32, n, k32, state, keep_flag);
+    return hipGetLastError();
+  }
+  switch (W) {
+#define CASE(w) \
+  case w: hipLaunchKernelGGL((k_select_first<w, false>), g, b, 0, st, sets, scores, c32, n, k32, state, keep_flag); break;
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+  }
   return hipGetLastError();
 }
 

@@ -158,11 +158,11 @@ void build_guide(uint32_t n, uint32_t k, const uint32_t* delta, Guide& out, hipS
 
 // Device buffers of the beam, kept by the context between calls.
 struct BeamWs {
-  DevBuf dtf, dtb, list, cand, table, slot, keep, pos, uniq, uidx, x, scratch, keys, keys2, vals, vals2, small,
-      scan_tmp, sort_tmp, hist, mf, mb;
+  DevBuf dtf, dtb, list, cand, table, slot, keep, pos, uniq, uidx, x, scratch, keys, vals, vals2, small, scan_tmp,
+      hist, mf, mb;
   DevBuf sb_in, sb_meta, sb_word;  // one-workgroup beam: seed and transitions, depth table, word
   std::vector<uint64_t> host_mf, host_mb;  // per-state masks of the tables, until the next call
-  size_t scan_bytes = 0, sort_bytes = 0;
+  size_t scan_bytes = 0;
   uint64_t* host = nullptr;  // pinned read-back
   BeamWs() { SW_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), 64, hipHostMallocDefault)); }
   ~BeamWs() {
@@ -576,9 +576,9 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
     sw::BeamWs& ws = sw::beam_workspace(ctx);
     DevBuf &dtf = ws.dtf, &dtb = ws.dtb, &list = ws.list, &cand = ws.cand, &table = ws.table, &slot = ws.slot,
            &keep = ws.keep, &pos = ws.pos, &uniq = ws.uniq, &uidx = ws.uidx, &x = ws.x, &scratch = ws.scratch,
-           &keys = ws.keys, &keys2 = ws.keys2, &vals = ws.vals, &vals2 = ws.vals2, &small = ws.small,
-           &scan_tmp = ws.scan_tmp, &sort_tmp = ws.sort_tmp, &histd = ws.hist;
-    size_t &scan_bytes = ws.scan_bytes, &sort_bytes = ws.sort_bytes;
+           &keys = ws.keys, &vals = ws.vals, &vals2 = ws.vals2, &small = ws.small, &scan_tmp = ws.scan_tmp,
+           &histd = ws.hist;
+    size_t &scan_bytes = ws.scan_bytes;
     if (m) sw::build_tables_device(n, k, delta, 0, ws.host_mf, ws.mf, dtf, st);  // images: the features only
     bool tables_built = false;
     auto build_preimage_tables = [&]() {
@@ -770,40 +770,29 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
           SW_CHECK_HIP(sw::beam_select_reference(uniq.as<uint64_t>(), uidx.as<uint32_t>(), nu, W, n, keepn,
               vals.as<uint32_t>(), list.as<uint64_t>(), histd.as<uint32_t>() + hist_len, st));
       } else {
-        const uint32_t* order = nullptr;
+        // Past the counted maximum: the kept sets found by selection (no
+        // sort; their stored order does not matter), then compacted in
+        // candidate order with their history indices.
         vals.ensure(uint64_t(nu) * 4 + 4);
         vals2.ensure(uint64_t(nu) * 4 + 4);
-        if (m) {
-          keys2.ensure(uint64_t(nu) * 4 + 4);
-          SW_CHECK_HIP(sw::launch_iota(vals.as<uint32_t>(), nu, st));
-          size_t sneed = 0;
-          SW_CHECK_HIP(sw::sort_scores(nullptr, nullptr, nullptr, nullptr, nu, nullptr, &sneed, st));
-          if (sneed > sort_bytes) {
-            sort_tmp.release();
-            sort_tmp.ensure(sneed);
-            sort_bytes = sneed;
-          }
-          SW_CHECK_HIP(sw::sort_scores(scores, keys2.as<float>(), vals.as<uint32_t>(), vals2.as<uint32_t>(), nu,
-              sort_tmp.p, &sort_bytes, st));
-          order = vals2.as<uint32_t>();
-        } else {
-          keys.ensure(uint64_t(nu) * 8 + 8);
-          keys2.ensure(uint64_t(nu) * 8 + 8);
-          size_t sneed = 0;
-          uint32_t* res = nullptr;
-          SW_CHECK_HIP(sw::beam_reference_order(uniq.as<uint64_t>(), nu, W, n, vals.as<uint32_t>(),
-              vals2.as<uint32_t>(), keys.as<uint64_t>(), keys2.as<uint64_t>(), nullptr, &sneed, &res, st));
-          if (sneed > sort_bytes) {
-            sort_tmp.release();
-            sort_tmp.ensure(sneed);
-            sort_bytes = sneed;
-          }
-          SW_CHECK_HIP(sw::beam_reference_order(uniq.as<uint64_t>(), nu, W, n, vals.as<uint32_t>(),
-              vals2.as<uint32_t>(), keys.as<uint64_t>(), keys2.as<uint64_t>(), sort_tmp.p, &sort_bytes, &res, st));
-          order = res;
+        keys.ensure(uint64_t(nu) + 8);
+        pos.ensure(uint64_t(nu) * 4 + 4);
+        SW_CHECK_HIP(sw::beam_select_first(uniq.as<uint64_t>(), scores, nu, W, n, keepn, keys.as<uint8_t>(),
+            vals.as<uint32_t>(), st));
+        size_t need = 0;
+        SW_CHECK_HIP(sw::keep_scan_temp_bytes(nu, &need));
+        if (need > scan_bytes) {
+          scan_tmp.release();
+          scan_tmp.ensure(need);
+          scan_bytes = need;
         }
-        SW_CHECK_HIP(sw::launch_gather_rows(uniq.as<uint64_t>(), order, keepn, W, list.as<uint64_t>(), st));
-        SW_CHECK_HIP(sw::launch_gather_u32(uidx.as<uint32_t>(), order, keepn, histd.as<uint32_t>() + hist_len, st));
+        uint32_t* total = small.as<uint32_t>() + 4;
+        SW_CHECK_HIP(sw::launch_keep_scan(0, nullptr, nullptr, nullptr, nu, vals.as<uint32_t>(), pos.as<uint32_t>(),
+            total, scan_tmp.p, &scan_bytes, st));
+        SW_CHECK_HIP(sw::launch_select(int(W), uniq.as<uint64_t>(), 0, vals.as<uint32_t>(), pos.as<uint32_t>(), nu,
+            list.as<uint64_t>(), vals2.as<uint32_t>(), st));
+        SW_CHECK_HIP(sw::launch_gather_u32(uidx.as<uint32_t>(), vals2.as<uint32_t>(), keepn,
+            histd.as<uint32_t>() + hist_len, st));
       }
       hist_off.push_back(hist_len);
       hist_m.push_back(m0);

@@ -55,20 +55,18 @@ hipError_t mlp_refresh_bf16(MlpDev& p, hipStream_t st);
 // Smallest index of a set with all n states (first must hold ~0).
 hipError_t launch_first_full(const uint64_t* sets, uint64_t count, uint32_t n, uint32_t W, unsigned long long* first,
     hipStream_t st);
-hipError_t launch_gather_rows(const uint64_t* rows, const uint32_t* idx, uint64_t count, uint32_t W, uint64_t* out,
-    hipStream_t st);
-hipError_t launch_iota(uint32_t* p, uint64_t count, hipStream_t st);
 // out[j] = src[idx[j]].
 hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint64_t count, uint32_t* out,
     hipStream_t st);
 // y[i] = -|sets[i]| (the cardinality order as an ascending key).
 hipError_t launch_neg_popcount(const uint64_t* sets, uint64_t count, uint32_t W, float* y, hipStream_t st);
-// The reference's Beam order of `count` distinct sets: larger first, then by
-// word 0, word 1, ... ascending (beam.hpp's comparator).  Leaves in *result
-// (perm_a or perm_b) the set indices in that order.  temp == NULL: size query.
-hipError_t beam_reference_order(const uint64_t* sets, uint64_t count, uint32_t W, uint32_t n, uint32_t* perm_a,
-    uint32_t* perm_b, uint64_t* key_a, uint64_t* key_b, void* temp, size_t* temp_bytes, uint32_t** result,
-    hipStream_t st);
+// Which `keep` of `count` distinct sets come first, by selection (one
+// workgroup, no sort): in the reference's Beam order (scores NULL: larger
+// first, then word 0, word 1, ... ascending, beam.hpp's comparator) or in
+// ascending stable order of float scores.  keep_flag[i] = 1 for the kept;
+// state: count bytes of scratch.
+hipError_t beam_select_first(const uint64_t* sets, const float* scores, uint64_t count, uint32_t W, uint32_t n,
+    uint64_t keep, uint8_t* state, uint32_t* keep_flag, hipStream_t st);
 // The first `keep` sets of a depth, by place, written to out_rows/out_idx
 // (sets[i], idx[i] of the set at place r < keep), for count <= the counted
 // maximum (beam_rank_fits): in the reference's Beam order, or in ascending
@@ -80,8 +78,5 @@ hipError_t beam_select_scores(const float* scores, const uint64_t* sets, const u
     uint32_t W, uint64_t keep, uint32_t* rank, uint64_t* out_rows, uint32_t* out_idx, hipStream_t st);
 // table[0 .. slots) = empty and *first = ~0 (the start of a beam depth).
 hipError_t launch_beam_prep(uint32_t* table, uint64_t slots, unsigned long long* first, hipStream_t st);
-// Stable ascending sort of float keys with u32 values (temp == NULL: size query).
-hipError_t sort_scores(const float* keys_in, float* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
-    uint64_t count, void* temp, size_t* temp_bytes, hipStream_t st);
 
 }  // namespace sw

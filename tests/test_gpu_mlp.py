@@ -298,3 +298,82 @@ def test_trained_scorer_gains_over_cardinality_order(ctx):
             assert b >= known[s]
     assert sum(learned) < sum(cardinality), (learned, cardinality)
     assert sum(l < c for l, c in zip(learned, cardinality)) > sum(l > c for l, c in zip(learned, cardinality))
+
+
+def _sink_states(n, k, d):
+    """States of the sink component: those reachable from every state (the
+    automata here are synchronizing, so it is unique)."""
+    dd = np.asarray(d, dtype=np.int64).reshape(n, k)
+    reach = []
+    for s in range(n):
+        seen = np.zeros(n, dtype=bool)
+        seen[s] = True
+        stack = [s]
+        while stack:
+            q = stack.pop()
+            for t in dd[q]:
+                if not seen[t]:
+                    seen[t] = True
+                    stack.append(int(t))
+        reach.append(seen)
+    return np.logical_and.reduce(reach)
+
+
+def _learned_beam_restated(m, n, k, d, width, cap):
+    """The learned beam (beam.hpp:98-188 with the network's order): from the
+    sink singletons with two preimages under some letter, each depth takes
+    the preimages (letter-major), stops at the full set, drops repeats (the
+    first kept), and keeps the `width` lowest scores (ties by position)."""
+    import oracle_lib as orc
+    dd = np.asarray(d, dtype=np.int64).reshape(n, k)
+    sink = _sink_states(n, k, d)
+    indeg = np.zeros((n, k), dtype=np.int64)
+    for a in range(k):
+        np.add.at(indeg[:, a], dd[:, a], 1)
+    W = sw.words(n)
+    seeds = [q for q in range(n) if sink[q] and (indeg[q] > 1).any()]
+    lst = np.zeros((len(seeds), W), dtype=np.uint64)
+    for i, q in enumerate(seeds):
+        lst[i, q // 64] = np.uint64(1) << np.uint64(q % 64)
+    full = np.zeros(W, dtype=np.uint64)
+    for q in range(n):
+        full[q // 64] |= np.uint64(1) << np.uint64(q % 64)
+    for length in range(1, cap):
+        cand = orc.apply(n, k, d, lst, backward=True)
+        if (cand == full).all(axis=1).any():
+            return length
+        _, first = np.unique(cand, axis=0, return_index=True)
+        uniq = cand[np.sort(first)]
+        if len(uniq) > width:
+            y = m.forward(m.features(d, uniq))
+            uniq = uniq[np.sort(np.argsort(y, kind="stable")[:width])]
+        lst = uniq
+        if not len(lst):
+            return cap
+    return cap
+
+
+@pytest.mark.parametrize("n,width,trained", [(300, 12000, True), (100, 9000, False)])
+def test_learned_beam_past_the_counted_order(ctx, n, width, trained):
+    """Depths of more than 16384 distinct candidates select the kept sets on
+    the device by radix selection over (score, position): the bound equals
+    a host restatement that orders the same network's scores with a stable
+    sort, for the committed network and for an untrained one."""
+    from pathlib import Path
+    from synchrowords_amd.train_scorer import load_scorer
+    k = 2
+    if trained:
+        m = load_scorer(ctx, Path(sw.__file__).parent / "models" / "scorer_n300_k2.npz")
+    else:
+        m = sw.Mlp(ctx, n, k, hidden=64, seed=3)
+    try:
+        for s in (11, 12):
+            d = gen.random_dfa(n, k, 9100 + s)
+            cap = n * 2
+            want = _learned_beam_restated(m, n, k, d, width, cap)
+            b, w = m.beam(d, width, cap)
+            assert b == want
+            if b < cap:
+                assert len(w) == b and replay(n, k, d, w) == 1
+    finally:
+        m.close()
