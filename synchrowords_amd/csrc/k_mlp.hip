@@ -7,9 +7,10 @@
 // (the length of the shortest word u with delta(Q, u) inside S), as measured
 // by the exact search's own forward levels.
 //
-// GEMM tiles use v_mfma_f32_16x16x16bf16_1k: a workgroup of 4 waves computes a
-// 64x64 tile of C, each wave a 32x32 quadrant of 2x2 MFMA blocks; K advances
-// 32 at a time through LDS.  The network is small (about 0.4 M parameters), so
+// GEMM tiles use gfx950's double-rate v_mfma_f32_16x16x32_bf16: a workgroup
+// of 4 waves computes a 64x64 tile of C, each wave a 32x32 quadrant of 2x2
+// MFMA blocks; K advances 32 at a time through LDS, one MFMA per block and
+// step, its operands one 16-byte LDS read per lane.  The network is small (about 0.4 M parameters), so
 // the layers are latency- rather than MFMA-bound at beam batch sizes.
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
@@ -24,7 +25,7 @@ namespace sw {
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline uint16_t f2bf(float f) {
@@ -101,25 +102,21 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, in
     }
     }
     __syncthreads();
+    static_assert(TK % 32 == 0 && (TK + 8) % 8 == 0, "k blocks of 32; rows of whole 16-byte lines");
 #pragma unroll
-    for (int kk = 0; kk < TK; kk += 16) {
-      // 16x16x16 block: lane l holds A[l%16][4*(l/16) .. +3] and B[4*(l/16) .. +3][l%16].
-      const int r = lane & 15, kq = kk + 4 * (lane >> 4);
-      bf16x4 a[2], b[2];
+    for (int kk = 0; kk < TK; kk += 32) {
+      // gfx950's double-rate 16x16x32 block: lane l holds A[l%16][8*(l/16) .. +7]
+      // and B[8*(l/16) .. +7][l%16], one 16-byte LDS read each.
+      const int r = lane & 15, kq = kk + 8 * (lane >> 4);
+      bf16x8 a[2], b[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const uint16_t* p = &As[wm + i * 16 + r][kq];
-        a[i] = bf16x4{short(p[0]), short(p[1]), short(p[2]), short(p[3])};
-      }
+      for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&As[wm + i * 16 + r][kq]);
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const uint16_t* p = &Bs[wn + j * 16 + r][kq];
-        b[j] = bf16x4{short(p[0]), short(p[1]), short(p[2]), short(p[3])};
-      }
+      for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn + j * 16 + r][kq]);
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
