@@ -246,6 +246,50 @@ def bench_times(ctx) -> dict:
     return out
 
 
+def bench_reduce(ctx) -> dict:
+    """The configs/exact_reduce.json plan (Reduce on the device, then Exact on
+    the reduced automaton from the Reduce's list) against configs/exact.json
+    on the same automata: random binary DFAs n = 300 (seeds 1000..1004) and
+    n = 200 (seeds 0..9).  Best of two warm calls each; the lengths must
+    agree with each other and the reduced N with the reference's
+    (tests/golden/reduce_plan.json)."""
+    from synchrowords_amd import gen
+    sys.path.insert(0, str(ROOT / "tests"))
+    from cli_helpers import EXACT_PLAN, EXACT_REDUCE_PLAN
+    plans = {"exact": json.dumps(EXACT_PLAN), "exact_reduce": json.dumps(EXACT_REDUCE_PLAN)}
+    golden = {c["name"]: c for c in json.loads((ROOT / "tests" / "golden" / "reduce_plan.json").read_text())["cases"]}
+    work = [(f"random300_s{s}", 300, gen.random_dfa(300, 2, s)) for s in range(1000, 1005)] + \
+           [(f"random200_s{s}", 200, gen.random_dfa(200, 2, s)) for s in range(10)]
+    ctx.pipeline(300, 2, work[0][2], plan=plans["exact_reduce"])  # warm
+    rows, tot = [], {"exact": 0.0, "exact_reduce": 0.0}
+    for name, n, d in work:
+        row = {"automaton": name}
+        for key, plan in plans.items():
+            best = None
+            for _ in range(2):
+                t0 = time.perf_counter()
+                r = ctx.pipeline(n, 2, d, plan=plan)
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            tot[key] += best
+            row[f"{key}_ms"] = round(best * 1e3, 2)
+            row[f"{key}_length"] = r.upper
+            if key == "exact_reduce":
+                row["reduced_n"], row["reduce_steps"] = r.reduced_n, r.reduce_steps
+                row["stage_ms"] = {a: round(b, 2) for a, b in r.times_ms}
+                g = golden.get(name)
+                if g is not None:
+                    row["reduced_n_equals_reference"] = r.reduced_n == g["reduced_n"]
+            assert r.word is not None and len(r.word) == r.upper and replay_ok(n, d, r.word)
+        assert row["exact_length"] == row["exact_reduce_length"]
+        rows.append(row)
+        print(f"[bench] reduce {row}", file=sys.stderr, flush=True)
+    return {"workload": "configs/exact_reduce.json against configs/exact.json, random binary DFAs n=300 "
+                        "(seeds 1000..1004) and n=200 (seeds 0..9)",
+            "exact_s_total": round(tot["exact"], 4), "exact_reduce_s_total": round(tot["exact_reduce"], 4),
+            "rows": rows}
+
+
 def replay_ok(n: int, d, word) -> bool:
     import synchrowords_amd as sw
     return sw.replay(n, 2, d, word) == 1
@@ -525,11 +569,12 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", choices=["subset", "batch", "learned", "exact", "brute", "eppstein", "times",
-                                           "frontier"],
+                                           "frontier", "reduce"],
                     default="subset",
                     help="subset: the headline line; batch / learned: the batched and learned-bound reports; "
                          "exact: one verbose exact search (--exact-n, --exact-seed); brute: power-set BFS; "
-                         "eppstein: Eppstein's bound; frontier: one search step at 2^23 frontier sets")
+                         "eppstein: Eppstein's bound; frontier: one search step at 2^23 frontier sets; "
+                         "reduce: the configs/exact_reduce.json plan against configs/exact.json")
     ap.add_argument("--exact-n", type=int, default=300)
     ap.add_argument("--exact-seed", type=int, default=1)
     ap.add_argument("--batch-count", type=int, default=4096)
@@ -587,6 +632,8 @@ def main() -> int:
             line = bench_times(ctx)
         elif args.workload == "frontier":
             line = bench_frontier(ctx, args.steps, args.warmup)
+        elif args.workload == "reduce":
+            line = bench_reduce(ctx)
         elif args.workload == "exact":
             d = sw.random_dfa(args.exact_n, 2, args.exact_seed)
             ub = ctx.pipeline(args.exact_n, 2, d).beam_bound  # warm; the plan's Beam bound caps the search
