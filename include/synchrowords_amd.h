@@ -261,6 +261,20 @@ int sw_exact_solve(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, u
     const uint32_t* upper_word, uint64_t upper_word_len, const sw_exact_config* cfg, sw_exact_result* out,
     uint32_t* word, uint64_t word_cap);
 
+/* The same search from a list of sets instead of {Q}: the length of a
+ * shortest word mapping some set of the list to a singleton, within
+ * [lower, upper] (upper: a known bound for that question).  `list` holds
+ * `list_count` host sets in the Subset<N> layout (ceil(n/64) words each,
+ * row-major), each of at least two states; repeated sets count once.  This
+ * is the Exact plugin's run after a Reduce (exact.cpp:131-140 of the
+ * reference: initialize_bfs_lists from data.result.reduce->list_bfs); the
+ * caller adds the Reduce's bfs_steps back (exact.cpp:83-87).  The word, when
+ * `word` is given and word_cap suffices, starts from a list set; its index
+ * is written to *list_index (may be NULL, -1 without a word). */
+int sw_exact_solve_from_list(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, const uint64_t* list,
+    uint64_t list_count, uint64_t lower, uint64_t upper, const sw_exact_config* cfg, sw_exact_result* out,
+    uint32_t* word, uint64_t word_cap, int64_t* list_index);
+
 /* Shortest reset length by breadth-first search over the subsets of Q,
  * the reference's Brute<N,K>::run (synchrolib/algorithm/brute/brute.hpp:28-122;
  * its BFS, :88-122): *length = the length, or -1 when the automaton is not

@@ -121,6 +121,9 @@ _SIGS = {
     "sw_trie_destroy": (None, [_P]),
     "sw_exact_solve": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint64, C.c_uint64, _P, C.c_uint64,
                                  C.POINTER(_ExactConfig), C.POINTER(_ExactResult), _P, C.c_uint64]),
+    "sw_exact_solve_from_list": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint64, C.c_uint64, C.c_uint64,
+                                           C.POINTER(_ExactConfig), C.POINTER(_ExactResult), _P, C.c_uint64,
+                                           C.POINTER(C.c_int64)]),
     "sw_pipeline_solve": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_char_p, C.POINTER(_Result), _P,
                                     C.c_uint64]),
     "sw_batch_solve": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, C.POINTER(_BatchConfig), _P, _P, _P, _P,
@@ -620,6 +623,25 @@ class Context:
         w = [int(x) for x in word[: r.word_len]] if r.word_len else None
         stats = {f: getattr(r, f) for f, _ in _ExactResult._fields_}
         return ExactResult(bool(r.found), int(r.length), w, bool(r.bound_word), stats)
+
+    def exact_from_list(self, n: int, k: int, delta, sets, lower: int = 0, upper: int | None = None,
+                        cfg: ExactConfig | None = None) -> tuple[ExactResult, int]:
+        """The exact search from a list of sets (each of at least two states)
+        instead of {Q}: the shortest length mapping some set to a singleton,
+        and the list index its word starts from (-1 without a word)."""
+        d = _delta(n, k, delta)
+        s = _sets(sets, n)
+        if upper is None:
+            upper = (n - 1) * (n - 1) + 1
+        word = np.zeros(min(max(int(upper), 1), 1 << 24), dtype=np.uint32)
+        r = _ExactResult()
+        c = (cfg or ExactConfig())._c()
+        li = C.c_int64()
+        _check(load().sw_exact_solve_from_list(self.handle, n, k, _ptr(d), _ptr(s), s.shape[0], lower, upper,
+                                               C.byref(c), C.byref(r), _ptr(word), word.size, C.byref(li)))
+        w = [int(x) for x in word[: r.word_len]] if r.word_len else None
+        stats = {f: getattr(r, f) for f, _ in _ExactResult._fields_}
+        return ExactResult(bool(r.found), int(r.length), w, bool(r.bound_word), stats), int(li.value)
 
     def brute(self, n: int, k: int, delta) -> int:
         """Shortest reset length by power-set BFS (n <= 32), -1 if not synchronizing."""

@@ -10,6 +10,7 @@
 #include "sw_gen.hpp"
 #include "sw_host.hpp"
 #include "sw_kernels.hpp"
+#include "sw_reduce.hpp"
 
 namespace sw {
 
@@ -465,6 +466,47 @@ int sw_exact_solve(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, u
     if (!upper_word || upper_word_len != upper) res.word.clear();
   }
   out->word_len = res.word.size();
+  if (word && res.word.size() <= word_cap) std::copy(res.word.begin(), res.word.end(), word);
+  SW_API_END
+}
+
+int sw_exact_solve_from_list(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, const uint64_t* list,
+    uint64_t list_count, uint64_t lower, uint64_t upper, const sw_exact_config* cfg, sw_exact_result* out,
+    uint32_t* word, uint64_t word_cap, int64_t* list_index) {
+  SW_API_BEGIN
+  SW_REQUIRE(ctx && delta && list && out, "NULL argument");
+  SW_REQUIRE(k >= 1 && k <= 255, "k must be in [1, 255]");
+  check_n(n);
+  for (size_t i = 0; i < size_t(n) * k; ++i) SW_REQUIRE(delta[i] < n, "transition out of range");
+  SW_REQUIRE(lower <= upper, "lower above upper");
+  SW_REQUIRE(list_count >= 1, "empty list");
+  const uint32_t W = (n + 63) / 64;
+  for (uint64_t i = 0; i < list_count; ++i) {
+    uint32_t c = 0;
+    for (uint32_t w = 0; w < W; ++w) {
+      const uint64_t x = list[i * W + w];
+      const uint32_t lo = w * 64;
+      const uint64_t valid = n >= lo + 64 ? ~0ull : ((1ull << (n - lo)) - 1);  // n > lo for w < W
+      SW_REQUIRE(!(x & ~valid), "list set with a state >= n");
+      c += uint32_t(__builtin_popcountll(x));
+    }
+    SW_REQUIRE(c >= 2, "list sets must hold at least two states");
+  }
+  sw_exact_config cc{};
+  if (cfg) cc = *cfg;
+  if (cc.dfs == 0 && !cfg) cc.dfs = 1;
+  sw::DevBuf rows;
+  uint64_t distinct = 0;
+  std::vector<uint32_t> src;
+  sw::distinct_rows(ctx, W, list, list_count, rows, distinct, src);
+  sw::ExactSeed seed;
+  seed.rows = rows.as<uint64_t>();
+  seed.count = distinct;
+  auto res = sw::exact_search(ctx, n, k, delta, lower, upper, cc, true, &seed);
+  *out = res.r;
+  if (res.r.bound_word) res.word.clear();  // the bound's word, if any, is the caller's
+  out->word_len = res.word.size();
+  if (list_index) *list_index = (!res.word.empty() && res.root >= 0) ? int64_t(src[size_t(res.root)]) : -1;
   if (word && res.word.size() <= word_cap) std::copy(res.word.begin(), res.word.end(), word);
   SW_API_END
 }

@@ -238,4 +238,46 @@ void reduce_stage(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, uint64
   rs.list_count = m;
 }
 
+void distinct_rows(sw_ctx* ctx, uint32_t W, const uint64_t* host_rows, uint64_t count, DevBuf& out,
+    uint64_t& out_count, std::vector<uint32_t>& src) {
+  use(ctx);
+  hipStream_t st = ctx->stream;
+  ReduceBufs& b = ctx->reduce;
+  const uint64_t rb = uint64_t(W) * 8;
+  SW_REQUIRE(count && count < (1ull << 31), "list of 1 to 2^31 - 1 sets");
+  b.cand.ensure(count * rb);
+  SW_CHECK_HIP(hipMemcpyAsync(b.cand.p, host_rows, count * rb, hipMemcpyHostToDevice, st));
+  uint64_t slots = 1024;
+  while (slots < 2 * count) slots <<= 1;
+  b.table.ensure(slots * 4);
+  b.slot.ensure(count * 4 + 4);
+  b.keep.ensure(count * 4 + 4);
+  b.pos.ensure(count * 4 + 4);
+  b.idx.ensure(count * 4 + 4);
+  b.small.ensure(128);
+  size_t tb = 0;
+  SW_CHECK_HIP(keep_scan_temp_bytes(count, &tb));
+  if (tb > b.scan_bytes) {
+    b.scan.release();
+    b.scan.ensure(tb);
+    b.scan_bytes = tb;
+  }
+  out.ensure(count * rb + 8);
+  uint32_t* total = b.small.as<uint32_t>();
+  SW_CHECK_HIP(launch_fill_u32(b.table.as<uint32_t>(), kEmpty, slots, st));
+  SW_CHECK_HIP(launch_hash_insert(int(W), b.cand.as<uint64_t>(), nullptr, count, b.table.as<uint32_t>(), slots - 1,
+      b.cand.as<uint64_t>(), b.slot.as<uint32_t>(), nullptr, st));
+  SW_CHECK_HIP(launch_keep_scan(1, b.slot.as<uint32_t>(), b.table.as<uint32_t>(), nullptr, count,
+      b.keep.as<uint32_t>(), b.pos.as<uint32_t>(), total, b.scan.p, &b.scan_bytes, st));
+  SW_CHECK_HIP(launch_select(int(W), b.cand.as<uint64_t>(), 0, b.keep.as<uint32_t>(), b.pos.as<uint32_t>(), count,
+      out.as<uint64_t>(), b.idx.as<uint32_t>(), st));
+  uint64_t* hs = b.host();
+  SW_CHECK_HIP(hipMemcpyAsync(hs, total, 4, hipMemcpyDeviceToHost, st));
+  SW_CHECK_HIP(hipStreamSynchronize(st));
+  out_count = uint32_t(hs[0]);
+  src.resize(out_count);
+  SW_CHECK_HIP(hipMemcpyAsync(src.data(), b.idx.p, out_count * 4, hipMemcpyDeviceToHost, st));
+  SW_CHECK_HIP(hipStreamSynchronize(st));
+}
+
 }  // namespace sw

@@ -32,4 +32,10 @@ struct ReduceState {
 void reduce_stage(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, uint64_t upper, uint64_t threshold,
     ReduceState& rs);
 
+// The distinct sets of a host list on the device (the first of equal sets
+// kept, in list order): out_count rows in `out`, src[j] = the list index of
+// row j.
+void distinct_rows(sw_ctx* ctx, uint32_t W, const uint64_t* host_rows, uint64_t count, DevBuf& out,
+    uint64_t& out_count, std::vector<uint32_t>& src);
+
 }  // namespace sw

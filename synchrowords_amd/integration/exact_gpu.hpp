@@ -13,7 +13,12 @@
 // (exact/config.hpp:24-46; the macros the JIT defines from the plan).
 // When the budget ends the search ("mlsw >= x"), a word the deepening found
 // on the way still lowers the upper bound.  Unlike the reference's Exact, it
-// also stores the shortest word it found in result.word.
+// also stores the shortest word it found in result.word.  After a Reduce
+// (data.result.reduce done: a smaller automaton and the Reduce's list), the
+// search starts from that list (sw_exact_solve_from_list, as the
+// reference's initialize_bfs_lists, exact.cpp:131-140) and the Reduce's
+// steps are added back to the bounds (exact.cpp:83-87); its word starts at a
+// list set, not at Q, so none is stored then.
 #pragma once
 #include <synchrolib/algorithm/algorithm.hpp>
 
@@ -48,6 +53,26 @@ class ExactGpu : public synchrolib::Algorithm<N, K> {
     cfg.dfs_min_list_size = ExactGpu::kDfsMinListSize;
     cfg.strict_memory_limit = ExactGpu::kStrict;
     sw_exact_result out{};
+    if (r.reduce && r.reduce->done) {
+      constexpr uint64_t W = (N + 63) / 64;
+      std::vector<uint64_t> list;
+      list.reserve(r.reduce->list_bfs.size() * W);
+      for (const auto& v : r.reduce->list_bfs) {
+        uint64_t row[W] = {};
+        for (size_t q = 0; q < v.size() && q < N; ++q)
+          if (v[q]) row[q / 64] |= uint64_t(1) << (q % 64);
+        list.insert(list.end(), row, row + W);
+      }
+      check(sw_exact_solve_from_list(context(), N, K, delta.data(), list.data(), r.reduce->list_bfs.size(),
+          r.mlsw_lower_bound, r.mlsw_upper_bound, &cfg, &out, nullptr, 0, nullptr));
+      const uint64_t steps = r.reduce->bfs_steps;
+      r.mlsw_lower_bound += steps;
+      r.mlsw_upper_bound += steps;
+      const uint64_t length = out.length + steps;
+      r.mlsw_lower_bound = std::max<uint64_t>(r.mlsw_lower_bound, length);
+      if (out.found) r.mlsw_upper_bound = length;
+      return;
+    }
     check(sw_exact_solve(context(), N, K, delta.data(), r.mlsw_lower_bound, r.mlsw_upper_bound,
         known.empty() ? nullptr : known.data(), known.size(), &cfg, &out, word.data(), word.size()));
     r.mlsw_lower_bound = std::max<uint64_t>(r.mlsw_lower_bound, out.length);

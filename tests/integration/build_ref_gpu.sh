@@ -18,10 +18,17 @@
 # and on the GPU its search gives its own lengths (tests/test_gpu_integration.py).
 # Test infrastructure; no reference file is copied or modified.
 #
-# usage: tests/integration/build_ref_gpu.sh N K
+# With PLAN "reduce" only the plugin is built, as plugin_reduce_<N>_<K>, with
+# the settings of configs/exact_reduce.json (Reduce before Exact; the test
+# runs it in two stages through oracle/ref_driver.cpp --reduce-out and
+# --reduce-in, one binary for each N, as the reference's app re-enters its
+# JIT for the reduced automaton).
+#
+# usage: tests/integration/build_ref_gpu.sh N K [PLAN]
 set -euo pipefail
 N=${1:?N}
 K=${2:?K}
+PLAN=${3:-exact}
 REF=${SYNCHRO_REFERENCE:-/root/reference}
 HERE="$(cd "$(dirname "$0")" && pwd)"
 ROOT="$(cd "$HERE/../.." && pwd)"
@@ -54,8 +61,13 @@ COMMON=(
   '-DBEAM_EXACT_RATIO=(0.01)' '-DMAX_ITER=(-1)' -DPRESORT_INDEG
   '-DDFS_SHORTCUT=(true)' '-DDFS=(true)' '-DSTRICT_MEMORY_LIMIT=(false)'
   '-DMAX_MEMORY=(12 * 1024)' '-DDFS_MIN_LIST_SIZE=(8192)'
-  '-DBFS_SMALL_LIST_SIZE=(AUT_N * AUT_K)'
 )
+if [ "$PLAN" = reduce ]; then
+  COMMON+=(-DCOMPILE_REDUCE '-DMIN_N=(80)' '-DLIST_SIZE_THRESHOLD=(AUT_N * AUT_K * 8)'
+    '-DBFS_SMALL_LIST_SIZE=(AUT_N * AUT_K * AUT_K * 8)')
+else
+  COMMON+=('-DBFS_SMALL_LIST_SIZE=(AUT_N * AUT_K)')
+fi
 FLAGS=(-std=c++17 -O2 -DNDEBUG -pthread -w -I "$REF" -I "$REF/jit" -I "$ROOT/include"
   -I "$ROOT/synchrowords_amd/integration")
 LINK=(-pthread -L "$ROOT/synchrowords_amd" -lsynchrowords_amd -Wl,-rpath,'$ORIGIN/../../../synchrowords_amd')
@@ -86,5 +98,9 @@ build() {  # name, GPU value, extra sources...
   mv "$OUT/$name.tmp" "$OUT/$name"
 }
 
-build "seams_${N}_${K}" true "$REF/jit/jitmain.cpp" "$ROOT/synchrowords_amd/integration/sw_seams.cpp"
-build "plugin_${N}_${K}" false "$ROOT/synchrowords_amd/integration/run_exact_gpu.cpp"
+if [ "$PLAN" = reduce ]; then
+  build "plugin_reduce_${N}_${K}" false "$ROOT/synchrowords_amd/integration/run_exact_gpu.cpp"
+else
+  build "seams_${N}_${K}" true "$REF/jit/jitmain.cpp" "$ROOT/synchrowords_amd/integration/sw_seams.cpp"
+  build "plugin_${N}_${K}" false "$ROOT/synchrowords_amd/integration/run_exact_gpu.cpp"
+fi

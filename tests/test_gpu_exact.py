@@ -338,3 +338,44 @@ def test_reduce_settles_the_length(ctx, case):
     _check_word(case, r.upper, r.word)
     _, decided, length, _ = orc.reduce(n, k, case["delta"], r.eppstein_bound, n * 100000)
     assert decided and length == case["upper"]
+
+
+
+def _images(n, k, d, sets):
+    """Distinct images of `sets` under every letter (oracle apply)."""
+    import numpy as np
+    import oracle_lib as orc
+    out = orc.apply(n, k, d, sets)
+    return np.unique(out, axis=0)
+
+
+@pytest.mark.parametrize("case", _cases("c1_random48.json")[:6] + _cases("c3_random200.json")[:3])
+@pytest.mark.parametrize("depth", [3, 7])
+def test_exact_from_a_list(ctx, case, depth):
+    """The search from a list of sets (sw_exact_solve_from_list) on the
+    images of Q under all words of one length d: any reset word passes
+    through one of them, so the shortest word from the list has the
+    reference's length minus d; the returned word takes the set the search
+    names to a singleton."""
+    import numpy as np
+    import oracle_lib as orc
+    n, k, d = case["n"], case["k"], case["delta"]
+    if depth >= case["upper"]:
+        pytest.skip("the length is below the depth")
+    W = (n + 63) // 64
+    full = np.zeros((1, W), dtype=np.uint64)
+    for q in range(n):
+        full[0, q // 64] |= np.uint64(1) << np.uint64(q % 64)
+    lst = full
+    for _ in range(depth):
+        lst = _images(n, k, d, lst)
+    # a repeat of the first set: counted once
+    lst = np.concatenate([lst, lst[:1]])
+    r, idx = ctx.exact_from_list(n, k, d, lst, upper=case["upper"] - depth + 1)
+    assert r.found and r.length == case["upper"] - depth
+    if r.word is not None:
+        assert len(r.word) == r.length and 0 <= idx < len(lst)
+        cur = lst[idx:idx + 1]
+        for a in r.word:
+            cur = orc.apply(n, k, d, cur)[a:a + 1]
+        assert int(sum(bin(int(x)).count("1") for x in cur[0])) == 1

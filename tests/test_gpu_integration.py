@@ -69,3 +69,35 @@ def test_reference_search_through_the_seams(fixture, n, count):
 def test_reference_plan_with_the_exact_plugin(fixture, n, count):
     cases = [c for c in golden(fixture) if c["n"] == n][:count]
     _check(_run(BIN / f"plugin_{n}_2", cases, timeout=240), cases)
+
+
+@pytest.mark.parametrize("name,n,reduced_n", [("random200_s0", 200, 166), ("random300_s1000", 300, 236)])
+def test_reference_reduce_plan_with_the_exact_plugin(name, n, reduced_n):
+    """configs/exact_reduce.json through the reference's own plan loop and
+    Reduce, in two stages as its app runs them (app/main.cpp:57-66): the
+    binary for N stops after the reference's Reduce with the reduced
+    automaton and its list (--reduce-out); the binary for the reduced N
+    continues (--reduce-in) with ExactGpu, which searches from that list
+    (sw_exact_solve_from_list) and adds the Reduce's steps back.  Both the
+    reduced N and the final bounds are the reference's
+    (tests/golden/reduce_plan.json)."""
+    case = [c for c in golden("reduce_plan.json") if c["name"] == name][0]
+    assert case["reduced_n"] == reduced_n
+    first, second = BIN / f"plugin_reduce_{n}_2", BIN / f"plugin_reduce_{reduced_n}_2"
+    for b in (first, second):
+        if not b.exists():
+            pytest.skip(f"{b.name} not built (tests/integration/build_ref_gpu.sh N K reduce)")
+    plan = "Brute,Eppstein,Beam,Reduce,Exact"
+    with tempfile.TemporaryDirectory() as d:
+        inp, red = Path(d) / "in.txt", Path(d) / "reduce.txt"
+        inp.write_text(f"{case['k']} {n}\n" + " ".join(str(x) for x in case["delta"]) + "\n")
+        out = subprocess.run([str(first), str(inp), "--plan", plan, "--reduce-out", str(red)], capture_output=True,
+                             text=True, timeout=240)
+        assert out.returncode == 0, out.stderr[-2000:]
+        assert f"reduce_pending=1 reduced_n={reduced_n}" in out.stdout, out.stdout[-1000:]
+        out = subprocess.run([str(second), "/dev/null", "--plan", plan, "--reduce-in", str(red)],
+                             capture_output=True, text=True, timeout=240)
+        assert out.returncode == 0, out.stderr[-2000:]
+    line = [l for l in out.stdout.splitlines() if "| non_synchro=" in l][-1]
+    fields = dict(kv.split("=") for kv in line.split("|", 1)[1].split())
+    assert (int(fields["lo"]), int(fields["hi"])) == (case["lower"], case["upper"]), line
