@@ -19,6 +19,7 @@
 #include "sw_mlp.hpp"
 
 
+#include <algorithm>
 #include <utility>
 
 namespace sw {
@@ -613,6 +614,7 @@ __device__ __forceinline__ uint64_t select_chunk(const uint64_t* __restrict__ se
 }
 
 constexpr uint32_t kSelectThreads = 1024;
+constexpr uint64_t kSelectOneWorkgroup = 1u << 17;  // past this many items, the device-wide passes
 
 template <int W, bool SCORES>
 __global__ __launch_bounds__(kSelectThreads) void k_select_first(const uint64_t* __restrict__ sets,
@@ -660,12 +662,131 @@ __global__ __launch_bounds__(kSelectThreads) void k_select_first(const uint64_t*
   for (uint32_t i = tid; i < count; i += kSelectThreads) keep_flag[i] = state[i] != 2 ? 1u : 0u;
 }
 
+// The same selection over the whole device, for lists too long for one
+// workgroup (a wide Beam at large n: the plan's second run reaches widths of
+// 10^5 and more at n = 512, k = 2).  Each pass is three launches: a
+// histogram of the undecided items' digit (LDS histograms added into
+// ctl.hist), the decision (one workgroup: the boundary digit, the need left,
+// done when the whole bucket is needed), and the marking of the items
+// decided by it.  A launch finds the pass done or not yet decided in ctl and
+// does nothing then, so the host enqueues a chunk's eight passes at once and
+// reads `done` back once per chunk.
+struct SelCtl {
+  uint32_t need, digit, done, pass;
+  uint32_t hist[256];
+};
+
+template <int W, bool SCORES>
+__global__ __launch_bounds__(256) void k_sel_hist(const uint64_t* __restrict__ sets,
+    const float* __restrict__ scores, uint32_t count, uint32_t n, uint32_t c, uint32_t shift,
+    const uint8_t* __restrict__ state, SelCtl* ctl) {
+  __shared__ uint32_t h[256];
+  if (ctl->done) return;  // uniform: written by an earlier launch
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256)
+    if (state[i] == 0) atomicAdd(&h[(select_chunk<W, SCORES>(sets, scores, i, n, c) >> shift) & 255u], 1u);
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&ctl->hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_sel_decide(SelCtl* ctl, uint32_t pass) {
+  __shared__ uint32_t s_done;
+  if (threadIdx.x == 0) {
+    s_done = ctl->done;
+    if (!s_done) {
+      uint32_t below = 0, d = 0;
+      for (; d < 255; ++d) {
+        if (below + ctl->hist[d] >= ctl->need) break;
+        below += ctl->hist[d];
+      }
+      ctl->digit = d;
+      ctl->need -= below;
+      ctl->done = ctl->hist[d] == ctl->need ? 1u : 0u;
+      ctl->pass = pass;
+    }
+  }
+  __syncthreads();
+  if (!s_done) ctl->hist[threadIdx.x] = 0;
+}
+
+template <int W, bool SCORES>
+__global__ __launch_bounds__(256) void k_sel_mark(const uint64_t* __restrict__ sets,
+    const float* __restrict__ scores, uint32_t count, uint32_t n, uint32_t c, uint32_t shift,
+    uint8_t* __restrict__ state, const SelCtl* ctl, uint32_t pass) {
+  if (ctl->pass != pass) return;  // this pass made no decision
+  const uint32_t ds = ctl->digit;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) {
+    if (state[i] != 0) continue;
+    const uint32_t dg = uint32_t(select_chunk<W, SCORES>(sets, scores, i, n, c) >> shift) & 255u;
+    if (dg != ds) state[i] = dg < ds ? 1 : 2;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sel_init(uint8_t* __restrict__ state, uint32_t count, SelCtl* ctl,
+    uint32_t keep) {
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) state[i] = 0;
+  if (blockIdx.x == 0) {
+    ctl->hist[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+      ctl->need = keep;
+      ctl->done = keep >= count ? 1u : 0u;
+      ctl->digit = 0;
+      ctl->pass = ~0u;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sel_flags(const uint8_t* __restrict__ state, uint32_t count,
+    uint32_t* __restrict__ keep_flag) {
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256)
+    keep_flag[i] = state[i] != 2 ? 1u : 0u;
+}
+
+template <int W, bool SCORES>
+static hipError_t select_wide(const uint64_t* sets, const float* scores, uint32_t count, uint32_t n, uint32_t keep,
+    uint8_t* state, uint32_t* keep_flag, void* ctl_mem, uint32_t* host_done, hipStream_t st) {
+  SelCtl* ctl = static_cast<SelCtl*>(ctl_mem);
+  const unsigned g = unsigned(std::min<uint64_t>(blocks(count), 2048));
+  hipLaunchKernelGGL(k_sel_init, dim3(g), dim3(256), 0, st, state, count, ctl, keep);
+  const uint32_t chunks = SCORES ? 1u : uint32_t(W) + 1u;
+  uint32_t pass = 0;
+  for (uint32_t c = 0; c < chunks; ++c) {
+    for (int dp = 7; dp >= 0; --dp, ++pass) {
+      const uint32_t shift = uint32_t(dp) * 8u;
+      hipLaunchKernelGGL((k_sel_hist<W, SCORES>), dim3(g), dim3(256), 0, st, sets, scores, count, n, c, shift, state,
+          ctl);
+      hipLaunchKernelGGL(k_sel_decide, dim3(1), dim3(256), 0, st, ctl, pass);
+      hipLaunchKernelGGL((k_sel_mark<W, SCORES>), dim3(g), dim3(256), 0, st, sets, scores, count, n, c, shift, state,
+          ctl, pass);
+    }
+    hipError_t e = hipMemcpyAsync(host_done, &ctl->done, 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    if (*host_done) break;
+  }
+  hipLaunchKernelGGL(k_sel_flags, dim3(g), dim3(256), 0, st, state, count, keep_flag);
+  return hipGetLastError();
+}
+
+size_t beam_select_ctl_bytes() { return sizeof(SelCtl); }
+
 hipError_t beam_select_first(const uint64_t* sets, const float* scores, uint64_t count, uint32_t W, uint32_t n,
-    uint64_t keep, uint8_t* state, uint32_t* keep_flag, hipStream_t st) {
+    uint64_t keep, uint8_t* state, uint32_t* keep_flag, void* ctl, uint32_t* host_done, hipStream_t st) {
   if (!count) return hipSuccess;
   if (count > 0xffffffffull || W < 1 || W > 8) return hipErrorInvalidValue;
   const dim3 g(1), b(kSelectThreads);
   const uint32_t c32 = uint32_t(count), k32 = uint32_t(keep < count ? keep : count);
+  if (count > kSelectOneWorkgroup) {
+    if (!ctl || !host_done) return hipErrorInvalidValue;
+    if (scores) return select_wide<1, true>(sets, scores, c32, n, k32, state, keep_flag, ctl, host_done, st);
+    switch (W) {
+#define CASE(w) case w: return select_wide<w, false>(sets, scores, c32, n, k32, state, keep_flag, ctl, host_done, st);
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    }
+    return hipErrorInvalidValue;
+  }
   if (scores) {
     hipLaunchKernelGGL((k_select_first<1, true>), g, b, 0, st, sets, scores, c32, n, k32, state, keep_flag);
     return hipGetLastError();

@@ -159,7 +159,7 @@ void build_guide(uint32_t n, uint32_t k, const uint32_t* delta, Guide& out, hipS
 // Device buffers of the beam, kept by the context between calls.
 struct BeamWs {
   DevBuf dtf, dtb, list, cand, table, slot, keep, pos, uniq, uidx, x, scratch, keys, vals, vals2, small, scan_tmp,
-      hist, mf, mb;
+      hist, mf, mb, sel;
   DevBuf sb_in, sb_meta, sb_word;  // one-workgroup beam: seed and transitions, depth table, word
   std::vector<uint64_t> host_mf, host_mb;  // per-state masks of the tables, until the next call
   size_t scan_bytes = 0;
@@ -777,8 +777,9 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
         vals2.ensure(uint64_t(nu) * 4 + 4);
         keys.ensure(uint64_t(nu) + 8);
         pos.ensure(uint64_t(nu) * 4 + 4);
+        ws.sel.ensure(sw::beam_select_ctl_bytes());
         SW_CHECK_HIP(sw::beam_select_first(uniq.as<uint64_t>(), scores, nu, W, n, keepn, keys.as<uint8_t>(),
-            vals.as<uint32_t>(), st));
+            vals.as<uint32_t>(), ws.sel.p, reinterpret_cast<uint32_t*>(ws.host + 7), st));
         size_t need = 0;
         SW_CHECK_HIP(sw::keep_scan_temp_bytes(nu, &need));
         if (need > scan_bytes) {

@@ -64,9 +64,13 @@ hipError_t launch_neg_popcount(const uint64_t* sets, uint64_t count, uint32_t W,
 // workgroup, no sort): in the reference's Beam order (scores NULL: larger
 // first, then word 0, word 1, ... ascending, beam.hpp's comparator) or in
 // ascending stable order of float scores.  keep_flag[i] = 1 for the kept;
-// state: count bytes of scratch.
+// state: count bytes of scratch.  Past 2^17 items the passes run over the
+// whole device: ctl (beam_select_ctl_bytes() bytes of device memory) and
+// host_done (one pinned word) are then required, and the call synchronizes
+// the stream once per key chunk.
+size_t beam_select_ctl_bytes();
 hipError_t beam_select_first(const uint64_t* sets, const float* scores, uint64_t count, uint32_t W, uint32_t n,
-    uint64_t keep, uint8_t* state, uint32_t* keep_flag, hipStream_t st);
+    uint64_t keep, uint8_t* state, uint32_t* keep_flag, void* ctl, uint32_t* host_done, hipStream_t st);
 // The first `keep` sets of a depth, by place, written to out_rows/out_idx
 // (sets[i], idx[i] of the set at place r < keep), for count <= the counted
 // maximum (beam_rank_fits): in the reference's Beam order, or in ascending

@@ -114,10 +114,13 @@ def test_dominance_pruning_keeps_the_length(ctx, n, k):
             assert replay(n, k, d, r.word) == 1
 
 
-@pytest.mark.parametrize("n,k,width", [(100, 2, 9000), (300, 2, 12000), (100, 3, 6000)])
+@pytest.mark.parametrize("n,k,width", [(100, 2, 9000), (300, 2, 12000), (100, 3, 6000), (300, 2, 100000),
+                                       (200, 3, 70000)])
 def test_device_beam_wide_crosses_the_counted_order(ctx, n, k, width):
-    """Widths whose depths hold more than 16384 distinct candidates take the
-    sorted order there and the counted order below: the oracle's bound."""
+    """Widths whose depths hold more than 16384 distinct candidates select
+    the kept sets by radix selection there (one workgroup up to 2^17
+    candidates, over the whole device past that) and count the order below:
+    the oracle's bound."""
     import oracle_lib as orc
     d = gen.random_dfa(n, k, 9300 + n)
     cap = (n - 1) ** 2 + 1
@@ -379,3 +382,27 @@ def test_exact_from_a_list(ctx, case, depth):
         for a in r.word:
             cur = orc.apply(n, k, d, cur)[a:a + 1]
         assert int(sum(bin(int(x)).count("1") for x in cur[0])) == 1
+
+
+
+@pytest.mark.parametrize("n,k", [(80, 4), (130, 3), (260, 2), (330, 2), (400, 2), (512, 2), (512, 3)])
+def test_reduce_stage_across_word_counts(ctx, n, k):
+    """The Reduce stage alone (a plan of Eppstein, Beam and Reduce, no
+    Exact) on random automata of 2 to 8 words per set and 2 to 4 letters:
+    the device's steps and reduced number of states equal the oracle's from
+    the same Beam bound."""
+    import json
+    import oracle_lib as orc
+    d = gen.random_dfa(n, k, 7300 + n + k)
+    plan = {"upper_bound": "1ULL * (AUT_N - 1) * (AUT_N - 1) + 1", "algorithms": [
+        {"name": "Eppstein", "config": {}},
+        {"name": "Beam", "config": {"presort": "indeg", "dynamic": True, "min_beam_size": "AUT_N * std::log2(AUT_N)",
+                                    "max_beam_size": "AUT_N * AUT_N * std::log2(AUT_N)", "beam_exact_ratio": 0.01}},
+        {"name": "Reduce", "config": {"min_n": "80", "list_size_threshold": "AUT_N * AUT_K * 8"}}]}
+    r = ctx.pipeline(n, k, d, plan=json.dumps(plan))
+    steps, decided, length, rn = orc.reduce(n, k, d, r.beam_bound, n * k * 8)
+    if decided:
+        assert (r.lower, r.upper) == (length, length) and r.reduced_n == 0
+    else:
+        assert (r.reduced_n, r.reduce_steps) == (rn, steps if rn else 0)
+        assert r.lower < r.upper == r.beam_bound

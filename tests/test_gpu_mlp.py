@@ -353,7 +353,7 @@ def _learned_beam_restated(m, n, k, d, width, cap):
     return cap
 
 
-@pytest.mark.parametrize("n,width,trained", [(300, 12000, True), (100, 9000, False)])
+@pytest.mark.parametrize("n,width,trained", [(300, 12000, True), (100, 9000, False), (300, 80000, True)])
 def test_learned_beam_past_the_counted_order(ctx, n, width, trained):
     """Depths of more than 16384 distinct candidates select the kept sets on
     the device by radix selection over (score, position): the bound equals
