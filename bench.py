@@ -36,6 +36,7 @@ HBM_PEAK_GBS = 8000.0
 
 
 LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12  # 256 CUs x 128 B/clk x 2.4 GHz
+IN_FLIGHT = 8  # plans in flight in the time-to-exact workload's second measurement (one context each)
 
 
 def lds_roofline(ctx, small, tile_count: int, kernel_ms: float, indexed: bool) -> dict:
@@ -212,6 +213,8 @@ def bench_times(ctx) -> dict:
     clock.  Reference: its own binary on the host CPU at one thread and at
     16 threads (min(64, host threads) on the box), its plan time per
     automaton.  Every length is compared."""
+    import concurrent.futures as cf
+    import synchrowords_amd as sw
     from synchrowords_amd import gen
     cpu = host_cpu()
     work = [("random n=48 k=2, seeds 0..19", 48, [gen.random_dfa(48, 2, s) for s in range(20)]),
@@ -233,6 +236,29 @@ def bench_times(ctx) -> dict:
             assert r.word is not None and len(r.word) == r.upper and replay_ok(n, d, r.word)
         row = {"workload": name, "automata": len(autos), "gpu_s_total": round(sum(gpu), 4),
                "gpu_s_max": round(max(gpu), 4), "gpu_stage_ms": {k: round(v, 2) for k, v in stages.items()}}
+        # The same automata with several plans in flight: one context (own
+        # stream) per host thread, each taking every j-th automaton in turn;
+        # the reference's app runs a file one automaton at a time.
+        if len(autos) > 1:
+            j = min(IN_FLIGHT, len(autos))
+            ctxs = [sw.Context(0) for _ in range(j)]
+            for c in ctxs:
+                c.pipeline(n, 2, autos[0])  # warm
+            got = [None] * len(autos)
+
+            def work(w):
+                for i in range(w, len(autos), j):
+                    r = ctxs[w].pipeline(n, 2, autos[i])
+                    got[i] = (r.lower, r.upper)
+
+            t0 = time.perf_counter()
+            with cf.ThreadPoolExecutor(j) as ex:
+                list(ex.map(work, range(j)))
+            wall = time.perf_counter() - t0
+            for c in ctxs:
+                c.close()
+            assert got == lengths
+            row[f"gpu_s_total_{j}_in_flight"] = round(wall, 4)
         for t in (1, 16):
             ref = run_reference(n, 2, t, autos)
             if ref is None:
@@ -240,6 +266,8 @@ def bench_times(ctx) -> dict:
             same = [tuple(x[:2]) for x in ref["rows"]] == [tuple(x) for x in lengths]
             row[f"reference_{t}t_s_total"] = ref["s_total"]
             row[f"speedup_vs_{t}t"] = round(ref["s_total"] / sum(gpu), 2)
+            if f"gpu_s_total_{IN_FLIGHT}_in_flight" in row:
+                row[f"speedup_in_flight_vs_{t}t"] = round(ref["s_total"] / row[f"gpu_s_total_{IN_FLIGHT}_in_flight"], 2)
             row[f"lengths_equal_{t}t"] = same
         print(f"[bench] times {row}", file=sys.stderr, flush=True)
         out["rows"].append(row)

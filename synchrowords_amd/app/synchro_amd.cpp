@@ -3,7 +3,7 @@
 // files, so that scripts written for the reference's `synchro` run unchanged.
 //
 //   synchro_amd -f automata.txt -c config.json [-o results.txt [--continue]]
-//               [-v | -q | -d] [-b suffix] [-h]
+//               [-v | -q | -d] [-b suffix] [-j jobs] [-h]
 //
 // Input: automata one after another, each "K N" then K*N transitions,
 // delta(q, a) at position q*K + a (app/io.hpp:50-77).  Output: one line per
@@ -12,19 +12,26 @@
 // (app/io.hpp:95-143).  With --continue the non-empty lines already in the
 // output file are counted and that many automata are skipped
 // (app/main.cpp:26-46).  -b is accepted for compatibility (there is no JIT
-// build here).  Exit codes follow the reference: 1 arguments or output file,
+// build here).  -j N (not in the reference) solves N automata at a time, each
+// with its own device context and stream, and still writes the lines in input
+// order.  Exit codes follow the reference: 1 arguments or output file,
 // 2 config, 3 input, 4 a failed run.
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cctype>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
 #include <fstream>
 #include <iostream>
+#include <mutex>
 #include <optional>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "synchrowords_amd.h"
@@ -52,6 +59,7 @@ struct Args {
   std::string input, config, build_suffix;
   std::optional<std::string> output;
   bool verbose = false, quiet = false, debug = false, cont = false;
+  int jobs = 1;
 };
 
 void usage() {
@@ -66,6 +74,7 @@ void usage() {
                "  -v, --verbose           Verbose output\n"
                "  -q, --quiet             Quiet output (only warnings and errors)\n"
                "  -d, --debug             Debug output (all messages and timers)\n"
+               "  -j, --jobs arg          Automata solved at a time, one device context each (default: 1)\n"
                "  -h, --help              Print usage\n";
 }
 
@@ -105,6 +114,12 @@ Args parse(int argc, char** argv) {
       a.quiet = true;
     } else if (is("-d", "--debug")) {
       a.debug = true;
+    } else if (is("-j", "--jobs")) {
+      const std::string v = value("jobs");
+      char* end = nullptr;
+      const long j = std::strtol(v.c_str(), &end, 10);
+      if (!end || *end || j < 1 || j > 64) fail(1, "Option 'jobs' needs an integer in [1, 64]");
+      a.jobs = int(j);
     } else {
       fail(1, "Option '" + s + "' does not exist");
     }
@@ -189,19 +204,67 @@ int main(int argc, char** argv) {
   }
   if (skip) info("Skipping " + std::to_string(skip) + " automata");
 
-  sw_ctx* ctx = nullptr;
-  if (skip < auts.size() && sw_ctx_create(0, &ctx) != SW_OK) fail(4, std::string("device: ") + sw_last_error());
-
-  std::vector<uint32_t> word;
-  for (size_t index = skip; index < auts.size(); ++index) {
-    const Automaton& a = auts[index];
-    const uint64_t cap = uint64_t(a.n) * a.n * a.n / 6 + uint64_t(a.n) * a.n + 8;  // longest bound a plan may give
-    word.assign(size_t(std::min<uint64_t>(cap, 1ull << 26)), 0);
+  // One solved automaton: its result, the word, or the error of a failed run.
+  struct Solved {
+    bool done = false, ok = false;
     sw_result r{};
-    if (sw_pipeline_solve(ctx, a.n, a.k, a.delta.data(), plan.c_str(), &r, word.data(), word.size()) != SW_OK) {
-      sw_ctx_destroy(ctx);
-      fail(4, "automaton " + std::to_string(index) + ": " + sw_last_error());
+    std::vector<uint32_t> word;
+    std::string error;
+  };
+  const size_t total = auts.size();
+  std::vector<Solved> solved(total);
+  std::mutex mu;
+  std::condition_variable cv;
+  std::atomic<size_t> next{skip};
+  std::atomic<bool> stop{false};
+  auto worker = [&]() {
+    sw_ctx* ctx = nullptr;
+    std::string ctx_error;
+    if (sw_ctx_create(0, &ctx) != SW_OK) ctx_error = std::string("device: ") + sw_last_error();
+    std::vector<uint32_t> word;
+    for (size_t index; !stop && (index = next.fetch_add(1)) < total;) {
+      Solved out;
+      if (ctx) {
+        const Automaton& a = auts[index];
+        const uint64_t cap = uint64_t(a.n) * a.n * a.n / 6 + uint64_t(a.n) * a.n + 8;  // longest bound a plan may give
+        word.assign(size_t(std::min<uint64_t>(cap, 1ull << 26)), 0);
+        if (sw_pipeline_solve(ctx, a.n, a.k, a.delta.data(), plan.c_str(), &out.r, word.data(), word.size()) == SW_OK) {
+          out.ok = true;
+          out.word.assign(word.begin(), word.begin() + std::min<uint64_t>(out.r.word_len, word.size()));
+        } else {
+          out.error = "automaton " + std::to_string(index) + ": " + sw_last_error();
+        }
+      } else {
+        out.error = ctx_error;
+      }
+      out.done = true;
+      if (!out.ok) stop = true;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        solved[index] = std::move(out);
+      }
+      cv.notify_all();
     }
+    if (ctx) sw_ctx_destroy(ctx);
+  };
+  std::vector<std::thread> pool;
+  const size_t jobs = std::min<size_t>(size_t(args.jobs), total > skip ? total - skip : 0);
+  for (size_t t = 0; t < jobs; ++t) pool.emplace_back(worker);
+
+  // Lines in input order, each as soon as it and every one before it is done.
+  std::string failure;
+  for (size_t index = skip; index < total; ++index) {
+    Solved s;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait(lk, [&] { return solved[index].done; });
+      s = std::move(solved[index]);
+    }
+    if (!s.ok) {
+      failure = s.error;
+      break;
+    }
+    const sw_result& r = s.r;
     if (r.non_synchro) {
       info("NON SYNCHRO");
     } else {
@@ -227,12 +290,14 @@ int main(int argc, char** argv) {
     if (r.word_len) {
       info("Saving synchronizing word of length " + std::to_string(r.word_len));
       out << " {";
-      for (uint64_t i = 0; i < r.word_len; ++i) out << (i ? " " : "") << word[i];
+      for (uint64_t i = 0; i < r.word_len; ++i) out << (i ? " " : "") << s.word[i];
       out << "}";
     }
     out << "\n";
     out.flush();
   }
-  if (ctx) sw_ctx_destroy(ctx);
+  stop = true;
+  for (auto& th : pool) th.join();
+  if (!failure.empty()) fail(4, failure);
   return 0;
 }

@@ -49,3 +49,12 @@ def test_continue_with_everything_done_needs_no_device(tmp_path):
     out = write(tmp_path, "out.txt", "0: [1, 1] ((Brute, 0))\n")
     r = run("-f", inp, "-c", plan_file(tmp_path, EXACT_PLAN), "-o", out, "--continue")
     assert r.returncode == 0 and out.read_text() == "0: [1, 1] ((Brute, 0))\n"
+
+
+def test_jobs_option_is_checked(tmp_path):
+    inp = write(tmp_path, "in.txt", "2 2\n1 1 0 0\n")
+    cfg = plan_file(tmp_path, EXACT_PLAN)
+    for bad in ("0", "65", "x", "2x"):
+        r = run("-f", inp, "-c", cfg, "-j", bad)
+        assert r.returncode == 1 and "jobs" in r.stderr
+    assert "-j, --jobs" in run("-h").stdout

@@ -64,3 +64,28 @@ def test_continue_appends_the_rest(tmp_path):
     assert [(x[0], x[1]) for x in a] == [(x[0], x[1]) for x in b]
     for (idx, bounds, _, _), c in zip(a, cases):
         assert bounds == (c["lower"], c["upper"])
+
+
+@pytest.mark.parametrize("jobs", [4, 8])
+def test_jobs_keep_the_lines_in_order(tmp_path, jobs):
+    """-j N solves N automata at a time (one context each) and still writes
+    one line per automaton in input order, with the reference's bounds and
+    words that replay; with --continue it resumes after the lines written."""
+    cases = golden("c1_random48.json")[:16] + golden("readme.json")
+    inp = write(tmp_path, "in.txt", automata_text(cases))
+    cfg = plan_file(tmp_path, EXACT_PLAN)
+    out = tmp_path / "out.txt"
+    run("-f", inp, "-c", cfg, "-o", out, "-q", "-j", jobs)
+    lines = parse(out.read_text())
+    assert [x[0] for x in lines] == list(range(len(cases)))
+    for (idx, bounds, algos, word), c in zip(lines, cases):
+        if c["non_synchro"]:
+            assert bounds is None
+            continue
+        assert bounds == (c["lower"], c["upper"])
+        if word is not None:
+            assert len(word) == bounds[1] and replay(c["n"], c["k"], c["delta"], word) == 1
+    part = tmp_path / "part.txt"
+    part.write_text("\n".join(out.read_text().splitlines()[:5]) + "\n")
+    run("-f", inp, "-c", cfg, "-o", part, "-q", "-j", jobs, "--continue")
+    assert [x[:2] for x in parse(part.read_text())] == [x[:2] for x in lines]
