@@ -331,6 +331,17 @@ typedef struct sw_result {
 int sw_pipeline_solve(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, const char* plan_json,
     sw_result* out, uint32_t* word, uint64_t word_cap);
 
+/* The same run, also giving the word the reference's AlgoResult.word holds
+ * after the plan (algorithm/algorithm.hpp:19-42): only Eppstein sets it,
+ * when the plan's Eppstein has find_word true and transition_tables false
+ * and it lowered the upper bound (eppstein.hpp:70-72).  *result_word_len is
+ * 0 when there is none; the word is written to result_word when
+ * result_word_cap suffices.  `word` keeps sw_pipeline_solve's meaning (a
+ * word of length `upper`, the shortest when Exact ran). */
+int sw_pipeline_solve_result_word(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta,
+    const char* plan_json, sw_result* out, uint32_t* word, uint64_t word_cap, uint32_t* result_word,
+    uint64_t result_word_cap, uint64_t* result_word_len);
+
 /* ----------------------------------------------------------- batched search */
 
 typedef struct sw_batch_config {

@@ -126,6 +126,8 @@ _SIGS = {
                                            C.POINTER(C.c_int64)]),
     "sw_pipeline_solve": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_char_p, C.POINTER(_Result), _P,
                                     C.c_uint64]),
+    "sw_pipeline_solve_result_word": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_char_p, C.POINTER(_Result), _P,
+                                                C.c_uint64, _P, C.c_uint64, _U64P]),
     "sw_batch_solve": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, C.POINTER(_BatchConfig), _P, _P, _P, _P,
                                  C.c_uint32, C.POINTER(_BatchStats)]),
     "sw_mlp_create": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(_P)]),

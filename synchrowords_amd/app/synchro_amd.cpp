@@ -3,7 +3,7 @@
 // files, so that scripts written for the reference's `synchro` run unchanged.
 //
 //   synchro_amd -f automata.txt -c config.json [-o results.txt [--continue]]
-//               [-v | -q | -d] [-b suffix] [-j jobs] [-h]
+//               [-v | -q | -d] [-b suffix] [-j jobs] [--reference-word] [-h]
 //
 // Input: automata one after another, each "K N" then K*N transitions,
 // delta(q, a) at position q*K + a (app/io.hpp:50-77).  Output: one line per
@@ -14,7 +14,9 @@
 // (app/main.cpp:26-46).  -b is accepted for compatibility (there is no JIT
 // build here).  -j N (not in the reference) solves N automata at a time, each
 // with its own device context and stream, and still writes the lines in input
-// order.  Exit codes follow the reference: 1 arguments or output file,
+// order.  The word field holds a shortest reset word whenever Exact ran; with
+// --reference-word it holds what the reference writes instead: Eppstein's
+// word when the plan asks for it (find_word), else nothing.  Exit codes follow the reference: 1 arguments or output file,
 // 2 config, 3 input, 4 a failed run.
 #include <algorithm>
 #include <atomic>
@@ -58,7 +60,7 @@ void info(const std::string& m) { log(1, "INFO", m); }
 struct Args {
   std::string input, config, build_suffix;
   std::optional<std::string> output;
-  bool verbose = false, quiet = false, debug = false, cont = false;
+  bool verbose = false, quiet = false, debug = false, cont = false, reference_word = false;
   int jobs = 1;
 };
 
@@ -75,6 +77,8 @@ void usage() {
                "  -q, --quiet             Quiet output (only warnings and errors)\n"
                "  -d, --debug             Debug output (all messages and timers)\n"
                "  -j, --jobs arg          Automata solved at a time, one device context each (default: 1)\n"
+               "      --reference-word    Write the word the reference writes (Eppstein's, with find_word)\n"
+               "                          instead of a shortest one\n"
                "  -h, --help              Print usage\n";
 }
 
@@ -108,6 +112,8 @@ Args parse(int argc, char** argv) {
       a.build_suffix = value("build-suffix");
     } else if (s == "--continue") {
       a.cont = true;
+    } else if (s == "--reference-word") {
+      a.reference_word = true;
     } else if (is("-v", "--verbose")) {
       a.verbose = true;
     } else if (is("-q", "--quiet")) {
@@ -221,16 +227,30 @@ int main(int argc, char** argv) {
     sw_ctx* ctx = nullptr;
     std::string ctx_error;
     if (sw_ctx_create(0, &ctx) != SW_OK) ctx_error = std::string("device: ") + sw_last_error();
-    std::vector<uint32_t> word;
+    std::vector<uint32_t> word, rword;
     for (size_t index; !stop && (index = next.fetch_add(1)) < total;) {
       Solved out;
       if (ctx) {
         const Automaton& a = auts[index];
         const uint64_t cap = uint64_t(a.n) * a.n * a.n / 6 + uint64_t(a.n) * a.n + 8;  // longest bound a plan may give
         word.assign(size_t(std::min<uint64_t>(cap, 1ull << 26)), 0);
-        if (sw_pipeline_solve(ctx, a.n, a.k, a.delta.data(), plan.c_str(), &out.r, word.data(), word.size()) == SW_OK) {
+        int rc;
+        uint64_t rwl = 0;
+        if (args.reference_word) {
+          rword.assign(word.size(), 0);
+          rc = sw_pipeline_solve_result_word(ctx, a.n, a.k, a.delta.data(), plan.c_str(), &out.r, word.data(),
+              word.size(), rword.data(), rword.size(), &rwl);
+        } else {
+          rc = sw_pipeline_solve(ctx, a.n, a.k, a.delta.data(), plan.c_str(), &out.r, word.data(), word.size());
+        }
+        if (rc == SW_OK) {
           out.ok = true;
-          out.word.assign(word.begin(), word.begin() + std::min<uint64_t>(out.r.word_len, word.size()));
+          if (args.reference_word) {
+            out.r.word_len = std::min<uint64_t>(rwl, rword.size());
+            out.word.assign(rword.begin(), rword.begin() + out.r.word_len);
+          } else {
+            out.word.assign(word.begin(), word.begin() + std::min<uint64_t>(out.r.word_len, word.size()));
+          }
         } else {
           out.error = "automaton " + std::to_string(index) + ": " + sw_last_error();
         }

@@ -359,7 +359,7 @@ const char* kDefaultPlan = R"PLAN({
 // [0, upper_bound] and every algorithm may only tighten them; an algorithm
 // whose bounds already meet returns at once (each plugin's early exit).
 void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char* plan_json, sw_result* out,
-    std::vector<uint32_t>& word) {
+    std::vector<uint32_t>& word, std::vector<uint32_t>* result_word) {
   const std::string text = plan_json ? plan_json : kDefaultPlan;
   JsonParser jp{text.data(), text.data() + text.size()};
   Json plan = jp.parse();
@@ -401,6 +401,7 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
       }
     } else if (name == "Eppstein") {
       const bool find_word = eval_bool(cfg->get("find_word"), false, n, k);
+      const bool tables = eval_bool(cfg->get("transition_tables"), false, n, k);
       if (open) {
         // Eppstein's greedy bound on the device (k_eppstein.hip).
         const uint64_t off = 0;
@@ -416,7 +417,10 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
           // The word is kept whatever find_word says: when Exact proves this
           // bound optimal, it is the shortest word.
           word.assign(w.begin(), w.begin() + std::min<uint64_t>(wl, cap));
-          (void)find_word;
+          // The reference's result.word: Eppstein's, when the plan asks for it
+          // (eppstein.hpp:70-72, FIND_WORD and not TRANSITION_TABLES); no
+          // other algorithm sets it.
+          if (result_word && find_word && !tables) *result_word = word;
         }
         out->eppstein_bound = upper;
       }
@@ -531,16 +535,22 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
 
 }  // namespace sw
 
-extern "C" int sw_pipeline_solve(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, const char* plan_json,
-    sw_result* out, uint32_t* word, uint64_t word_cap) {
+namespace {
+
+int pipeline_api(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, const char* plan_json, sw_result* out,
+    uint32_t* word, uint64_t word_cap, uint32_t* result_word, uint64_t result_word_cap, uint64_t* result_word_len) {
   try {
     SW_REQUIRE(ctx && delta && out, "NULL argument");
     SW_REQUIRE(n >= 1 && k >= 1 && k <= 255, "n must be >= 1 and k in [1, 255]");
     if (n > 512) throw sw::Error{SW_ERR_UNSUPPORTED, "n above 512 states is not compiled"};
     for (size_t i = 0; i < size_t(n) * k; ++i) SW_REQUIRE(delta[i] < n, "transition out of range");
-    std::vector<uint32_t> w;
-    sw::pipeline(ctx, n, k, delta, plan_json, out, w);
+    std::vector<uint32_t> w, rw;
+    sw::pipeline(ctx, n, k, delta, plan_json, out, w, result_word_len ? &rw : nullptr);
     if (word && w.size() <= word_cap) std::copy(w.begin(), w.end(), word);
+    if (result_word_len) {
+      *result_word_len = out->non_synchro ? 0 : rw.size();
+      if (result_word && rw.size() <= result_word_cap) std::copy(rw.begin(), rw.end(), result_word);
+    }
     return SW_OK;
   } catch (const sw::Error& e) {
     sw::set_error(e.msg);
@@ -549,4 +559,21 @@ extern "C" int sw_pipeline_solve(sw_ctx* ctx, uint32_t n, uint32_t k, const uint
     sw::set_error(e.what());
     return SW_ERR_INVALID;
   }
+}
+
+}  // namespace
+
+extern "C" int sw_pipeline_solve(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, const char* plan_json,
+    sw_result* out, uint32_t* word, uint64_t word_cap) {
+  return pipeline_api(ctx, n, k, delta, plan_json, out, word, word_cap, nullptr, 0, nullptr);
+}
+
+extern "C" int sw_pipeline_solve_result_word(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta,
+    const char* plan_json, sw_result* out, uint32_t* word, uint64_t word_cap, uint32_t* result_word,
+    uint64_t result_word_cap, uint64_t* result_word_len) {
+  if (!result_word_len) {
+    sw::set_error("NULL argument");
+    return SW_ERR_INVALID;
+  }
+  return pipeline_api(ctx, n, k, delta, plan_json, out, word, word_cap, result_word, result_word_cap, result_word_len);
 }

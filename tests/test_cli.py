@@ -57,4 +57,4 @@ def test_jobs_option_is_checked(tmp_path):
     for bad in ("0", "65", "x", "2x"):
         r = run("-f", inp, "-c", cfg, "-j", bad)
         assert r.returncode == 1 and "jobs" in r.stderr
-    assert "-j, --jobs" in run("-h").stdout
+    assert "-j, --jobs" in run("-h").stdout and "--reference-word" in run("-h").stdout

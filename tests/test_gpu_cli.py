@@ -89,3 +89,27 @@ def test_jobs_keep_the_lines_in_order(tmp_path, jobs):
     part.write_text("\n".join(out.read_text().splitlines()[:5]) + "\n")
     run("-f", inp, "-c", cfg, "-o", part, "-q", "-j", jobs, "--continue")
     assert [x[:2] for x in parse(part.read_text())] == [x[:2] for x in lines]
+
+
+def test_reference_word_reproduces_the_readme_lines(tmp_path):
+    """--reference-word: the word field is what the reference writes, so
+    that with configs/readme_config.json every line of the README example
+    equals docs/install.md:101-105 but for the times: the same bounds and
+    algorithms, no word for the first three automata, and Eppstein's
+    25-letter word for the fourth."""
+    cases = golden("readme.json")
+    inp = write(tmp_path, "in.txt", automata_text(cases))
+    out = tmp_path / "out.txt"
+    run("-f", inp, "-c", plan_file(tmp_path, README_PLAN), "-o", out, "-q", "--reference-word")
+    lines = parse(out.read_text())
+    names = [a["name"] for a in README_PLAN["algorithms"]]
+    assert [x[0] for x in lines] == list(range(len(cases)))
+    for (idx, bounds, algos, word), c in zip(lines, cases):
+        if c["non_synchro"]:
+            assert bounds is None
+            continue
+        assert bounds == tuple(c["docs"]) and algos == names
+        if c.get("docs_eppstein_word"):
+            assert word == c["docs_eppstein_word"]
+        else:
+            assert word is None
