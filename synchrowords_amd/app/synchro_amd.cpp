@@ -233,11 +233,13 @@ int main(int argc, char** argv) {
       if (ctx) {
         const Automaton& a = auts[index];
         const uint64_t cap = uint64_t(a.n) * a.n * a.n / 6 + uint64_t(a.n) * a.n + 8;  // longest bound a plan may give
-        word.assign(size_t(std::min<uint64_t>(cap, 1ull << 26)), 0);
+        // Grown, never cleared: the library writes the words it reports.
+        const size_t want = size_t(std::min<uint64_t>(cap, 1ull << 26));
+        if (word.size() < want) word.resize(want);
         int rc;
         uint64_t rwl = 0;
         if (args.reference_word) {
-          rword.assign(word.size(), 0);
+          if (rword.size() < word.size()) rword.resize(word.size());
           rc = sw_pipeline_solve_result_word(ctx, a.n, a.k, a.delta.data(), plan.c_str(), &out.r, word.data(),
               word.size(), rword.data(), rword.size(), &rwl);
         } else {
