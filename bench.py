@@ -416,6 +416,68 @@ def bench_batch(ctx, count: int, cap_rows: int = 0, max_pairs: int = 0) -> dict:
     return out
 
 
+def bench_overflow(ctx, count: int, in_flight: int) -> dict:
+    """The batch's overflow path on its own: every instance of the batch as a
+    whole-device search under its Eppstein bound, one at a time; the ones
+    with more than 2^16 sets on a side (those that leave the persistent
+    launch) again, one at a time and `in_flight` at a time (one context and
+    host thread each; 0 skips that leg)."""
+    import threading
+    import synchrowords_amd as sw
+    from synchrowords_amd import gen
+    specs = gen.batch_specs(4096, 2025)[:count]
+    inst = [(n, k, sw.random_dfa(n, k, seed)) for n, k, seed in specs]
+    nsy, bounds, _ = ctx.eppstein_batch(inst)
+    todo = [i for i, (n, _, _) in enumerate(inst) if not nsy[i] and n > 1]
+    ctx.exact(*inst[todo[0]], upper=int(bounds[todo[0]]))  # warm
+    t0 = time.perf_counter()
+    stats = {}
+    for i in todo:
+        stats[i] = ctx.exact(*inst[i], upper=int(bounds[i])).stats
+    t_all = time.perf_counter() - t0
+    big = [i for i in todo if max(stats[i]["sets_forward"], stats[i]["sets_backward"]) > (1 << 16)]
+    big.sort(key=lambda i: (-inst[i][1], -inst[i][0]))
+    t0 = time.perf_counter()
+    for i in big:
+        ctx.exact(*inst[i], upper=int(bounds[i]))
+    t_big = time.perf_counter() - t0
+    print(f"[bench] overflow: {len(todo)} searches in {t_all:.2f} s, {len(big)} large in {t_big:.2f} s",
+          file=sys.stderr, flush=True)
+    if in_flight <= 0:
+        ctxs = []
+    else:
+        ctxs = [sw.Context(0) for _ in range(in_flight)]
+    for c in ctxs:
+        c.exact(*inst[big[0]], upper=int(bounds[big[0]]))  # warm each context
+    nxt, lock = [0], threading.Lock()
+
+    def work(c):
+        while True:
+            with lock:
+                j = nxt[0]
+                nxt[0] += 1
+            if j >= len(big):
+                return
+            c.exact(*inst[big[j]], upper=int(bounds[big[j]]))
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=work, args=(c,)) for c in ctxs]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    t_in_flight = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    agg = lambda key: round(sum(stats[i][key] for i in big) / 1e3, 3)
+    return {"workload": f"whole-device searches of the batch of {count} (batch_specs(4096, 2025)), Eppstein bounds",
+            "searches": len(todo), "serial_s_all": round(t_all, 3), "large": len(big),
+            "large_serial_s": round(t_big, 3), "in_flight": in_flight,
+            "large_in_flight_s": round(t_in_flight, 3) if ctxs else None,
+            "large_expand_s": agg("ms_expand"), "large_goal_s": agg("ms_goal"),
+            "large_median_sets_per_side": int(np.median([max(stats[i]["sets_forward"], stats[i]["sets_backward"])
+                                                          for i in big]))}
+
+
 def bench_brute(ctx, count: int, no_cpu: bool) -> dict:
     """Power-set BFS: a batch of small automata in one launch (one workgroup
     each, visited set in LDS) and single automata of 24 states (per-level
@@ -597,7 +659,7 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", choices=["subset", "batch", "learned", "exact", "brute", "eppstein", "times",
-                                           "frontier", "reduce"],
+                                           "frontier", "reduce", "overflow"],
                     default="subset",
                     help="subset: the headline line; batch / learned: the batched and learned-bound reports; "
                          "exact: one verbose exact search (--exact-n, --exact-seed); brute: power-set BFS; "
@@ -606,6 +668,7 @@ def main() -> int:
     ap.add_argument("--exact-n", type=int, default=300)
     ap.add_argument("--exact-seed", type=int, default=1)
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--in-flight", type=int, default=8, help="overflow workload: searches at a time (0: skip)")
     ap.add_argument("--batch-cap-rows", type=int, default=0, help="sets per side inside a workgroup (0: default)")
     ap.add_argument("--batch-max-pairs", type=int, default=0, help="largest goal check inside a workgroup (0: default)")
     ap.add_argument("--scorer", type=str, default=str(ROOT / "synchrowords_amd" / "models" / "scorer_n300_k2.npz"))
@@ -652,6 +715,8 @@ def main() -> int:
         ctx = sw.Context(0)
         if args.workload == "batch":
             line = bench_batch(ctx, args.batch_count, args.batch_cap_rows, args.batch_max_pairs)
+        elif args.workload == "overflow":
+            line = bench_overflow(ctx, args.batch_count, args.in_flight)
         elif args.workload == "brute":
             line = bench_brute(ctx, args.batch_count, args.no_cpu)
         elif args.workload == "eppstein":
