@@ -45,14 +45,26 @@ void run_subset(sw_ctx* ctx, int W, VsubParams& p) {
 #ifndef SW_SORT_COUNTS
 #define SW_SORT_COUNTS 1
 #endif
+// Launches of at least 2^SW_IX_SORT_LOG2 pairs sort both lists by size;
+// anchor buckets from 2^SW_IX_ANCHOR_LOG2 pairs (64: never).  Measured on the
+// batch's overflow searches (n <= 160, lists of 10^4..10^6 sets: 9 % less
+// time than with both at 2^34, the bucket pass's round trip no longer paid
+// below 2^38 pairs) and on the n = 300 searches (unchanged).
+#ifndef SW_IX_SORT_LOG2
+#define SW_IX_SORT_LOG2 30
+#endif
+#ifndef SW_IX_ANCHOR_LOG2
+#define SW_IX_ANCHOR_LOG2 38
+#endif
   // Counts walk every pair's states: sorted by size, the two sets of a pair
   // are of like size, so the shorter one reads fewer padding rows.
   if (SW_SORT_COUNTS && ctx->ix_walk == 0 && p.mode == VSUB_COUNT) p.sort_by_size = 1;
   const uint64_t s = p.s_count, pairs = (s + 1) / 2;
-  // Sorting costs two passes over each list and a few launches: worth it
-  // for launches of more than about a millisecond (unless forced).
+  // Sorting costs two passes over each list and a few launches; the
+  // workgroups' early stop at their tile's largest set pays for it from the
+  // indexed kernel's smallest launches on.
   const bool want_sort =
-      p.sort_by_size && (ctx->ix_walk >= 2 || tests >= (1ull << 34)) && p.t_count < (1ull << 32);
+      p.sort_by_size && (ctx->ix_walk >= 2 || tests >= (1ull << SW_IX_SORT_LOG2)) && p.t_count < (1ull << 32);
   if (want_sort && p.diag != INT64_MIN && p.distinct_sets && vsub_ix_fits(p.n, W, true)) {
     p.diag = INT64_MIN;
     p.proper_by_size = 1;
@@ -65,7 +77,8 @@ void run_subset(sw_ctx* ctx, int W, VsubParams& p) {
   uint32_t* f = nullptr;
   if (p.bucket_anchor && (ctx->ix_walk == 0 || ctx->ix_walk == 4) && p.diag == INT64_MIN &&
       (p.mode == VSUB_ANY || p.mode == VSUB_FLAGS || p.mode == VSUB_SFLAGS) &&
-      (ctx->ix_walk == 4 || tests >= (1ull << 34)) && s < (1ull << 32) && p.t_count < (1ull << 32)) {
+      (ctx->ix_walk == 4 || (SW_IX_ANCHOR_LOG2 < 64 && tests >= (1ull << (SW_IX_ANCHOR_LOG2 & 63)))) &&
+      s < (1ull << 32) && p.t_count < (1ull << 32)) {
     // ---- anchor buckets
     const uint32_t n = p.n, n1 = n + 1;
     ctx->ix_order.ensure(3 * 512 * 4);
