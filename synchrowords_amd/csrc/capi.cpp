@@ -1,6 +1,7 @@
 // C ABI: housekeeping and the kernel-level entry points.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <climits>
 #include <cstdio>
 #include <cstring>
@@ -63,6 +64,20 @@ const uint64_t* to_device(sw_ctx* ctx, DevBuf& buf, const uint64_t* p, uint64_t 
 
 }  // namespace
 
+namespace {
+std::atomic<int> g_user_contexts{0};
+}  // namespace
+
+namespace sw {
+int user_contexts() { return g_user_contexts.load(); }
+void mark_helper_context(sw_ctx* ctx) {
+  if (!ctx->helper_ctx) {
+    ctx->helper_ctx = true;
+    g_user_contexts.fetch_sub(1);
+  }
+}
+}  // namespace sw
+
 extern "C" {
 
 const char* sw_last_error(void) { return sw::g_error.c_str(); }
@@ -90,11 +105,15 @@ int sw_ctx_create(int device, sw_ctx** out) {
   ctx->mem = prop.totalGlobalMem;
   sw::noalloc_init(device);  // the no-allocation entry point's scratch address
   *out = ctx.release();
+  g_user_contexts.fetch_add(1);
   SW_API_END
 }
 
 void sw_ctx_destroy(sw_ctx* ctx) {
   if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->helper && ctx->helper_free) ctx->helper_free(ctx->helper);  // joins its thread, ends its context
+  ctx->helper = nullptr;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->a.release();
@@ -113,6 +132,7 @@ void sw_ctx_destroy(sw_ctx* ctx) {
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
   (void)hipStreamDestroy(ctx->stream);
+  if (!ctx->helper_ctx) g_user_contexts.fetch_sub(1);
   delete ctx;
 }
 

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kThreadsBS) void k_beam_small(BeamSmallArgs a) {
   __shared__ uint32_t tab[kTab];
   __shared__ uint32_t wsum[kThreadsBS / 64];
   __shared__ uint32_t pch[65];  // sets per size, for the selection
-  __shared__ uint32_t s_first, s_cls, s_cstar, s_need;
+  __shared__ uint32_t s_first, s_cls, s_cstar, s_need, s_halt;
   const uint32_t tid = threadIdx.x;
   const uint32_t n = a.n, k = a.k;
   const uint64_t full = n == 64 ? ~0ull : ((1ull << n) - 1);
@@ -78,6 +78,15 @@ __global__ __launch_bounds__(kThreadsBS) void k_beam_small(BeamSmallArgs a) {
   uint32_t m = a.m0, cur = 0, hist_len = 0, len = 1;
   uint32_t status = kBeamSmallExhausted;
   for (; len < a.upper; ++len) {
+    if (a.stop && (len & 15u) == 0) {
+      // Every 16 depths: has the caller's other stream set a smaller cap?
+      if (tid == 0) {
+        const unsigned long long v = __hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_halt = (uint32_t(v >> 32) == a.stop_gen && len >= uint32_t(v)) ? 1u : 0u;
+      }
+      __syncthreads();
+      if (s_halt) break;  // as if upper had been that depth: exhausted
+    }
     const uint32_t cnt = m * k;
     if (cnt > kBeamSmallCand || hist_len + cnt > a.hist_cap || len > a.meta_cap) {
       status = kBeamSmallHandover;
@@ -233,6 +242,15 @@ __global__ __launch_bounds__(kThreadsBS) void k_beam_small(BeamSmallArgs a) {
     a.out[2] = m;
     a.out[3] = hist_len;
   }
+}
+
+__global__ void k_set_stop(unsigned long long* cell, unsigned long long value) {
+  __hip_atomic_store(cell, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+hipError_t launch_set_stop(unsigned long long* cell, uint64_t value, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_stop, dim3(1), dim3(1), 0, st, cell, (unsigned long long)value);
+  return hipGetLastError();
 }
 
 hipError_t launch_beam_small(const BeamSmallArgs& a, hipStream_t st) {

@@ -554,14 +554,22 @@ int sw_mlp_samples(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* delta, u
   }
 }
 
+}  // extern "C"
+
+namespace sw {
+
 // Beam search on the device: from the sink singletons with two preimages
 // under a letter, expand by preimages, drop duplicates, keep the first
 // `width` candidates — in the network's score order, or (m == NULL) in the
 // reference's Beam order (larger sets first, then by their words,
 // beam.hpp:98-188); the first depth producing Q bounds the length.  Returns
-// *bound (upper when not improved) and the word of that length.
-int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t width,
-    uint64_t upper, uint64_t* bound, uint32_t* word, uint64_t word_cap, uint64_t* word_len) {
+// *bound (upper when not improved) and the word of that length.  With a
+// stop cell, the beam also ends at the cell's depth once the cell holds the
+// stop's generation (read every 16 depths in the one-workgroup kernel, every
+// depth on the device-wide path): the caller then treats any bound at or past
+// that depth as no improvement.
+int beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t width,
+    uint64_t upper, uint64_t* bound, uint32_t* word, uint64_t word_cap, uint64_t* word_len, const BeamStop* stop) {
   try {
     SW_REQUIRE(ctx && delta && bound && width, "NULL argument");
     SW_REQUIRE(!m || (m->n == n && m->k == k), "network built for another (n, k)");
@@ -643,6 +651,10 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
       a.out = sword.as<uint32_t>();
       a.word = sword.as<uint32_t>() + 4;
       a.list_out = list.as<uint64_t>();
+      if (stop) {
+        a.stop = stop->cell;
+        a.stop_gen = stop->gen;
+      }
       SW_CHECK_HIP(sw::launch_beam_small(a, st));
       std::vector<uint32_t> back(4 + size_t(upper));
       SW_CHECK_HIP(hipMemcpyAsync(back.data(), sword.p, back.size() * 4, hipMemcpyDeviceToHost, st));
@@ -703,9 +715,11 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
           uniq.as<uint64_t>(), uidx.as<uint32_t>(), st));
       // first (8 bytes) and total (4 bytes at +16) in one copy
       SW_CHECK_HIP(hipMemcpyAsync(ws.host, small.p, 24, hipMemcpyDeviceToHost, st));
+      if (stop) SW_CHECK_HIP(hipMemcpyAsync(ws.host + 4, stop->cell, 8, hipMemcpyDeviceToHost, st));
       SW_CHECK_HIP(hipStreamSynchronize(st));
       const unsigned long long f = ws.host[0];
       const uint32_t nu = uint32_t(ws.host[2]);
+      if (stop && uint32_t(ws.host[4] >> 32) == stop->gen && len >= uint32_t(ws.host[4])) break;  // capped there
       if (f != ~0ull) {
         // Word: letter of this candidate, then back through the kept parents.
         std::vector<uint32_t> h(hist_len);
@@ -805,6 +819,15 @@ int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_
   } catch (const Error& e) {
     return fail(e);
   }
+}
+
+}  // namespace sw
+
+extern "C" {
+
+int sw_beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t width,
+    uint64_t upper, uint64_t* bound, uint32_t* word, uint64_t word_cap, uint64_t* word_len) {
+  return sw::beam_device(ctx, m, n, k, delta, width, upper, bound, word, word_cap, word_len, nullptr);
 }
 
 }  // extern "C"

@@ -10,17 +10,24 @@
 // its C++ expressions, keeps [lower, upper] and the word, and does the
 // Beam's presort and dynamic sizing on the host.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <exception>
+#include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <queue>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "sw_host.hpp"
+#include "sw_kernels.hpp"
 #include "sw_reduce.hpp"
 
 namespace sw {
@@ -277,13 +284,13 @@ struct DeviceBeam {
   sw_ctx* ctx;
   uint32_t n, k;
   std::vector<uint32_t> d;  // relabelled automaton
-  BeamResult run(uint64_t width, uint64_t max_len) const {
+  BeamResult run(uint64_t width, uint64_t max_len, const BeamStop* stop = nullptr) const {
     BeamResult r{max_len, {}};
     if (max_len <= 1 || width == 0) return r;
     std::vector<uint32_t> w(std::min<uint64_t>(max_len, 1u << 22));
     uint64_t b = max_len, wl = 0;
     const uint32_t wd = uint32_t(std::min<uint64_t>(width, 0xffffffffu));
-    if (sw_beam_device(ctx, nullptr, n, k, d.data(), wd, max_len, &b, w.data(), w.size(), &wl) != SW_OK)
+    if (beam_device(ctx, nullptr, n, k, d.data(), wd, max_len, &b, w.data(), w.size(), &wl, stop) != SW_OK)
       throw Error{SW_ERR_HIP, std::string("device beam: ") + sw_last_error()};
     r.bound = b;
     if (b < max_len && wl == b) r.word.assign(w.begin(), w.begin() + wl);
@@ -291,9 +298,114 @@ struct DeviceBeam {
   }
 };
 
-BeamResult beam(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d0, uint64_t upper, const BeamSettings& bs) {
-  // Presort "indeg": relabel states by ascending in-degree, ties by index
-  // (beam.hpp:351-369).  The order of equal-size sets depends on the labels.
+// A host thread with a device context of its own that runs one task at a
+// time for a context (the Beam's first pass beside Eppstein), made on first
+// use and ended with the context.  Both sides spin for a short while before
+// they sleep: a pass lasts a tenth of a millisecond, about what waking a
+// sleeping thread can take.
+class Helper {
+ public:
+  explicit Helper(int device) {
+    if (sw_ctx_create(device, &ctx_) != SW_OK) throw Error{SW_ERR_HIP, sw_last_error()};
+    mark_helper_context(ctx_);
+    th_ = std::thread([this, device] {
+      (void)hipSetDevice(device);
+      uint64_t seen = 0;
+      for (;;) {
+        if (!spin([&] { return posted_.load(std::memory_order_acquire) != seen || quit_.load(); })) {
+          std::unique_lock<std::mutex> lk(mu_);
+          cv_.wait(lk, [&] { return posted_.load(std::memory_order_acquire) != seen || quit_.load(); });
+        }
+        if (quit_.load()) return;
+        std::function<void()> f;
+        {
+          std::lock_guard<std::mutex> lk(mu_);
+          seen = posted_.load(std::memory_order_acquire);
+          f = std::move(task_);
+          task_ = nullptr;
+        }
+        std::exception_ptr e;
+        try {
+          f();
+        } catch (...) {
+          e = std::current_exception();
+        }
+        {
+          std::lock_guard<std::mutex> lk(mu_);
+          err_ = e;
+          done_.store(seen, std::memory_order_release);
+        }
+        cv_.notify_all();
+      }
+    });
+  }
+  ~Helper() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      quit_.store(true);
+    }
+    cv_.notify_all();
+    th_.join();
+    sw_ctx_destroy(ctx_);
+  }
+  sw_ctx* ctx() const { return ctx_; }
+  void start(std::function<void()> f) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      task_ = std::move(f);
+      posted_.fetch_add(1, std::memory_order_release);
+    }
+    cv_.notify_all();
+  }
+  // Until the task has returned; rethrows what it threw.
+  void wait() {
+    const uint64_t want = posted_.load(std::memory_order_acquire);
+    auto done = [&] { return done_.load(std::memory_order_acquire) == want; };
+    if (!spin(done)) {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_.wait(lk, done);
+    }
+    std::lock_guard<std::mutex> lk(mu_);
+    if (err_) {
+      std::exception_ptr e = err_;
+      err_ = nullptr;
+      std::rethrow_exception(e);
+    }
+  }
+
+ private:
+  // Polls `ready` for up to two milliseconds.
+  template <typename F>
+  static bool spin(F ready) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0;; ++i) {
+      if (ready()) return true;
+      if ((i & 255) == 255 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) return false;
+      __builtin_ia32_pause();
+    }
+  }
+  sw_ctx* ctx_ = nullptr;
+  std::thread th_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::function<void()> task_;
+  std::atomic<uint64_t> posted_{0}, done_{0};
+  std::atomic<bool> quit_{false};
+  std::exception_ptr err_;
+};
+
+Helper& helper_of(sw_ctx* ctx) {
+  if (!ctx->helper) {
+    ctx->helper = new Helper(ctx->device);
+    ctx->helper_free = [](void* h) { delete static_cast<Helper*>(h); };
+  }
+  return *static_cast<Helper*>(ctx->helper);
+}
+
+// The state relabelling of the Beam's presort "indeg": states by ascending
+// in-degree, ties by index (beam.hpp:351-369).  The order of equal-size sets
+// depends on the labels.
+DeviceBeam relabelled(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d0, const BeamSettings& bs) {
   std::vector<uint32_t> order(n);
   if (bs.presort_indeg) {
     std::vector<std::pair<double, int>> indeg(n);
@@ -308,11 +420,37 @@ BeamResult beam(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d0, uint64_
   DeviceBeam br{ctx, n, k, std::vector<uint32_t>(size_t(n) * k, 0)};
   for (uint32_t q = 0; q < n; ++q)
     for (uint32_t a = 0; a < k; ++a) br.d[size_t(order[q]) * k + a] = order[d0[size_t(q) * k + a]];
+  return br;
+}
+
+// The Beam's depth cap under the upper bound `upper` (max_iter, when set,
+// caps the depths too).
+uint64_t beam_maxlen(uint64_t upper, const BeamSettings& bs) {
   uint64_t maxlen = upper;
   if (bs.max_iter >= 0 && uint64_t(bs.max_iter) + 1 < maxlen) maxlen = uint64_t(bs.max_iter) + 1;
+  return maxlen;
+}
+
+// Width of the Beam's first (or only) pass.
+uint64_t beam_first_width(const BeamSettings& bs) {
+  return bs.dynamic ? uint64_t(static_cast<unsigned>(bs.min_beam)) : uint64_t(bs.beam_size);
+}
+
+// `first`: the first pass already run (under this upper), else NULL.
+BeamResult beam(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d0, uint64_t upper, const BeamSettings& bs,
+    const BeamResult* first) {
+  const DeviceBeam br = relabelled(ctx, n, k, d0, bs);
+  const uint64_t maxlen = beam_maxlen(upper, bs);
+  // A pass under the bound `cap` that reached no depth below its depth cap
+  // (max_iter may end it first) returns `cap` itself, with no word
+  // (beam.hpp:125-131, 186).
+  auto settle = [&](BeamResult o, uint64_t cap) {
+    if (o.bound >= beam_maxlen(cap, bs)) o = BeamResult{cap, {}};
+    return o;
+  };
   BeamResult res;
   if (!bs.dynamic) {
-    auto o = br.run(uint64_t(bs.beam_size), maxlen);
+    auto o = settle(first ? *first : br.run(uint64_t(bs.beam_size), maxlen), upper);
     res.bound = std::min(o.bound, upper);
     res.word = o.word;
     return res;
@@ -322,7 +460,7 @@ BeamResult beam(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d0, uint64_
   using cost_t = long double;
   const unsigned min_beam = static_cast<unsigned>(bs.min_beam);
   const unsigned max_beam = static_cast<unsigned>(bs.max_beam);
-  auto o1 = br.run(min_beam, maxlen);
+  auto o1 = settle(first ? *first : br.run(min_beam, maxlen), upper);
   unsigned bound = static_cast<unsigned>(o1.bound);
   res.bound = bound;
   res.word = o1.word;
@@ -330,7 +468,7 @@ BeamResult beam(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d0, uint64_
   cost_t proposed = std::round(exact_cost * bs.ratio / bound);
   unsigned size = static_cast<unsigned>(std::clamp<cost_t>(proposed, min_beam, max_beam));
   if (size > min_beam) {
-    auto o2 = br.run(size, bound);
+    auto o2 = settle(br.run(size, beam_maxlen(bound, bs)), bound);
     if (o2.bound < bound) {
       res.bound = o2.bound;
       res.word = o2.word;
@@ -354,6 +492,21 @@ const char* kDefaultPlan = R"PLAN({
 })PLAN";
 
 }  // namespace
+
+BeamSettings beam_settings(const Json* cfg, uint32_t n, uint32_t k) {
+  BeamSettings bs;
+  bs.beam_size = eval_num(cfg->get("beam_size"), "std::log2(AUT_N)", n, k);
+  bs.dynamic = eval_bool(cfg->get("dynamic"), false, n, k);
+  bs.min_beam = eval_num(cfg->get("min_beam_size"), "AUT_N * std::log2(AUT_N)", n, k);
+  bs.max_beam = eval_num(cfg->get("max_beam_size"), "AUT_N * AUT_N * std::log2(AUT_N)", n, k);
+  bs.ratio = eval_num(cfg->get("beam_exact_ratio"), "0.01", n, k);
+  bs.max_iter = (long long)eval_num(cfg->get("max_iter"), "-1", n, k);
+  const Json* ps = cfg->get("presort");
+  std::string presort = ps && ps->kind == Json::STR ? ps->str : "none";
+  SW_REQUIRE(presort == "none" || presort == "indeg", "presort value not in [none, indeg]");
+  bs.presort_indeg = presort == "indeg";
+  return bs;
+}
 
 // The reference's run() over a plan (jit/jitmain.cpp:11-49): bounds start at
 // [0, upper_bound] and every algorithm may only tighten them; an algorithm
@@ -379,13 +532,27 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
   // algorithms return at once then: brute.hpp:39, eppstein.hpp:36,
   // beam.hpp:48, reduce.hpp:35).
   ReduceState red;
-  for (const Json& a : algos->arr) {
+  // The Beam's first pass, when it ran beside Eppstein (the stage before it):
+  // automata of at most 64 states, whose Eppstein and first Beam pass are
+  // one workgroup each.  It runs on the helper's context under the bound
+  // Eppstein started from, and ends early at the bound Eppstein found (a
+  // stop cell the beam reads); a depth at or past that bound counts as no
+  // improvement, so the result is the pass run after Eppstein under its
+  // bound.
+  struct {
+    bool ran = false;
+    BeamResult r;
+  } first;
+  std::chrono::steady_clock::duration carry{0};  // time of the next stage spent in this one
+  for (size_t ai = 0; ai < algos->arr.size(); ++ai) {
+    const Json& a = algos->arr[ai];
     const Json* nm = a.get("name");
     SW_REQUIRE(nm && nm->kind == Json::STR, "algorithm without a name");
     const Json empty;
     const Json* cfg = a.get("config") ? a.get("config") : &empty;
     const std::string name = nm->str;
-    auto t0 = std::chrono::steady_clock::now();
+    auto t0 = std::chrono::steady_clock::now() - carry;
+    carry = std::chrono::steady_clock::duration{0};
     const bool open = !non_synchro && lower != upper;
     if (red.reduced && name != "Exact") {
       // Not run on the reduced automaton.
@@ -409,7 +576,69 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
         uint64_t bound = upper, wl = 0;
         const uint64_t cap = std::min<uint64_t>(upper, 1u << 22);
         std::vector<uint32_t> w(cap ? cap : 1);
+        // The next stage a Beam that can run beside this one?
+        const Json* next = ai + 1 < algos->arr.size() ? &algos->arr[ai + 1] : nullptr;
+        const Json* next_name = next ? next->get("name") : nullptr;
+        // Only for a context alone in its process: with several busy, the
+        // device's few hardware queues are shared (two passes on one queue
+        // run one after the other) and the plans already overlap each other.
+        const bool beside = n >= 2 && n <= 64 && k <= kBeamSmallMaxK && next_name &&
+            next_name->kind == Json::STR && next_name->str == "Beam" && user_contexts() == 1;
+        BeamSettings bs;
+        DeviceBeam br{nullptr, n, k, {}};
+        uint64_t width = 0, pass_len = 0;
+        BeamResult pass;
+        Helper* helper = nullptr;
+        uint32_t gen = 0;
+        if (beside) {
+          const Json empty_cfg;
+          bs = beam_settings(next->get("config") ? next->get("config") : &empty_cfg, n, k);
+          width = beam_first_width(bs);
+          pass_len = beam_maxlen(upper, bs);
+          if (width && pass_len > 1) {
+            helper = &helper_of(ctx);
+            br = relabelled(helper->ctx(), n, k, d, bs);
+            if (!ctx->stop_cell.p) {
+              ctx->stop_cell.ensure(8);
+              SW_CHECK_HIP(launch_set_stop(ctx->stop_cell.as<unsigned long long>(), 0, ctx->stream));
+              SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+            }
+            gen = ++ctx->stop_gen;
+            if (gen == 0) gen = ++ctx->stop_gen;  // 0 is the cell's first value
+            const BeamStop stop{ctx->stop_cell.as<unsigned long long>(), gen};
+            helper->start([&br, &pass, width, pass_len, stop] { pass = br.run(width, pass_len, &stop); });
+          }
+        }
+        // Whatever ends this stage, the pass ends (at its next check) and is
+        // waited for before the buffers it uses go.
+        struct PassGuard {
+          Helper* h;
+          sw_ctx* ctx;
+          uint32_t gen;
+          ~PassGuard() {
+            if (!h) return;
+            (void)launch_set_stop(ctx->stop_cell.as<unsigned long long>(), uint64_t(gen) << 32, ctx->stream);
+            try {
+              h->wait();
+            } catch (...) {
+            }
+          }
+        } guard{helper, ctx, gen};
         eppstein_batch(ctx, 1, &n, &k, &off, d, &upper, cap, &ns, &imp, &bound, &wl, w.data());
+        if (helper) {
+          // The depth the pass would have stopped at, run after this stage.
+          const auto t_split = std::chrono::steady_clock::now();
+          const uint64_t after = ns ? 0 : beam_maxlen(imp ? bound : upper, bs);
+          SW_CHECK_HIP(launch_set_stop(ctx->stop_cell.as<unsigned long long>(),
+              (uint64_t(gen) << 32) | std::min<uint64_t>(after, 0xffffffffu), ctx->stream));
+          guard.h = nullptr;
+          helper->wait();
+          first.ran = !ns;
+          first.r = pass.bound < after ? pass : BeamResult{after, {}};
+          // The wait is the Beam's time.
+          carry = std::chrono::steady_clock::now() - t_split;
+          t0 += carry;
+        }
         if (ns) {
           non_synchro = true;
         } else if (imp) {
@@ -426,21 +655,11 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
       }
     } else if (name == "Beam") {
       if (open) {
-        BeamSettings bs;
-        bs.beam_size = eval_num(cfg->get("beam_size"), "std::log2(AUT_N)", n, k);
-        bs.dynamic = eval_bool(cfg->get("dynamic"), false, n, k);
-        bs.min_beam = eval_num(cfg->get("min_beam_size"), "AUT_N * std::log2(AUT_N)", n, k);
-        bs.max_beam = eval_num(cfg->get("max_beam_size"), "AUT_N * AUT_N * std::log2(AUT_N)", n, k);
-        bs.ratio = eval_num(cfg->get("beam_exact_ratio"), "0.01", n, k);
-        bs.max_iter = (long long)eval_num(cfg->get("max_iter"), "-1", n, k);
-        const Json* ps = cfg->get("presort");
-        std::string presort = ps && ps->kind == Json::STR ? ps->str : "none";
-        SW_REQUIRE(presort == "none" || presort == "indeg", "presort value not in [none, indeg]");
-        bs.presort_indeg = presort == "indeg";
+        const BeamSettings bs = beam_settings(cfg, n, k);
         if (n == 1) {
           lower = upper = 0;
         } else {
-          BeamResult b = beam(ctx, n, k, d, upper, bs);
+          BeamResult b = beam(ctx, n, k, d, upper, bs, first.ran ? &first.r : nullptr);
           if (b.bound < upper || (b.bound == upper && word.size() != upper && b.word.size() == b.bound)) {
             upper = b.bound;
             word = b.word;
@@ -448,6 +667,7 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
         }
         out->beam_bound = upper;
       }
+      first.ran = false;
     } else if (name == "Exact") {
       if (open) {
         sw_exact_config ec{};

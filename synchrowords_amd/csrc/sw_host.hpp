@@ -139,12 +139,36 @@ struct sw_ctx {
   uint64_t ix_min_pairs = 1ull << 30;
   // The plan's Reduce stage.
   sw::ReduceBufs reduce;
+  // The plan's helper (pipeline.cpp): a host thread with a context of its
+  // own that runs the Beam's first pass while Eppstein runs on this one;
+  // the stop cell that Eppstein's bound is written to, and the generation
+  // of its latest use.
+  void* helper = nullptr;
+  void (*helper_free)(void*) = nullptr;
+  sw::DevBuf stop_cell;
+  uint32_t stop_gen = 0;
+  bool helper_ctx = false;  // this context is such a helper's own
 };
 
 namespace sw {
 
 // Make the context's device current.
 void use(sw_ctx* ctx);
+
+// Contexts alive in this process that callers made (helpers' own not
+// counted), and marking a context as a helper's own.
+int user_contexts();
+void mark_helper_context(sw_ctx* ctx);
+
+// A stop cell for a beam that runs beside other work: the cell holds
+// (generation << 32) | depth, written by the other work's stream.
+struct BeamStop {
+  const unsigned long long* cell;
+  uint32_t gen;
+};
+// sw_beam_device with an optional stop cell (mlp.cpp).
+int beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* delta, uint32_t width,
+    uint64_t upper, uint64_t* bound, uint32_t* word, uint64_t word_cap, uint64_t* word_len, const BeamStop* stop);
 
 // Look up the device address of the no-allocation entry point's static
 // scratch on `device` (current); once per device (noalloc.hip).
