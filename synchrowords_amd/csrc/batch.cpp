@@ -69,16 +69,23 @@ bool exact_search_workgroup(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t*
   uint8_t* dbytes = din + sizeof(BatchInstance) + 64;
   BatchResult* dres = ctx->wg_out.as<BatchResult>();
   uint8_t* dwords = ctx->wg_out.as<uint8_t>() + sizeof(BatchResult) + 64;
-  std::vector<uint8_t> head(sizeof(BatchInstance) + 64, 0);
-  std::memcpy(head.data(), &b, sizeof(b));
-  SW_CHECK_HIP(hipMemcpyAsync(din, head.data(), head.size(), hipMemcpyHostToDevice, s));
-  SW_CHECK_HIP(hipMemcpyAsync(dbytes, bytes.data(), bytes.size(), hipMemcpyHostToDevice, s));
+  // One copy in (instance, queue words, transitions: wg_in's layout) and one
+  // out (result, then the word), through pinned staging.
+  const size_t in_bytes = sizeof(BatchInstance) + 64 + bytes.size();
+  const size_t out_bytes = sizeof(BatchResult) + 64 + wcap;
+  uint8_t* stage = static_cast<uint8_t*>(stage_bytes(ctx, in_bytes + out_bytes));
+  std::memset(stage, 0, sizeof(BatchInstance) + 64);
+  std::memcpy(stage, &b, sizeof(b));
+  std::memcpy(stage + sizeof(BatchInstance) + 64, bytes.data(), bytes.size());
+  SW_CHECK_HIP(hipMemcpyAsync(din, stage, in_bytes, hipMemcpyHostToDevice, s));
   SW_CHECK_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(dres), 0u, sizeof(BatchResult) / 4, s));
   SW_CHECK_HIP(launch_batch(1, dinst, dbytes, dorder, 1, dhead, dres, dwords, wcap, ctx->wg_scratch.as<uint8_t>(),
       stride, L, 1, s, 1024));
-  BatchResult r{};
-  SW_CHECK_HIP(hipMemcpyAsync(&r, dres, sizeof(r), hipMemcpyDeviceToHost, s));
+  uint8_t* back = stage + in_bytes;
+  SW_CHECK_HIP(hipMemcpyAsync(back, dres, out_bytes, hipMemcpyDeviceToHost, s));
   SW_CHECK_HIP(hipStreamSynchronize(s));
+  BatchResult r;
+  std::memcpy(&r, back, sizeof(r));
   if (r.status != kBatchFound && r.status != kBatchBound) return false;
   std::memset(&out.r, 0, sizeof(out.r));
   out.word.clear();
@@ -86,9 +93,9 @@ bool exact_search_workgroup(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t*
   out.r.length = r.length;
   out.r.steps_forward = r.steps;
   if (r.status == kBatchFound) {
-    std::vector<uint8_t> w(r.word_len);
-    if (r.word_len) SW_CHECK_HIP(hipMemcpy(w.data(), dwords, r.word_len, hipMemcpyDeviceToHost));
-    out.word.assign(w.begin(), w.end());
+    SW_REQUIRE(r.word_len <= wcap, "workgroup search: word past its buffer");
+    const uint8_t* w = back + sizeof(BatchResult) + 64;
+    out.word.assign(w, w + r.word_len);
     out.r.word_len = out.word.size();
   } else {
     out.r.bound_word = 1;

@@ -69,6 +69,18 @@ std::atomic<int> g_user_contexts{0};
 }  // namespace
 
 namespace sw {
+void* stage_bytes(sw_ctx* ctx, size_t bytes) {
+  if (bytes > ctx->stage_cap) {
+    size_t cap = ctx->stage_cap ? ctx->stage_cap : 16384;
+    while (cap < bytes) cap *= 2;
+    if (ctx->stage) SW_CHECK_HIP(hipHostFree(ctx->stage));
+    ctx->stage = nullptr;
+    ctx->stage_cap = 0;
+    SW_CHECK_HIP(hipHostMalloc(&ctx->stage, cap, hipHostMallocDefault));
+    ctx->stage_cap = cap;
+  }
+  return ctx->stage;
+}
 int user_contexts() { return g_user_contexts.load(); }
 void mark_helper_context(sw_ctx* ctx) {
   if (!ctx->helper_ctx) {
@@ -129,6 +141,8 @@ void sw_ctx_destroy(sw_ctx* ctx) {
   ctx->frontier_ws = nullptr;
   if (ctx->ix_host) (void)hipHostFree(ctx->ix_host);
   ctx->ix_host = nullptr;
+  if (ctx->stage) (void)hipHostFree(ctx->stage);
+  ctx->stage = nullptr;
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
   (void)hipStreamDestroy(ctx->stream);

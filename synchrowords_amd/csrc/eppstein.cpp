@@ -134,17 +134,24 @@ void eppstein_batch(sw_ctx* ctx, uint32_t count, const uint32_t* ns, const uint3
     ctx->epp_scratch.ensure(scratch);
     ctx->epp_words.ensure(m * word_cap * 4 + 4);
     ctx->epp_out.ensure(m * sizeof(EppOut));
-    SW_CHECK_HIP(hipMemcpyAsync(ctx->epp_jobs.p, jobs.data(), m * sizeof(EppJob), hipMemcpyHostToDevice, st));
-    SW_CHECK_HIP(hipMemcpyAsync(ctx->epp_delta.p, dpack.data(), dpack.size() * 4, hipMemcpyHostToDevice, st));
+    // Copies through the context's pinned staging: jobs and transitions in,
+    // results and words out.
+    const size_t jb = m * sizeof(EppJob), db = dpack.size() * 4, ob = m * sizeof(EppOut);
+    const size_t wb = words && word_cap ? m * word_cap * 4 : 0;
+    unsigned char* stage = static_cast<unsigned char*>(stage_bytes(ctx, jb + db + ob + wb));
+    std::memcpy(stage, jobs.data(), jb);
+    std::memcpy(stage + jb, dpack.data(), db);
+    SW_CHECK_HIP(hipMemcpyAsync(ctx->epp_jobs.p, stage, jb, hipMemcpyHostToDevice, st));
+    SW_CHECK_HIP(hipMemcpyAsync(ctx->epp_delta.p, stage + jb, db, hipMemcpyHostToDevice, st));
     SW_CHECK_HIP(launch_eppstein_wg(ctx->epp_jobs.as<EppJob>(), uint32_t(m), ctx->epp_delta.as<uint32_t>(),
         ctx->epp_scratch.as<unsigned char>(), ctx->epp_words.as<uint32_t>(), ctx->epp_out.as<EppOut>(), list_lds,
         eppstein_letter_lds(max_n, list_lds), max_n <= 64, st));
-    std::vector<EppOut> out(m);
-    SW_CHECK_HIP(hipMemcpyAsync(out.data(), ctx->epp_out.p, m * sizeof(EppOut), hipMemcpyDeviceToHost, st));
-    if (words && word_cap)
-      SW_CHECK_HIP(hipMemcpyAsync(words + uint64_t(at) * word_cap, ctx->epp_words.p, m * word_cap * 4,
-          hipMemcpyDeviceToHost, st));
+    SW_CHECK_HIP(hipMemcpyAsync(stage + jb + db, ctx->epp_out.p, ob, hipMemcpyDeviceToHost, st));
+    if (wb) SW_CHECK_HIP(hipMemcpyAsync(stage + jb + db + ob, ctx->epp_words.p, wb, hipMemcpyDeviceToHost, st));
     SW_CHECK_HIP(hipStreamSynchronize(st));
+    std::vector<EppOut> out(m);
+    std::memcpy(out.data(), stage + jb + db, ob);
+    if (wb) std::memcpy(words + uint64_t(at) * word_cap, stage + jb + db + ob, wb);
     for (size_t j = 0; j < m; ++j) {
       const uint32_t i = at + uint32_t(j);
       if (non_synchro) non_synchro[i] = out[j].non_synchro;

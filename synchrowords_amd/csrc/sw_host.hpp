@@ -135,6 +135,9 @@ struct sw_ctx {
   int ix_walk = 0;
   size_t ix_tmp_bytes = 0;
   uint32_t* ix_host = nullptr;  // pinned: the index length read back
+  // Pinned staging of the one-automaton stages' small copies (stage_bytes).
+  void* stage = nullptr;
+  size_t stage_cap = 0;
   int ix_enabled = 1;
   uint64_t ix_min_pairs = 1ull << 30;
   // The plan's Reduce stage.
@@ -154,6 +157,11 @@ namespace sw {
 
 // Make the context's device current.
 void use(sw_ctx* ctx);
+
+// At least `bytes` of the context's pinned staging memory, for the small
+// copies of one call; call it while the context's stream holds no copy from
+// it (growing frees the old buffer).
+void* stage_bytes(sw_ctx* ctx, size_t bytes);
 
 // Contexts alive in this process that callers made (helpers' own not
 // counted), and marking a context as a helper's own.
