@@ -301,7 +301,7 @@ struct DeviceBeam {
 // A host thread with a device context of its own that runs one task at a
 // time for a context (the Beam's first pass beside Eppstein), made on first
 // use and ended with the context.  Both sides spin for a short while before
-// they sleep: a pass lasts a tenth of a millisecond, about what waking a
+// they sleep: a pass can last a tenth of a millisecond, about what waking a
 // sleeping thread can take.
 class Helper {
  public:
@@ -532,13 +532,12 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
   // algorithms return at once then: brute.hpp:39, eppstein.hpp:36,
   // beam.hpp:48, reduce.hpp:35).
   ReduceState red;
-  // The Beam's first pass, when it ran beside Eppstein (the stage before it):
-  // automata of at most 64 states, whose Eppstein and first Beam pass are
-  // one workgroup each.  It runs on the helper's context under the bound
-  // Eppstein started from, and ends early at the bound Eppstein found (a
-  // stop cell the beam reads); a depth at or past that bound counts as no
-  // improvement, so the result is the pass run after Eppstein under its
-  // bound.
+  // The Beam's first pass, when it ran beside Eppstein (the stage before it;
+  // both are latency-bound chains of small launches, or one workgroup each up
+  // to 64 states).  It runs on the helper's context under the bound Eppstein
+  // started from, and ends early at the bound Eppstein found (a stop cell the
+  // beam reads); a depth at or past that bound counts as no improvement, so
+  // the result is the pass run after Eppstein under its bound.
   struct {
     bool ran = false;
     BeamResult r;
@@ -582,7 +581,7 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
         // Only for a context alone in its process: with several busy, the
         // device's few hardware queues are shared (two passes on one queue
         // run one after the other) and the plans already overlap each other.
-        const bool beside = n >= 2 && n <= 64 && k <= kBeamSmallMaxK && next_name &&
+        const bool beside = n >= 2 && next_name &&
             next_name->kind == Json::STR && next_name->str == "Beam" && user_contexts() == 1;
         BeamSettings bs;
         DeviceBeam br{nullptr, n, k, {}};

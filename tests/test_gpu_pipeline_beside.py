@@ -1,7 +1,8 @@
-"""The Beam's first pass beside Eppstein (pipeline.cpp): for automata of at
-most 64 states, a context alone in its process runs the Beam's first pass on
-a helper context while Eppstein runs, and cuts it at Eppstein's bound through
-a stop cell.  The plan's results must be those of the stages run one after
+"""The Beam's first pass beside Eppstein (pipeline.cpp): a context alone in
+its process runs the Beam's first pass on a helper context while Eppstein
+runs, and cuts it at Eppstein's bound through a stop cell (read by the
+one-workgroup beam up to 64 states, by the host loop of the device-wide beam
+above).  The plan's results must be those of the stages run one after
 the other, which is what a process with two contexts alive does.
 
 The lone-context side runs in a child process (one context there); this
@@ -59,9 +60,15 @@ def _cases():
         out.append((40, 2, _two_copies(20, 700 + s), plain))  # not synchronizing
     out.append((24, 2, [int(x) for x in gen.cerny(24)], plain))
     out.append((16, 16, [int(x) for x in sw.random_dfa(16, 16, 31)], plain))
+    out.append((40, 20, [int(x) for x in sw.random_dfa(40, 20, 32)], plain))  # past the one-workgroup beam's k
+    for s in range(3):  # the device-wide beam
+        out.append((100, 3, [int(x) for x in sw.random_dfa(100, 3, 40 + s)], plain))
+        out.append((200, 2, [int(x) for x in sw.random_dfa(200, 2, s)], plain))
+        out.append((90 + s, 2, [int(x) for x in sw.random_dfa(90 + s, 2, 60 + s)], json.dumps(capped)))
+        out.append((128, 2, _two_copies(64, 80 + s), plain))  # not synchronizing
+    out.append((70, 2, [int(x) for x in gen.cerny(70)], plain))
     for c in golden("random_misc.json"):
-        if c["n"] <= 64:
-            out.append((c["n"], c["k"], c["delta"], plain))
+        out.append((c["n"], c["k"], c["delta"], plain))
     return [{"n": n, "k": k, "delta": d, "plan": p} for n, k, d, p in out]
 
 
@@ -92,8 +99,6 @@ def test_beam_beside_eppstein_matches_stages_in_order(ctx, tmp_path):
         other.close()
     # And the reference's own results where it has them (configs/exact.json).
     for c in golden("random_misc.json"):
-        if c["n"] > 64:
-            continue
         i = next(j for j, x in enumerate(cases) if x["delta"] == c["delta"])
         assert bool(beside[i][0]) == bool(c["non_synchro"]), c["name"]
         if not c["non_synchro"]:
