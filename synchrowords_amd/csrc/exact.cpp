@@ -421,6 +421,14 @@ class Search {
     return uint32_t(ws_.host[0]);
   }
 
+  // A scan's total written by the scan's last kernel straight into pinned
+  // host memory (no copy launch): the host reads it once the stream is idle.
+  uint32_t* host_total() { return reinterpret_cast<uint32_t*>(ws_.host + 6); }
+  uint32_t take_host_total() {
+    SW_CHECK_HIP(hipStreamSynchronize(st_));
+    return *reinterpret_cast<volatile uint32_t*>(host_total());
+  }
+
   void ensure_table(Side& S, uint64_t incoming) {
     const uint64_t need = (S.rows + incoming) * 2 + 1024;
     if (S.tcap >= need) return;
@@ -542,10 +550,10 @@ class Search {
     ensure_table(S, cnt);
     SW_CHECK_HIP(launch_hash_insert(W, cand_.as<uint64_t>(), valid_.as<uint32_t>(), cnt, S.table.as<uint32_t>(),
         S.tcap - 1, S.arena.as<uint64_t>(), slot_.as<uint32_t>(), probes_.as<unsigned long long>(), st_));
-    uint32_t* total = small_.as<uint32_t>();
+    uint32_t* total = host_total();
     SW_CHECK_HIP(launch_keep_scan(1, slot_.as<uint32_t>(), S.table.as<uint32_t>(), nullptr, cnt, keep_.as<uint32_t>(),
         pos_.as<uint32_t>(), total, scan_tmp_.p, &scan_bytes_, st_));
-    const uint64_t nnew = read_u32(total);
+    const uint64_t nnew = take_host_total();
     ensure_rows(S, S.rows + nnew);
     const uint64_t lo = S.rows;
     const uint64_t pairs = (lo + nnew) * nnew;
@@ -623,7 +631,7 @@ class Search {
           total, scan_tmp_.p, &scan_bytes_, st_));
       SW_CHECK_HIP(launch_select(W, S.arena.as<uint64_t>(), lo, keep_.as<uint32_t>(), pos_.as<uint32_t>(), nnew,
           S.front.as<uint64_t>(), S.front_idx.as<uint32_t>(), st_));
-      S.fcount = read_u32(total);
+      S.fcount = take_host_total();
       ms_dom_ += ms_since(t1);
     } else {
       S.fcount = nnew;
