@@ -184,14 +184,44 @@ class Search {
     uint64_t t = 0;
     bool found = false;
     uint64_t wit = 0;
+    // The goal check of length t is read only after the step of length t + 1
+    // has been enqueued up to its new-set count (one wait for both); a meet
+    // found then ends the search at t and that step is dropped before its
+    // commit.
+    auto read_goal = [&](uint64_t len) {
+      const auto g0 = std::chrono::steady_clock::now();
+      const uint64_t fc = F_.fcount, bc = B_.fcount;
+      const bool met = goal_read(&wit);
+      ms_goal_ += ms_since(g0);
+      if (cfg_.verbose) {
+        std::fprintf(stderr, "[sw exact] len %llu goal |F| %llu x |B| %llu\n", (unsigned long long)len,
+            (unsigned long long)fc, (unsigned long long)bc);
+        if (W_ <= 5) {
+          std::fprintf(stderr, "[sw exact]   forward sizes %s\n", size_quantiles(F_, false).c_str());
+          std::fprintf(stderr, "[sw exact]   backward sizes %s\n", size_quantiles(B_, true).c_str());
+        }
+      }
+      return met;
+    };
     // The supplied upper bound is the length of a known word: search below it.
-    while (t + 1 < upper) {
-      if (F_.fcount == 0 || B_.fcount == 0) break;  // every continuation pruned
+    while (true) {
+      if (t + 1 >= upper || F_.fcount == 0 || B_.fcount == 0) {  // (no sets: every continuation pruned)
+        if (goal_pending_ && read_goal(t)) found = true;
+        break;
+      }
       bool fwd;
       if (F_.frozen) fwd = false;
       else if (side_rule_ == 2) fwd = plan_forward(t, upper);
       else fwd = F_.fcount <= B_.fcount;
       ++t;
+      // A large step is not enqueued on a guess: when the goal check meets,
+      // its work would be thrown away (a search that ends by a meet ends one
+      // step into the guess), and next to its cost the wait it saves is small.
+      if (goal_pending_ && (fwd ? F_ : B_).fcount * k_ > kGuessCandidates && read_goal(t - 1)) {
+        --t;
+        found = true;
+        break;
+      }
       // A step of the other side when the chosen one would pass the budget:
       // every level that still fits shortens the deepening that follows.
       bool stepped = false;
@@ -199,33 +229,26 @@ class Search {
         const bool f = attempt == 0 ? fwd : !fwd;
         if (f && F_.frozen) continue;
         try {
-          step(f ? F_ : B_, !f, t);
+          step_launch(f ? F_ : B_, !f);
           fwd = f;
           stepped = true;
         } catch (const BudgetExceeded&) {
         }
+      }
+      if (goal_pending_ && read_goal(t - 1)) {
+        --t;
+        found = true;
+        break;
       }
       if (!stepped) {
         --t;  // lengths up to t were checked
         budget_end_ = true;
         break;
       }
+      step_finish(fwd ? F_ : B_, !fwd, t);
       const auto g0 = std::chrono::steady_clock::now();
-      const bool met = goal_check(&wit, fwd);
-      const double gms = ms_since(g0);
-      ms_goal_ += gms;
-      if (cfg_.verbose) {
-        std::fprintf(stderr, "[sw exact] len %llu goal |F| %llu x |B| %llu  %.3f ms\n", (unsigned long long)t,
-            (unsigned long long)F_.fcount, (unsigned long long)B_.fcount, gms);
-        if (W_ <= 5) {
-          std::fprintf(stderr, "[sw exact]   forward sizes %s\n", size_quantiles(F_, false).c_str());
-          std::fprintf(stderr, "[sw exact]   backward sizes %s\n", size_quantiles(B_, true).c_str());
-        }
-      }
-      if (met) {
-        found = true;
-        break;
-      }
+      goal_launch(fwd);
+      ms_goal_ += ms_since(g0);
     }
     bool oom = false;
     uint64_t length = 0;
@@ -350,6 +373,14 @@ class Search {
 
   uint64_t pair_tests_ = 0;
   double ms_expand_ = 0, ms_dom_ = 0, ms_goal_ = 0;  // host time between synchronizations
+  bool goal_pending_ = false;  // a goal check launched and not read yet
+  // Largest step (candidates) enqueued before the goal check before it is read.
+  static constexpr uint64_t kGuessCandidates = 1ull << 16;
+  // The step whose expansion is enqueued (step_launch) and not yet taken.
+  struct {
+    uint64_t m = 0, cnt = 0;
+    std::chrono::steady_clock::time_point t0;
+  } pend_;
   Workspace& ws_;
   DevBuf (&tables_)[2];
   Side &F_, &B_;
@@ -528,6 +559,14 @@ class Search {
 
   // One expansion step of side S (backward: complemented sets).
   void step(Side& S, bool backward, uint64_t t) {
+    step_launch(S, backward);
+    step_finish(S, backward, t);
+  }
+  // The step up to its new-set count: budget tests (BudgetExceeded before
+  // anything is enqueued), expansion, hash insertion and the scan whose
+  // total lands in pinned memory.  Nothing of the side changes that a
+  // search ending here would need: the commit is step_finish's.
+  void step_launch(Side& S, bool backward) {
     const auto t0 = std::chrono::steady_clock::now();
     const uint64_t m = S.fcount;
     const uint64_t cnt = m * k_;
@@ -550,9 +589,17 @@ class Search {
     ensure_table(S, cnt);
     SW_CHECK_HIP(launch_hash_insert(W, cand_.as<uint64_t>(), valid_.as<uint32_t>(), cnt, S.table.as<uint32_t>(),
         S.tcap - 1, S.arena.as<uint64_t>(), slot_.as<uint32_t>(), probes_.as<unsigned long long>(), st_));
-    uint32_t* total = host_total();
     SW_CHECK_HIP(launch_keep_scan(1, slot_.as<uint32_t>(), S.table.as<uint32_t>(), nullptr, cnt, keep_.as<uint32_t>(),
-        pos_.as<uint32_t>(), total, scan_tmp_.p, &scan_bytes_, st_));
+        pos_.as<uint32_t>(), host_total(), scan_tmp_.p, &scan_bytes_, st_));
+    pend_.m = m;
+    pend_.cnt = cnt;
+    pend_.t0 = t0;
+  }
+  void step_finish(Side& S, bool backward, uint64_t t) {
+    const auto t0 = pend_.t0;
+    const uint64_t m = pend_.m, cnt = pend_.cnt;
+    const int W = int(W_);
+    uint32_t* total = host_total();
     const uint64_t nnew = take_host_total();
     ensure_rows(S, S.rows + nnew);
     const uint64_t lo = S.rows;
@@ -724,8 +771,24 @@ class Search {
     ++trie_builds_;
   }
 
+  // The goal check in two halves: the launch (kernels, then the found flag
+  // and witness copied into pinned memory) and the read, which waits for the
+  // stream; the search enqueues the next step's expansion in between.
   bool goal_check(uint64_t* witness, bool stepped_forward) {
-    if (!F_.fcount || !B_.fcount) return false;
+    goal_launch(stepped_forward);
+    return goal_read(witness);
+  }
+  bool goal_read(uint64_t* witness) {
+    if (!goal_pending_) return false;
+    goal_pending_ = false;
+    SW_CHECK_HIP(hipStreamSynchronize(st_));
+    const volatile uint64_t* h = ws_.host + 2;
+    *witness = h[1];
+    return uint32_t(h[0]) != 0;
+  }
+  void goal_launch(bool stepped_forward) {
+    goal_pending_ = false;
+    if (!F_.fcount || !B_.fcount) return;
     int* found = small_.as<int>() + 2;
     unsigned long long* wit = reinterpret_cast<unsigned long long*>(small_.as<char>() + 16);
     if (!goal_armed_) SW_CHECK_HIP(launch_goal_reset(found, wit, st_));
@@ -771,11 +834,8 @@ class Search {
       pair_tests_ += pairs;
     }
     // found (4 bytes at +8) and witness (8 bytes at +16) in one copy
-    SW_CHECK_HIP(hipMemcpyAsync(ws_.host, small_.as<char>() + 8, 16, hipMemcpyDeviceToHost, st_));
-    SW_CHECK_HIP(hipStreamSynchronize(st_));
-    const int f = int(uint32_t(ws_.host[0]));
-    *witness = ws_.host[1];
-    return f != 0;
+    SW_CHECK_HIP(hipMemcpyAsync(ws_.host + 2, small_.as<char>() + 8, 16, hipMemcpyDeviceToHost, st_));
+    goal_pending_ = true;
   }
 
   // Device bytes this search holds, from its counts (buffers the context
