@@ -308,6 +308,14 @@ class Helper {
   explicit Helper(int device) {
     if (sw_ctx_create(device, &ctx_) != SW_OK) throw Error{SW_ERR_HIP, sw_last_error()};
     mark_helper_context(ctx_);
+    try {
+      start_thread(device);
+    } catch (...) {
+      sw_ctx_destroy(ctx_);
+      throw;
+    }
+  }
+  void start_thread(int device) {
     th_ = std::thread([this, device] {
       (void)hipSetDevice(device);
       uint64_t seen = 0;
@@ -394,12 +402,18 @@ class Helper {
   std::exception_ptr err_;
 };
 
-Helper& helper_of(sw_ctx* ctx) {
-  if (!ctx->helper) {
-    ctx->helper = new Helper(ctx->device);
-    ctx->helper_free = [](void* h) { delete static_cast<Helper*>(h); };
+// The context's helper, made on first use; NULL when one cannot be made
+// (then the stages run in order, and no helper is tried again).
+Helper* helper_of(sw_ctx* ctx) {
+  if (!ctx->helper && !ctx->helper_failed) {
+    try {
+      ctx->helper = new Helper(ctx->device);
+      ctx->helper_free = [](void* h) { delete static_cast<Helper*>(h); };
+    } catch (const std::exception&) {
+      ctx->helper_failed = true;
+    }
   }
-  return *static_cast<Helper*>(ctx->helper);
+  return static_cast<Helper*>(ctx->helper);
 }
 
 // The state relabelling of the Beam's presort "indeg": states by ascending
@@ -594,8 +608,8 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
           bs = beam_settings(next->get("config") ? next->get("config") : &empty_cfg, n, k);
           width = beam_first_width(bs);
           pass_len = beam_maxlen(upper, bs);
-          if (width && pass_len > 1) {
-            helper = &helper_of(ctx);
+          if (width && pass_len > 1) helper = helper_of(ctx);
+          if (helper) {
             br = relabelled(helper->ctx(), n, k, d, bs);
             if (!ctx->stop_cell.p) {
               ctx->stop_cell.ensure(8);

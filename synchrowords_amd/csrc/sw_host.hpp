@@ -150,7 +150,8 @@ struct sw_ctx {
   void (*helper_free)(void*) = nullptr;
   sw::DevBuf stop_cell;
   uint32_t stop_gen = 0;
-  bool helper_ctx = false;  // this context is such a helper's own
+  bool helper_ctx = false;     // this context is such a helper's own
+  bool helper_failed = false;  // no helper could be made: the stages run in order
 };
 
 namespace sw {
