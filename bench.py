@@ -604,11 +604,12 @@ def bench_eppstein(ctx, count: int, no_cpu: bool) -> dict:
 def bench_learned(ctx, scorer: str, seeds, width: int) -> dict:
     """Learned-bound against plain-bound exact search on the GPU (n=300, k=2).
 
-    Both pipelines run Eppstein (host), a device beam of the same width and the
-    device exact search capped by the smaller bound; they differ only in the
-    beam's order: cardinality (the reference's) or the trained network.  The
-    configs/exact.json plan (host Beam of the reference's dynamic width) is
-    timed beside them."""
+    Both pipelines run Eppstein, a device beam of the same width and the
+    device exact search capped by the smaller bound, one after the other;
+    they differ only in the beam's order: cardinality (the reference's) or
+    the trained network.  The configs/exact.json plan (the device Beam of the
+    reference's dynamic width, its first pass beside Eppstein) is timed
+    beside them."""
     import synchrowords_amd as sw
     from synchrowords_amd.train_scorer import load_scorer
     m = load_scorer(ctx, scorer)
@@ -672,7 +673,8 @@ def main() -> int:
     ap.add_argument("--batch-cap-rows", type=int, default=0, help="sets per side inside a workgroup (0: default)")
     ap.add_argument("--batch-max-pairs", type=int, default=0, help="largest goal check inside a workgroup (0: default)")
     ap.add_argument("--scorer", type=str, default=str(ROOT / "synchrowords_amd" / "models" / "scorer_n300_k2.npz"))
-    ap.add_argument("--learned-width", type=int, default=64)
+    ap.add_argument("--learned-width", type=str, default="64,2048",
+                    help="learned workload: beam widths, comma-separated (one report each)")
     ap.add_argument("--learned-seeds", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (hardware-counter calibration)")
@@ -734,7 +736,12 @@ def main() -> int:
             line = {"workload": f"exact search, random binary DFA n={args.exact_n} seed={args.exact_seed}, "
                                 f"upper bound {ub} (Beam)", "length": r.length, "stats": r.stats}
         else:
-            line = bench_learned(ctx, args.scorer, list(range(1000, 1000 + args.learned_seeds)), args.learned_width)
+            seeds = list(range(1000, 1000 + args.learned_seeds))
+            widths = [int(w) for w in args.learned_width.split(",")]
+            reports = [bench_learned(ctx, args.scorer, seeds, w) for w in widths]
+            line = reports[0] if len(reports) == 1 else {
+                "workload": "learned against plain bound at several beam widths", "widths": widths,
+                "reports": reports}
         ctx.close()
         print(json.dumps(line))
         return 0
