@@ -64,7 +64,7 @@ constexpr int kIxPrefetch = SW_IX_PREFETCH;  // uint4 groups of offsets (32 stat
 #define SW_IX_EE_CHUNK 131072
 #endif
 #ifndef SW_IX_EE_WGS
-#define SW_IX_EE_WGS 1024
+#define SW_IX_EE_WGS 512
 #endif
 constexpr int kIxHead = SW_IX_HEAD;
 constexpr int kIxRing = SW_IX_RING;
