@@ -369,6 +369,26 @@ __global__ __launch_bounds__(256) void k_select(const uint64_t* __restrict__ row
 }
 
 template <int W>
+__global__ __launch_bounds__(256) void k_beam_select(const uint64_t* __restrict__ cand,
+    const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos, uint64_t count, uint32_t n,
+    uint64_t* __restrict__ out, uint32_t* __restrict__ idx_out, uint32_t* __restrict__ idx_copy,
+    unsigned long long* first) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count || !keep[i]) return;
+  const uint64_t j = pos[i];
+  bool full = true;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const uint64_t x = cand[i * W + w];
+    out[j * W + w] = x;
+    full &= x == word_mask(n, w);
+  }
+  idx_out[j] = uint32_t(i);
+  idx_copy[j] = uint32_t(i);
+  if (full) *first = i;  // one kept candidate at most is Q
+}
+
+template <int W>
 __global__ __launch_bounds__(256) void k_count_bits(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n,
     unsigned long long* acc) {
   __shared__ uint32_t hist[kMaxW * 64];
@@ -810,6 +830,21 @@ static hipError_t select_w(const uint64_t* rows, uint64_t lo, const uint32_t* ke
 hipError_t launch_select(int W, const uint64_t* rows, uint64_t lo, const uint32_t* keep, const uint32_t* pos,
     uint64_t count, uint64_t* out, uint32_t* idx_out, hipStream_t st) {
   SW_DISPATCH_W(W, select_w<w_>(rows, lo, keep, pos, count, out, idx_out, st))
+}
+
+template <int W>
+static hipError_t beam_select_w(const uint64_t* cand, const uint32_t* keep, const uint32_t* pos, uint64_t count,
+    uint32_t n, uint64_t* out, uint32_t* idx_out, uint32_t* idx_copy, unsigned long long* first, hipStream_t st) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_beam_select<W>, dim3(blocks_for(count)), dim3(256), 0, st, cand, keep, pos, count, n, out,
+      idx_out, idx_copy, first);
+  return hipGetLastError();
+}
+
+hipError_t launch_beam_select(int W, const uint64_t* cand, const uint32_t* keep, const uint32_t* pos,
+    uint64_t count, uint32_t n, uint64_t* out, uint32_t* idx_out, uint32_t* idx_copy, unsigned long long* first,
+    hipStream_t st) {
+  SW_DISPATCH_W(W, beam_select_w<w_>(cand, keep, pos, count, n, out, idx_out, idx_copy, first, st))
 }
 
 template <int W>

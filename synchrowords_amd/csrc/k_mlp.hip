@@ -352,15 +352,6 @@ hipError_t launch_features(int W, const uint64_t* sets, const uint64_t* images, 
 
 namespace {
 
-__global__ void k_first_full(const uint64_t* __restrict__ sets, uint64_t count, uint32_t n, uint32_t W,
-    unsigned long long* first) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  int c = 0;
-  for (uint32_t w = 0; w < W; ++w) c += __popcll(sets[i * W + w]);
-  if (uint32_t(c) == n) atomicMin(first, (unsigned long long)i);
-}
-
 __global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t count,
     uint32_t* __restrict__ out) {
   const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -376,13 +367,6 @@ __global__ void k_neg_popcount(const uint64_t* __restrict__ sets, uint64_t count
 }
 
 }  // namespace
-
-hipError_t launch_first_full(const uint64_t* sets, uint64_t count, uint32_t n, uint32_t W, unsigned long long* first,
-    hipStream_t st) {
-  if (!count) return hipSuccess;
-  hipLaunchKernelGGL(k_first_full, dim3(blocks(count)), dim3(256), 0, st, sets, count, n, W, first);
-  return hipGetLastError();
-}
 
 hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint64_t count, uint32_t* out,
     hipStream_t st) {

@@ -694,7 +694,6 @@ int beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* 
       SW_CHECK_HIP(sw::launch_beam_prep(table.as<uint32_t>(), tcap, first, st));
       SW_CHECK_HIP(sw::launch_expand(int(W), list.as<uint64_t>(), m0, dtb.as<uint64_t>(), S, k, nullptr, n, 0,
           cand.as<uint64_t>(), nullptr, st));
-      SW_CHECK_HIP(sw::launch_first_full(cand.as<uint64_t>(), cnt, n, W, first, st));
       // Dedup within the step (the smallest candidate index of equal sets
       // survives), launched before the one read-back of the step.
       SW_CHECK_HIP(sw::launch_hash_insert(int(W), cand.as<uint64_t>(), nullptr, cnt, table.as<uint32_t>(), tcap - 1,
@@ -711,8 +710,12 @@ int beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* 
           keep.as<uint32_t>(), pos.as<uint32_t>(), total, scan_tmp.p, &scan_bytes, st));
       uniq.ensure(cnt * W * 8 + 8);
       uidx.ensure(cnt * 4 + 4);
-      SW_CHECK_HIP(sw::launch_select(int(W), cand.as<uint64_t>(), 0, keep.as<uint32_t>(), pos.as<uint32_t>(), cnt,
-          uniq.as<uint64_t>(), uidx.as<uint32_t>(), st));
+      // The compaction also finds Q (the first full candidate) and writes the
+      // history indices of this depth in place, which stand when every
+      // distinct set is kept.
+      histd.ensure((hist_len + cnt) * 4 + 4, st, hist_len * 4);
+      SW_CHECK_HIP(sw::launch_beam_select(int(W), cand.as<uint64_t>(), keep.as<uint32_t>(), pos.as<uint32_t>(), cnt,
+          n, uniq.as<uint64_t>(), uidx.as<uint32_t>(), histd.as<uint32_t>() + hist_len, first, st));
       // first (8 bytes) and total (4 bytes at +16) in one copy
       SW_CHECK_HIP(hipMemcpyAsync(ws.host, small.p, 24, hipMemcpyDeviceToHost, st));
       if (stop) SW_CHECK_HIP(hipMemcpyAsync(ws.host + 4, stop->cell, 8, hipMemcpyDeviceToHost, st));
@@ -755,10 +758,7 @@ int beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* 
         // list is the compacted buffer itself.
         std::swap(list.p, uniq.p);
         std::swap(list.cap, uniq.cap);
-        if (nu)
-          SW_CHECK_HIP(hipMemcpyAsync(histd.as<uint32_t>() + hist_len, uidx.p, size_t(nu) * 4,
-              hipMemcpyDeviceToDevice, st));
-        hist_off.push_back(hist_len);
+        hist_off.push_back(hist_len);  // this depth's history: written by the compaction
         hist_m.push_back(m0);
         hist_len += keepn;
         m0 = keepn;

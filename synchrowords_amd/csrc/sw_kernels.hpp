@@ -386,6 +386,13 @@ hipError_t launch_goal_reset(int* found, unsigned long long* witness, hipStream_
 // out[j] = rows[lo + i] for kept i (j = pos[i]); idx_out[j] = lo + i.
 hipError_t launch_select(int W, const uint64_t* rows, uint64_t lo, const uint32_t* keep, const uint32_t* pos,
     uint64_t count, uint64_t* out, uint32_t* idx_out, hipStream_t st);
+// The device beam's compaction of one depth's distinct candidates: as
+// launch_select with lo = 0, and idx_copy[j] = i too; *first = i for the kept
+// candidate that is Q (all n states), if any (the smallest index of the full
+// candidates, since the kept one of equal candidates is the smallest).
+hipError_t launch_beam_select(int W, const uint64_t* cand, const uint32_t* keep, const uint32_t* pos,
+    uint64_t count, uint32_t n, uint64_t* out, uint32_t* idx_out, uint32_t* idx_copy, unsigned long long* first,
+    hipStream_t st);
 // Per-state frequency over a list (acc must be zeroed): acc[q] += #sets with q.
 hipError_t launch_count_bits(int W, const uint64_t* sets, uint64_t count, uint32_t n, unsigned long long* acc,
     hipStream_t st);

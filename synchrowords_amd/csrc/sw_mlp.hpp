@@ -52,9 +52,6 @@ hipError_t mlp_backward(const MlpDev& p, const uint16_t* x, uint64_t B, const fl
     float* loss, uint32_t rank_group, hipStream_t st);
 hipError_t mlp_adam(MlpDev& p, MlpGrads& g, MlpAdam& a, float lr, hipStream_t st);
 hipError_t mlp_refresh_bf16(MlpDev& p, hipStream_t st);
-// Smallest index of a set with all n states (first must hold ~0).
-hipError_t launch_first_full(const uint64_t* sets, uint64_t count, uint32_t n, uint32_t W, unsigned long long* first,
-    hipStream_t st);
 // out[j] = src[idx[j]].
 hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint64_t count, uint32_t* out,
     hipStream_t st);
