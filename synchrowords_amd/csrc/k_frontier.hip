@@ -372,9 +372,10 @@ template <int W>
 __global__ __launch_bounds__(256) void k_beam_select(const uint64_t* __restrict__ cand,
     const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos, uint64_t count, uint32_t n,
     uint64_t* __restrict__ out, uint32_t* __restrict__ idx_out, uint32_t* __restrict__ idx_copy,
-    unsigned long long* first) {
+    unsigned long long* first, uint32_t* __restrict__ table, const uint32_t* __restrict__ slot_of) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= count || !keep[i]) return;
+  table[slot_of[i]] = kEmpty;
   const uint64_t j = pos[i];
   bool full = true;
 #pragma unroll
@@ -834,17 +835,18 @@ hipError_t launch_select(int W, const uint64_t* rows, uint64_t lo, const uint32_
 
 template <int W>
 static hipError_t beam_select_w(const uint64_t* cand, const uint32_t* keep, const uint32_t* pos, uint64_t count,
-    uint32_t n, uint64_t* out, uint32_t* idx_out, uint32_t* idx_copy, unsigned long long* first, hipStream_t st) {
+    uint32_t n, uint64_t* out, uint32_t* idx_out, uint32_t* idx_copy, unsigned long long* first, uint32_t* table,
+    const uint32_t* slot_of, hipStream_t st) {
   if (!count) return hipSuccess;
   hipLaunchKernelGGL(k_beam_select<W>, dim3(blocks_for(count)), dim3(256), 0, st, cand, keep, pos, count, n, out,
-      idx_out, idx_copy, first);
+      idx_out, idx_copy, first, table, slot_of);
   return hipGetLastError();
 }
 
 hipError_t launch_beam_select(int W, const uint64_t* cand, const uint32_t* keep, const uint32_t* pos,
     uint64_t count, uint32_t n, uint64_t* out, uint32_t* idx_out, uint32_t* idx_copy, unsigned long long* first,
-    hipStream_t st) {
-  SW_DISPATCH_W(W, beam_select_w<w_>(cand, keep, pos, count, n, out, idx_out, idx_copy, first, st))
+    uint32_t* table, const uint32_t* slot_of, hipStream_t st) {
+  SW_DISPATCH_W(W, beam_select_w<w_>(cand, keep, pos, count, n, out, idx_out, idx_copy, first, table, slot_of, st))
 }
 
 template <int W>

@@ -681,17 +681,25 @@ int beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* 
       len0 = o[1];
     }
     build_preimage_tables();
+    // The hash table is emptied whole once (and when it grows); after that
+    // each depth's compaction empties the slots it filled.  The full-set
+    // index is reset once: a depth that sets it ends the beam.
+    unsigned long long* first = small.as<unsigned long long>();
+    bool table_clean = false;
     for (uint64_t len = len0; len < upper; ++len) {
       const uint64_t cnt = m0 * k;
       cand.ensure(cnt * W * 8 + 8);
       uint64_t tcap = 1024;
       while (tcap < 2 * cnt) tcap <<= 1;
+      const void* table_before = table.p;
       table.ensure(tcap * 4);
       slot.ensure(cnt * 4 + 4);
       keep.ensure(cnt * 4 + 4);
       pos.ensure(cnt * 4 + 4);
-      unsigned long long* first = small.as<unsigned long long>();
-      SW_CHECK_HIP(sw::launch_beam_prep(table.as<uint32_t>(), tcap, first, st));
+      if (!table_clean || table.p != table_before) {
+        SW_CHECK_HIP(sw::launch_beam_prep(table.as<uint32_t>(), table.cap / 4, first, st));
+        table_clean = true;
+      }
       SW_CHECK_HIP(sw::launch_expand(int(W), list.as<uint64_t>(), m0, dtb.as<uint64_t>(), S, k, nullptr, n, 0,
           cand.as<uint64_t>(), nullptr, st));
       // Dedup within the step (the smallest candidate index of equal sets
@@ -715,7 +723,8 @@ int beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* 
       // distinct set is kept.
       histd.ensure((hist_len + cnt) * 4 + 4, st, hist_len * 4);
       SW_CHECK_HIP(sw::launch_beam_select(int(W), cand.as<uint64_t>(), keep.as<uint32_t>(), pos.as<uint32_t>(), cnt,
-          n, uniq.as<uint64_t>(), uidx.as<uint32_t>(), histd.as<uint32_t>() + hist_len, first, st));
+          n, uniq.as<uint64_t>(), uidx.as<uint32_t>(), histd.as<uint32_t>() + hist_len, first, table.as<uint32_t>(),
+          slot.as<uint32_t>(), st));
       // first (8 bytes) and total (4 bytes at +16) in one copy
       SW_CHECK_HIP(hipMemcpyAsync(ws.host, small.p, 24, hipMemcpyDeviceToHost, st));
       if (stop) SW_CHECK_HIP(hipMemcpyAsync(ws.host + 4, stop->cell, 8, hipMemcpyDeviceToHost, st));

@@ -389,10 +389,12 @@ hipError_t launch_select(int W, const uint64_t* rows, uint64_t lo, const uint32_
 // The device beam's compaction of one depth's distinct candidates: as
 // launch_select with lo = 0, and idx_copy[j] = i too; *first = i for the kept
 // candidate that is Q (all n states), if any (the smallest index of the full
-// candidates, since the kept one of equal candidates is the smallest).
+// candidates, since the kept one of equal candidates is the smallest); and
+// table[slot_of[i]] = empty for every kept i, which empties every slot the
+// depth's hash insertion filled (each holds one kept candidate).
 hipError_t launch_beam_select(int W, const uint64_t* cand, const uint32_t* keep, const uint32_t* pos,
     uint64_t count, uint32_t n, uint64_t* out, uint32_t* idx_out, uint32_t* idx_copy, unsigned long long* first,
-    hipStream_t st);
+    uint32_t* table, const uint32_t* slot_of, hipStream_t st);
 // Per-state frequency over a list (acc must be zeroed): acc[q] += #sets with q.
 hipError_t launch_count_bits(int W, const uint64_t* sets, uint64_t count, uint32_t n, unsigned long long* acc,
     hipStream_t st);
