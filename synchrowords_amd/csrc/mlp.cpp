@@ -683,8 +683,11 @@ int beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* 
     build_preimage_tables();
     // The hash table is emptied whole once (and when it grows); after that
     // each depth's compaction empties the slots it filled.  The full-set
-    // index is reset once: a depth that sets it ends the beam.
-    unsigned long long* first = small.as<unsigned long long>();
+    // index is reset once: a depth that sets it ends the beam.  The index and
+    // the depth's number of distinct sets are written by the kernels straight
+    // into pinned host memory: a depth waits once and copies nothing.
+    unsigned long long* first = reinterpret_cast<unsigned long long*>(ws.host + 6);
+    uint32_t* total = reinterpret_cast<uint32_t*>(ws.host + 5);
     bool table_clean = false;
     for (uint64_t len = len0; len < upper; ++len) {
       const uint64_t cnt = m0 * k;
@@ -713,7 +716,6 @@ int beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* 
         scan_tmp.ensure(need);
         scan_bytes = need;
       }
-      uint32_t* total = small.as<uint32_t>() + 4;
       SW_CHECK_HIP(sw::launch_keep_scan(1, slot.as<uint32_t>(), table.as<uint32_t>(), nullptr, cnt,
           keep.as<uint32_t>(), pos.as<uint32_t>(), total, scan_tmp.p, &scan_bytes, st));
       uniq.ensure(cnt * W * 8 + 8);
@@ -725,12 +727,10 @@ int beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* 
       SW_CHECK_HIP(sw::launch_beam_select(int(W), cand.as<uint64_t>(), keep.as<uint32_t>(), pos.as<uint32_t>(), cnt,
           n, uniq.as<uint64_t>(), uidx.as<uint32_t>(), histd.as<uint32_t>() + hist_len, first, table.as<uint32_t>(),
           slot.as<uint32_t>(), st));
-      // first (8 bytes) and total (4 bytes at +16) in one copy
-      SW_CHECK_HIP(hipMemcpyAsync(ws.host, small.p, 24, hipMemcpyDeviceToHost, st));
       if (stop) SW_CHECK_HIP(hipMemcpyAsync(ws.host + 4, stop->cell, 8, hipMemcpyDeviceToHost, st));
       SW_CHECK_HIP(hipStreamSynchronize(st));
-      const unsigned long long f = ws.host[0];
-      const uint32_t nu = uint32_t(ws.host[2]);
+      const unsigned long long f = *reinterpret_cast<volatile unsigned long long*>(first);
+      const uint32_t nu = *reinterpret_cast<volatile uint32_t*>(total);
       if (stop && uint32_t(ws.host[4] >> 32) == stop->gen && len >= uint32_t(ws.host[4])) break;  // capped there
       if (f != ~0ull) {
         // Word: letter of this candidate, then back through the kept parents.
@@ -810,9 +810,9 @@ int beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* 
           scan_tmp.ensure(need);
           scan_bytes = need;
         }
-        uint32_t* total = small.as<uint32_t>() + 4;
+        uint32_t* kept_total = small.as<uint32_t>() + 4;  // (keepn, known already)
         SW_CHECK_HIP(sw::launch_keep_scan(0, nullptr, nullptr, nullptr, nu, vals.as<uint32_t>(), pos.as<uint32_t>(),
-            total, scan_tmp.p, &scan_bytes, st));
+            kept_total, scan_tmp.p, &scan_bytes, st));
         SW_CHECK_HIP(sw::launch_select(int(W), uniq.as<uint64_t>(), 0, vals.as<uint32_t>(), pos.as<uint32_t>(), nu,
             list.as<uint64_t>(), vals2.as<uint32_t>(), st));
         SW_CHECK_HIP(sw::launch_gather_u32(uidx.as<uint32_t>(), vals2.as<uint32_t>(), keepn,
