@@ -143,6 +143,8 @@ void sw_ctx_destroy(sw_ctx* ctx) {
   ctx->ix_host = nullptr;
   if (ctx->stage) (void)hipHostFree(ctx->stage);
   ctx->stage = nullptr;
+  if (ctx->stop_cell) (void)hipHostFree(ctx->stop_cell);
+  ctx->stop_cell = nullptr;
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
   (void)hipStreamDestroy(ctx->stream);

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kThreadsBS) void k_beam_small(BeamSmallArgs a) {
     if (a.stop && (len & 15u) == 0) {
       // Every 16 depths: has the caller's other stream set a smaller cap?
       if (tid == 0) {
-        const unsigned long long v = __hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long v = __hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         s_halt = (uint32_t(v >> 32) == a.stop_gen && len >= uint32_t(v)) ? 1u : 0u;
       }
       __syncthreads();
@@ -242,15 +242,6 @@ __global__ __launch_bounds__(kThreadsBS) void k_beam_small(BeamSmallArgs a) {
     a.out[2] = m;
     a.out[3] = hist_len;
   }
-}
-
-__global__ void k_set_stop(unsigned long long* cell, unsigned long long value) {
-  __hip_atomic_store(cell, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-hipError_t launch_set_stop(unsigned long long* cell, uint64_t value, hipStream_t st) {
-  hipLaunchKernelGGL(k_set_stop, dim3(1), dim3(1), 0, st, cell, (unsigned long long)value);
-  return hipGetLastError();
 }
 
 hipError_t launch_beam_small(const BeamSmallArgs& a, hipStream_t st) {

@@ -727,11 +727,13 @@ int beam_device(sw_ctx* ctx, sw_mlp* m, uint32_t n, uint32_t k, const uint32_t* 
       SW_CHECK_HIP(sw::launch_beam_select(int(W), cand.as<uint64_t>(), keep.as<uint32_t>(), pos.as<uint32_t>(), cnt,
           n, uniq.as<uint64_t>(), uidx.as<uint32_t>(), histd.as<uint32_t>() + hist_len, first, table.as<uint32_t>(),
           slot.as<uint32_t>(), st));
-      if (stop) SW_CHECK_HIP(hipMemcpyAsync(ws.host + 4, stop->cell, 8, hipMemcpyDeviceToHost, st));
       SW_CHECK_HIP(hipStreamSynchronize(st));
       const unsigned long long f = *reinterpret_cast<volatile unsigned long long*>(first);
       const uint32_t nu = *reinterpret_cast<volatile uint32_t*>(total);
-      if (stop && uint32_t(ws.host[4] >> 32) == stop->gen && len >= uint32_t(ws.host[4])) break;  // capped there
+      if (stop) {
+        const unsigned long long v = __atomic_load_n(stop->cell, __ATOMIC_ACQUIRE);
+        if (uint32_t(v >> 32) == stop->gen && len >= uint32_t(v)) break;  // capped there
+      }
       if (f != ~0ull) {
         // Word: letter of this candidate, then back through the kept parents.
         std::vector<uint32_t> h(hist_len);

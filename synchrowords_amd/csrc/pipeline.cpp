@@ -402,6 +402,11 @@ class Helper {
   std::exception_ptr err_;
 };
 
+// The stop cell's new value, one 64-bit store the beam reads whole.
+void set_stop(sw_ctx* ctx, uint64_t value) {
+  __atomic_store_n(ctx->stop_cell, (unsigned long long)value, __ATOMIC_RELEASE);
+}
+
 // The context's helper, made on first use; NULL when one cannot be made
 // (then the stages run in order, and no helper is tried again).
 Helper* helper_of(sw_ctx* ctx) {
@@ -611,14 +616,15 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
           if (width && pass_len > 1) helper = helper_of(ctx);
           if (helper) {
             br = relabelled(helper->ctx(), n, k, d, bs);
-            if (!ctx->stop_cell.p) {
-              ctx->stop_cell.ensure(8);
-              SW_CHECK_HIP(launch_set_stop(ctx->stop_cell.as<unsigned long long>(), 0, ctx->stream));
-              SW_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+            if (!ctx->stop_cell) {
+              // Pinned host memory: this thread writes it, the beam's kernel
+              // (or its host loop) reads it.
+              SW_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->stop_cell), 64, hipHostMallocDefault));
+              set_stop(ctx, 0);
             }
             gen = ++ctx->stop_gen;
             if (gen == 0) gen = ++ctx->stop_gen;  // 0 is the cell's first value
-            const BeamStop stop{ctx->stop_cell.as<unsigned long long>(), gen};
+            const BeamStop stop{ctx->stop_cell, gen};
             helper->start([&br, &pass, width, pass_len, stop] { pass = br.run(width, pass_len, &stop); });
           }
         }
@@ -630,7 +636,7 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
           uint32_t gen;
           ~PassGuard() {
             if (!h) return;
-            (void)launch_set_stop(ctx->stop_cell.as<unsigned long long>(), uint64_t(gen) << 32, ctx->stream);
+            set_stop(ctx, uint64_t(gen) << 32);
             try {
               h->wait();
             } catch (...) {
@@ -642,8 +648,7 @@ void pipeline(sw_ctx* ctx, uint32_t n, uint32_t k, const uint32_t* d, const char
           // The depth the pass would have stopped at, run after this stage.
           const auto t_split = std::chrono::steady_clock::now();
           const uint64_t after = ns ? 0 : beam_maxlen(imp ? bound : upper, bs);
-          SW_CHECK_HIP(launch_set_stop(ctx->stop_cell.as<unsigned long long>(),
-              (uint64_t(gen) << 32) | std::min<uint64_t>(after, 0xffffffffu), ctx->stream));
+          set_stop(ctx, (uint64_t(gen) << 32) | std::min<uint64_t>(after, 0xffffffffu));
           guard.h = nullptr;
           helper->wait();
           first.ran = !ns;

@@ -148,7 +148,7 @@ struct sw_ctx {
   // of its latest use.
   void* helper = nullptr;
   void (*helper_free)(void*) = nullptr;
-  sw::DevBuf stop_cell;
+  unsigned long long* stop_cell = nullptr;  // pinned host memory, read by the beam's kernel
   uint32_t stop_gen = 0;
   bool helper_ctx = false;     // this context is such a helper's own
   bool helper_failed = false;  // no helper could be made: the stages run in order
@@ -169,8 +169,8 @@ void* stage_bytes(sw_ctx* ctx, size_t bytes);
 int user_contexts();
 void mark_helper_context(sw_ctx* ctx);
 
-// A stop cell for a beam that runs beside other work: the cell holds
-// (generation << 32) | depth, written by the other work's stream.
+// A stop cell for a beam that runs beside other work: the cell, in pinned
+// host memory, holds (generation << 32) | depth, written by the host.
 struct BeamStop {
   const unsigned long long* cell;
   uint32_t gen;

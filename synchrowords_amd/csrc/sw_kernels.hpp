@@ -217,15 +217,13 @@ struct BeamSmallArgs {
   uint32_t* word;
   uint64_t* list_out;
   uint32_t* out;
-  // Optional stop cell, (generation << 32) | depth, written by another
-  // stream while the beam runs: every 16 depths the beam reads it and ends
-  // (no Q) once it holds stop_gen and len >= depth.
+  // Optional stop cell in pinned host memory, (generation << 32) | depth,
+  // written by the host while the beam runs: every 16 depths the beam reads
+  // it and ends (no Q) once it holds stop_gen and len >= depth.
   const unsigned long long* stop;
   uint32_t stop_gen;
 };
 hipError_t launch_beam_small(const BeamSmallArgs& a, hipStream_t st);
-// *cell = value as one 64-bit store (a stop cell that a running kernel reads).
-hipError_t launch_set_stop(unsigned long long* cell, uint64_t value, hipStream_t st);
 
 // Power-set BFS (k_brute.hip).  One automaton per workgroup with its bitmap
 // in LDS (brute_lds_bytes(n, k) <= kBruteMaxLds): frontier_off
